@@ -1,0 +1,210 @@
+/*
+ * scphd.h -- C ABI of the MI355X (gfx950) single-cluster PHD stereo camera
+ * calibration filter.
+ *
+ * Every entry point below replaces one operation of the reference class
+ * SCPHDCameraCalibration (cheesinglee/scphd-camera-calibration,
+ * src/scphdcameracalibration.cuh:29-122).  The citation next to each
+ * declaration names the reference member (file:line) it stands in for.
+ *
+ * Conventions
+ *   - Return codes are int: SCPHD_OK (0) or a negative SCPHD_ERR_* value.
+ *     scphd_last_error() returns a message for the last failing call on a
+ *     handle.  No call throws or exits the process.
+ *   - Caller buffers (host pointers unless the name says _device) are only
+ *     read or written during the call.
+ *   - A handle owns all device state and one HIP stream.  Distinct handles
+ *     may be used from distinct threads; one handle is not reentrant.
+ *   - Arrays of parent extrinsics are parent-major, 12 doubles per parent:
+ *     cartesian x y z vx vy vz, angular x y z vx vy vz (the reference
+ *     Extrinsics struct, src/types.h:137-140).
+ *   - Sample clouds cross the ABI as 7 structure-of-arrays planes of doubles
+ *     (world x y z vx vy vz, sample weight), Gaussian-major in canonical
+ *     order: parent ascending, then Gaussian within parent, then sample.
+ */
+#ifndef SCPHD_H
+#define SCPHD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SCPHD_OK 0
+#define SCPHD_ERR_ARG (-1)       /* invalid argument or size                 */
+#define SCPHD_ERR_HIP (-2)       /* HIP runtime failure                      */
+#define SCPHD_ERR_NAN (-3)       /* NaN parent weight (checkStuff)           */
+#define SCPHD_ERR_CAPACITY (-4)  /* max_gaussians per parent exceeded        */
+#define SCPHD_ERR_CONFIG (-5)    /* configuration file missing or malformed  */
+#define SCPHD_ERR_STATE (-6)     /* call not valid in the current state      */
+#define SCPHD_ERR_NODEVICE (-7)  /* no usable GPU                            */
+
+#define SCPHD_STORAGE_FP32 0     /* sample clouds stored as float  */
+#define SCPHD_STORAGE_FP64 1     /* sample clouds stored as double */
+
+/* Filter parameters: the keys the reference reads from its libconfig file
+ * (src/scphdcameracalibration.cu:827-1005, :1145, :1246-1247, :1343, :1744,
+ * :1920, :2596-2610, :2655; src/main.cpp:74,77).  Layout is part of the ABI. */
+typedef struct scphd_params {
+    int32_t n_particles;            /* parents N_p                              */
+    int32_t particles_per_feature;  /* samples per Gaussian S                   */
+    int32_t max_gaussians;          /* capacity per parent, 0 = unbounded       */
+    int32_t storage;                /* SCPHD_STORAGE_*                          */
+    int32_t verbosity;
+    int32_t slam;                   /* main loop: update with camera 2 only     */
+    int32_t max_steps;
+    int32_t save_all_maps;          /* accepted, no effect (map dumps)          */
+    double min_weight;              /* prune threshold                          */
+    double min_separation;          /* merge threshold (Mahalanobis, squared)   */
+    double min_neff;                /* resample when n_eff < min_neff           */
+    double ps;                      /* survival probability                     */
+    double prior_a[12];             /* uniform prior lower bounds, Extrinsics order */
+    double prior_b[12];             /* uniform prior upper bounds               */
+    double cam_accel_std[6];        /* camera CV model: cartesian ax ay az, angular ax ay az */
+    double feat_accel_std[3];       /* landmark CV model: ax ay az              */
+    double fx, fy, u0, v0;          /* disparity measurement model              */
+    double std_u, std_v, pd, lambda;
+    double birth_w0, birth_d0, birth_vu0, birth_vv0, birth_vd0;
+    double birth_var_d0, birth_var_vu0, birth_var_vv0, birth_var_vd0;
+    double max_velocity, min_range; /* read by the reference, unused            */
+    uint64_t seed;                  /* Philox4x32-10 key for every noise stream */
+} scphd_params;
+
+typedef struct scphd_filter scphd_t;
+
+/* Fill *p with the defaults of the reference configuration file
+ * (config/calibration.cfg) and seed 0. */
+int scphd_params_init(scphd_params* p);
+
+/* Parse a libconfig-syntax file into *p.  Accepts the keys the reference code
+ * reads (prior.*.{a,b}) and the shipped file's prior.*.{mean,std}, which maps
+ * to a = mean - sqrt(3) std, b = mean + sqrt(3) std.  Replaces the libconfig
+ * lookups of SCPHDCameraCalibration::SCPHDCameraCalibration
+ * (src/scphdcameracalibration.cu:823-1006). err may be NULL. */
+int scphd_params_load(const char* cfg_path, scphd_params* p, char* err, size_t errlen);
+
+/* Construct a filter: pick the GPU, allocate device state, draw the uniform
+ * prior over parent extrinsics.  Replaces the constructor
+ * src/scphdcameracalibration.cu:823-1006 and initializeCuda :1008-1032. */
+int scphd_create(const scphd_params* p, scphd_t** out);
+
+/* Free all device state.  Replaces the (implicit) destructor of the
+ * reference class, src/scphdcameracalibration.cuh:29. */
+void scphd_destroy(scphd_t* f);
+
+/* Message for the last failing call on f (never NULL). */
+const char* scphd_last_error(const scphd_t* f);
+
+/* Constant-velocity prediction of parents and landmark samples, survival
+ * reweighting.  Replaces predict(double dt), src/scphdcameracalibration.cu:1034-1151. */
+int scphd_predict(scphd_t* f, double dt);
+
+/* GM-PHD update of every daughter map with one camera's measurement set and,
+ * when fixed_camera == 0, the single-cluster parent reweighting.  u and v are
+ * host arrays of n_meas pixel coordinates.  Replaces
+ * update(vector<double> u, vector<double> v, bool fixed_camera),
+ * src/scphdcameracalibration.cu:1576-2094. */
+int scphd_update(scphd_t* f, const double* u, const double* v, int32_t n_meas,
+                 int32_t fixed_camera);
+
+/* Same as scphd_update with u and v already resident in device memory
+ * (no counterpart in the reference, whose measurements arrive as host
+ * vectors at src/scphdcameracalibration.cu:1659-1660). */
+int scphd_update_device(scphd_t* f, const double* u_device, const double* v_device,
+                        int32_t n_meas, int32_t fixed_camera);
+
+/* Systematic resampling of parents when n_eff < min_neff.  *resampled (may be
+ * NULL) is set to 1 when parents were resampled.  Replaces resample(),
+ * src/scphdcameracalibration.cu:2651-2817 and computeNeff :2641-2649. */
+int scphd_resample(scphd_t* f, int32_t* resampled);
+
+/* NaN guard on the parent weights: SCPHD_ERR_NAN instead of exit(2).
+ * Replaces checkStuff(), src/scphdcameracalibration.cu:812-821. */
+int scphd_check(scphd_t* f);
+
+/* Block until all work queued on the handle's stream has finished. */
+int scphd_sync(scphd_t* f);
+
+typedef struct scphd_counts {
+    int64_t n_particles;
+    int64_t particles_per_feature;
+    int64_t n_gaussians;      /* N_g, Gaussians over all parents              */
+    int64_t n_samples;        /* N_g * S                                      */
+    int64_t n_predict;        /* predict() calls so far (noise step counters) */
+    int64_t n_update;
+    int64_t n_resample;
+    int64_t pending_predict;  /* 1 when a predict is deferred into the next update */
+} scphd_counts;
+
+int scphd_get_counts(scphd_t* f, scphd_counts* c);
+
+/* Parent states (N_p x 12, parent-major) and normalised parent weights.
+ * Replaces the particle/weight read-back of writeMat,
+ * src/scphdcameracalibration.cu:2200-2260. Either pointer may be NULL. */
+int scphd_get_parents(scphd_t* f, double* states, double* weights);
+
+/* Daughter maps in canonical order: gcount[N_p] Gaussians per parent,
+ * wg[N_g] Gaussian (feature) weights, samples[7 * N_s] planes
+ * x y z vx vy vz w.  Any pointer may be NULL.  Materialises a deferred
+ * predict first.  Replaces the map read-back of writeMat,
+ * src/scphdcameracalibration.cu:2262-2311. */
+int scphd_get_maps(scphd_t* f, int32_t* gcount, double* wg, double* samples);
+
+/* Replace the whole filter state (parents, weights, maps, noise counters).
+ * Arrays as in scphd_get_parents / scphd_get_maps; n_gaussians must equal the
+ * sum of gcount.  No counterpart in the reference (its state is private);
+ * used for lock-step comparison and rig snapshot/resume. */
+int scphd_set_state(scphd_t* f, const double* states, const double* weights,
+                    const int32_t* gcount, int64_t n_gaussians, const double* wg,
+                    const double* samples, int64_t n_predict, int64_t n_update,
+                    int64_t n_resample);
+
+/* Diagnostics of the last update (all counts are totals over parents). */
+typedef struct scphd_update_stats {
+    int64_t n_meas;            /* M                                            */
+    int64_t n_gaussians_in;    /* N_g entering the update                      */
+    int64_t n_pairs;           /* N_g * M component updates                    */
+    int64_t n_survivors;       /* detection + birth terms kept for merging     */
+    int64_t n_merged;          /* Gaussians out of the merge                   */
+    int64_t n_nondetect;       /* non-detection Gaussians kept                 */
+    int64_t n_gaussians_out;   /* n_merged + n_nondetect                       */
+    int64_t max_survivors_per_parent;
+    int64_t cap_exceeded;      /* parents whose map exceeded max_gaussians     */
+} scphd_update_stats;
+
+int scphd_get_update_stats(scphd_t* f, scphd_update_stats* s);
+
+/* Per-parent intermediates of the last update, for verification: eta
+ * [N_p * M] log normalisers (parent-major), loglik[N_p] parent
+ * log-likelihood, nmerged/nnondetect/nsurvivors[N_p] counts.  Any pointer may
+ * be NULL.  No counterpart in the reference. */
+int scphd_get_update_debug(scphd_t* f, double* eta, double* loglik, int32_t* nmerged,
+                           int32_t* nnondetect, int32_t* nsurvivors);
+
+/* Per-Gaussian fit of the last update, 28 planes of n_gaussians_in values:
+ * disparity-space mean (6), covariance (21, packed lower triangle row by
+ * row), detection probability pd_g.  For verification of the refit
+ * (fitGaussiansKernel, src/kernels.cu:102-438, and computePdKernel
+ * :82-99).  out must hold 28 * n_gaussians_in doubles. */
+int scphd_get_fit_debug(scphd_t* f, double* out);
+
+/* Ancestor index of every parent chosen by the last scphd_resample that
+ * resampled (identity otherwise).  No counterpart in the reference (host
+ * array idx_resample, src/scphdcameracalibration.cu:2659). */
+int scphd_get_ancestors(scphd_t* f, int32_t* ancestors);
+
+/* Wall time in milliseconds of the phases of the last update, measured with
+ * HIP events on the handle's stream: [0] reproject+fit, [1] normalisers and
+ * survivors, [2] merge, [3] posterior sampling, [4] map assembly, [5] total.
+ * Requires the handle to have been created with timing enabled (see
+ * scphd_set_timing). */
+int scphd_set_timing(scphd_t* f, int32_t enabled);
+int scphd_get_phase_ms(scphd_t* f, double* ms6);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SCPHD_H */

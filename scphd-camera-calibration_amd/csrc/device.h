@@ -1,0 +1,251 @@
+/*
+ * device.h -- numerics shared by the gfx950 kernels of the SC-PHD filter:
+ * counter-based noise, safe logarithm, packed 6x6 symmetric algebra.
+ *
+ * Covariances are stored packed, lower triangle, row by row:
+ *   idx(i, j) = i (i + 1) / 2 + j   for i >= j   (21 values).
+ */
+#ifndef SCPHD_DEVICE_H
+#define SCPHD_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <float.h>
+
+#include "scphd_models.h"
+
+#define SCPHD_LOG0 (-(double)FLT_MAX) /* log(0) value, src/device_math.cu:8 */
+
+namespace scphd {
+
+/* ----------------------------------------------------------------- noise */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+ * easy as 1, 2, 3", SC'11): 10 rounds of two 32x32->64 multiplies, key
+ * bumped by the Weyl constants between rounds. */
+SCPHD_HD inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                   uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t a = (uint64_t)0xD2511F53u * c0;
+        const uint64_t b = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(b >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(a >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)b;
+        c3 = (uint32_t)a;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+/* Noise streams.  Counter words: c0 = element << 2 | block,
+ * c1 = Gaussian index, c2 = parent, c3 = step << 4 | stream; key = seed. */
+enum NoiseStream : uint32_t {
+    NS_PRIOR = 0,
+    NS_CAMERA = 1,
+    NS_FEATURE = 2,
+    NS_POSTERIOR = 3,
+    NS_RESAMPLE = 4,
+};
+
+struct NoiseKey {
+    uint32_t k0, k1;
+    SCPHD_HD static NoiseKey from_seed(uint64_t s) {
+        NoiseKey k;
+        k.k0 = (uint32_t)s;
+        k.k1 = (uint32_t)(s >> 32);
+        return k;
+    }
+};
+
+SCPHD_HD inline void noise_block(NoiseKey key, uint32_t stream, uint64_t step, uint32_t parent,
+                                 uint32_t gauss, uint32_t elem, uint32_t block, uint32_t out[4]) {
+    philox4x32_10((elem << 2) | block, gauss, parent, (uint32_t)((step << 4) | stream), key.k0, key.k1, out);
+}
+
+/* 53-bit uniforms from two 32-bit words. */
+SCPHD_HD inline double uniform_co(uint32_t lo, uint32_t hi) { /* [0, 1) */
+    return (double)((((uint64_t)hi << 32) | lo) >> 11) * 0x1.0p-53;
+}
+SCPHD_HD inline double uniform_oc(uint32_t lo, uint32_t hi) { /* (0, 1] */
+    return (double)(((((uint64_t)hi << 32) | lo) >> 11) + 1) * 0x1.0p-53;
+}
+
+/* N (<= 6) standard normals by the Box-Muller transform, two per block. */
+template <int N>
+SCPHD_HD inline void normals(NoiseKey key, uint32_t stream, uint64_t step, uint32_t parent, uint32_t gauss,
+                             uint32_t elem, double z[N]) {
+#pragma unroll
+    for (int b = 0; 2 * b < N; ++b) {
+        uint32_t x[4];
+        noise_block(key, stream, step, parent, gauss, elem, (uint32_t)b, x);
+        const double a = uniform_oc(x[0], x[1]);
+        const double c = uniform_co(x[2], x[3]);
+        const double r = sqrt(-2.0 * log(a));
+        const double th = (2.0 * M_PI) * c;
+        z[2 * b] = r * cos(th);
+        if (2 * b + 1 < N) z[2 * b + 1] = r * sin(th);
+    }
+}
+
+/* ------------------------------------------------------------------ math */
+SCPHD_HD inline double safe_log(double x) { return x <= 0 ? SCPHD_LOG0 : log(x); }
+
+SCPHD_HD constexpr int pidx(int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
+
+/* Lower Cholesky factor (packed) of a packed symmetric 6x6, row by row
+ * (Cholesky-Banachiewicz), no PD check: a non-PD input leaves NaN. */
+SCPHD_HD inline void chol6(const double* A, double* L) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+#pragma unroll
+        for (int j = 0; j <= i; ++j) {
+            double t = A[pidx(i, j)];
+            if (i == j) {
+#pragma unroll
+                for (int k = 0; k < j; ++k) t -= L[pidx(j, k)] * L[pidx(j, k)];
+                L[pidx(i, j)] = sqrt(t);
+            } else {
+#pragma unroll
+                for (int k = 0; k < j; ++k) t -= L[pidx(i, k)] * L[pidx(j, k)];
+                L[pidx(i, j)] = t / L[pidx(j, j)];
+            }
+        }
+    }
+}
+
+/* Positive-definiteness test (isPosDef, src/device_math.cu:800-812, which
+ * checks the Cholesky factor for NaN).  Here every Cholesky pivot (the value
+ * under the square root) must exceed PD_REL times its diagonal entry: the
+ * same answer as the NaN test for every matrix whose smallest relative pivot
+ * is above 1e-9, and a deterministic "not PD" for numerically singular ones
+ * (rank-deficient refits of clouds with a few in-range samples sit at 1e-13
+ * and below, where the NaN test is decided by rounding). */
+static constexpr double PD_REL = 1e-9;
+
+SCPHD_HD inline bool is_pd6(const double* A) {
+    double L[21];
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+#pragma unroll
+        for (int j = 0; j <= i; ++j) {
+            double t = A[pidx(i, j)];
+            if (i == j) {
+#pragma unroll
+                for (int k = 0; k < j; ++k) t -= L[pidx(j, k)] * L[pidx(j, k)];
+                ok = ok && (t > PD_REL * A[pidx(i, i)]);
+                L[pidx(i, j)] = sqrt(t);
+            } else {
+#pragma unroll
+                for (int k = 0; k < j; ++k) t -= L[pidx(i, k)] * L[pidx(j, k)];
+                L[pidx(i, j)] = t / L[pidx(j, j)];
+            }
+        }
+    }
+    return ok;
+}
+
+/* Inverse of a packed lower-triangular 6x6 by forward substitution. */
+SCPHD_HD inline void trinv6(const double* L, double* Li) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+#pragma unroll
+        for (int i = c; i < 6; ++i) {
+            double v = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+            for (int j = c; j < i; ++j) v -= L[pidx(i, j)] * Li[pidx(j, c)];
+            Li[pidx(i, c)] = v / L[pidx(i, i)];
+        }
+    }
+}
+
+/* Squared Mahalanobis distance with Sigma = Pa + Pb (packed covariances). */
+SCPHD_HD inline double mahal6(const double* ma, const double* Pa, const double* mb, const double* Pb) {
+    double d[6], S[21], L[21], Li[21];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) d[i] = ma[i] - mb[i];
+#pragma unroll
+    for (int i = 0; i < 21; ++i) S[i] = Pa[i] + Pb[i];
+    chol6(S, L);
+    trinv6(L, Li);
+    double dist = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j <= i; ++j) s += d[j] * Li[pidx(i, j)];
+        dist += s * s;
+    }
+    return dist;
+}
+
+/* Measurement-independent part of the 2-D Kalman detection update with
+ * H = [I2 0], R = diag(var_u, var_v) (computeDetectionsKernel,
+ * src/kernels.cu:492-595): symmetrised innovation covariance and its
+ * inverse, gain K = P H^T S^-1 (6x2, column 0 then column 1) and the
+ * Joseph-form covariance (I-KH) P (I-KH)^T + K R K^T. */
+struct DetectionTerms {
+    double s0, s1, s2, s3; /* S^-1, column-major */
+    double det;
+    double K[12];
+    double Pn[21];
+};
+
+SCPHD_HD inline void detection_terms(const double* P, double var_u, double var_v, DetectionTerms& r) {
+    double g0 = P[pidx(0, 0)] + var_u;
+    double g1 = P[pidx(1, 0)];
+    double g2 = P[pidx(0, 1)];
+    double g3 = P[pidx(1, 1)] + var_v;
+    g1 = (g1 + g2) / 2;
+    g2 = g1;
+    const double det = g0 * g3 - g1 * g2;
+    r.det = det;
+    r.s0 = g3 / det;
+    r.s1 = -g1 / det;
+    r.s2 = -g2 / det;
+    r.s3 = g0 / det;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        r.K[i] = P[pidx(i, 0)] * r.s0 + P[pidx(i, 1)] * r.s1;
+        r.K[6 + i] = P[pidx(i, 0)] * r.s2 + P[pidx(i, 1)] * r.s3;
+    }
+    /* A = I - K H: columns 0, 1 are e_c - K(:, c); columns 2..5 are e_c. */
+    double AP[36]; /* row-major A P */
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const double a = ((i == k) ? 1.0 : 0.0) - (k == 0 ? r.K[i] : (k == 1 ? r.K[6 + i] : 0.0));
+                s += a * P[pidx(k, j)];
+            }
+            AP[i * 6 + j] = s;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+#pragma unroll
+        for (int j = 0; j <= i; ++j) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const double a = ((j == k) ? 1.0 : 0.0) - (k == 0 ? r.K[j] : (k == 1 ? r.K[6 + j] : 0.0));
+                s += AP[i * 6 + k] * a;
+            }
+            s += r.K[i] * var_u * r.K[j] + r.K[6 + i] * var_v * r.K[6 + j];
+            r.Pn[pidx(i, j)] = s;
+        }
+    }
+}
+
+}  // namespace scphd
+
+#endif /* SCPHD_DEVICE_H */

@@ -1,0 +1,803 @@
+/*
+ * filter.hip -- C ABI of the SC-PHD calibration filter (include/scphd.h):
+ * device state ownership and the host-side sequencing of the kernels of
+ * one filter step on the handle's stream.
+ *
+ * The step follows SCPHDCameraCalibration (cheesinglee/scphd-camera-calibration
+ * src/scphdcameracalibration.cu): predict :1034-1151, update :1576-2094,
+ * resample :2651-2817.  Host<->device traffic per update is the measurement
+ * set and two small count read-backs (per-parent survivor offsets and the
+ * output totals) used to size buffers.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "scphd.h"
+
+using namespace scphd;
+
+namespace {
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    int64_t n = 0;
+    hipError_t ensure(int64_t need) {
+        if (need <= n) return hipSuccess;
+        release();
+        const int64_t cap = std::max<int64_t>(need + need / 4, 64);
+        hipError_t e = hipMalloc((void**)&p, (size_t)cap * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        n = cap;
+        return hipSuccess;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+/* One generation of sample blocks: 7 planes of float or double. */
+struct Generation {
+    void* pl[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int64_t cap_blocks = 0;
+    void release() {
+        for (auto& x : pl) {
+            if (x) (void)hipFree(x);
+            x = nullptr;
+        }
+        cap_blocks = 0;
+    }
+    /* grow to at least need blocks keeping the first keep blocks */
+    hipError_t ensure(int64_t need, int64_t keep, int S, size_t esz, hipStream_t s) {
+        if (need <= cap_blocks) return hipSuccess;
+        const int64_t cap = std::max<int64_t>(need + need / 4, 16);
+        void* np[7] = {};
+        for (int c = 0; c < 7; ++c) {
+            hipError_t e = hipMalloc(&np[c], (size_t)cap * S * esz);
+            if (e != hipSuccess) {
+                for (int d = 0; d < c; ++d) (void)hipFree(np[d]);
+                return e;
+            }
+            if (keep > 0 && pl[c]) {
+                e = hipMemcpyAsync(np[c], pl[c], (size_t)keep * S * esz, hipMemcpyDeviceToDevice, s);
+                if (e != hipSuccess) return e;
+            }
+        }
+        hipError_t e = hipStreamSynchronize(s);
+        release();
+        for (int c = 0; c < 7; ++c) pl[c] = np[c];
+        cap_blocks = cap;
+        return e;
+    }
+    template <typename T>
+    Planes<T> planes() const {
+        Planes<T> r;
+        for (int c = 0; c < 7; ++c) r.p[c] = (T*)pl[c];
+        return r;
+    }
+};
+
+}  // namespace
+
+struct scphd_filter {
+    scphd_params prm;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    FilterModels models;
+    BirthModel birth;
+    NoiseKey key;
+    int Np = 0, S = 0;
+    size_t esz = 4;
+
+    /* parents */
+    DBuf<double> ext, pw, pose, loglik, cum, ext_tmp;
+    DBuf<int32_t> anc, newcnt;
+    /* map table and its successor */
+    int64_t Ng = 0;
+    DBuf<int32_t> goff, gparent, gblock;
+    DBuf<double> gw;
+    DBuf<int32_t> goff2, gparent2, gblock2;
+    DBuf<double> gw2;
+    /* sample generations */
+    Generation gen[2];
+    int cur = 0;
+    /* per-Gaussian intermediates */
+    DBuf<double> fit, pre;
+    DBuf<int32_t> pdok, keep, keepscan, ndlist;
+    /* measurements, normalisers, survivors, merge */
+    DBuf<double> du, dv, eta;
+    int capM = 0;
+    DBuf<int32_t> nsurv, survoff, nmerged, mergeoff, outcnt, scantmp, sg, sm, dflag;
+    DBuf<double> sw, smu, mw, mmu, mP, dneff;
+    DBuf<char> mscratch;
+    int32_t* hbuf = nullptr; /* pinned read-back area */
+    int64_t hbuf_n = 0;
+    std::vector<int32_t> hoff;
+
+    /* counters and deferred prediction */
+    int64_t n_predict = 0, n_update = 0, n_resample = 0;
+    bool pending = false;
+    double pdt = 0.0;
+    uint64_t pstep = 0;
+
+    scphd_update_stats stats{};
+    int64_t last_M = 0;
+    bool timing = false;
+    hipEvent_t ev[6] = {};
+    double phase_ms[6] = {};
+    int lds_cap = 0;
+};
+
+namespace {
+
+int fail(scphd_filter* f, int code, const std::string& msg) {
+    if (f) f->err = msg;
+    return code;
+}
+
+int hipfail(scphd_filter* f, hipError_t e, const char* what) {
+    return fail(f, SCPHD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPTRY(expr)                                              \
+    do {                                                          \
+        hipError_t e__ = (expr);                                  \
+        if (e__ != hipSuccess) return hipfail(f, e__, #expr);     \
+    } while (0)
+
+int ensure_hbuf(scphd_filter* f, int64_t n) {
+    if (n <= f->hbuf_n) return SCPHD_OK;
+    if (f->hbuf) (void)hipHostFree(f->hbuf);
+    f->hbuf = nullptr;
+    const int64_t cap = std::max<int64_t>(n + n / 4, 1024);
+    HIPTRY(hipHostMalloc((void**)&f->hbuf, (size_t)cap * sizeof(int32_t), hipHostMallocDefault));
+    f->hbuf_n = cap;
+    return SCPHD_OK;
+}
+
+void set_models(scphd_filter* f) {
+    const scphd_params& p = f->prm;
+    f->models.meas = DisparityMeasurementModel(p.fx, p.fy, p.u0, p.v0, p.std_u, p.std_v, p.pd, p.lambda);
+    f->models.cam = OrientedLinearCVMotionModel3D(p.cam_accel_std[0], p.cam_accel_std[1], p.cam_accel_std[2],
+                                                  p.cam_accel_std[3], p.cam_accel_std[4], p.cam_accel_std[5]);
+    f->models.feat = LinearCVMotionModel3D(p.feat_accel_std[0], p.feat_accel_std[1], p.feat_accel_std[2]);
+    BirthModel& b = f->birth;
+    b.mean[0] = b.mean[1] = 0.0;
+    b.mean[2] = p.birth_d0;
+    b.mean[3] = p.birth_vu0;
+    b.mean[4] = p.birth_vv0;
+    b.mean[5] = p.birth_vd0;
+    for (double& x : b.P) x = 0.0;
+    b.P[pidx(0, 0)] = p.std_u * p.std_u;
+    b.P[pidx(1, 1)] = p.std_v * p.std_v;
+    b.P[pidx(2, 2)] = p.birth_var_d0;
+    b.P[pidx(3, 3)] = p.birth_var_vu0;
+    b.P[pidx(4, 4)] = p.birth_var_vv0;
+    b.P[pidx(5, 5)] = p.birth_var_vd0;
+    b.logw0 = safe_log(p.birth_w0);
+    b.pd_ok = is_pd6(b.P) ? 1 : 0;
+    f->key = NoiseKey::from_seed(p.seed);
+    /* largest survivor count whose merge workspace fits 60 KB of LDS */
+    int cap = 64;
+    while (merge_lds_bytes(cap * 2) <= 61440) cap *= 2;
+    while (merge_lds_bytes(cap + 64) <= 61440) cap += 64;
+    f->lds_cap = cap;
+}
+
+/* Apply a deferred sample prediction into fresh, unshared blocks. */
+template <typename T>
+int materialize_t(scphd_filter* f) {
+    if (!f->pending) return SCPHD_OK;
+    f->pending = false;
+    if (f->Ng == 0) return SCPHD_OK;
+    const int nxt = 1 - f->cur;
+    HIPTRY(f->gen[nxt].ensure(f->Ng, 0, f->S, f->esz, f->stream));
+    HIPTRY(launch_materialize<T>(f->stream, f->gen[f->cur].planes<T>(), f->gen[nxt].planes<T>(), f->gblock.p,
+                                 f->gparent.p, f->goff.p, f->Ng, f->S, f->models.feat, f->pdt, f->key, f->pstep));
+    HIPTRY(launch_iota(f->stream, f->gblock.p, f->Ng));
+    f->cur = nxt;
+    return SCPHD_OK;
+}
+
+int materialize(scphd_filter* f) {
+    return f->esz == 8 ? materialize_t<double>(f) : materialize_t<float>(f);
+}
+
+void record(scphd_filter* f, int i) {
+    if (f->timing) (void)hipEventRecord(f->ev[i], f->stream);
+}
+
+template <typename T>
+int update_t(scphd_filter* f, int M, int fixed) {
+    hipStream_t s = f->stream;
+    const int Np = f->Np, S = f->S;
+    const int64_t Ng = f->Ng;
+    const scphd_params& P = f->prm;
+    const int nxt = 1 - f->cur;
+    record(f, 0);
+
+    HIPTRY(launch_pose(s, f->ext.p, f->pose.p, Np, fixed));
+    HIPTRY(f->keepscan.ensure(Ng + 1));
+    HIPTRY(f->scantmp.ensure(scan_tmp_size(std::max<int64_t>(Ng, Np) + 1) + 16));
+    if (Ng > 0) {
+        HIPTRY(f->gen[nxt].ensure(Ng, 0, S, f->esz, s));
+        HIPTRY(f->fit.ensure((int64_t)FP_COUNT * Ng));
+        HIPTRY(f->pre.ensure((int64_t)PP_COUNT * Ng));
+        HIPTRY(f->pdok.ensure(Ng));
+        HIPTRY(f->keep.ensure(Ng));
+        HIPTRY(f->ndlist.ensure(Ng));
+        FitArgs fa;
+        fa.gblock = f->gblock.p;
+        fa.gparent = f->gparent.p;
+        fa.goff = f->goff.p;
+        fa.gw = f->gw.p;
+        fa.pose = f->pose.p;
+        fa.Ng = Ng;
+        fa.S = S;
+        fa.meas = f->models.meas;
+        fa.feat = f->models.feat;
+        fa.pending = f->pending ? 1 : 0;
+        fa.dt = f->pdt;
+        fa.key = f->key;
+        fa.pstep = f->pstep;
+        fa.min_weight = P.min_weight;
+        fa.fit = f->fit.p;
+        fa.keep = f->keep.p;
+        fa.nd_base = 0;
+        HIPTRY(launch_fit<T>(s, fa, f->gen[f->cur].planes<T>(), f->gen[nxt].planes<T>()));
+        HIPTRY(launch_precompute(s, f->fit.p, f->gw.p, Ng, P.std_u * P.std_u, P.std_v * P.std_v, f->pre.p,
+                                 f->pdok.p));
+    }
+    f->pending = false;
+    record(f, 1);
+
+    /* normalisers, parent log-likelihood, survivor counts */
+    HIPTRY(f->eta.ensure((int64_t)Np * std::max(M, 1)));
+    HIPTRY(f->nsurv.ensure(Np + 1));
+    HIPTRY(f->survoff.ensure(Np + 1));
+    HIPTRY(f->nmerged.ensure(Np + 1));
+    HIPTRY(f->mergeoff.ensure(Np + 1));
+    HIPTRY(f->outcnt.ensure(Np + 1));
+    HIPTRY(f->goff2.ensure(Np + 1));
+    HIPTRY(f->dflag.ensure(4));
+    EtaArgs ea;
+    ea.goff = f->goff.p;
+    ea.gw = f->gw.p;
+    ea.pre = f->pre.p;
+    ea.pdok = f->pdok.p;
+    ea.Ng = Ng;
+    ea.zu = f->du.p;
+    ea.zv = f->dv.p;
+    ea.M = M;
+    ea.Np = Np;
+    ea.bpc = safe_log(M * P.birth_w0 + f->models.meas.kappa());
+    ea.min_weight = P.min_weight;
+    ea.birth = f->birth;
+    ea.eta = f->eta.p;
+    ea.loglik = f->loglik.p;
+    ea.nsurv = f->nsurv.p;
+    HIPTRY(launch_eta(s, ea));
+    HIPTRY(launch_scan(s, f->nsurv.p, f->survoff.p, Np, f->scantmp.p));
+    if (ensure_hbuf(f, 2 * (int64_t)Np + 16) != SCPHD_OK) return SCPHD_ERR_HIP;
+    HIPTRY(hipMemcpyAsync(f->hbuf, f->survoff.p, (size_t)(Np + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    const int64_t Stot = f->hbuf[Np];
+    int maxU = 0;
+    for (int p = 0; p < Np; ++p) maxU = std::max(maxU, f->hbuf[p + 1] - f->hbuf[p]);
+
+    HIPTRY(f->sw.ensure(Stot + 1));
+    HIPTRY(f->sg.ensure(Stot + 1));
+    HIPTRY(f->sm.ensure(Stot + 1));
+    const int64_t scap = f->sw.n;
+    HIPTRY(f->smu.ensure(6 * scap));
+    HIPTRY(f->mw.ensure(scap));
+    HIPTRY(f->mmu.ensure(6 * scap));
+    HIPTRY(f->mP.ensure(21 * scap));
+    HIPTRY(hipMemsetAsync(f->dflag.p, 0, 4 * sizeof(int32_t), s));
+    if (Stot > 0) {
+        SurvArgs sa;
+        sa.e = ea;
+        sa.fit = f->fit.p;
+        sa.survoff = f->survoff.p;
+        sa.sw = f->sw.p;
+        sa.sg = f->sg.p;
+        sa.sm = f->sm.p;
+        sa.smu = f->smu.p;
+        sa.cap = scap;
+        sa.err = f->dflag.p;
+        HIPTRY(launch_survivors(s, sa));
+    }
+    record(f, 2);
+
+    /* merge */
+    MergeArgs ma;
+    ma.survoff = f->survoff.p;
+    ma.sw = f->sw.p;
+    ma.sg = f->sg.p;
+    ma.sm = f->sm.p;
+    ma.smu = f->smu.p;
+    ma.scap = scap;
+    ma.pre = f->pre.p;
+    ma.Ng = Ng;
+    ma.birth = f->birth;
+    ma.min_separation = P.min_separation;
+    ma.Np = Np;
+    ma.lds_cap = f->lds_cap;
+    ma.scratch = nullptr;
+    ma.mw = f->mw.p;
+    ma.mmu = f->mmu.p;
+    ma.mP = f->mP.p;
+    ma.nmerged = f->nmerged.p;
+    ma.big = 0;
+    HIPTRY(launch_merge(s, ma, merge_lds_bytes(std::min(std::max(maxU, 1), f->lds_cap))));
+    if (maxU > f->lds_cap) {
+        HIPTRY(f->mscratch.ensure(merge_scratch_bytes(Stot, Np)));
+        ma.scratch = f->mscratch.p;
+        ma.big = 1;
+        HIPTRY(launch_merge(s, ma, 0));
+    }
+    if (!fixed) HIPTRY(launch_parent_weights(s, f->pw.p, f->loglik.p, Np));
+    record(f, 3);
+
+    /* output counts */
+    HIPTRY(launch_scan(s, f->nmerged.p, f->mergeoff.p, Np, f->scantmp.p));
+    if (Ng > 0)
+        HIPTRY(launch_scan(s, f->keep.p, f->keepscan.p, Ng, f->scantmp.p));
+    else
+        HIPTRY(hipMemsetAsync(f->keepscan.p, 0, sizeof(int32_t), s));
+    HIPTRY(launch_outcount(s, f->nmerged.p, f->goff.p, f->keepscan.p, Np, f->outcnt.p));
+    HIPTRY(launch_scan(s, f->outcnt.p, f->goff2.p, Np, f->scantmp.p));
+    HIPTRY(hipMemcpyAsync(f->hbuf, f->goff2.p, (size_t)(Np + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipMemcpyAsync(f->hbuf + Np + 1, f->mergeoff.p + Np, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipMemcpyAsync(f->hbuf + Np + 2, f->dflag.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    const int64_t Nout = f->hbuf[Np];
+    const int64_t Kt = f->hbuf[Np + 1];
+    if (f->hbuf[Np + 2] != 0) return fail(f, SCPHD_ERR_STATE, "survivor count mismatch between passes");
+    int64_t capx = 0;
+    for (int p = 0; p < Np; ++p)
+        if (P.max_gaussians > 0 && f->hbuf[p + 1] - f->hbuf[p] > P.max_gaussians) ++capx;
+    f->stats.cap_exceeded = capx;
+    if (capx > 0) return fail(f, SCPHD_ERR_CAPACITY, "a parent map exceeds max_gaussians");
+
+    /* posterior samples into [Ng, Ng + Kt) of the next generation */
+    HIPTRY(f->gen[nxt].ensure(Ng + Kt, Ng, S, f->esz, s));
+    SampleArgs pa;
+    pa.mergeoff = f->mergeoff.p;
+    pa.survoff = f->survoff.p;
+    pa.mmu = f->mmu.p;
+    pa.mP = f->mP.p;
+    pa.scap = scap;
+    pa.pose = f->pose.p;
+    pa.Np = Np;
+    pa.Kt = Kt;
+    pa.S = S;
+    pa.meas = f->models.meas;
+    pa.key = f->key;
+    pa.ustep = (uint64_t)f->n_update;
+    pa.post_base = Ng;
+    HIPTRY(launch_sample<T>(s, pa, f->gen[nxt].planes<T>()));
+    record(f, 4);
+
+    /* next map table */
+    HIPTRY(launch_ndlist(s, f->keep.p, f->keepscan.p, Ng, f->ndlist.p));
+    HIPTRY(f->gparent2.ensure(Nout + 1));
+    HIPTRY(f->gblock2.ensure(Nout + 1));
+    HIPTRY(f->gw2.ensure(Nout + 1));
+    TableArgs ta;
+    ta.outoff = f->goff2.p;
+    ta.mergeoff = f->mergeoff.p;
+    ta.survoff = f->survoff.p;
+    ta.goff = f->goff.p;
+    ta.keepscan = f->keepscan.p;
+    ta.ndlist = f->ndlist.p;
+    ta.mw = f->mw.p;
+    ta.gw = f->gw.p;
+    ta.fit = f->fit.p;
+    ta.Ng_in = Ng;
+    ta.Np = Np;
+    ta.Nout = Nout;
+    ta.post_base = Ng;
+    ta.nd_base = 0;
+    ta.gparent = f->gparent2.p;
+    ta.gblock = f->gblock2.p;
+    ta.gwo = f->gw2.p;
+    HIPTRY(launch_table(s, ta));
+    record(f, 5);
+    std::swap(f->goff, f->goff2);
+    std::swap(f->gparent, f->gparent2);
+    std::swap(f->gblock, f->gblock2);
+    std::swap(f->gw, f->gw2);
+    f->cur = nxt;
+    f->Ng = Nout;
+    f->n_update++;
+
+    f->stats.n_meas = M;
+    f->stats.n_gaussians_in = Ng;
+    f->stats.n_pairs = Ng * M;
+    f->stats.n_survivors = Stot;
+    f->stats.n_merged = Kt;
+    f->stats.n_nondetect = Nout - Kt;
+    f->stats.n_gaussians_out = Nout;
+    f->stats.max_survivors_per_parent = maxU;
+    f->last_M = M;
+    if (f->timing) {
+        HIPTRY(hipEventSynchronize(f->ev[5]));
+        for (int i = 0; i < 5; ++i) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, f->ev[i], f->ev[i + 1]);
+            f->phase_ms[i] = ms;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, f->ev[0], f->ev[5]);
+        f->phase_ms[5] = ms;
+    }
+    return SCPHD_OK;
+}
+
+int update_common(scphd_filter* f, int M, int fixed) {
+    int rc = f->esz == 8 ? update_t<double>(f, M, fixed) : update_t<float>(f, M, fixed);
+    return rc;
+}
+
+int check_handle(scphd_filter* f) { return f ? SCPHD_OK : SCPHD_ERR_ARG; }
+
+}  // namespace
+
+/* ================================================================== ABI */
+extern "C" {
+
+int scphd_create(const scphd_params* p, scphd_t** out) {
+    if (!p || !out) return SCPHD_ERR_ARG;
+    *out = nullptr;
+    if (p->n_particles <= 0 || p->particles_per_feature <= 0 || p->particles_per_feature > 512)
+        return SCPHD_ERR_ARG;
+    if (p->storage != SCPHD_STORAGE_FP32 && p->storage != SCPHD_STORAGE_FP64) return SCPHD_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SCPHD_ERR_NODEVICE;
+    /* the device with the most compute units (src/scphdcameracalibration.cu:1013-1024) */
+    int best = 0, best_cu = -1;
+    for (int d = 0; d < ndev; ++d) {
+        int cu = 0;
+        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, d) == hipSuccess && cu > best_cu) {
+            best_cu = cu;
+            best = d;
+        }
+    }
+    scphd_filter* f = new scphd_filter();
+    f->prm = *p;
+    f->dev = best;
+    f->Np = p->n_particles;
+    f->S = p->particles_per_feature;
+    f->esz = p->storage == SCPHD_STORAGE_FP64 ? 8 : 4;
+    set_models(f);
+    auto bail = [&](int code) {
+        scphd_destroy(f);
+        return code;
+    };
+    if (hipSetDevice(f->dev) != hipSuccess) return bail(SCPHD_ERR_NODEVICE);
+    if (hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking) != hipSuccess) return bail(SCPHD_ERR_HIP);
+    for (auto& e : f->ev)
+        if (hipEventCreate(&e) != hipSuccess) return bail(SCPHD_ERR_HIP);
+    const int Np = f->Np;
+    if (f->ext.ensure(12 * (int64_t)Np) || f->ext_tmp.ensure(12 * (int64_t)Np) || f->pw.ensure(Np) ||
+        f->pose.ensure(12 * (int64_t)Np) || f->loglik.ensure(Np) || f->cum.ensure(Np) || f->anc.ensure(Np) ||
+        f->newcnt.ensure(Np + 1) || f->goff.ensure(Np + 1) || f->goff2.ensure(Np + 1) || f->dneff.ensure(1) ||
+        f->dflag.ensure(4) || f->gparent.ensure(1) || f->gblock.ensure(1) || f->gw.ensure(1))
+        return bail(SCPHD_ERR_HIP);
+    if (launch_prior(f->stream, f->ext.p, f->pw.p, Np, p->prior_a, p->prior_b, f->key) != hipSuccess)
+        return bail(SCPHD_ERR_HIP);
+    if (hipMemsetAsync(f->goff.p, 0, (size_t)(Np + 1) * sizeof(int32_t), f->stream) != hipSuccess)
+        return bail(SCPHD_ERR_HIP);
+    if (launch_iota(f->stream, f->anc.p, Np) != hipSuccess) return bail(SCPHD_ERR_HIP);
+    if (hipStreamSynchronize(f->stream) != hipSuccess) return bail(SCPHD_ERR_HIP);
+    *out = f;
+    return SCPHD_OK;
+}
+
+void scphd_destroy(scphd_t* f) {
+    if (!f) return;
+    (void)hipSetDevice(f->dev);
+    if (f->stream) (void)hipStreamSynchronize(f->stream);
+    DBuf<double>* dbs[] = {&f->ext, &f->pw, &f->pose, &f->loglik, &f->cum, &f->ext_tmp, &f->gw, &f->gw2,
+                           &f->fit, &f->pre, &f->du, &f->dv, &f->eta, &f->sw, &f->smu, &f->mw, &f->mmu, &f->mP,
+                           &f->dneff};
+    for (auto* b : dbs) b->release();
+    DBuf<int32_t>* ibs[] = {&f->anc, &f->newcnt, &f->goff, &f->gparent, &f->gblock, &f->goff2, &f->gparent2,
+                            &f->gblock2, &f->pdok, &f->keep, &f->keepscan, &f->ndlist, &f->nsurv, &f->survoff,
+                            &f->nmerged, &f->mergeoff, &f->outcnt, &f->scantmp, &f->sg, &f->sm, &f->dflag};
+    for (auto* b : ibs) b->release();
+    f->mscratch.release();
+    f->gen[0].release();
+    f->gen[1].release();
+    if (f->hbuf) (void)hipHostFree(f->hbuf);
+    for (auto& e : f->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (f->stream) (void)hipStreamDestroy(f->stream);
+    delete f;
+}
+
+const char* scphd_last_error(const scphd_t* f) { return f ? f->err.c_str() : "null handle"; }
+
+int scphd_predict(scphd_t* f, double dt) {
+    if (check_handle(f)) return SCPHD_ERR_ARG;
+    HIPTRY(hipSetDevice(f->dev));
+    int rc = materialize(f);
+    if (rc) return rc;
+    HIPTRY(launch_parent_predict(f->stream, f->ext.p, f->Np, f->models.cam, dt, f->key, (uint64_t)f->n_predict));
+    HIPTRY(launch_scale(f->stream, f->gw.p, f->Ng, f->prm.ps));
+    f->pending = f->Ng > 0;
+    f->pdt = dt;
+    f->pstep = (uint64_t)f->n_predict;
+    f->n_predict++;
+    return SCPHD_OK;
+}
+
+int scphd_update_device(scphd_t* f, const double* u_device, const double* v_device, int32_t n_meas,
+                        int32_t fixed_camera) {
+    if (check_handle(f) || n_meas < 0 || (n_meas > 0 && (!u_device || !v_device))) return SCPHD_ERR_ARG;
+    HIPTRY(hipSetDevice(f->dev));
+    HIPTRY(f->du.ensure(std::max(n_meas, 1)));
+    HIPTRY(f->dv.ensure(std::max(n_meas, 1)));
+    if (n_meas > 0 && u_device != f->du.p) {
+        HIPTRY(hipMemcpyAsync(f->du.p, u_device, n_meas * sizeof(double), hipMemcpyDeviceToDevice, f->stream));
+        HIPTRY(hipMemcpyAsync(f->dv.p, v_device, n_meas * sizeof(double), hipMemcpyDeviceToDevice, f->stream));
+    }
+    return update_common(f, n_meas, fixed_camera ? 1 : 0);
+}
+
+int scphd_update(scphd_t* f, const double* u, const double* v, int32_t n_meas, int32_t fixed_camera) {
+    if (check_handle(f) || n_meas < 0 || (n_meas > 0 && (!u || !v))) return SCPHD_ERR_ARG;
+    HIPTRY(hipSetDevice(f->dev));
+    HIPTRY(f->du.ensure(std::max(n_meas, 1)));
+    HIPTRY(f->dv.ensure(std::max(n_meas, 1)));
+    if (n_meas > 0) {
+        HIPTRY(hipMemcpyAsync(f->du.p, u, n_meas * sizeof(double), hipMemcpyHostToDevice, f->stream));
+        HIPTRY(hipMemcpyAsync(f->dv.p, v, n_meas * sizeof(double), hipMemcpyHostToDevice, f->stream));
+    }
+    return update_common(f, n_meas, fixed_camera ? 1 : 0);
+}
+
+int scphd_resample(scphd_t* f, int32_t* resampled) {
+    if (check_handle(f)) return SCPHD_ERR_ARG;
+    HIPTRY(hipSetDevice(f->dev));
+    int rc = materialize(f);
+    if (rc) return rc;
+    hipStream_t s = f->stream;
+    const int Np = f->Np;
+    ResampleArgs ra;
+    ra.pw = f->pw.p;
+    ra.ext = f->ext.p;
+    ra.ext_tmp = f->ext_tmp.p;
+    ra.cum = f->cum.p;
+    ra.anc = f->anc.p;
+    ra.newcnt = f->newcnt.p;
+    ra.goff = f->goff.p;
+    ra.Np = Np;
+    ra.min_neff = f->prm.min_neff;
+    ra.key = f->key;
+    ra.rstep = (uint64_t)f->n_resample;
+    ra.flag = f->dflag.p;
+    ra.neff = f->dneff.p;
+    HIPTRY(launch_resample(s, ra));
+    HIPTRY(f->scantmp.ensure(scan_tmp_size(Np + 1) + 16));
+    HIPTRY(launch_scan(s, f->newcnt.p, f->goff2.p, Np, f->scantmp.p));
+    if (ensure_hbuf(f, 8) != SCPHD_OK) return SCPHD_ERR_HIP;
+    HIPTRY(hipMemcpyAsync(f->hbuf, f->dflag.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipMemcpyAsync(f->hbuf + 1, f->goff2.p + Np, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    f->n_resample++;
+    const int go = f->hbuf[0];
+    if (resampled) *resampled = go;
+    if (!go) return SCPHD_OK;
+    const int64_t Nnew = f->hbuf[1];
+    HIPTRY(f->gparent2.ensure(Nnew + 1));
+    HIPTRY(f->gblock2.ensure(Nnew + 1));
+    HIPTRY(f->gw2.ensure(Nnew + 1));
+    HIPTRY(launch_table_resample(s, f->anc.p, f->goff.p, f->goff2.p, f->gblock.p, f->gw.p, Np, Nnew,
+                                 f->gparent2.p, f->gblock2.p, f->gw2.p));
+    std::swap(f->goff, f->goff2);
+    std::swap(f->gparent, f->gparent2);
+    std::swap(f->gblock, f->gblock2);
+    std::swap(f->gw, f->gw2);
+    f->Ng = Nnew;
+    return SCPHD_OK;
+}
+
+int scphd_check(scphd_t* f) {
+    if (check_handle(f)) return SCPHD_ERR_ARG;
+    HIPTRY(hipSetDevice(f->dev));
+    std::vector<double> w(f->Np);
+    HIPTRY(hipMemcpyAsync(w.data(), f->pw.p, f->Np * sizeof(double), hipMemcpyDeviceToHost, f->stream));
+    HIPTRY(hipStreamSynchronize(f->stream));
+    for (double x : w)
+        if (x != x) return fail(f, SCPHD_ERR_NAN, "nan weight detected");
+    return SCPHD_OK;
+}
+
+int scphd_sync(scphd_t* f) {
+    if (check_handle(f)) return SCPHD_ERR_ARG;
+    HIPTRY(hipSetDevice(f->dev));
+    HIPTRY(hipStreamSynchronize(f->stream));
+    return SCPHD_OK;
+}
+
+int scphd_get_counts(scphd_t* f, scphd_counts* c) {
+    if (check_handle(f) || !c) return SCPHD_ERR_ARG;
+    c->n_particles = f->Np;
+    c->particles_per_feature = f->S;
+    c->n_gaussians = f->Ng;
+    c->n_samples = f->Ng * f->S;
+    c->n_predict = f->n_predict;
+    c->n_update = f->n_update;
+    c->n_resample = f->n_resample;
+    c->pending_predict = f->pending ? 1 : 0;
+    return SCPHD_OK;
+}
+
+int scphd_get_parents(scphd_t* f, double* states, double* weights) {
+    if (check_handle(f)) return SCPHD_ERR_ARG;
+    HIPTRY(hipSetDevice(f->dev));
+    if (states)
+        HIPTRY(hipMemcpyAsync(states, f->ext.p, (size_t)f->Np * 12 * sizeof(double), hipMemcpyDeviceToHost,
+                              f->stream));
+    if (weights)
+        HIPTRY(hipMemcpyAsync(weights, f->pw.p, (size_t)f->Np * sizeof(double), hipMemcpyDeviceToHost, f->stream));
+    HIPTRY(hipStreamSynchronize(f->stream));
+    return SCPHD_OK;
+}
+
+int scphd_get_maps(scphd_t* f, int32_t* gcount, double* wg, double* samples) {
+    if (check_handle(f)) return SCPHD_ERR_ARG;
+    HIPTRY(hipSetDevice(f->dev));
+    int rc = materialize(f);
+    if (rc) return rc;
+    hipStream_t s = f->stream;
+    if (gcount) {
+        std::vector<int32_t> off(f->Np + 1);
+        HIPTRY(hipMemcpyAsync(off.data(), f->goff.p, off.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        HIPTRY(hipStreamSynchronize(s));
+        for (int p = 0; p < f->Np; ++p) gcount[p] = off[p + 1] - off[p];
+    }
+    if (wg && f->Ng > 0)
+        HIPTRY(hipMemcpyAsync(wg, f->gw.p, (size_t)f->Ng * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (samples && f->Ng > 0) {
+        const int64_t n = f->Ng * f->S;
+        double* d = nullptr;
+        HIPTRY(hipMalloc((void**)&d, (size_t)n * 7 * sizeof(double)));
+        hipError_t e = f->esz == 8 ? launch_extract<double>(s, f->gen[f->cur].planes<double>(), f->gblock.p, f->Ng,
+                                                            f->S, d)
+                                   : launch_extract<float>(s, f->gen[f->cur].planes<float>(), f->gblock.p, f->Ng,
+                                                           f->S, d);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(samples, d, (size_t)n * 7 * sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        (void)hipFree(d);
+        if (e != hipSuccess) return hipfail(f, e, "get_maps");
+    }
+    HIPTRY(hipStreamSynchronize(s));
+    return SCPHD_OK;
+}
+
+int scphd_set_state(scphd_t* f, const double* states, const double* weights, const int32_t* gcount,
+                    int64_t n_gaussians, const double* wg, const double* samples, int64_t n_predict,
+                    int64_t n_update, int64_t n_resample) {
+    if (check_handle(f) || !states || !weights || !gcount || n_gaussians < 0) return SCPHD_ERR_ARG;
+    if (n_gaussians > 0 && (!wg || !samples)) return SCPHD_ERR_ARG;
+    HIPTRY(hipSetDevice(f->dev));
+    const int Np = f->Np, S = f->S;
+    std::vector<int32_t> off(Np + 1, 0);
+    std::vector<int32_t> par((size_t)std::max<int64_t>(n_gaussians, 1));
+    for (int p = 0; p < Np; ++p) {
+        if (gcount[p] < 0) return fail(f, SCPHD_ERR_ARG, "negative gaussian count");
+        off[p + 1] = off[p] + gcount[p];
+        for (int k = 0; k < gcount[p] && off[p] + k < n_gaussians; ++k) par[off[p] + k] = p;
+    }
+    if (off[Np] != n_gaussians) return fail(f, SCPHD_ERR_ARG, "gcount does not sum to n_gaussians");
+    hipStream_t s = f->stream;
+    const int64_t Ng = n_gaussians;
+    HIPTRY(f->goff.ensure(Np + 1));
+    HIPTRY(f->gparent.ensure(Ng + 1));
+    HIPTRY(f->gblock.ensure(Ng + 1));
+    HIPTRY(f->gw.ensure(Ng + 1));
+    HIPTRY(hipMemcpyAsync(f->ext.p, states, (size_t)Np * 12 * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(f->pw.p, weights, (size_t)Np * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(f->goff.p, off.data(), (size_t)(Np + 1) * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    if (Ng > 0) {
+        HIPTRY(hipMemcpyAsync(f->gparent.p, par.data(), (size_t)Ng * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        HIPTRY(hipMemcpyAsync(f->gw.p, wg, (size_t)Ng * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPTRY(launch_iota(s, f->gblock.p, Ng));
+        HIPTRY(f->gen[f->cur].ensure(Ng, 0, S, f->esz, s));
+        double* d = nullptr;
+        const int64_t n = Ng * S;
+        HIPTRY(hipMalloc((void**)&d, (size_t)n * 7 * sizeof(double)));
+        hipError_t e = hipMemcpyAsync(d, samples, (size_t)n * 7 * sizeof(double), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess)
+            e = f->esz == 8 ? launch_inject<double>(s, d, Ng, S, f->gen[f->cur].planes<double>())
+                            : launch_inject<float>(s, d, Ng, S, f->gen[f->cur].planes<float>());
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        (void)hipFree(d);
+        if (e != hipSuccess) return hipfail(f, e, "set_state");
+    }
+    HIPTRY(hipStreamSynchronize(s));
+    f->Ng = Ng;
+    f->n_predict = n_predict;
+    f->n_update = n_update;
+    f->n_resample = n_resample;
+    f->pending = false;
+    return SCPHD_OK;
+}
+
+int scphd_get_update_stats(scphd_t* f, scphd_update_stats* st) {
+    if (check_handle(f) || !st) return SCPHD_ERR_ARG;
+    *st = f->stats;
+    return SCPHD_OK;
+}
+
+int scphd_get_update_debug(scphd_t* f, double* eta, double* loglik, int32_t* nmerged, int32_t* nnondetect,
+                           int32_t* nsurvivors) {
+    if (check_handle(f)) return SCPHD_ERR_ARG;
+    HIPTRY(hipSetDevice(f->dev));
+    hipStream_t s = f->stream;
+    const int Np = f->Np;
+    if (eta && f->last_M > 0)
+        HIPTRY(hipMemcpyAsync(eta, f->eta.p, (size_t)Np * f->last_M * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (loglik) HIPTRY(hipMemcpyAsync(loglik, f->loglik.p, (size_t)Np * sizeof(double), hipMemcpyDeviceToHost, s));
+    std::vector<int32_t> nm(Np), oc(Np);
+    HIPTRY(hipMemcpyAsync(nm.data(), f->nmerged.p, (size_t)Np * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipMemcpyAsync(oc.data(), f->outcnt.p, (size_t)Np * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    if (nsurvivors)
+        HIPTRY(hipMemcpyAsync(nsurvivors, f->nsurv.p, (size_t)Np * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    for (int p = 0; p < Np; ++p) {
+        if (nmerged) nmerged[p] = nm[p];
+        if (nnondetect) nnondetect[p] = oc[p] - nm[p];
+    }
+    return SCPHD_OK;
+}
+
+int scphd_get_fit_debug(scphd_t* f, double* out) {
+    if (check_handle(f) || !out) return SCPHD_ERR_ARG;
+    HIPTRY(hipSetDevice(f->dev));
+    const int64_t n = f->stats.n_gaussians_in;
+    for (int q = 0; q < FP_COUNT && n > 0; ++q)
+        HIPTRY(hipMemcpyAsync(out + q * n, f->fit.p + q * n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+                              f->stream));
+    HIPTRY(hipStreamSynchronize(f->stream));
+    return SCPHD_OK;
+}
+
+int scphd_get_ancestors(scphd_t* f, int32_t* ancestors) {
+    if (check_handle(f) || !ancestors) return SCPHD_ERR_ARG;
+    HIPTRY(hipSetDevice(f->dev));
+    HIPTRY(hipMemcpyAsync(ancestors, f->anc.p, (size_t)f->Np * sizeof(int32_t), hipMemcpyDeviceToHost, f->stream));
+    HIPTRY(hipStreamSynchronize(f->stream));
+    return SCPHD_OK;
+}
+
+int scphd_set_timing(scphd_t* f, int32_t enabled) {
+    if (check_handle(f)) return SCPHD_ERR_ARG;
+    f->timing = enabled != 0;
+    return SCPHD_OK;
+}
+
+int scphd_get_phase_ms(scphd_t* f, double* ms6) {
+    if (check_handle(f) || !ms6) return SCPHD_ERR_ARG;
+    for (int i = 0; i < 6; ++i) ms6[i] = f->phase_ms[i];
+    return SCPHD_OK;
+}
+
+}  /* extern "C" */

@@ -1,0 +1,235 @@
+/*
+ * kernels.h -- launch interface of the gfx950 kernels of the SC-PHD filter
+ * step.  Every launcher enqueues on the given stream and returns the HIP
+ * status of the launch.
+ *
+ * Device data layout (all arrays structure-of-arrays, HBM resident):
+ *   parents   ext[Np*12] (parent-major Extrinsics), pw[Np] linear weights,
+ *             pose[Np*12] (R row-major 9, t 3) of the camera used by the
+ *             current update.
+ *   map table one entry per Gaussian in canonical order (parent ascending):
+ *             goff[Np+1] offsets, gparent[Ng], gblock[Ng] sample block,
+ *             gw[Ng] Gaussian weight (FP64).
+ *   samples   a generation buffer of 7 planes (x y z vx vy vz w) of T
+ *             (float or double); block b holds samples [b*S, (b+1)*S).
+ *             Blocks may be shared by several table entries after
+ *             resampling; kernels never write a block they read.
+ */
+#ifndef SCPHD_KERNELS_H
+#define SCPHD_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device.h"
+
+namespace scphd {
+
+/* per-Gaussian fit outputs, stride Ng (SoA planes) */
+enum FitPlane { FP_MU = 0, FP_P = 6, FP_PD = 27, FP_COUNT = 28 };
+/* per-Gaussian detection terms, stride Ng */
+enum PrePlane {
+    PP_MUU = 0, PP_MUV, PP_S0, PP_S12, PP_S3, PP_C, PP_H, /* likelihood record */
+    PP_K = 7,                                           /* gain, 12 */
+    PP_PN = 19,                                         /* updated covariance, 21 packed */
+    PP_COUNT = 40
+};
+
+template <typename T>
+struct Planes {
+    T* p[7];
+};
+
+struct FilterModels {
+    DisparityMeasurementModel meas;
+    OrientedLinearCVMotionModel3D cam;
+    LinearCVMotionModel3D feat;
+};
+
+struct BirthModel {
+    double mean[6]; /* u, v filled per measurement; d0 vu0 vv0 vd0 */
+    double P[21];   /* packed diagonal */
+    double logw0;
+    int pd_ok;
+};
+
+/* ---- prediction ---- */
+hipError_t launch_parent_predict(hipStream_t s, double* ext, int Np, OrientedLinearCVMotionModel3D cam,
+                                 double dt, NoiseKey key, uint64_t step);
+hipError_t launch_scale(hipStream_t s, double* x, int64_t n, double a);
+hipError_t launch_prior(hipStream_t s, double* ext, double* pw, int Np, const double* a12, const double* b12,
+                        NoiseKey key);
+hipError_t launch_pose(hipStream_t s, const double* ext, double* pose, int Np, int fixed);
+
+/* Apply a deferred sample prediction, writing every table entry to its own
+ * block (block id = table index) of the destination generation. */
+template <typename T>
+hipError_t launch_materialize(hipStream_t s, Planes<T> src, Planes<T> dst, const int32_t* gblock,
+                              const int32_t* gparent, const int32_t* goff, int64_t Ng, int S,
+                              LinearCVMotionModel3D feat, double dt, NoiseKey key, uint64_t step);
+
+/* ---- update ---- */
+struct FitArgs {
+    const int32_t* gblock;
+    const int32_t* gparent;
+    const int32_t* goff;
+    const double* gw;
+    const double* pose;
+    int64_t Ng;
+    int S;
+    DisparityMeasurementModel meas;
+    LinearCVMotionModel3D feat;
+    int pending;
+    double dt;
+    NoiseKey key;
+    uint64_t pstep;
+    double min_weight;
+    double* fit;     /* FP_COUNT planes */
+    int32_t* keep;   /* non-detection term kept */
+    int64_t nd_base; /* destination block of the non-detection cloud of entry g: nd_base + g */
+};
+
+template <typename T>
+hipError_t launch_fit(hipStream_t s, const FitArgs& a, Planes<T> src, Planes<T> dst);
+
+hipError_t launch_precompute(hipStream_t s, const double* fit, const double* gw, int64_t Ng, double var_u,
+                             double var_v, double* pre, int32_t* pdok);
+
+struct EtaArgs {
+    const int32_t* goff;
+    const double* gw;
+    const double* pre;
+    const int32_t* pdok;
+    int64_t Ng; /* plane stride */
+    const double* zu;
+    const double* zv;
+    int M;
+    int Np;
+    double bpc;        /* log(M w0 + kappa) */
+    double min_weight;
+    BirthModel birth;
+    double* eta;       /* Np*M */
+    double* loglik;    /* Np   */
+    int32_t* nsurv;    /* Np   */
+};
+hipError_t launch_eta(hipStream_t s, const EtaArgs& a);
+
+struct SurvArgs {
+    EtaArgs e;
+    const double* fit;
+    const int32_t* survoff; /* Np+1 */
+    double* sw;
+    int32_t* sg;            /* table index, or -1 - m for a birth term */
+    int32_t* sm;
+    double* smu;            /* 6 planes, stride cap */
+    int64_t cap;
+    int32_t* err;           /* set to 1 when a parent's count differs from nsurv */
+};
+hipError_t launch_survivors(hipStream_t s, const SurvArgs& a);
+
+struct MergeArgs {
+    const int32_t* survoff;
+    const double* sw;
+    const int32_t* sg;
+    const int32_t* sm;
+    const double* smu;
+    int64_t scap;
+    const double* pre; /* PP_PN planes of the table entries */
+    int64_t Ng;
+    BirthModel birth;
+    double min_separation;
+    int Np;
+    int32_t lds_cap;   /* parents with more survivors use global scratch */
+    int32_t big;       /* 0: parents with <= lds_cap survivors, 1: the others */
+    char* scratch;     /* global scratch for big parents: merge_scratch_bytes(Stot, Np) */
+    double* mw;        /* merged outputs at survoff[p] + k */
+    double* mmu;       /* 6 planes, stride scap */
+    double* mP;        /* 21 planes, stride scap */
+    int32_t* nmerged;  /* Np */
+};
+hipError_t launch_merge(hipStream_t s, const MergeArgs& a, int lds_bytes);
+int merge_lds_bytes(int cap);
+int64_t merge_scratch_bytes(int64_t Stot, int Np);
+hipError_t launch_iota(hipStream_t s, int32_t* x, int64_t n);
+
+hipError_t launch_parent_weights(hipStream_t s, double* pw, const double* loglik, int Np);
+
+struct SampleArgs {
+    const int32_t* mergeoff; /* Np+1 */
+    const int32_t* survoff;
+    const double* mmu;
+    const double* mP;
+    int64_t scap;
+    const double* pose;
+    int Np;
+    int64_t Kt;
+    int S;
+    DisparityMeasurementModel meas;
+    NoiseKey key;
+    uint64_t ustep;
+    int64_t post_base;
+};
+template <typename T>
+hipError_t launch_sample(hipStream_t s, const SampleArgs& a, Planes<T> dst);
+
+hipError_t launch_outcount(hipStream_t s, const int32_t* nmerged, const int32_t* goff, const int32_t* keepscan,
+                           int Np, int32_t* outcnt);
+hipError_t launch_ndlist(hipStream_t s, const int32_t* keep, const int32_t* keepscan, int64_t Ng, int32_t* ndlist);
+
+struct TableArgs {
+    const int32_t* outoff;   /* Np+1 */
+    const int32_t* mergeoff; /* Np+1 */
+    const int32_t* survoff;  /* Np+1 */
+    const int32_t* goff;     /* input table offsets */
+    const int32_t* keepscan; /* Ng+1 */
+    const int32_t* ndlist;
+    const double* mw;
+    const double* gw;        /* input table weights */
+    const double* fit;       /* FP_PD plane */
+    int64_t Ng_in;
+    int Np;
+    int64_t Nout;
+    int64_t post_base;
+    int64_t nd_base;
+    int32_t* gparent;
+    int32_t* gblock;
+    double* gwo;
+};
+hipError_t launch_table(hipStream_t s, const TableArgs& a);
+
+/* ---- resampling ---- */
+struct ResampleArgs {
+    double* pw;
+    double* ext;
+    double* ext_tmp;
+    double* cum;
+    int32_t* anc;
+    int32_t* newcnt;
+    const int32_t* goff;
+    int Np;
+    double min_neff;
+    NoiseKey key;
+    uint64_t rstep;
+    int32_t* flag; /* out: 1 if resampled */
+    double* neff;  /* out */
+};
+hipError_t launch_resample(hipStream_t s, const ResampleArgs& a);
+hipError_t launch_table_resample(hipStream_t s, const int32_t* anc, const int32_t* goff_old,
+                                 const int32_t* goff_new, const int32_t* gblock_old, const double* gw_old,
+                                 int Np, int64_t Nnew, int32_t* gparent, int32_t* gblock, double* gw);
+
+/* ---- read-back ---- */
+template <typename T>
+hipError_t launch_extract(hipStream_t s, Planes<T> src, const int32_t* gblock, int64_t Ng, int S, double* out);
+template <typename T>
+hipError_t launch_inject(hipStream_t s, const double* in, int64_t Ng, int S, Planes<T> dst);
+
+/* ---- scan ---- */
+/* out[0..n] = exclusive prefix sums of in[0..n), out[n] = total.
+ * tmp needs scan_tmp_size(n) ints. */
+int64_t scan_tmp_size(int64_t n);
+hipError_t launch_scan(hipStream_t s, const int32_t* in, int32_t* out, int64_t n, int32_t* tmp);
+
+}  // namespace scphd
+
+#endif

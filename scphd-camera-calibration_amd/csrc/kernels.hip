@@ -1,0 +1,1257 @@
+/*
+ * kernels.hip -- gfx950 kernels of the SC-PHD camera calibration step.
+ *
+ * Reference: cheesinglee/scphd-camera-calibration.  Each kernel names the
+ * reference code whose semantics it implements; none is a translation of it:
+ * the reference launches ~40 thrust/CUDA kernels per step over dense
+ * N_g x M intermediates, this file fuses them into per-Gaussian wavefront
+ * kernels and per-parent workgroup kernels that never materialise the
+ * dense (Gaussian, measurement) array.
+ */
+#include "kernels.h"
+
+#include <float.h>
+
+namespace scphd {
+
+static constexpr int NT = 256; /* threads per workgroup for per-parent kernels */
+static constexpr int WPB = 4;  /* wavefronts (Gaussians) per workgroup for per-Gaussian kernels */
+
+/* ------------------------------------------------------------ reductions */
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+/* Sum over a workgroup of NT threads; every thread gets the result. */
+__device__ inline double block_sum(double v, double* sh) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
+    return r;
+}
+
+__device__ inline double block_max(double v, double* sh) {
+    v = wave_max(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double r = -INFINITY;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r = fmax(r, sh[i]);
+    return r;
+}
+
+/* Exclusive scan over a workgroup of NT threads; *total gets the sum. */
+__device__ inline int block_exscan(int v, int* sh, int* total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    __syncthreads();
+    if (lane == 63) sh[w] = x;
+    __syncthreads();
+    int base = 0, tot = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+        if (i < w) base += sh[i];
+        tot += sh[i];
+    }
+    *total = tot;
+    return base + x - v;
+}
+
+__device__ inline int upper_parent(const int32_t* off, int n, int64_t x) {
+    /* largest p in [0, n) with off[p] <= x, for nondecreasing off[0..n] with off[n] > x */
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if ((int64_t)off[mid] <= x)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+/* ------------------------------------------------------------ prediction */
+struct Bounds12 {
+    double a[12], b[12];
+};
+
+/* Uniform prior over [a, b] per extrinsic component, weights 1/N
+ * (src/scphdcameracalibration.cu:835-980). */
+__global__ void k_prior(double* ext, double* pw, int Np, Bounds12 ab, NoiseKey key) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= Np) return;
+    for (int c = 0; c < 12; ++c) {
+        uint32_t x[4];
+        noise_block(key, NS_PRIOR, 0, (uint32_t)p, 0, 0, (uint32_t)(c / 2), x);
+        const double r = (c % 2 == 0) ? uniform_co(x[0], x[1]) : uniform_co(x[2], x[3]);
+        ext[(int64_t)p * 12 + c] = r * (ab.b[c] - ab.a[c]) + ab.a[c];
+    }
+    pw[p] = 1.0 / Np;
+}
+
+/* Parent extrinsics through OrientedLinearCVMotionModel3D::computeNoisyMotion
+ * with N(0, std^2) accelerations (predict_camera functor,
+ * src/scphdcameracalibration.cu:129-152, :1062-1100). */
+__global__ void k_parent_predict(double* ext, int Np, OrientedLinearCVMotionModel3D cam, double dt, NoiseKey key,
+                                 uint64_t step) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= Np) return;
+    double a[6];
+    normals<6>(key, NS_CAMERA, step, (uint32_t)p, 0, 0, a);
+    double* e = ext + (int64_t)p * 12;
+    EuclideanPoint c{e[0], e[1], e[2], e[3], e[4], e[5]};
+    EuclideanPoint g{e[6], e[7], e[8], e[9], e[10], e[11]};
+    cam.computeNoisyMotion(c, g, dt, a[0] * cam.std_ax_cartesian(), a[1] * cam.std_ay_cartesian(),
+                           a[2] * cam.std_az_cartesian(), a[3] * cam.std_ax_angular(), a[4] * cam.std_ay_angular(),
+                           a[5] * cam.std_az_angular());
+    e[0] = c.x; e[1] = c.y; e[2] = c.z; e[3] = c.vx; e[4] = c.vy; e[5] = c.vz;
+    e[6] = g.x; e[7] = g.y; e[8] = g.z; e[9] = g.vx; e[10] = g.vy; e[11] = g.vz;
+}
+
+__global__ void k_scale(double* x, int64_t n, double a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] *= a;
+}
+
+/* Camera used by an update: the parent's extrinsics, or all-zero extrinsics
+ * for the fixed camera (src/scphdcameracalibration.cu:1167-1176). */
+__global__ void k_pose(const double* ext, double* pose, int Np, int fixed) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= Np) return;
+    CameraPose c;
+    if (fixed) {
+        c = CameraPose::identity();
+    } else {
+        const double* e = ext + (int64_t)p * 12;
+        Extrinsics x{{e[0], e[1], e[2], e[3], e[4], e[5]}, {e[6], e[7], e[8], e[9], e[10], e[11]}};
+        c = CameraPose::fromExtrinsics(x);
+    }
+    double* o = pose + (int64_t)p * 12;
+    for (int i = 0; i < 9; ++i) o[i] = c.R[i];
+    for (int i = 0; i < 3; ++i) o[9 + i] = c.t[i];
+}
+
+__device__ inline CameraPose load_pose(const double* pose, int p) {
+    CameraPose c;
+    const double* o = pose + (int64_t)p * 12;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) c.R[i] = o[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) c.t[i] = o[9 + i];
+    return c;
+}
+
+/* Landmark sample prediction (predict_features functor,
+ * src/scphdcameracalibration.cu:154-178) for the sample (p, k, s). */
+__device__ inline EuclideanPoint predict_sample(EuclideanPoint x, const LinearCVMotionModel3D& feat, double dt,
+                                                NoiseKey key, uint64_t step, uint32_t p, uint32_t k, uint32_t s) {
+    double a[3];
+    normals<3>(key, NS_FEATURE, step, p, k, s, a);
+    return feat.computeNoisyMotion(x, dt, a[0] * feat.std_ax(), a[1] * feat.std_ay(), a[2] * feat.std_az());
+}
+
+template <typename T>
+__global__ void k_materialize(Planes<T> src, Planes<T> dst, const int32_t* gblock, const int32_t* gparent,
+                              const int32_t* goff, int64_t Ng, int S, LinearCVMotionModel3D feat, double dt,
+                              NoiseKey key, uint64_t step) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Ng * S) return;
+    const int64_t g = i / S;
+    const int s = (int)(i - g * S);
+    const int p = gparent[g];
+    const int k = (int)(g - goff[p]);
+    const int64_t si = (int64_t)gblock[g] * S + s;
+    EuclideanPoint x{(double)src.p[0][si], (double)src.p[1][si], (double)src.p[2][si],
+                     (double)src.p[3][si], (double)src.p[4][si], (double)src.p[5][si]};
+    x = predict_sample(x, feat, dt, key, step, (uint32_t)p, (uint32_t)k, (uint32_t)s);
+    const int64_t di = g * S + s;
+    dst.p[0][di] = (T)x.x; dst.p[1][di] = (T)x.y; dst.p[2][di] = (T)x.z;
+    dst.p[3][di] = (T)x.vx; dst.p[4][di] = (T)x.vy; dst.p[5][di] = (T)x.vz;
+    dst.p[6][di] = src.p[6][si];
+}
+
+/* ------------------------------------------------------- reproject + fit */
+/* One wavefront per Gaussian: deferred prediction, world -> disparity with
+ * the parent's camera and the field-of-view Pd (worldToDisparityKernel,
+ * src/kernels.cu:6-47), per-Gaussian Pd (computePdKernel :82-99), sample
+ * reweighting and normalisation (src/scphdcameracalibration.cu:1602-1608,
+ * normalizeWeightsKernel src/kernels.cu:996-1018), weighted moment fit
+ * (fitGaussiansKernel :102-438) and, when the non-detection term is kept
+ * (w (1 - pd) >= min_weight, computeNonDetections
+ * src/scphdcameracalibration.cu:1290-1420), the non-detection cloud
+ * (disparity -> world round trip, weights 1 - pd_i) written to its own
+ * block.  Samples stay in registers: one HBM read per sample. */
+template <typename T, int NS>
+__global__ __launch_bounds__(256) void k_fit(FitArgs a, Planes<T> src, Planes<T> dst) {
+    const int lane = threadIdx.x & 63;
+    const int64_t g = (int64_t)blockIdx.x * WPB + (threadIdx.x >> 6);
+    if (g >= a.Ng) return; /* whole wavefront; no workgroup barriers below */
+    const int p = a.gparent[g];
+    const int k = (int)(g - a.goff[p]);
+    const int64_t b0 = (int64_t)a.gblock[g] * a.S;
+    const CameraPose pose = load_pose(a.pose, p);
+    const int S = a.S;
+
+    double q[NS][6], pdv[NS], wv[NS];
+    double spd = 0.0, sw = 0.0;
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        const int s = lane + 64 * i;
+        pdv[i] = 0.0;
+        wv[i] = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) q[i][c] = 0.0;
+        if (s < S) {
+            const int64_t si = b0 + s;
+            EuclideanPoint x{(double)src.p[0][si], (double)src.p[1][si], (double)src.p[2][si],
+                             (double)src.p[3][si], (double)src.p[4][si], (double)src.p[5][si]};
+            const double w = (double)src.p[6][si];
+            if (a.pending) x = predict_sample(x, a.feat, a.dt, a.key, a.pstep, (uint32_t)p, (uint32_t)k, (uint32_t)s);
+            const DisparityPoint d = a.meas.computeMeasurement(x, pose);
+            q[i][0] = d.u; q[i][1] = d.v; q[i][2] = d.d; q[i][3] = d.vu; q[i][4] = d.vv; q[i][5] = d.vd;
+            const double pd = a.meas.inRange(d) ? a.meas.pd() : 0.0;
+            pdv[i] = pd;
+            wv[i] = w * pd;
+            spd += pd;
+            sw += wv[i];
+        }
+    }
+    spd = wave_sum(spd);
+    sw = wave_sum(sw);
+    const double pdg = spd / S;
+    const double lsum = safe_log(sw);
+
+    /* normalised weights, weighted mean, W, sum of squares */
+    double m[6] = {0, 0, 0, 0, 0, 0}, W = 0.0, W2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        const int s = lane + 64 * i;
+        double wn = 0.0;
+        if (s < S) wn = exp(safe_log(wv[i]) - lsum);
+        wv[i] = wn;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) m[c] += q[i][c] * wn;
+        W += wn;
+        W2 += wn * wn;
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) m[c] = wave_sum(m[c]);
+    W = wave_sum(W);
+    W2 = wave_sum(W2);
+    const double cn = W / (W * W + W2);
+
+    /* centred second moments */
+    double P[21];
+#pragma unroll
+    for (int e = 0; e < 21; ++e) P[e] = 0.0;
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        double dv[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) dv[c] = q[i][c] - m[c];
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+#pragma unroll
+            for (int c = 0; c <= r; ++c) P[pidx(r, c)] += dv[r] * dv[c] * wv[i];
+    }
+#pragma unroll
+    for (int e = 0; e < 21; ++e) P[e] = wave_sum(P[e]);
+
+    /* lanes 0..27 write one output plane each */
+    {
+        double v = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            if (lane == FP_MU + c) v = m[c];
+#pragma unroll
+        for (int e = 0; e < 21; ++e)
+            if (lane == FP_P + e) v = P[e] / cn;
+        if (lane == FP_PD) v = pdg;
+        if (lane < FP_COUNT) a.fit[(int64_t)lane * a.Ng + g] = v;
+    }
+
+    /* non-detection term */
+    const double wg = a.gw[g];
+    const bool keep = wg * (1 - pdg) >= a.min_weight;
+    if (lane == 0) a.keep[g] = keep ? 1 : 0;
+    if (keep) {
+        const int64_t d0 = (a.nd_base + g) * (int64_t)S;
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            const int s = lane + 64 * i;
+            if (s < S) {
+                const DisparityPoint d{q[i][0], q[i][1], q[i][2], q[i][3], q[i][4], q[i][5]};
+                const EuclideanPoint x = a.meas.invertMeasurement(d, pose);
+                const int64_t di = d0 + s;
+                dst.p[0][di] = (T)x.x; dst.p[1][di] = (T)x.y; dst.p[2][di] = (T)x.z;
+                dst.p[3][di] = (T)x.vx; dst.p[4][di] = (T)x.vy; dst.p[5][di] = (T)x.vz;
+                dst.p[6][di] = (T)(1.0 - pdv[i]);
+            }
+        }
+    }
+}
+
+/* Measurement-independent detection terms of each fitted Gaussian
+ * (computeDetectionsKernel, src/kernels.cu:492-598): innovation covariance,
+ * gain, Joseph covariance and its PD test, log(pd) + log(w), log det. */
+__global__ void k_precompute(const double* fit, const double* gw, int64_t Ng, double var_u, double var_v,
+                             double* pre, int32_t* pdok) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= Ng) return;
+    double P[21];
+#pragma unroll
+    for (int e = 0; e < 21; ++e) P[e] = fit[(FP_P + e) * Ng + g];
+    DetectionTerms r;
+    detection_terms(P, var_u, var_v, r);
+    const bool ok = is_pd6(r.Pn);
+    const double pdg = fit[FP_PD * Ng + g];
+    pre[PP_MUU * Ng + g] = fit[(FP_MU + 0) * Ng + g];
+    pre[PP_MUV * Ng + g] = fit[(FP_MU + 1) * Ng + g];
+    pre[PP_S0 * Ng + g] = r.s0;
+    pre[PP_S12 * Ng + g] = r.s1 + r.s2;
+    pre[PP_S3 * Ng + g] = r.s3;
+    pre[PP_C * Ng + g] = safe_log(pdg) + safe_log(gw[g]);
+    pre[PP_H * Ng + g] = 0.5 * safe_log(r.det);
+#pragma unroll
+    for (int i = 0; i < 12; ++i) pre[(PP_K + i) * Ng + g] = r.K[i];
+#pragma unroll
+    for (int e = 0; e < 21; ++e) pre[(PP_PN + e) * Ng + g] = r.Pn[e];
+    pdok[g] = ok ? 1 : 0;
+}
+
+/* ------------------------------------------- normalisers and survivors */
+static constexpr int ETA_CH = 512; /* Gaussians staged in LDS per chunk */
+
+struct LikChunk {
+    double v[7][ETA_CH];
+    int ok[ETA_CH];
+};
+
+__device__ inline void stage_chunk(LikChunk& c, const EtaArgs& a, int64_t g0, int n) {
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+#pragma unroll
+        for (int q = 0; q < 7; ++q) c.v[q][j] = a.pre[(int64_t)q * a.Ng + g0 + j];
+        c.ok[j] = a.pdok[g0 + j];
+    }
+}
+
+/* log weight of the detection term of staged Gaussian j and measurement
+ * (zu, zv): log pd + log w - dist/2 - log 2 pi - log det / 2
+ * (src/kernels.cu:527-539); log(0) when the updated covariance is not PD
+ * (:597-598). */
+__device__ inline double det_lw(const LikChunk& c, int j, double zu, double zv) {
+    const double i0 = zu - c.v[PP_MUU][j], i1 = zv - c.v[PP_MUV][j];
+    const double dist = i0 * i0 * c.v[PP_S0][j] + i0 * i1 * c.v[PP_S12][j] + i1 * i1 * c.v[PP_S3][j];
+    const double lw = c.v[PP_C][j] - 0.5 * dist - 1.8378770664093453 - c.v[PP_H][j];
+    return c.ok[j] ? lw : SCPHD_LOG0;
+}
+
+/* Per parent: eta_{p,m} = log( sum_g exp(lw_{g,m}) + M w0 + kappa ) for
+ * every measurement (normaliser reduce_by_key + log_sum_exp,
+ * src/scphdcameracalibration.cu:1729-1768, exact per-parent segments), the
+ * single-cluster log-likelihood sum_m eta + sum_g w_g (:1813-1849) and the
+ * number of detection/birth terms whose normalised weight reaches
+ * min_weight (updateKernel's prune flag, src/kernels.cu:608-659).
+ * One workgroup per parent, one thread per measurement, Gaussians staged
+ * through LDS. */
+__global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
+    __shared__ LikChunk ch;
+    __shared__ double red[16];
+    const int p = blockIdx.x;
+    const int t = threadIdx.x;
+    const int64_t g0 = a.goff[p];
+    const int G = a.goff[p + 1] - a.goff[p];
+    double eta_sum = 0.0;
+    int cnt = 0;
+    const int rounds = (a.M + NT - 1) / NT;
+    const bool single = G <= ETA_CH; /* whole map staged once */
+    if (single && G > 0) {
+        stage_chunk(ch, a, g0, G);
+        __syncthreads();
+    }
+    for (int r = 0; r < rounds; ++r) {
+        const int m = r * NT + t;
+        const bool valid = m < a.M;
+        const double zu = valid ? a.zu[m] : 0.0, zv = valid ? a.zv[m] : 0.0;
+        /* eta = mx + log1p(sum of the other terms' exp(l - mx)), mx the
+         * first maximum of (bpc, l_0, ...), terms summed g ascending then
+         * bpc: the normaliser definition the oracle uses bit for bit */
+        double mx = a.bpc;
+        int am = -1;
+        for (int c0 = 0; c0 < G; c0 += ETA_CH) {
+            const int n = min(ETA_CH, G - c0);
+            if (!single) {
+                __syncthreads();
+                stage_chunk(ch, a, g0 + c0, n);
+                __syncthreads();
+            }
+            if (valid)
+                for (int j = 0; j < n; ++j) {
+                    const double l = det_lw(ch, j, zu, zv);
+                    if (l > mx) {
+                        mx = l;
+                        am = c0 + j;
+                    }
+                }
+        }
+        double T = 0.0;
+        for (int c0 = 0; c0 < G; c0 += ETA_CH) {
+            const int n = min(ETA_CH, G - c0);
+            if (!single) {
+                __syncthreads();
+                stage_chunk(ch, a, g0 + c0, n);
+                __syncthreads();
+            }
+            if (valid)
+                for (int j = 0; j < n; ++j)
+                    if (c0 + j != am) T += exp(det_lw(ch, j, zu, zv) - mx);
+        }
+        if (am >= 0) T += exp(a.bpc - mx);
+        const double eta = mx + log1p(T);
+        if (valid) {
+            a.eta[(int64_t)p * a.M + m] = eta;
+            eta_sum += eta;
+        }
+        for (int c0 = 0; c0 < G; c0 += ETA_CH) {
+            const int n = min(ETA_CH, G - c0);
+            if (!single) {
+                __syncthreads();
+                stage_chunk(ch, a, g0 + c0, n);
+                __syncthreads();
+            }
+            if (valid) {
+                for (int j = 0; j < n; ++j) {
+                    const double w = exp(det_lw(ch, j, zu, zv) - eta);
+                    cnt += (w >= a.min_weight && ch.ok[j]) ? 1 : 0;
+                }
+            }
+        }
+        if (valid) {
+            const double w = exp(a.birth.logw0 - eta);
+            cnt += (w >= a.min_weight && a.birth.pd_ok) ? 1 : 0;
+        }
+    }
+    double swg = 0.0;
+    for (int j = t; j < G; j += NT) swg += a.gw[g0 + j];
+    const double se = block_sum(eta_sum, red);
+    const double sg = block_sum(swg, red);
+    const double sc = block_sum((double)cnt, red);
+    if (t == 0) {
+        a.loglik[p] = se + sg;
+        a.nsurv[p] = (int32_t)sc;
+    }
+}
+
+/* Writes the surviving detection terms (Kalman-updated mean, weight
+ * exp(lw - eta)) and birth terms of every parent, compacted in the
+ * reference's pack order (updateKernel, src/kernels.cu:608-659: detection
+ * terms measurement-major, then births). */
+__global__ __launch_bounds__(NT) void k_survivors(SurvArgs a) {
+    __shared__ LikChunk ch;
+    __shared__ int sh[8];
+    const EtaArgs& e = a.e;
+    const int p = blockIdx.x;
+    const int t = threadIdx.x;
+    const int64_t g0 = e.goff[p];
+    const int G = e.goff[p + 1] - e.goff[p];
+    const int64_t base = a.survoff[p];
+    const int cap = a.survoff[p + 1] - a.survoff[p];
+    int run = 0;
+    const int rounds = (e.M + NT - 1) / NT;
+    for (int r = 0; r < rounds; ++r) {
+        const int m = r * NT + t;
+        const bool valid = m < e.M;
+        const double zu = valid ? e.zu[m] : 0.0, zv = valid ? e.zv[m] : 0.0;
+        const double eta = valid ? e.eta[(int64_t)p * e.M + m] : 0.0;
+        int cnt = 0;
+        for (int c0 = 0; c0 < G; c0 += ETA_CH) {
+            const int n = min(ETA_CH, G - c0);
+            __syncthreads();
+            stage_chunk(ch, e, g0 + c0, n);
+            __syncthreads();
+            if (valid)
+                for (int j = 0; j < n; ++j) {
+                    const double w = exp(det_lw(ch, j, zu, zv) - eta);
+                    cnt += (w >= e.min_weight && ch.ok[j]) ? 1 : 0;
+                }
+        }
+        int tot;
+        int o = run + block_exscan(cnt, sh, &tot);
+        for (int c0 = 0; c0 < G; c0 += ETA_CH) {
+            const int n = min(ETA_CH, G - c0);
+            if (G > ETA_CH) {
+                __syncthreads();
+                stage_chunk(ch, e, g0 + c0, n);
+                __syncthreads();
+            }
+            if (valid)
+                for (int j = 0; j < n; ++j) {
+                    const double w = exp(det_lw(ch, j, zu, zv) - eta);
+                    if (w >= e.min_weight && ch.ok[j]) {
+                        if (o < cap) {
+                            const int64_t gi = g0 + c0 + j;
+                            const int64_t so = base + o;
+                            const double i0 = zu - ch.v[PP_MUU][j], i1 = zv - ch.v[PP_MUV][j];
+                            a.sw[so] = w;
+                            a.sg[so] = (int32_t)gi;
+                            a.sm[so] = m;
+#pragma unroll
+                            for (int c = 0; c < 6; ++c) {
+                                const double mu = a.fit[(FP_MU + c) * e.Ng + gi];
+                                a.smu[c * a.cap + so] = mu + i0 * e.pre[(PP_K + c) * e.Ng + gi] +
+                                                        i1 * e.pre[(PP_K + 6 + c) * e.Ng + gi];
+                            }
+                        }
+                        ++o;
+                    }
+                }
+        }
+        run += tot;
+    }
+    for (int r = 0; r < rounds; ++r) {
+        const int m = r * NT + t;
+        const bool valid = m < e.M;
+        const double eta = valid ? e.eta[(int64_t)p * e.M + m] : 0.0;
+        const double w = valid ? exp(e.birth.logw0 - eta) : 0.0;
+        const bool ok = valid && w >= e.min_weight && e.birth.pd_ok;
+        int tot;
+        const int o = run + block_exscan(ok ? 1 : 0, sh, &tot);
+        if (ok && o < cap) {
+            const int64_t so = base + o;
+            a.sw[so] = w;
+            a.sg[so] = -1 - m;
+            a.sm[so] = m;
+            a.smu[0 * a.cap + so] = e.zu[m];
+            a.smu[1 * a.cap + so] = e.zv[m];
+#pragma unroll
+            for (int c = 2; c < 6; ++c) a.smu[c * a.cap + so] = e.birth.mean[c];
+        }
+        run += tot;
+    }
+    if (t == 0 && run != cap) *a.err = 1;
+}
+
+/* -------------------------------------------------------------- merging */
+/* Workspace of one parent's merge, U slots (Up = next power of two >= U):
+ *   double key_w[Up]; int ord[Up]; double pf_u[U]; float pf_su[U];
+ *   uint8 flag[U]; uint8 memb[U]; int mlist[U]   (8-byte aligned pieces) */
+__host__ __device__ inline int next_pow2(int x) {
+    int v = 1;
+    while (v < x) v <<= 1;
+    return v;
+}
+
+struct MergeLayout {
+    int64_t ord, pf_u, pf_su, flag, memb, mlist, bytes;
+    __host__ __device__ explicit MergeLayout(int U) {
+        const int64_t Up = next_pow2(U > 0 ? U : 1);
+        ord = Up * 8;
+        pf_u = (ord + Up * 4 + 7) & ~(int64_t)7;
+        pf_su = pf_u + (int64_t)U * 8;
+        flag = pf_su + (int64_t)U * 4;
+        memb = flag + U;
+        mlist = (memb + U + 7) & ~(int64_t)7;
+        bytes = mlist + (int64_t)U * 4;
+    }
+};
+
+/* global scratch for parents that do not fit in LDS: 48 bytes per survivor
+ * slot plus 128 per parent bounds MergeLayout(U).bytes since Up < 2U */
+static constexpr int64_t MERGE_WS_PER_SLOT = 48, MERGE_WS_PER_PARENT = 128;
+
+int merge_lds_bytes(int cap) { return (int)MergeLayout(cap).bytes; }
+
+int64_t merge_scratch_bytes(int64_t Stot, int Np) {
+    return Stot * MERGE_WS_PER_SLOT + (int64_t)Np * MERGE_WS_PER_PARENT + 256;
+}
+
+__device__ inline void survivor_P(const MergeArgs& a, int64_t si, double* P) {
+    const int g = a.sg[si];
+    if (g >= 0) {
+#pragma unroll
+        for (int e = 0; e < 21; ++e) P[e] = a.pre[(int64_t)(PP_PN + e) * a.Ng + g];
+    } else {
+#pragma unroll
+        for (int e = 0; e < 21; ++e) P[e] = a.birth.P[e];
+    }
+}
+
+/* Greedy Gaussian-mixture merge of one parent's survivors
+ * (phdUpdateMergeKernel, src/kernels.cu:662-874): repeatedly take the
+ * heaviest unmerged term (ties: lowest pack index), merge every unmerged
+ * term within squared Mahalanobis distance min_separation of it
+ * (Sigma = P_a + P_b, computeMahalDist src/device_math.cu:374-453) by
+ * moment matching, emit the merge if PD else the leader.  The survivors are
+ * sorted once by (weight desc, index asc) so the leader is the next
+ * unmerged entry in that order; a conservative one-dimensional bound on the
+ * u coordinate skips the exact 6x6 test for far-away terms. */
+__global__ __launch_bounds__(NT) void k_merge(MergeArgs a) {
+    extern __shared__ char lds[];
+    __shared__ int s_lead, s_pos, s_nm;
+    __shared__ double s_W, s_lw, s_lmu[6], s_lP[21], s_mu[6], s_P[21];
+    const int p = blockIdx.x;
+    const int t = threadIdx.x;
+    const int64_t sbase = a.survoff[p];
+    const int U = a.survoff[p + 1] - a.survoff[p];
+    const bool big = U > a.lds_cap;
+    if ((a.big != 0) != big) return; /* the other launch handles this parent (uniform) */
+    if (U == 0) {
+        if (t == 0) a.nmerged[p] = 0;
+        return;
+    }
+    char* ws = big ? (a.scratch + sbase * MERGE_WS_PER_SLOT + (int64_t)p * MERGE_WS_PER_PARENT) : lds;
+    const int Up = next_pow2(U);
+    const MergeLayout ly(U);
+    double* key_w = (double*)ws;
+    int* ord = (int*)(ws + ly.ord);
+    double* pf_u = (double*)(ws + ly.pf_u);
+    float* pf_su = (float*)(ws + ly.pf_su);
+    uint8_t* flag = (uint8_t*)(ws + ly.flag);
+    uint8_t* memb = (uint8_t*)(ws + ly.memb);
+    int* mlist = (int*)(ws + ly.mlist);
+    const double thr = a.min_separation;
+
+    for (int i = t; i < Up; i += NT) {
+        if (i < U) {
+            key_w[i] = a.sw[sbase + i];
+            ord[i] = i;
+            pf_u[i] = a.smu[0 * a.scap + sbase + i];
+            const int g = a.sg[sbase + i];
+            pf_su[i] = (float)(g >= 0 ? a.pre[(int64_t)(PP_PN + pidx(0, 0)) * a.Ng + g] : a.birth.P[pidx(0, 0)]);
+            flag[i] = 0;
+        } else {
+            key_w[i] = -INFINITY;
+            ord[i] = 0x7fffffff;
+        }
+    }
+    __syncthreads();
+    /* bitonic sort, "before" = (w desc, index asc) */
+    for (int k = 2; k <= Up; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = t; i < Up; i += NT) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const double wi = key_w[i], wl = key_w[l];
+                    const int ii = ord[i], il = ord[l];
+                    const bool l_before_i = (wl > wi) || (wl == wi && il < ii);
+                    const bool asc = (i & k) == 0;
+                    if (asc == l_before_i) {
+                        key_w[i] = wl; key_w[l] = wi;
+                        ord[i] = il; ord[l] = ii;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (t == 0) s_pos = 0;
+    __syncthreads();
+
+    const int lane = t & 63, wave = t >> 6;
+    int K = 0;
+    const int64_t mbase = sbase;
+    for (;;) {
+        if (wave == 0) {
+            int found = -1;
+            for (int r0 = s_pos; r0 < U; r0 += 64) {
+                const int r = r0 + lane;
+                const bool c = r < U && !flag[ord[r]];
+                const uint64_t b = __ballot(c);
+                if (b) {
+                    found = r0 + __ffsll((unsigned long long)b) - 1;
+                    break;
+                }
+            }
+            if (lane == 0) {
+                /* the next search restarts at the leader's rank: a leader
+                 * that is not its own member stays selectable, and the
+                 * following round then finds no members and stops, as the
+                 * reference's argmax loop does */
+                s_lead = found >= 0 ? ord[found] : -1;
+                s_pos = found >= 0 ? found : U;
+            }
+        }
+        __syncthreads();
+        const int lead = s_lead;
+        if (lead < 0) break;
+        if (t < 6) s_lmu[t] = a.smu[t * a.scap + sbase + lead];
+        if (t == 0) s_lw = a.sw[sbase + lead];
+        if (t < 21) {
+            const int g = a.sg[sbase + lead];
+            s_lP[t] = g >= 0 ? a.pre[(int64_t)(PP_PN + t) * a.Ng + g] : a.birth.P[t];
+        }
+        __syncthreads();
+        const double lu = pf_u[lead];
+        const double lsu = pf_su[lead];
+        for (int i = t; i < U; i += NT) {
+            uint8_t mb = 0;
+            if (!flag[i]) {
+                const double du = pf_u[i] - lu;
+                const double su = (double)pf_su[i] + lsu;
+                /* d^2 >= du^2 / Sigma_uu: reject only with a margin */
+                if (!(du * du > 1.001 * thr * su)) {
+                    double mu[6], P[21];
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) mu[c] = a.smu[c * a.scap + sbase + i];
+                    survivor_P(a, sbase + i, P);
+                    const double d = mahal6(s_lmu, s_lP, mu, P);
+                    mb = d < thr ? 1 : 0;
+                }
+            }
+            memb[i] = mb;
+        }
+        __syncthreads();
+        if (wave == 0) {
+            int nm = 0;
+            for (int r0 = 0; r0 < U; r0 += 64) {
+                const int i = r0 + lane;
+                const bool mbit = i < U && memb[i];
+                const uint64_t b = __ballot(mbit);
+                if (mbit) mlist[nm + __popcll(b & ((1ull << lane) - 1ull))] = i;
+                nm += __popcll(b);
+            }
+            if (lane == 0) s_nm = nm;
+        }
+        __syncthreads();
+        const int nm = s_nm;
+        if (t == 0) {
+            double W = 0.0;
+            for (int j = 0; j < nm; ++j) W += a.sw[sbase + mlist[j]];
+            s_W = W;
+        }
+        __syncthreads();
+        const double W = s_W;
+        if (W == 0) break;
+        if (t < 6) {
+            double acc = 0.0;
+            for (int j = 0; j < nm; ++j) {
+                const int64_t si = sbase + mlist[j];
+                acc += a.sw[si] * a.smu[t * a.scap + si];
+            }
+            s_mu[t] = acc / W;
+        }
+        __syncthreads();
+        if (t < 21) {
+            int r = 0;
+            while ((r + 1) * (r + 2) / 2 <= t) ++r;
+            const int c = t - r * (r + 1) / 2;
+            double acc = 0.0;
+            for (int j = 0; j < nm; ++j) {
+                const int64_t si = sbase + mlist[j];
+                const int g = a.sg[si];
+                const double Pe = g >= 0 ? a.pre[(int64_t)(PP_PN + t) * a.Ng + g] : a.birth.P[t];
+                const double dc = s_mu[c] - a.smu[c * a.scap + si];
+                const double dr = s_mu[r] - a.smu[r * a.scap + si];
+                acc += a.sw[si] * (Pe + dc * dr);
+            }
+            s_P[t] = acc / W;
+        }
+        __syncthreads();
+        if (t == 0) {
+            const bool ok = is_pd6(s_P);
+            const int64_t o = mbase + K;
+            a.mw[o] = ok ? W : s_lw;
+            for (int c = 0; c < 6; ++c) a.mmu[c * a.scap + o] = ok ? s_mu[c] : s_lmu[c];
+            for (int e = 0; e < 21; ++e) a.mP[e * a.scap + o] = ok ? s_P[e] : s_lP[e];
+        }
+        for (int j = t; j < nm; j += NT) flag[mlist[j]] = 1;
+        ++K;
+        __syncthreads();
+    }
+    if (t == 0) a.nmerged[p] = K;
+}
+
+/* ------------------------------------------------------ parent weights */
+/* w_p <- w_p exp(loglik_p), normalised, in the log domain with a max shift
+ * (src/scphdcameracalibration.cu:1851-1875 multiplies in the linear
+ * domain).  One workgroup. */
+__global__ __launch_bounds__(1024) void k_parent_weights(double* pw, const double* loglik, int Np) {
+    __shared__ double red[16];
+    double mx = -INFINITY;
+    for (int p = threadIdx.x; p < Np; p += blockDim.x) mx = fmax(mx, safe_log(pw[p]) + loglik[p]);
+    mx = block_max(mx, red);
+    double s = 0.0;
+    for (int p = threadIdx.x; p < Np; p += blockDim.x) s += exp(safe_log(pw[p]) + loglik[p] - mx);
+    s = block_sum(s, red);
+    __syncthreads();
+    for (int p = threadIdx.x; p < Np; p += blockDim.x) pw[p] = exp(safe_log(pw[p]) + loglik[p] - mx) / s;
+}
+
+/* ---------------------------------------------------- posterior samples */
+/* One wavefront per merged Gaussian: S draws x = L z + mu with L the
+ * Cholesky factor of the merged covariance (sample_disparity_gaussian,
+ * src/scphdcameracalibration.cu:640-690) mapped to the world by the
+ * parent's camera (disparityToWorldKernel, src/kernels.cu:49-80), sample
+ * weight 1/S (:2009-2010). */
+template <typename T, int NS>
+__global__ __launch_bounds__(256) void k_sample(SampleArgs a, Planes<T> dst) {
+    const int lane = threadIdx.x & 63;
+    const int64_t gm = (int64_t)blockIdx.x * WPB + (threadIdx.x >> 6);
+    if (gm >= a.Kt) return;
+    const int p = upper_parent(a.mergeoff, a.Np, gm);
+    const int k = (int)(gm - a.mergeoff[p]);
+    const int64_t rec = (int64_t)a.survoff[p] + k;
+    double mu[6], P[21], L[21];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) mu[c] = a.mmu[c * a.scap + rec];
+#pragma unroll
+    for (int e = 0; e < 21; ++e) P[e] = a.mP[e * a.scap + rec];
+    chol6(P, L);
+    const CameraPose pose = load_pose(a.pose, p);
+    const int64_t d0 = (a.post_base + gm) * (int64_t)a.S;
+    const double w = 1.0 / a.S;
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        const int s = lane + 64 * i;
+        if (s < a.S) {
+            double z[6];
+            normals<6>(a.key, NS_POSTERIOR, a.ustep, (uint32_t)p, (uint32_t)k, (uint32_t)s, z);
+            double q[6];
+#pragma unroll
+            for (int r = 0; r < 6; ++r) {
+                double acc = 0.0;
+#pragma unroll
+                for (int c = 0; c <= r; ++c) acc += L[pidx(r, c)] * z[c];
+                q[r] = acc + mu[r];
+            }
+            const DisparityPoint d{q[0], q[1], q[2], q[3], q[4], q[5]};
+            const EuclideanPoint x = a.meas.invertMeasurement(d, pose);
+            const int64_t di = d0 + s;
+            dst.p[0][di] = (T)x.x; dst.p[1][di] = (T)x.y; dst.p[2][di] = (T)x.z;
+            dst.p[3][di] = (T)x.vx; dst.p[4][di] = (T)x.vy; dst.p[5][di] = (T)x.vz;
+            dst.p[6][di] = (T)w;
+        }
+    }
+}
+
+/* -------------------------------------------------------- map assembly */
+__global__ void k_outcount(const int32_t* nmerged, const int32_t* goff, const int32_t* keepscan, int Np,
+                           int32_t* outcnt) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= Np) return;
+    outcnt[p] = nmerged[p] + (keepscan[goff[p + 1]] - keepscan[goff[p]]);
+}
+
+__global__ void k_ndlist(const int32_t* keep, const int32_t* keepscan, int64_t Ng, int32_t* ndlist) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < Ng && keep[g]) ndlist[keepscan[g]] = (int32_t)g;
+}
+
+/* New map table per parent: [merged posterior Gaussians | kept
+ * non-detection Gaussians in input order] (recombineNonDetections,
+ * src/scphdcameracalibration.cu:1422-1574, weights w (1 - pd) for the
+ * non-detection terms, :1314-1318). */
+__global__ void k_table(TableArgs a) {
+    const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= a.Nout) return;
+    const int p = upper_parent(a.outoff, a.Np, o);
+    const int j = (int)(o - a.outoff[p]);
+    const int K = a.mergeoff[p + 1] - a.mergeoff[p];
+    if (j < K) {
+        a.gblock[o] = (int32_t)(a.post_base + a.mergeoff[p] + j);
+        a.gwo[o] = a.mw[(int64_t)a.survoff[p] + j];
+    } else {
+        const int g = a.ndlist[a.keepscan[a.goff[p]] + (j - K)];
+        a.gblock[o] = (int32_t)(a.nd_base + g);
+        a.gwo[o] = a.gw[g] * (1 - a.fit[(int64_t)FP_PD * a.Ng_in + g]);
+    }
+    a.gparent[o] = p;
+}
+
+/* ---------------------------------------------------------- resampling */
+/* n_eff = 1 / (N sum w^2) (computeNeff :2641-2649); when below min_neff,
+ * systematic resampling with points u + j/N against the sequential
+ * cumulative sum of the weights (resample :2651-2702), the overrun fallback
+ * to the first maximum of exp(w), gather of parent states, and weights 1/N.
+ * One workgroup.  The cumulative sum keeps the reference's left-to-right
+ * rounding: thread 0 accumulates chunks staged in LDS. */
+static constexpr int RS_CH = 2048;
+
+__global__ __launch_bounds__(1024) void k_resample(ResampleArgs a) {
+    __shared__ double red[16];
+    __shared__ double s_u;
+    __shared__ double chunk[RS_CH];
+    __shared__ double s_em[16];
+    __shared__ int s_am[16];
+    const int N = a.Np;
+    const int t = threadIdx.x;
+    double ss = 0.0;
+    for (int p = t; p < N; p += blockDim.x) ss += a.pw[p] * a.pw[p];
+    ss = block_sum(ss, red);
+    const double neff = 1.0 / (N * ss);
+    const bool go = !(neff >= a.min_neff);
+    if (t == 0) {
+        *a.neff = neff;
+        *a.flag = go ? 1 : 0;
+    }
+    if (!go) {
+        for (int p = t; p < N; p += blockDim.x) {
+            a.anc[p] = p;
+            a.newcnt[p] = a.goff[p + 1] - a.goff[p];
+        }
+        return;
+    }
+    /* first index of the maximum of exp(w) */
+    double em = -1.0;
+    int am = -1;
+    for (int i = t; i < N; i += blockDim.x) {
+        const double e = exp(a.pw[i]);
+        if (e > em) {
+            em = e;
+            am = i;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double e2 = __shfl_xor(em, o, 64);
+        const int a2 = __shfl_xor(am, o, 64);
+        if (e2 > em || (e2 == em && a2 >= 0 && (am < 0 || a2 < am))) {
+            em = e2;
+            am = a2;
+        }
+    }
+    if ((t & 63) == 0) {
+        s_em[t >> 6] = em;
+        s_am[t >> 6] = am;
+    }
+    if (t == 0) {
+        uint32_t x[4];
+        noise_block(a.key, NS_RESAMPLE, a.rstep, 0, 0, 0, 0, x);
+        s_u = uniform_co(x[0], x[1]) * (1.0 / N);
+    }
+    /* sequential cumulative sum */
+    double c = 0.0;
+    for (int c0 = 0; c0 < N; c0 += RS_CH) {
+        const int n = min(RS_CH, N - c0);
+        __syncthreads();
+        for (int i = t; i < n; i += blockDim.x) chunk[i] = a.pw[c0 + i];
+        __syncthreads();
+        if (t == 0) {
+            int i = 0;
+            if (c0 == 0) {
+                c = chunk[0];
+                a.cum[0] = c;
+                i = 1;
+            }
+            for (; i < n; ++i) {
+                c += chunk[i];
+                a.cum[c0 + i] = c;
+            }
+        }
+    }
+    __syncthreads();
+    int amax = -1;
+    {
+        double bm = -1.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            if (s_am[w] >= 0 && (s_em[w] > bm || (s_em[w] == bm && s_am[w] < amax))) {
+                bm = s_em[w];
+                amax = s_am[w];
+            }
+        }
+    }
+    const double interval = 1.0 / N;
+    for (int j = t; j < N; j += blockDim.x) {
+        const double r = s_u + j * interval;
+        int lo = 0, hi = N;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (!(r > a.cum[mid]))
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        const int anc = lo < N ? lo : amax;
+        a.anc[j] = anc;
+        for (int q = 0; q < 12; ++q) a.ext_tmp[(int64_t)j * 12 + q] = a.ext[(int64_t)anc * 12 + q];
+        a.newcnt[j] = a.goff[anc + 1] - a.goff[anc];
+    }
+    __syncthreads();
+    for (int j = t; j < N; j += blockDim.x) {
+        for (int q = 0; q < 12; ++q) a.ext[(int64_t)j * 12 + q] = a.ext_tmp[(int64_t)j * 12 + q];
+        a.pw[j] = 1.0 / N;
+    }
+}
+
+__global__ void k_iota(int32_t* x, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = (int32_t)i;
+}
+
+__global__ void k_table_resample(const int32_t* anc, const int32_t* goff_old, const int32_t* goff_new,
+                                 const int32_t* gblock_old, const double* gw_old, int Np, int64_t Nnew,
+                                 int32_t* gparent, int32_t* gblock, double* gw) {
+    const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= Nnew) return;
+    const int p = upper_parent(goff_new, Np, o);
+    const int64_t src = (int64_t)goff_old[anc[p]] + (o - goff_new[p]);
+    gparent[o] = p;
+    gblock[o] = gblock_old[src];
+    gw[o] = gw_old[src];
+}
+
+/* ------------------------------------------------------------ read-back */
+template <typename T>
+__global__ void k_extract(Planes<T> src, const int32_t* gblock, int64_t Ng, int S, double* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = Ng * S;
+    if (i >= n) return;
+    const int64_t g = i / S;
+    const int64_t si = (int64_t)gblock[g] * S + (i - g * S);
+#pragma unroll
+    for (int c = 0; c < 7; ++c) out[c * n + i] = (double)src.p[c][si];
+}
+
+template <typename T>
+__global__ void k_inject(const double* in, int64_t Ng, int S, Planes<T> dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = Ng * S;
+    if (i >= n) return;
+#pragma unroll
+    for (int c = 0; c < 7; ++c) dst.p[c][i] = (T)in[c * n + i];
+}
+
+/* ----------------------------------------------------------------- scan */
+static constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_TILE = SCAN_T * SCAN_I;
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_tiles(const int32_t* in, int32_t* out, int64_t n, int32_t* sums) {
+    __shared__ int sh[8];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_I;
+    int v[SCAN_I];
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_I; ++i) {
+        v[i] = (base + i < n) ? in[base + i] : 0;
+        s += v[i];
+    }
+    int tot;
+    int off = block_exscan(s, sh, &tot);
+#pragma unroll
+    for (int i = 0; i < SCAN_I; ++i) {
+        if (base + i < n) out[base + i] = off;
+        off += v[i];
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_sums(int32_t* sums, int64_t nb, int32_t* total_out) {
+    __shared__ int sh[8];
+    int carry = 0;
+    for (int64_t c0 = 0; c0 < nb; c0 += SCAN_T) {
+        const int64_t i = c0 + threadIdx.x;
+        const int v = i < nb ? sums[i] : 0;
+        int tot;
+        const int o = block_exscan(v, sh, &tot);
+        if (i < nb) sums[i] = carry + o;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total_out = carry;
+}
+
+__global__ void k_scan_add(int32_t* out, int64_t n, const int32_t* sums) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += sums[i / SCAN_TILE];
+}
+
+/* ============================================================ launchers */
+static inline unsigned nblocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+#define LAUNCH_CHECK() return hipGetLastError()
+
+hipError_t launch_prior(hipStream_t s, double* ext, double* pw, int Np, const double* a12, const double* b12,
+                        NoiseKey key) {
+    Bounds12 ab;
+    for (int i = 0; i < 12; ++i) {
+        ab.a[i] = a12[i];
+        ab.b[i] = b12[i];
+    }
+    k_prior<<<nblocks(Np, 256), 256, 0, s>>>(ext, pw, Np, ab, key);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_parent_predict(hipStream_t s, double* ext, int Np, OrientedLinearCVMotionModel3D cam, double dt,
+                                 NoiseKey key, uint64_t step) {
+    k_parent_predict<<<nblocks(Np, 256), 256, 0, s>>>(ext, Np, cam, dt, key, step);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_scale(hipStream_t s, double* x, int64_t n, double a) {
+    if (n <= 0) return hipSuccess;
+    k_scale<<<nblocks(n, 256), 256, 0, s>>>(x, n, a);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_pose(hipStream_t s, const double* ext, double* pose, int Np, int fixed) {
+    k_pose<<<nblocks(Np, 256), 256, 0, s>>>(ext, pose, Np, fixed);
+    LAUNCH_CHECK();
+}
+
+template <typename T>
+hipError_t launch_materialize(hipStream_t s, Planes<T> src, Planes<T> dst, const int32_t* gblock,
+                              const int32_t* gparent, const int32_t* goff, int64_t Ng, int S,
+                              LinearCVMotionModel3D feat, double dt, NoiseKey key, uint64_t step) {
+    if (Ng <= 0) return hipSuccess;
+    k_materialize<T><<<nblocks(Ng * S, 256), 256, 0, s>>>(src, dst, gblock, gparent, goff, Ng, S, feat, dt, key,
+                                                          step);
+    LAUNCH_CHECK();
+}
+
+template <typename T>
+hipError_t launch_fit(hipStream_t s, const FitArgs& a, Planes<T> src, Planes<T> dst) {
+    if (a.Ng <= 0) return hipSuccess;
+    const unsigned nb = nblocks(a.Ng, WPB);
+    const int ns = (a.S + 63) / 64;
+    if (ns <= 1)
+        k_fit<T, 1><<<nb, 256, 0, s>>>(a, src, dst);
+    else if (ns <= 2)
+        k_fit<T, 2><<<nb, 256, 0, s>>>(a, src, dst);
+    else if (ns <= 4)
+        k_fit<T, 4><<<nb, 256, 0, s>>>(a, src, dst);
+    else if (ns <= 8)
+        k_fit<T, 8><<<nb, 256, 0, s>>>(a, src, dst);
+    else
+        return hipErrorInvalidValue;
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_precompute(hipStream_t s, const double* fit, const double* gw, int64_t Ng, double var_u,
+                             double var_v, double* pre, int32_t* pdok) {
+    if (Ng <= 0) return hipSuccess;
+    k_precompute<<<nblocks(Ng, 128), 128, 0, s>>>(fit, gw, Ng, var_u, var_v, pre, pdok);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_eta(hipStream_t s, const EtaArgs& a) {
+    k_eta<<<a.Np, NT, 0, s>>>(a);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_survivors(hipStream_t s, const SurvArgs& a) {
+    k_survivors<<<a.e.Np, NT, 0, s>>>(a);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_merge(hipStream_t s, const MergeArgs& a, int lds_bytes) {
+    k_merge<<<a.Np, NT, a.big ? 0 : lds_bytes, s>>>(a);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_parent_weights(hipStream_t s, double* pw, const double* loglik, int Np) {
+    k_parent_weights<<<1, 1024, 0, s>>>(pw, loglik, Np);
+    LAUNCH_CHECK();
+}
+
+template <typename T>
+hipError_t launch_sample(hipStream_t s, const SampleArgs& a, Planes<T> dst) {
+    if (a.Kt <= 0) return hipSuccess;
+    const unsigned nb = nblocks(a.Kt, WPB);
+    const int ns = (a.S + 63) / 64;
+    if (ns <= 1)
+        k_sample<T, 1><<<nb, 256, 0, s>>>(a, dst);
+    else if (ns <= 2)
+        k_sample<T, 2><<<nb, 256, 0, s>>>(a, dst);
+    else if (ns <= 4)
+        k_sample<T, 4><<<nb, 256, 0, s>>>(a, dst);
+    else if (ns <= 8)
+        k_sample<T, 8><<<nb, 256, 0, s>>>(a, dst);
+    else
+        return hipErrorInvalidValue;
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_outcount(hipStream_t s, const int32_t* nmerged, const int32_t* goff, const int32_t* keepscan,
+                           int Np, int32_t* outcnt) {
+    k_outcount<<<nblocks(Np, 256), 256, 0, s>>>(nmerged, goff, keepscan, Np, outcnt);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_ndlist(hipStream_t s, const int32_t* keep, const int32_t* keepscan, int64_t Ng, int32_t* ndlist) {
+    if (Ng <= 0) return hipSuccess;
+    k_ndlist<<<nblocks(Ng, 256), 256, 0, s>>>(keep, keepscan, Ng, ndlist);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_table(hipStream_t s, const TableArgs& a) {
+    if (a.Nout <= 0) return hipSuccess;
+    k_table<<<nblocks(a.Nout, 256), 256, 0, s>>>(a);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_resample(hipStream_t s, const ResampleArgs& a) {
+    k_resample<<<1, 1024, 0, s>>>(a);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_iota(hipStream_t s, int32_t* x, int64_t n) {
+    if (n <= 0) return hipSuccess;
+    k_iota<<<nblocks(n, 256), 256, 0, s>>>(x, n);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_table_resample(hipStream_t s, const int32_t* anc, const int32_t* goff_old,
+                                 const int32_t* goff_new, const int32_t* gblock_old, const double* gw_old, int Np,
+                                 int64_t Nnew, int32_t* gparent, int32_t* gblock, double* gw) {
+    if (Nnew <= 0) return hipSuccess;
+    k_table_resample<<<nblocks(Nnew, 256), 256, 0, s>>>(anc, goff_old, goff_new, gblock_old, gw_old, Np, Nnew,
+                                                         gparent, gblock, gw);
+    LAUNCH_CHECK();
+}
+
+template <typename T>
+hipError_t launch_extract(hipStream_t s, Planes<T> src, const int32_t* gblock, int64_t Ng, int S, double* out) {
+    if (Ng <= 0) return hipSuccess;
+    k_extract<T><<<nblocks(Ng * S, 256), 256, 0, s>>>(src, gblock, Ng, S, out);
+    LAUNCH_CHECK();
+}
+
+template <typename T>
+hipError_t launch_inject(hipStream_t s, const double* in, int64_t Ng, int S, Planes<T> dst) {
+    if (Ng <= 0) return hipSuccess;
+    k_inject<T><<<nblocks(Ng * S, 256), 256, 0, s>>>(in, Ng, S, dst);
+    LAUNCH_CHECK();
+}
+
+int64_t scan_tmp_size(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
+
+hipError_t launch_scan(hipStream_t s, const int32_t* in, int32_t* out, int64_t n, int32_t* tmp) {
+    const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb > 0) k_scan_tiles<<<(unsigned)nb, SCAN_T, 0, s>>>(in, out, n, tmp);
+    k_scan_sums<<<1, SCAN_T, 0, s>>>(tmp, nb, out + n);
+    if (nb > 1) k_scan_add<<<nblocks(n, 256), 256, 0, s>>>(out, n, tmp);
+    LAUNCH_CHECK();
+}
+
+/* explicit instantiations */
+template hipError_t launch_materialize<float>(hipStream_t, Planes<float>, Planes<float>, const int32_t*,
+                                              const int32_t*, const int32_t*, int64_t, int, LinearCVMotionModel3D,
+                                              double, NoiseKey, uint64_t);
+template hipError_t launch_materialize<double>(hipStream_t, Planes<double>, Planes<double>, const int32_t*,
+                                               const int32_t*, const int32_t*, int64_t, int, LinearCVMotionModel3D,
+                                               double, NoiseKey, uint64_t);
+template hipError_t launch_fit<float>(hipStream_t, const FitArgs&, Planes<float>, Planes<float>);
+template hipError_t launch_fit<double>(hipStream_t, const FitArgs&, Planes<double>, Planes<double>);
+template hipError_t launch_sample<float>(hipStream_t, const SampleArgs&, Planes<float>);
+template hipError_t launch_sample<double>(hipStream_t, const SampleArgs&, Planes<double>);
+template hipError_t launch_extract<float>(hipStream_t, Planes<float>, const int32_t*, int64_t, int, double*);
+template hipError_t launch_extract<double>(hipStream_t, Planes<double>, const int32_t*, int64_t, int, double*);
+template hipError_t launch_inject<float>(hipStream_t, const double*, int64_t, int, Planes<float>);
+template hipError_t launch_inject<double>(hipStream_t, const double*, int64_t, int, Planes<double>);
+
+}  // namespace scphd

@@ -1,0 +1,288 @@
+"""Python front end of the gfx950 SC-PHD calibration filter.
+
+Thin ctypes binding of the C ABI in include/scphd.h (libscphd_hip.so, built
+in this directory by the Makefile).  The class Filter mirrors the reference
+driver SCPHDCameraCalibration (cheesinglee/scphd-camera-calibration
+src/scphdcameracalibration.cuh:29-122): predict(dt), update(u, v, fixed),
+resample(), check() plus state read-back.  There is no CPU fallback: a
+missing library raises ImportError.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libscphd_hip.so")
+
+SCPHD_OK = 0
+ERRORS = {-1: "SCPHD_ERR_ARG", -2: "SCPHD_ERR_HIP", -3: "SCPHD_ERR_NAN", -4: "SCPHD_ERR_CAPACITY",
+          -5: "SCPHD_ERR_CONFIG", -6: "SCPHD_ERR_STATE", -7: "SCPHD_ERR_NODEVICE"}
+
+_D = ctypes.c_double
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_U64 = ctypes.c_uint64
+_P = ctypes.c_void_p
+
+EXPORTED = [
+    "scphd_params_init", "scphd_params_load", "scphd_create", "scphd_destroy", "scphd_last_error",
+    "scphd_predict", "scphd_update", "scphd_update_device", "scphd_resample", "scphd_check", "scphd_sync",
+    "scphd_get_counts", "scphd_get_parents", "scphd_get_maps", "scphd_set_state", "scphd_get_update_stats",
+    "scphd_get_update_debug", "scphd_get_ancestors", "scphd_set_timing", "scphd_get_phase_ms",
+    "scphd_get_fit_debug",
+]
+
+
+class Params(ctypes.Structure):
+    """scphd_params (include/scphd.h)."""
+
+    _fields_ = [
+        ("n_particles", _I32), ("particles_per_feature", _I32), ("max_gaussians", _I32),
+        ("storage", _I32), ("verbosity", _I32), ("slam", _I32), ("max_steps", _I32),
+        ("save_all_maps", _I32),
+        ("min_weight", _D), ("min_separation", _D), ("min_neff", _D), ("ps", _D),
+        ("prior_a", _D * 12), ("prior_b", _D * 12),
+        ("cam_accel_std", _D * 6), ("feat_accel_std", _D * 3),
+        ("fx", _D), ("fy", _D), ("u0", _D), ("v0", _D),
+        ("std_u", _D), ("std_v", _D), ("pd", _D), ("lambda_", _D),
+        ("birth_w0", _D), ("birth_d0", _D), ("birth_vu0", _D), ("birth_vv0", _D), ("birth_vd0", _D),
+        ("birth_var_d0", _D), ("birth_var_vu0", _D), ("birth_var_vv0", _D), ("birth_var_vd0", _D),
+        ("max_velocity", _D), ("min_range", _D),
+        ("seed", _U64),
+    ]
+
+    @classmethod
+    def from_dict(cls, d):
+        p = cls()
+        lib().scphd_params_init(ctypes.byref(p))
+        for name, _ in cls._fields_:
+            key = "lambda" if name == "lambda_" else name
+            if key not in d:
+                continue
+            fld = getattr(p, name)
+            if isinstance(fld, ctypes.Array):
+                for i, x in enumerate(d[key]):
+                    fld[i] = x
+            else:
+                setattr(p, name, d[key])
+        return p
+
+    def to_dict(self):
+        out = {}
+        for name, _ in self._fields_:
+            key = "lambda" if name == "lambda_" else name
+            v = getattr(self, name)
+            out[key] = list(v) if isinstance(v, ctypes.Array) else v
+        return out
+
+
+class Counts(ctypes.Structure):
+    _fields_ = [(n, _I64) for n in ("n_particles", "particles_per_feature", "n_gaussians", "n_samples",
+                                    "n_predict", "n_update", "n_resample", "pending_predict")]
+
+
+class UpdateStats(ctypes.Structure):
+    _fields_ = [(n, _I64) for n in ("n_meas", "n_gaussians_in", "n_pairs", "n_survivors", "n_merged",
+                                    "n_nondetect", "n_gaussians_out", "max_survivors_per_parent",
+                                    "cap_exceeded")]
+
+
+_lib = None
+
+
+def lib():
+    """Load libscphd_hip.so; raises ImportError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `make -C {HERE}` (no CPU fallback exists)")
+    L = ctypes.CDLL(LIB_PATH)
+    dp = ctypes.POINTER(_D)
+    ip = ctypes.POINTER(_I32)
+    pp = ctypes.POINTER(Params)
+    sig = {
+        "scphd_params_init": (_I32, [pp]),
+        "scphd_params_load": (_I32, [ctypes.c_char_p, pp, ctypes.c_char_p, ctypes.c_size_t]),
+        "scphd_create": (_I32, [pp, ctypes.POINTER(_P)]),
+        "scphd_destroy": (None, [_P]),
+        "scphd_last_error": (ctypes.c_char_p, [_P]),
+        "scphd_predict": (_I32, [_P, _D]),
+        "scphd_update": (_I32, [_P, dp, dp, _I32, _I32]),
+        "scphd_update_device": (_I32, [_P, _P, _P, _I32, _I32]),
+        "scphd_resample": (_I32, [_P, ip]),
+        "scphd_check": (_I32, [_P]),
+        "scphd_sync": (_I32, [_P]),
+        "scphd_get_counts": (_I32, [_P, ctypes.POINTER(Counts)]),
+        "scphd_get_parents": (_I32, [_P, dp, dp]),
+        "scphd_get_maps": (_I32, [_P, ip, dp, dp]),
+        "scphd_set_state": (_I32, [_P, dp, dp, ip, _I64, dp, dp, _I64, _I64, _I64]),
+        "scphd_get_update_stats": (_I32, [_P, ctypes.POINTER(UpdateStats)]),
+        "scphd_get_update_debug": (_I32, [_P, dp, dp, ip, ip, ip]),
+        "scphd_get_ancestors": (_I32, [_P, ip]),
+        "scphd_set_timing": (_I32, [_P, _I32]),
+        "scphd_get_phase_ms": (_I32, [_P, dp]),
+        "scphd_get_fit_debug": (_I32, [_P, dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(_D))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(_I32))
+
+
+class ScphdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def load_params(path):
+    """Parse a libconfig file into a parameter dict (scphd_params_load)."""
+    p = Params()
+    err = ctypes.create_string_buffer(512)
+    rc = lib().scphd_params_load(path.encode(), ctypes.byref(p), err, 512)
+    if rc != SCPHD_OK:
+        raise ScphdError(rc, err.value.decode())
+    return p.to_dict()
+
+
+class Filter:
+    """One calibration filter (one stereo rig) resident on the GPU."""
+
+    def __init__(self, params):
+        L = lib()
+        self.params = Params.from_dict(params) if isinstance(params, dict) else params
+        h = _P()
+        rc = L.scphd_create(ctypes.byref(self.params), ctypes.byref(h))
+        if rc != SCPHD_OK:
+            raise ScphdError(rc, "scphd_create failed")
+        self.h = h
+        self.Np = self.params.n_particles
+        self.S = self.params.particles_per_feature
+
+    def close(self):
+        h = getattr(self, "h", None)
+        if h:
+            lib().scphd_destroy(h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != SCPHD_OK:
+            msg = lib().scphd_last_error(self.h)
+            raise ScphdError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def predict(self, dt):
+        self._check(lib().scphd_predict(self.h, dt), "predict")
+
+    def update(self, u, v, fixed):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        if u.shape != v.shape:
+            raise ValueError("u and v differ in length")
+        self._check(lib().scphd_update(self.h, _dp(u), _dp(v), len(u), int(bool(fixed))), "update")
+
+    def update_device(self, u_ptr, v_ptr, n, fixed):
+        self._check(lib().scphd_update_device(self.h, u_ptr, v_ptr, n, int(bool(fixed))), "update_device")
+
+    def resample(self):
+        r = _I32(0)
+        self._check(lib().scphd_resample(self.h, ctypes.byref(r)), "resample")
+        return bool(r.value)
+
+    def check(self):
+        return lib().scphd_check(self.h)
+
+    def sync(self):
+        self._check(lib().scphd_sync(self.h), "sync")
+
+    def counts(self):
+        c = Counts()
+        self._check(lib().scphd_get_counts(self.h, ctypes.byref(c)), "get_counts")
+        return {n: getattr(c, n) for n, _ in Counts._fields_}
+
+    def parents(self):
+        st = np.zeros((self.Np, 12))
+        w = np.zeros(self.Np)
+        self._check(lib().scphd_get_parents(self.h, _dp(st), _dp(w)), "get_parents")
+        return st, w
+
+    def maps(self):
+        self._check(lib().scphd_get_maps(self.h, None, None, None), "get_maps")  # materialise first
+        c = self.counts()
+        gc = np.zeros(self.Np, dtype=np.int32)
+        wg = np.zeros(max(c["n_gaussians"], 1))
+        smp = np.zeros((7, max(c["n_samples"], 1)))
+        self._check(lib().scphd_get_maps(self.h, _ip(gc), _dp(wg), _dp(smp)), "get_maps")
+        return gc, wg[:c["n_gaussians"]], smp[:, :c["n_samples"]]
+
+    def state(self):
+        st, w = self.parents()
+        gc, wg, smp = self.maps()
+        c = self.counts()
+        return dict(states=st, weights=w, gcount=gc, wg=wg, samples=np.ascontiguousarray(smp),
+                    n_predict=c["n_predict"], n_update=c["n_update"], n_resample=c["n_resample"])
+
+    def set_state(self, s):
+        st = np.ascontiguousarray(s["states"], dtype=np.float64)
+        w = np.ascontiguousarray(s["weights"], dtype=np.float64)
+        gc = np.ascontiguousarray(s["gcount"], dtype=np.int32)
+        wg = np.ascontiguousarray(s["wg"], dtype=np.float64)
+        smp = np.ascontiguousarray(s["samples"], dtype=np.float64)
+        n = int(gc.sum())
+        if wg.size == 0:
+            wg = np.zeros(1)
+        if smp.size == 0:
+            smp = np.zeros((7, 1))
+        self._check(lib().scphd_set_state(self.h, _dp(st), _dp(w), _ip(gc), n, _dp(wg), _dp(smp),
+                                          int(s["n_predict"]), int(s["n_update"]), int(s["n_resample"])),
+                    "set_state")
+
+    def update_stats(self):
+        s = UpdateStats()
+        self._check(lib().scphd_get_update_stats(self.h, ctypes.byref(s)), "get_update_stats")
+        return {n: getattr(s, n) for n, _ in UpdateStats._fields_}
+
+    def update_debug(self):
+        M = self.update_stats()["n_meas"]
+        eta = np.zeros(max(self.Np * M, 1))
+        ll = np.zeros(self.Np)
+        nm = np.zeros(self.Np, dtype=np.int32)
+        nn = np.zeros(self.Np, dtype=np.int32)
+        ns = np.zeros(self.Np, dtype=np.int32)
+        self._check(lib().scphd_get_update_debug(self.h, _dp(eta), _dp(ll), _ip(nm), _ip(nn), _ip(ns)),
+                    "get_update_debug")
+        return dict(eta=eta[:self.Np * M].reshape(self.Np, M), loglik=ll, nmerged=nm, nnondetect=nn,
+                    nsurvivors=ns)
+
+    def fit_debug(self):
+        n = self.update_stats()["n_gaussians_in"]
+        out = np.zeros(28 * max(n, 1))
+        self._check(lib().scphd_get_fit_debug(self.h, _dp(out)), "get_fit_debug")
+        return out[:28 * n].reshape(28, n)
+
+    def ancestors(self):
+        a = np.zeros(self.Np, dtype=np.int32)
+        self._check(lib().scphd_get_ancestors(self.h, _ip(a)), "get_ancestors")
+        return a
+
+    def set_timing(self, on):
+        self._check(lib().scphd_set_timing(self.h, int(bool(on))), "set_timing")
+
+    def phase_ms(self):
+        out = np.zeros(6)
+        self._check(lib().scphd_get_phase_ms(self.h, _dp(out)), "get_phase_ms")
+        return out

@@ -1,0 +1,137 @@
+"""Filter parameters and synthetic-scene settings of the named workloads.
+
+The filter keys are those the reference reads from its libconfig file
+(cheesinglee/scphd-camera-calibration src/scphdcameracalibration.cu:827-1005).
+Rig: 640x480 image (u0 = 320, v0 = 240, image = 2u0 x 2v0 as in
+src/disparitymeasurementmodel.cu:159-160), focal length f with the
+reference's sign convention fx = fy = -f (config/calibration.cfg:74-75), so
+that d = -fx/z = f/z > 0 in front of the camera.
+"""
+import math
+
+DEG = math.pi / 180.0
+
+
+def base_params(n_particles, samples, max_gaussians, lam, f=500.0, baseline=0.2, seed=1):
+    """Filter parameters shared by every workload.
+
+    Prior on the right-camera extrinsics: uniform over nominal +- 3 sigma with
+    sigma = (0.02 m, 1 deg); rates start at zero.  Birth terms sit at the
+    measurement with d = f / 4.5 m and wide disparity-rate variances.
+    """
+    sx, sa = 0.02, 1.0 * DEG
+    nominal = [baseline, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0]
+    half = [3 * sx] * 3 + [0.0] * 3 + [3 * sa] * 3 + [0.0] * 3
+    return {
+        "n_particles": n_particles,
+        "particles_per_feature": samples,
+        "max_gaussians": max_gaussians,
+        "storage": 0,
+        "verbosity": 0,
+        "slam": 0,
+        "max_steps": 100000,
+        "save_all_maps": 0,
+        "min_weight": 1e-4,
+        "min_separation": 4.0,
+        "min_neff": 0.75,
+        "ps": 0.95,
+        "prior_a": [n - h for n, h in zip(nominal, half)],
+        "prior_b": [n + h for n, h in zip(nominal, half)],
+        "cam_accel_std": [0.004, 0.004, 0.004, 0.2 * DEG, 0.2 * DEG, 0.2 * DEG],
+        "feat_accel_std": [0.02, 0.02, 0.02],
+        "fx": -f, "fy": -f, "u0": 320.0, "v0": 240.0,
+        "std_u": 1.0, "std_v": 1.0, "pd": 0.95, "lambda": float(lam),
+        "birth_w0": 1e-3,
+        "birth_d0": f / 4.5, "birth_vu0": 0.0, "birth_vv0": 0.0, "birth_vd0": 0.0,
+        "birth_var_d0": (f / 8.0) ** 2, "birth_var_vu0": 25.0, "birth_var_vv0": 9.0, "birth_var_vd0": 1.0,
+        "max_velocity": 5.0, "min_range": 0.5,
+        "seed": seed,
+    }
+
+
+# name -> (filter parameters, scene settings)
+WORKLOADS = {
+    # Plain one-rig, small: 30 landmarks, lambda 10, 256 parents, S = 100,
+    # cap 500 Gaussians per parent, 50 frames.
+    "c0": dict(
+        params=base_params(256, 100, 500, 10.0),
+        scene=dict(n_landmarks=30, lam=10.0, f=500.0, baseline=0.2, n_frames=50, dt=0.5),
+    ),
+    # Plain one-rig, large: 200 landmarks, lambda 50, 4096 parents, S = 200,
+    # cap 2000, 100 frames.
+    "c1": dict(
+        params=base_params(4096, 200, 2000, 50.0),
+        scene=dict(n_landmarks=200, lam=50.0, f=500.0, baseline=0.2, n_frames=100, dt=0.5),
+    ),
+    # Precision-study size: 16384 parents, S = 100, 300 landmarks, lambda 100.
+    "c3": dict(
+        params=base_params(16384, 100, 2000, 100.0),
+        scene=dict(n_landmarks=300, lam=100.0, f=500.0, baseline=0.2, n_frames=200, dt=0.5),
+    ),
+}
+
+
+def workload(name, **overrides):
+    w = WORKLOADS[name]
+    p = dict(w["params"])
+    s = dict(w["scene"])
+    for k, v in overrides.items():
+        if k in p:
+            p[k] = v
+        elif k in s:
+            s[k] = v
+        else:
+            raise KeyError(k)
+    return p, s
+
+
+def to_cfg(p):
+    """Render parameters as a libconfig file in the schema the reference code
+    reads (prior.{cartesian,angular}.{x..vz}.{a,b})."""
+    comp = ["x", "y", "z", "vx", "vy", "vz"]
+    lines = [
+        f"verbosity = {p['verbosity']}",
+        f"save_all_maps = {'true' if p['save_all_maps'] else 'false'}",
+        f"slam = {'true' if p['slam'] else 'false'}",
+        f"max_steps = {p['max_steps']}",
+        f"n_particles = {p['n_particles']}",
+        f"particles_per_feature = {p['particles_per_feature']}",
+        f"max_gaussians = {p['max_gaussians']}",
+        f"min_weight = {p['min_weight']!r}",
+        f"min_separation = {p['min_separation']!r}",
+        f"max_velocity = {p['max_velocity']!r} ;",
+        f"min_range = {p['min_range']!r} ;",
+        f"min_neff = {p['min_neff']!r} ;",
+        f"ps = {p['ps']!r}",
+        f"seed = {p['seed']}",
+        "prior : {",
+    ]
+    for grp, base in (("cartesian", 0), ("angular", 6)):
+        lines.append(f"    {grp} : {{")
+        for i, c in enumerate(comp):
+            lines.append(f"        {c} : {{a = {p['prior_a'][base + i]!r} ; b = {p['prior_b'][base + i]!r}}};")
+        lines.append("    }")
+    lines.append("}")
+    ca = p["cam_accel_std"]
+    lines += [
+        "camera_motion_model : {",
+        f"    cartesian : {{ ax = {ca[0]!r}\n        ay = {ca[1]!r}\n        az = {ca[2]!r} }}",
+        f"    angular : {{ ax = {ca[3]!r}\n        ay = {ca[4]!r}\n        az = {ca[5]!r} }}",
+        "}",
+        "feature_motion_model : {",
+        f"    ax = {p['feat_accel_std'][0]!r}",
+        f"    ay = {p['feat_accel_std'][1]!r}",
+        f"    az = {p['feat_accel_std'][2]!r}",
+        "}",
+        "measurement_model : {",
+    ]
+    for k in ("fx", "fy", "u0", "v0", "std_u", "std_v"):
+        lines.append(f"    {k} = {p[k]!r}")
+    lines.append(f"    lambda = {p['lambda']!r}")
+    lines.append(f"    pd = {p['pd']!r}")
+    lines.append("}")
+    lines.append("birth : {")
+    for k in ("w0", "d0", "vu0", "vv0", "vd0", "var_d0", "var_vu0", "var_vv0", "var_vd0"):
+        lines.append(f"    {k} = {p['birth_' + k]!r} ;")
+    lines.append("}")
+    return "\n".join(lines) + "\n"

@@ -1,0 +1,146 @@
+"""Lock-step parity of the gfx950 filter step against the FP64 oracle.
+
+Every frame starts both sides from the oracle's state (the GPU gets it
+through scphd_set_state), runs predict -> update -> resample with the same
+Philox noise streams, and compares:
+  * bit-exact: survivor counts, merged counts, kept non-detection counts,
+    map sizes per parent, the resampling decision and ancestor indices;
+  * within tolerance: parent states and weights, normalisers eta,
+    parent log-likelihoods, Gaussian weights, sample clouds.
+Tolerances: FP64 storage: 1e-9 relative on weights / normalisers, 1e-7 on
+samples (sample positions go through 1/d); FP32 storage: samples within 4
+float ulps (2.5e-7 relative) plus 1e-6 absolute.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import scene
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def small(storage, n_particles=48, samples=100, seed=7, **kw):
+    p, s = workloads.workload("c0", n_particles=n_particles, particles_per_feature=samples, **kw)
+    p["storage"] = storage
+    p["seed"] = seed
+    return p, s
+
+
+def _close(a, b, rtol, atol, what):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    assert a.shape == b.shape, f"{what}: shape {a.shape} vs {b.shape}"
+    if a.size == 0:
+        return 0.0
+    err = np.abs(a - b)
+    lim = atol + rtol * np.maximum(np.abs(a), np.abs(b))
+    bad = err > lim
+    assert not bad.any(), (f"{what}: {bad.sum()} of {a.size} differ; worst {err.max():.3e} at "
+                           f"{int(np.argmax(err))} ({a.flat[np.argmax(err)]!r} vs {b.flat[np.argmax(err)]!r})")
+    return float((err / np.maximum(np.maximum(np.abs(a), np.abs(b)), 1e-300)).max())
+
+
+def run_lockstep(p, s, frames, warm=0):
+    import scphd_amd
+
+    sc = scene.generate(seed=p["seed"], **s)
+    orc = oracle.Oracle(p, threads=8)
+    gpu = scphd_amd.Filter(p)
+    # same prior draw
+    so, wo = orc.parents()
+    sg, wg = gpu.parents()
+    _close(sg, so, 1e-14, 1e-15, "prior")
+    for k in range(warm):
+        u, v, fixed = scene.frame_measurements(sc, k)
+        orc.predict(s["dt"])
+        orc.update(u, v, fixed)
+        orc.resample()
+    f32 = p["storage"] == 0
+    rs = 1e-6 if f32 else 1e-7
+    report = []
+    for k in range(warm, warm + frames):
+        st = orc.state()
+        gpu.set_state(st)
+        u, v, fixed = scene.frame_measurements(sc, k)
+        orc.predict(s["dt"])
+        gpu.predict(s["dt"])
+        _close(gpu.parents()[0], orc.parents()[0], 1e-13, 1e-13, f"frame {k} predicted parents")
+        orc.update(u, v, fixed)
+        gpu.update(u, v, fixed)
+        do, dg = orc.update_debug(), gpu.update_debug()
+        for key in ("nsurvivors", "nmerged", "nnondetect"):
+            np.testing.assert_array_equal(dg[key], do[key], err_msg=f"frame {k} {key}")
+        _close(dg["eta"], do["eta"], 1e-11, 1e-11, f"frame {k} eta")
+        _close(dg["loglik"], do["loglik"], 1e-11, 1e-9, f"frame {k} loglik")
+        gco, wgo, smo = orc.maps()
+        gcg, wgg, smg = gpu.maps()
+        np.testing.assert_array_equal(gcg, gco, err_msg=f"frame {k} map sizes")
+        _close(wgg, wgo, 1e-9, 1e-15, f"frame {k} gaussian weights")
+        e = _close(smg, smo, rs, 1e-6 if f32 else 1e-9, f"frame {k} samples")
+        _close(gpu.parents()[1], orc.parents()[1], 1e-9, 1e-15, f"frame {k} parent weights")
+        ro, rg = orc.resample(), gpu.resample()
+        assert ro == rg, f"frame {k} resample decision"
+        if ro:
+            np.testing.assert_array_equal(gpu.ancestors(), orc.ancestors()[0], err_msg=f"frame {k} ancestors")
+        report.append((k, int(gco.sum()), e))
+    gpu.close()
+    return report
+
+
+@pytest.mark.parametrize("storage", [1, 0], ids=["fp64", "fp32"])
+def test_lockstep_small(storage):
+    """c0 geometry with 48 parents: births-only first frame, then frames
+    alternating fixed / moving camera with detection, merge, non-detection
+    and resampling terms."""
+    p, s = small(storage)
+    rep = run_lockstep(p, s, frames=6)
+    assert rep[-1][1] > 0
+
+
+def test_lockstep_after_warmup_fp32():
+    """Lock-step frames from a state grown by 8 oracle frames (maps of
+    ~100 Gaussians per parent)."""
+    p, s = small(0, n_particles=32)
+    run_lockstep(p, s, frames=3, warm=8)
+
+
+def test_lockstep_resampling_path():
+    """min_neff = 1.01 forces systematic resampling every frame, so the
+    ancestor indices and the re-indexed maps are compared every frame."""
+    p, s = small(1, n_particles=40, min_neff=1.01)
+    run_lockstep(p, s, frames=4)
+
+
+def test_lockstep_samples_per_gaussian_200():
+    """S = 200 (four samples per lane in the reproject/fit kernel)."""
+    p, s = small(0, n_particles=16, samples=200)
+    run_lockstep(p, s, frames=4)
+
+
+def test_empty_measurement_set():
+    """M = 0: no births, no detection terms; only non-detection terms and
+    the cardinality term of the parent likelihood remain."""
+    import scphd_amd
+
+    p, s = small(1, n_particles=16)
+    sc = scene.generate(seed=p["seed"], **s)
+    orc = oracle.Oracle(p)
+    gpu = scphd_amd.Filter(p)
+    for k in range(3):
+        u, v, fixed = scene.frame_measurements(sc, k)
+        orc.predict(0.5)
+        orc.update(u, v, fixed)
+        orc.resample()
+    gpu.set_state(orc.state())
+    orc.predict(0.5)
+    gpu.predict(0.5)
+    orc.update([], [], False)
+    gpu.update([], [], False)
+    gco, wgo, _ = orc.maps()
+    gcg, wgg, _ = gpu.maps()
+    np.testing.assert_array_equal(gcg, gco)
+    _close(wgg, wgo, 1e-9, 1e-15, "weights")
+    _close(gpu.parents()[1], orc.parents()[1], 1e-9, 1e-15, "parent weights")
+    gpu.close()
