@@ -195,13 +195,16 @@ int scphd_get_fit_debug(scphd_t* f, double* out);
  * array idx_resample, src/scphdcameracalibration.cu:2659). */
 int scphd_get_ancestors(scphd_t* f, int32_t* ancestors);
 
-/* Wall time in milliseconds of the phases of the last update, measured with
- * HIP events on the handle's stream: [0] reproject+fit, [1] normalisers and
- * survivors, [2] merge, [3] posterior sampling, [4] map assembly, [5] total.
- * Requires the handle to have been created with timing enabled (see
- * scphd_set_timing). */
+/* Wall time in milliseconds of the last update, measured with HIP events on
+ * the handle's stream while timing is enabled (scphd_set_timing):
+ *   [0] reproject + fit phase   [1] normalisers and survivors
+ *   [2] merge and parent weights [3] posterior sampling
+ *   [4] map assembly            [5] whole update
+ *   [6] reproject/fit kernel alone   [7] posterior-sampling kernel alone
+ * No counterpart in the reference (its NVTX ranges,
+ * src/scphdcameracalibration.cu:1062-2636, carry no timings). */
 int scphd_set_timing(scphd_t* f, int32_t enabled);
-int scphd_get_phase_ms(scphd_t* f, double* ms6);
+int scphd_get_phase_ms(scphd_t* f, double* ms8);
 
 #ifdef __cplusplus
 }

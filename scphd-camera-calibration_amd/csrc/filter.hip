@@ -136,8 +136,8 @@ struct scphd_filter {
     scphd_update_stats stats{};
     int64_t last_M = 0;
     bool timing = false;
-    hipEvent_t ev[6] = {};
-    double phase_ms[6] = {};
+    hipEvent_t ev[10] = {};  /* 0..5 phase boundaries, 6..7 fit kernel, 8..9 sample kernel */
+    double phase_ms[8] = {};
     int lds_cap = 0;
 };
 
@@ -257,7 +257,9 @@ int update_t(scphd_filter* f, int M, int fixed) {
         fa.fit = f->fit.p;
         fa.keep = f->keep.p;
         fa.nd_base = 0;
+        record(f, 6);
         HIPTRY(launch_fit<T>(s, fa, f->gen[f->cur].planes<T>(), f->gen[nxt].planes<T>()));
+        record(f, 7);
         HIPTRY(launch_precompute(s, f->fit.p, f->gw.p, Ng, P.std_u * P.std_u, P.std_v * P.std_v, f->pre.p,
                                  f->pdok.p));
     }
@@ -389,7 +391,9 @@ int update_t(scphd_filter* f, int M, int fixed) {
     pa.key = f->key;
     pa.ustep = (uint64_t)f->n_update;
     pa.post_base = Ng;
+    record(f, 8);
     HIPTRY(launch_sample<T>(s, pa, f->gen[nxt].planes<T>()));
+    record(f, 9);
     record(f, 4);
 
     /* next map table */
@@ -444,6 +448,12 @@ int update_t(scphd_filter* f, int M, int fixed) {
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, f->ev[0], f->ev[5]);
         f->phase_ms[5] = ms;
+        ms = 0.f;
+        if (Ng > 0) (void)hipEventElapsedTime(&ms, f->ev[6], f->ev[7]);
+        f->phase_ms[6] = ms;
+        ms = 0.f;
+        if (Kt > 0) (void)hipEventElapsedTime(&ms, f->ev[8], f->ev[9]);
+        f->phase_ms[7] = ms;
     }
     return SCPHD_OK;
 }
@@ -794,9 +804,9 @@ int scphd_set_timing(scphd_t* f, int32_t enabled) {
     return SCPHD_OK;
 }
 
-int scphd_get_phase_ms(scphd_t* f, double* ms6) {
-    if (check_handle(f) || !ms6) return SCPHD_ERR_ARG;
-    for (int i = 0; i < 6; ++i) ms6[i] = f->phase_ms[i];
+int scphd_get_phase_ms(scphd_t* f, double* ms8) {
+    if (check_handle(f) || !ms8) return SCPHD_ERR_ARG;
+    for (int i = 0; i < 8; ++i) ms8[i] = f->phase_ms[i];
     return SCPHD_OK;
 }
 

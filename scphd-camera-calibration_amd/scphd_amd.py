@@ -283,6 +283,6 @@ class Filter:
         self._check(lib().scphd_set_timing(self.h, int(bool(on))), "set_timing")
 
     def phase_ms(self):
-        out = np.zeros(6)
+        out = np.zeros(8)
         self._check(lib().scphd_get_phase_ms(self.h, _dp(out)), "get_phase_ms")
         return out
