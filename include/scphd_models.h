@@ -201,16 +201,20 @@ public:
         : fx_(fx), fy_(fy), u0_(u0), v0_(v0), img_width_(2 * u0), img_height_(2 * v0), std_u_(std_u),
           std_v_(std_v), pd_(pd), kappa_(lambda / ((2 * v0) * (2 * v0))) {}
 
-    /* Camera-frame point -> disparity point. */
+    /* Camera-frame point -> disparity point:
+     *   u = u0 - fx x / z, v = v0 - fy y / z, d = -fx / z,
+     *   u' = -fx (x' z - z' x) / z^2, v' = -fy (y' z - z' y) / z^2, d' = fx z' / z^2.
+     * The z^2 quotients multiply by 1 / z^2 (one division instead of three). */
     SCPHD_HD DisparityPoint computeMeasurement(EuclideanPoint p) const {
         const double x = p.x, y = p.y, z = p.z;
+        const double fz = fx_ / z, gz = fy_ / z, iz2 = 1.0 / (z * z);
         DisparityPoint q;
-        q.u = u0_ - fx_ / z * x;
-        q.v = v0_ - fy_ / z * y;
-        q.d = -fx_ / z;
-        q.vu = -fx_ * (p.vx * z - p.vz * x) / (z * z);
-        q.vv = -fy_ * (p.vy * z - p.vz * y) / (z * z);
-        q.vd = fx_ * p.vz / (z * z);
+        q.u = u0_ - fz * x;
+        q.v = v0_ - gz * y;
+        q.d = -fz;
+        q.vu = -fx_ * (p.vx * z - p.vz * x) * iz2;
+        q.vv = -fy_ * (p.vy * z - p.vz * y) * iz2;
+        q.vd = fx_ * p.vz * iz2;
         return q;
     }
 
@@ -223,16 +227,21 @@ public:
         return computeMeasurement(c.worldToCamera(p_world));
     }
 
-    /* Disparity point -> camera-frame point. */
+    /* Disparity point -> camera-frame point:
+     *   x = (u - u0) / d, y = (fx/fy) (v - v0) / d, z = -fx / d,
+     *   x' = (u' d - d' (u - u0)) / d^2, y' = (fx/fy) (v' d - d' (v - v0)) / d^2,
+     *   z' = fx d' / d^2,
+     * with the quotients taken as products by 1/d and 1/d^2. */
     SCPHD_HD EuclideanPoint invertToCamera(DisparityPoint q) const {
         const double u = q.u, v = q.v, d = q.d;
+        const double id = 1.0 / d, id2 = 1.0 / (d * d), r = fx_ / fy_;
         EuclideanPoint c;
-        c.x = (u - u0_) / d;
-        c.y = (fx_ / fy_) * (v - v0_) / d;
-        c.z = -fx_ / d;
-        c.vx = (q.vu * d - q.vd * (u - u0_)) / (d * d);
-        c.vy = (fx_ / fy_) * (q.vv * d - q.vd * (v - v0_)) / (d * d);
-        c.vz = fx_ * q.vd / (d * d);
+        c.x = (u - u0_) * id;
+        c.y = r * (v - v0_) * id;
+        c.z = -fx_ * id;
+        c.vx = (q.vu * d - q.vd * (u - u0_)) * id2;
+        c.vy = r * (q.vv * d - q.vd * (v - v0_)) * id2;
+        c.vz = fx_ * q.vd * id2;
         return c;
     }
 

@@ -30,6 +30,7 @@
  *   D10 the per-(parent, measurement) normaliser is summed in a fixed order
  *       with log1p (see log_normaliser); the reference's reduce_by_key order
  *       is unspecified.
+ *   D11 normalised sample weights as w / sum (normalise_weights).
  *   D9  positive-definiteness: relative Cholesky pivot > 1e-9 instead of
  *       "no NaN in the factor" (isPosDef, src/device_math.cu:800-812); see
  *       is_pos_def6.
@@ -202,6 +203,16 @@ static double log_normaliser(const double* l, int G, double bpc) {
     return mx + std::log1p(T);
 }
 
+/* normalizeWeightsKernel, src/kernels.cu:996-1018: w_i <- exp(log w_i -
+ * log sum w), with log 0 = -FLT_MAX so that an all-zero cloud gets weight 1
+ * everywhere.  D11: evaluated as w_i / sum (equal to the reference's form
+ * to rounding), 0 for w_i = 0, and 1 everywhere when the sum is 0. */
+static void normalise_weights(double* w, int S) {
+    double sum = 0.0;
+    for (int s = 0; s < S; ++s) sum += w[s];
+    for (int s = 0; s < S; ++s) w[s] = sum > 0 ? (w[s] > 0 ? w[s] / sum : 0.0) : 1.0;
+}
+
 /* ------------------------------------------------------------- geometry */
 /* Rotation camera->world R = Rx(theta) Ry(phi) Rz(psi), row-major, from the
  * coefficients of transformCameraToWorld, src/disparitymeasurementmodel.cu:82-144 */
@@ -296,27 +307,29 @@ static inline void draw(uint64_t seed, uint32_t stream, uint64_t step, uint32_t 
     philox4x32_10(ctr, key, out);
 }
 
-static inline double u01_open_low(uint32_t lo, uint32_t hi) {  /* (0, 1] */
-    const uint64_t b = (((uint64_t)hi << 32) | lo) >> 11;
-    return (double)(b + 1) * 0x1.0p-53;
-}
 static inline double u01(uint32_t lo, uint32_t hi) {  /* [0, 1) */
     const uint64_t b = (((uint64_t)hi << 32) | lo) >> 11;
     return (double)b * 0x1.0p-53;
 }
 
-/* n (<= 6) standard normals by Box-Muller, two per Philox block. */
+/* n (<= 8) standard normals by Box-Muller, four per Philox block: the pair
+ * (x0, x1) gives a = (x0 + 1) 2^-32 in (0, 1] and c = x1 2^-32 in [0, 1),
+ * r = sqrt(-2 log a), z = (r cos 2 pi c, r sin 2 pi c); (x2, x3) likewise. */
 static void normals(uint64_t seed, uint32_t stream, uint64_t step, uint32_t parent, uint32_t gauss,
                     uint32_t elem, int n, double* z) {
-    for (int b = 0; 2 * b < n; ++b) {
+    for (int b = 0; 4 * b < n; ++b) {
         uint32_t x[4];
         draw(seed, stream, step, parent, gauss, elem, (uint32_t)b, x);
-        const double a = u01_open_low(x[0], x[1]);
-        const double c = u01(x[2], x[3]);
-        const double r = std::sqrt(-2.0 * std::log(a));
-        const double th = (2.0 * M_PI) * c;
-        z[2 * b] = r * std::cos(th);
-        if (2 * b + 1 < n) z[2 * b + 1] = r * std::sin(th);
+        for (int h = 0; h < 2; ++h) {
+            const int i = 4 * b + 2 * h;
+            if (i >= n) break;
+            const double a = ((double)x[2 * h] + 1.0) * 0x1.0p-32;
+            const double c = (double)x[2 * h + 1] * 0x1.0p-32;
+            const double r = std::sqrt(-2.0 * std::log(a));
+            const double th = (2.0 * M_PI) * c;
+            z[i] = r * std::cos(th);
+            if (i + 1 < n) z[i + 1] = r * std::sin(th);
+        }
     }
 }
 
@@ -603,11 +616,7 @@ static int update(State& st, const double* zu, const double* zv, int M, int fixe
                 w[s] = st.smp[6][i] * pd;
             }
             pdg[k] = pdsum / S;
-            /* normalizeWeightsKernel, src/kernels.cu:996-1018 */
-            double sum = 0.0;
-            for (int s = 0; s < S; ++s) sum += w[s];
-            const double lsum = safe_log(sum);
-            for (int s = 0; s < S; ++s) w[s] = std::exp(safe_log(w[s]) - lsum);
+            normalise_weights(w.data(), S);
             /* fitGaussiansKernel, src/kernels.cu:102-438 */
             Comp& f = pred[k];
             f.w = st.wg[g];
@@ -1046,10 +1055,7 @@ void orc_systematic(const double* w, int N, double u, int32_t* a) { systematic(w
  * cov[36]. */
 void orc_fit(int S, const double* q, const double* w_in, double* mean, double* cov) {
     std::vector<double> w(w_in, w_in + S);
-    double sum = 0.0;
-    for (int s = 0; s < S; ++s) sum += w[s];
-    const double lsum = safe_log(sum);
-    for (int s = 0; s < S; ++s) w[s] = std::exp(safe_log(w[s]) - lsum);
+    normalise_weights(w.data(), S);
     for (int c = 0; c < 6; ++c) {
         double a = 0.0;
         for (int s = 0; s < S; ++s) a += q[s * 6 + c] * w[s];

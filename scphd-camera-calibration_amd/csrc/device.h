@@ -76,20 +76,31 @@ SCPHD_HD inline double uniform_oc(uint32_t lo, uint32_t hi) { /* (0, 1] */
     return (double)(((((uint64_t)hi << 32) | lo) >> 11) + 1) * 0x1.0p-53;
 }
 
-/* N (<= 6) standard normals by the Box-Muller transform, two per block. */
+/* N (<= 8) standard normals by the Box-Muller transform, four per Philox
+ * block: words (x0, x1) give a = (x0 + 1) 2^-32 in (0, 1], c = x1 2^-32 in
+ * [0, 1), r = sqrt(-2 log a), (z0, z1) = r (cos 2 pi c, sin 2 pi c); words
+ * (x2, x3) give (z2, z3).  cos/sin of 2 pi c are evaluated as sincospi(2c),
+ * which needs no argument reduction. */
 template <int N>
-SCPHD_HD inline void normals(NoiseKey key, uint32_t stream, uint64_t step, uint32_t parent, uint32_t gauss,
-                             uint32_t elem, double z[N]) {
+__device__ inline void normals(NoiseKey key, uint32_t stream, uint64_t step, uint32_t parent, uint32_t gauss,
+                               uint32_t elem, double z[N]) {
 #pragma unroll
-    for (int b = 0; 2 * b < N; ++b) {
+    for (int b = 0; 4 * b < N; ++b) {
         uint32_t x[4];
         noise_block(key, stream, step, parent, gauss, elem, (uint32_t)b, x);
-        const double a = uniform_oc(x[0], x[1]);
-        const double c = uniform_co(x[2], x[3]);
-        const double r = sqrt(-2.0 * log(a));
-        const double th = (2.0 * M_PI) * c;
-        z[2 * b] = r * cos(th);
-        if (2 * b + 1 < N) z[2 * b + 1] = r * sin(th);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int i = 4 * b + 2 * h;
+            if (i < N) {
+                const double a = ((double)x[2 * h] + 1.0) * 0x1.0p-32;
+                const double c = (double)x[2 * h + 1] * 0x1.0p-32;
+                const double r = sqrt(-2.0 * log(a));
+                double sn, cs;
+                sincospi(2.0 * c, &sn, &cs);
+                z[i] = r * cs;
+                if (i + 1 < N) z[i + 1] = r * sn;
+            }
+        }
     }
 }
 

@@ -121,6 +121,7 @@ struct scphd_filter {
     DBuf<double> du, dv, eta;
     int capM = 0;
     DBuf<int32_t> nsurv, survoff, nmerged, mergeoff, outcnt, scantmp, sg, sm, dflag;
+    DBuf<int32_t> ngroups, glead, gsta, gend, gmem;
     DBuf<double> sw, smu, mw, mmu, mP, dneff;
     DBuf<char> mscratch;
     int32_t* hbuf = nullptr; /* pinned read-back area */
@@ -343,14 +344,23 @@ int update_t(scphd_filter* f, int M, int fixed) {
     ma.mmu = f->mmu.p;
     ma.mP = f->mP.p;
     ma.nmerged = f->nmerged.p;
+    HIPTRY(f->ngroups.ensure(Np + 1));
+    HIPTRY(f->glead.ensure(Stot + 1));
+    HIPTRY(f->gsta.ensure(Stot + 1));
+    HIPTRY(f->gend.ensure(Stot + 1));
+    HIPTRY(f->gmem.ensure(Stot + 1));
+    ma.ngroups = f->ngroups.p;
+    ma.glead = f->glead.p;
+    ma.gsta = f->gsta.p;
+    ma.gend = f->gend.p;
+    ma.gmem = f->gmem.p;
+    ma.Stot = Stot;
     ma.big = 0;
-    HIPTRY(launch_merge(s, ma, merge_lds_bytes(std::min(std::max(maxU, 1), f->lds_cap))));
     if (maxU > f->lds_cap) {
         HIPTRY(f->mscratch.ensure(merge_scratch_bytes(Stot, Np)));
         ma.scratch = f->mscratch.p;
-        ma.big = 1;
-        HIPTRY(launch_merge(s, ma, 0));
     }
+    HIPTRY(launch_merge(s, ma, merge_lds_bytes(std::min(std::max(maxU, 1), f->lds_cap))));
     if (!fixed) HIPTRY(launch_parent_weights(s, f->pw.p, f->loglik.p, Np));
     record(f, 3);
 
@@ -528,7 +538,8 @@ void scphd_destroy(scphd_t* f) {
     for (auto* b : dbs) b->release();
     DBuf<int32_t>* ibs[] = {&f->anc, &f->newcnt, &f->goff, &f->gparent, &f->gblock, &f->goff2, &f->gparent2,
                             &f->gblock2, &f->pdok, &f->keep, &f->keepscan, &f->ndlist, &f->nsurv, &f->survoff,
-                            &f->nmerged, &f->mergeoff, &f->outcnt, &f->scantmp, &f->sg, &f->sm, &f->dflag};
+                            &f->nmerged, &f->mergeoff, &f->outcnt, &f->scantmp, &f->sg, &f->sm, &f->dflag,
+                            &f->ngroups, &f->glead, &f->gsta, &f->gend, &f->gmem};
     for (auto* b : ibs) b->release();
     f->mscratch.release();
     f->gen[0].release();

@@ -145,8 +145,17 @@ struct MergeArgs {
     double* mw;        /* merged outputs at survoff[p] + k */
     double* mmu;       /* 6 planes, stride scap */
     double* mP;        /* 21 planes, stride scap */
-    int32_t* nmerged;  /* Np */
+    int32_t* nmerged;  /* Np: merged count */
+    int32_t* ngroups;  /* Np: groups formed by the walk, -1 when the round loop is needed */
+    int32_t* glead;    /* per survivor slot: group leader (survivor index) */
+    int32_t* gsta;     /* per survivor slot: group member range [gsta, gend) in gmem */
+    int32_t* gend;
+    int32_t* gmem;     /* per survivor slot: group members (survivor indices) */
+    int64_t Stot;
 };
+/* Greedy merge of every parent's survivors: groups (sort, neighbour lists,
+ * walk), moments per group, and the round loop for parents whose
+ * neighbour lists overflow the workspace. */
 hipError_t launch_merge(hipStream_t s, const MergeArgs& a, int lds_bytes);
 int merge_lds_bytes(int cap);
 int64_t merge_scratch_bytes(int64_t Stot, int Np);

@@ -18,6 +18,32 @@ static constexpr int NT = 256; /* threads per workgroup for per-parent kernels *
 static constexpr int WPB = 4;  /* wavefronts (Gaussians) per workgroup for per-Gaussian kernels */
 
 /* ------------------------------------------------------------ reductions */
+/* DPP lane moves on the two 32-bit halves of a double (full EXEC assumed). */
+template <int CTRL>
+__device__ inline double dpp_f64(double x) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ inline double readlane_f64(double x, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
+    return __hiloint2double(hi, lo);
+}
+
+/* Sum over a full wavefront: four DPP steps inside each 16-lane row (swap
+ * neighbours, swap pairs, half-row mirror, row mirror) leave every lane
+ * with its row's sum; the four row sums are read into scalar registers and
+ * added.  Every lane gets the result.  All 64 lanes must be active. */
+__device__ inline double wave_sum_dpp(double x) {
+    x += dpp_f64<0xB1>(x);  /* quad_perm [1,0,3,2] */
+    x += dpp_f64<0x4E>(x);  /* quad_perm [2,3,0,1] */
+    x += dpp_f64<0x141>(x); /* row_half_mirror */
+    x += dpp_f64<0x140>(x); /* row_mirror */
+    return (readlane_f64(x, 0) + readlane_f64(x, 16)) + (readlane_f64(x, 32) + readlane_f64(x, 48));
+}
+
 __device__ inline double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -209,12 +235,12 @@ __global__ __launch_bounds__(256) void k_fit(FitArgs a, Planes<T> src, Planes<T>
     const CameraPose pose = load_pose(a.pose, p);
     const int S = a.S;
 
-    double q[NS][6], pdv[NS], wv[NS];
+    double q[NS][6], wv[NS];
+    uint32_t inmask = 0; /* bit i: sample lane + 64 i is inside the field of view */
     double spd = 0.0, sw = 0.0;
 #pragma unroll
     for (int i = 0; i < NS; ++i) {
         const int s = lane + 64 * i;
-        pdv[i] = 0.0;
         wv[i] = 0.0;
 #pragma unroll
         for (int c = 0; c < 6; ++c) q[i][c] = 0.0;
@@ -226,25 +252,27 @@ __global__ __launch_bounds__(256) void k_fit(FitArgs a, Planes<T> src, Planes<T>
             if (a.pending) x = predict_sample(x, a.feat, a.dt, a.key, a.pstep, (uint32_t)p, (uint32_t)k, (uint32_t)s);
             const DisparityPoint d = a.meas.computeMeasurement(x, pose);
             q[i][0] = d.u; q[i][1] = d.v; q[i][2] = d.d; q[i][3] = d.vu; q[i][4] = d.vv; q[i][5] = d.vd;
-            const double pd = a.meas.inRange(d) ? a.meas.pd() : 0.0;
-            pdv[i] = pd;
+            const bool in = a.meas.inRange(d);
+            const double pd = in ? a.meas.pd() : 0.0;
+            inmask |= in ? (1u << i) : 0u;
             wv[i] = w * pd;
             spd += pd;
             sw += wv[i];
         }
     }
-    spd = wave_sum(spd);
-    sw = wave_sum(sw);
+    spd = wave_sum_dpp(spd);
+    sw = wave_sum_dpp(sw);
     const double pdg = spd / S;
-    const double lsum = safe_log(sw);
 
-    /* normalised weights, weighted mean, W, sum of squares */
+    /* normalised weights w / sum (1 everywhere for an all-zero cloud),
+     * weighted mean, W, sum of squares */
+    const double rsw = sw > 0 ? 1.0 / sw : 0.0;
     double m[6] = {0, 0, 0, 0, 0, 0}, W = 0.0, W2 = 0.0;
 #pragma unroll
     for (int i = 0; i < NS; ++i) {
         const int s = lane + 64 * i;
         double wn = 0.0;
-        if (s < S) wn = exp(safe_log(wv[i]) - lsum);
+        if (s < S) wn = sw > 0 ? wv[i] * rsw : 1.0;
         wv[i] = wn;
 #pragma unroll
         for (int c = 0; c < 6; ++c) m[c] += q[i][c] * wn;
@@ -252,9 +280,9 @@ __global__ __launch_bounds__(256) void k_fit(FitArgs a, Planes<T> src, Planes<T>
         W2 += wn * wn;
     }
 #pragma unroll
-    for (int c = 0; c < 6; ++c) m[c] = wave_sum(m[c]);
-    W = wave_sum(W);
-    W2 = wave_sum(W2);
+    for (int c = 0; c < 6; ++c) m[c] = wave_sum_dpp(m[c]);
+    W = wave_sum_dpp(W);
+    W2 = wave_sum_dpp(W2);
     const double cn = W / (W * W + W2);
 
     /* centred second moments */
@@ -272,7 +300,7 @@ __global__ __launch_bounds__(256) void k_fit(FitArgs a, Planes<T> src, Planes<T>
             for (int c = 0; c <= r; ++c) P[pidx(r, c)] += dv[r] * dv[c] * wv[i];
     }
 #pragma unroll
-    for (int e = 0; e < 21; ++e) P[e] = wave_sum(P[e]);
+    for (int e = 0; e < 21; ++e) P[e] = wave_sum_dpp(P[e]);
 
     /* lanes 0..27 write one output plane each */
     {
@@ -302,7 +330,7 @@ __global__ __launch_bounds__(256) void k_fit(FitArgs a, Planes<T> src, Planes<T>
                 const int64_t di = d0 + s;
                 dst.p[0][di] = (T)x.x; dst.p[1][di] = (T)x.y; dst.p[2][di] = (T)x.z;
                 dst.p[3][di] = (T)x.vx; dst.p[4][di] = (T)x.vy; dst.p[5][di] = (T)x.vz;
-                dst.p[6][di] = (T)(1.0 - pdv[i]);
+                dst.p[6][di] = (T)(1.0 - (((inmask >> i) & 1u) ? a.meas.pd() : 0.0));
             }
         }
     }
@@ -549,9 +577,13 @@ __global__ __launch_bounds__(NT) void k_survivors(SurvArgs a) {
 }
 
 /* -------------------------------------------------------------- merging */
-/* Workspace of one parent's merge, U slots (Up = next power of two >= U):
- *   double key_w[Up]; int ord[Up]; double pf_u[U]; float pf_su[U];
- *   uint8 flag[U]; uint8 memb[U]; int mlist[U]   (8-byte aligned pieces) */
+/* Workspace of one parent's merge, U survivor slots (Up = next power of two):
+ *   key_w double[Up]  sort keys;            ord  int[Up]   (weight desc, index asc) order
+ *   pf_u  double[U]   mean u coordinate;    pf_su float[U] its variance
+ *   flag  uint8[U]    merged;               selfm uint8[U] member of its own group
+ *   nbo   int[U+1]    neighbour offsets;    mem  int[U]    group members
+ *   glead int[U]      group leaders;        gsta int[U+1]  group starts in mem
+ *   nbr   int[ecap]   neighbour lists, ecap = 4 U + 64                        */
 __host__ __device__ inline int next_pow2(int x) {
     int v = 1;
     while (v < x) v <<= 1;
@@ -559,22 +591,27 @@ __host__ __device__ inline int next_pow2(int x) {
 }
 
 struct MergeLayout {
-    int64_t ord, pf_u, pf_su, flag, memb, mlist, bytes;
+    int64_t ord, pf_u, pf_su, flag, selfm, nbo, mem, glead, gsta, nbr, ecap, bytes;
     __host__ __device__ explicit MergeLayout(int U) {
         const int64_t Up = next_pow2(U > 0 ? U : 1);
         ord = Up * 8;
         pf_u = (ord + Up * 4 + 7) & ~(int64_t)7;
         pf_su = pf_u + (int64_t)U * 8;
         flag = pf_su + (int64_t)U * 4;
-        memb = flag + U;
-        mlist = (memb + U + 7) & ~(int64_t)7;
-        bytes = mlist + (int64_t)U * 4;
+        selfm = flag + U;
+        nbo = (selfm + U + 7) & ~(int64_t)7;
+        mem = nbo + ((int64_t)U + 1) * 4;
+        glead = mem + (int64_t)U * 4;
+        gsta = glead + (int64_t)U * 4;
+        nbr = gsta + ((int64_t)U + 1) * 4;
+        ecap = 4 * (int64_t)U + 64;
+        bytes = nbr + ecap * 4;
     }
 };
 
-/* global scratch for parents that do not fit in LDS: 48 bytes per survivor
- * slot plus 128 per parent bounds MergeLayout(U).bytes since Up < 2U */
-static constexpr int64_t MERGE_WS_PER_SLOT = 48, MERGE_WS_PER_PARENT = 128;
+/* global scratch for parents that do not fit in LDS: MergeLayout(U).bytes
+ * < 66 U + 300 since Up < 2 U */
+static constexpr int64_t MERGE_WS_PER_SLOT = 72, MERGE_WS_PER_PARENT = 512;
 
 int merge_lds_bytes(int cap) { return (int)MergeLayout(cap).bytes; }
 
@@ -593,19 +630,123 @@ __device__ inline void survivor_P(const MergeArgs& a, int64_t si, double* P) {
     }
 }
 
-/* Greedy Gaussian-mixture merge of one parent's survivors
- * (phdUpdateMergeKernel, src/kernels.cu:662-874): repeatedly take the
- * heaviest unmerged term (ties: lowest pack index), merge every unmerged
- * term within squared Mahalanobis distance min_separation of it
- * (Sigma = P_a + P_b, computeMahalDist src/device_math.cu:374-453) by
- * moment matching, emit the merge if PD else the leader.  The survivors are
- * sorted once by (weight desc, index asc) so the leader is the next
- * unmerged entry in that order; a conservative one-dimensional bound on the
- * u coordinate skips the exact 6x6 test for far-away terms. */
-__global__ __launch_bounds__(NT) void k_merge(MergeArgs a) {
+__device__ inline void survivor_mu(const MergeArgs& a, int64_t si, double* mu) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) mu[c] = a.smu[c * a.scap + si];
+}
+
+/* Merge membership of survivors i and j: squared Mahalanobis distance with
+ * Sigma = P_i + P_j below min_separation (computeMahalDist,
+ * src/device_math.cu:374-453; the computed value is symmetric in i and j
+ * to the last bit).  A one-dimensional bound skips the 6x6 evaluation:
+ * d^2 >= du^2 / Sigma_uu, rejected only beyond a 0.1 % margin. */
+__device__ inline bool merge_pair(const MergeArgs& a, int64_t sbase, const double* pf_u, const float* pf_su, int i,
+                                  int j, double thr) {
+    const double du = pf_u[j] - pf_u[i];
+    const double su = (double)pf_su[j] + (double)pf_su[i];
+    if (du * du > 1.001 * thr * su) return false;
+    double mi[6], mj[6], Pi[21], Pj[21];
+    survivor_mu(a, sbase + i, mi);
+    survivor_mu(a, sbase + j, mj);
+    survivor_P(a, sbase + i, Pi);
+    survivor_P(a, sbase + j, Pj);
+    return mahal6(mi, Pi, mj, Pj) < thr;
+}
+
+/* Workspace pointers of one parent's merge. */
+struct MergeWs {
+    double* key_w;
+    int* ord;
+    double* pf_u;
+    float* pf_su;
+    uint8_t* flag;
+    uint8_t* selfm;
+    int* nbo;
+    int* mem;
+    int* nbr;
+    int64_t ecap;
+    __device__ MergeWs(char* ws, int U) {
+        const MergeLayout ly(U);
+        key_w = (double*)ws;
+        ord = (int*)(ws + ly.ord);
+        pf_u = (double*)(ws + ly.pf_u);
+        pf_su = (float*)(ws + ly.pf_su);
+        flag = (uint8_t*)(ws + ly.flag);
+        selfm = (uint8_t*)(ws + ly.selfm);
+        nbo = (int*)(ws + ly.nbo);
+        mem = (int*)(ws + ly.mem);
+        nbr = (int*)(ws + ly.nbr);
+        ecap = ly.ecap;
+    }
+};
+
+/* Load one parent's survivors into the workspace and sort them by
+ * (weight desc, index asc), the order in which the reference's argmax
+ * loop picks leaders (ties: lowest pack index). */
+__device__ inline void merge_load_sort(const MergeArgs& a, int64_t sbase, int U, MergeWs& w) {
+    const int t = threadIdx.x;
+    const int Up = next_pow2(U);
+    for (int i = t; i < Up; i += NT) {
+        if (i < U) {
+            w.key_w[i] = a.sw[sbase + i];
+            w.ord[i] = i;
+            w.pf_u[i] = a.smu[0 * a.scap + sbase + i];
+            const int g = a.sg[sbase + i];
+            w.pf_su[i] = (float)(g >= 0 ? a.pre[(int64_t)(PP_PN + pidx(0, 0)) * a.Ng + g] : a.birth.P[pidx(0, 0)]);
+            w.flag[i] = 0;
+        } else {
+            w.key_w[i] = -INFINITY;
+            w.ord[i] = 0x7fffffff;
+        }
+    }
+    __syncthreads();
+    for (int k = 2; k <= Up; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = t; i < Up; i += NT) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const double wi = w.key_w[i], wl = w.key_w[l];
+                    const int ii = w.ord[i], il = w.ord[l];
+                    const bool l_before_i = (wl > wi) || (wl == wi && il < ii);
+                    const bool asc = (i & k) == 0;
+                    if (asc == l_before_i) {
+                        w.key_w[i] = wl;
+                        w.key_w[l] = wi;
+                        w.ord[i] = il;
+                        w.ord[l] = ii;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__device__ inline bool merge_self(const MergeArgs& a, int64_t si, double thr) {
+    double m[6], P[21];
+    survivor_mu(a, si, m);
+    survivor_P(a, si, P);
+    return mahal6(m, P, m, P) < thr;
+}
+
+/* Greedy Gaussian-mixture merge (phdUpdateMergeKernel, src/kernels.cu:662-874):
+ * repeatedly take the heaviest unmerged term (ties: lowest pack index),
+ * merge every unmerged term within squared Mahalanobis distance
+ * min_separation of it by moment matching, emit the merge if PD else the
+ * leader, stop when nothing is left or a group's weight is zero.
+ *
+ * Evaluated in phases instead of the reference's barrier-bound rounds.
+ * k_merge_groups (one workgroup per parent): sort; all pairs' membership
+ * into neighbour lists; one lane walks the sorted order forming each
+ * leader's group from its unmerged neighbours -- the same groups as the
+ * round loop, since membership depends only on the pair.  k_merge_moments
+ * (one thread per group): moments with members summed in ascending index
+ * order.  k_merge_rounds: the round loop, for parents whose neighbour
+ * lists overflow the workspace. */
+__global__ __launch_bounds__(NT) void k_merge_groups(MergeArgs a) {
     extern __shared__ char lds[];
-    __shared__ int s_lead, s_pos, s_nm;
-    __shared__ double s_W, s_lw, s_lmu[6], s_lP[21], s_mu[6], s_P[21];
+    __shared__ int s_tot;
+    __shared__ int sh[8];
     const int p = blockIdx.x;
     const int t = threadIdx.x;
     const int64_t sbase = a.survoff[p];
@@ -613,78 +754,188 @@ __global__ __launch_bounds__(NT) void k_merge(MergeArgs a) {
     const bool big = U > a.lds_cap;
     if ((a.big != 0) != big) return; /* the other launch handles this parent (uniform) */
     if (U == 0) {
-        if (t == 0) a.nmerged[p] = 0;
+        if (t == 0) {
+            a.ngroups[p] = 0;
+            a.nmerged[p] = 0;
+        }
         return;
     }
     char* ws = big ? (a.scratch + sbase * MERGE_WS_PER_SLOT + (int64_t)p * MERGE_WS_PER_PARENT) : lds;
-    const int Up = next_pow2(U);
-    const MergeLayout ly(U);
-    double* key_w = (double*)ws;
-    int* ord = (int*)(ws + ly.ord);
-    double* pf_u = (double*)(ws + ly.pf_u);
-    float* pf_su = (float*)(ws + ly.pf_su);
-    uint8_t* flag = (uint8_t*)(ws + ly.flag);
-    uint8_t* memb = (uint8_t*)(ws + ly.memb);
-    int* mlist = (int*)(ws + ly.mlist);
+    MergeWs w(ws, U);
     const double thr = a.min_separation;
+    merge_load_sort(a, sbase, U, w);
 
-    for (int i = t; i < Up; i += NT) {
-        if (i < U) {
-            key_w[i] = a.sw[sbase + i];
-            ord[i] = i;
-            pf_u[i] = a.smu[0 * a.scap + sbase + i];
-            const int g = a.sg[sbase + i];
-            pf_su[i] = (float)(g >= 0 ? a.pre[(int64_t)(PP_PN + pidx(0, 0)) * a.Ng + g] : a.birth.P[pidx(0, 0)]);
-            flag[i] = 0;
-        } else {
-            key_w[i] = -INFINITY;
-            ord[i] = 0x7fffffff;
-        }
+    /* neighbour counts, rows [r0, r1) per thread */
+    const int rc = (U + NT - 1) / NT;
+    const int r0 = min(U, t * rc), r1 = min(U, r0 + rc);
+    int mine = 0;
+    for (int i = r0; i < r1; ++i) {
+        int ci = 0;
+        for (int j = 0; j < U; ++j)
+            if (j != i && merge_pair(a, sbase, w.pf_u, w.pf_su, i, j, thr)) ++ci;
+        w.selfm[i] = merge_self(a, sbase + i, thr) ? 1 : 0;
+        w.nbo[i] = ci;
+        mine += ci;
+    }
+    int tot;
+    int base = block_exscan(mine, sh, &tot);
+    for (int i = r0; i < r1; ++i) {
+        const int ci = w.nbo[i];
+        w.nbo[i] = base;
+        base += ci;
+    }
+    if (t == 0) {
+        w.nbo[U] = tot;
+        s_tot = tot;
     }
     __syncthreads();
-    /* bitonic sort, "before" = (w desc, index asc) */
-    for (int k = 2; k <= Up; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = t; i < Up; i += NT) {
-                const int l = i ^ j;
-                if (l > i) {
-                    const double wi = key_w[i], wl = key_w[l];
-                    const int ii = ord[i], il = ord[l];
-                    const bool l_before_i = (wl > wi) || (wl == wi && il < ii);
-                    const bool asc = (i & k) == 0;
-                    if (asc == l_before_i) {
-                        key_w[i] = wl; key_w[l] = wi;
-                        ord[i] = il; ord[l] = ii;
-                    }
-                }
-            }
-            __syncthreads();
+    if (s_tot > w.ecap) {
+        if (t == 0) {
+            a.ngroups[p] = -1;
+            a.nmerged[p] = 0;
         }
+        return;
     }
+    for (int i = r0; i < r1; ++i) {
+        int o = w.nbo[i];
+        for (int j = 0; j < U; ++j)
+            if (j != i && merge_pair(a, sbase, w.pf_u, w.pf_su, i, j, thr)) w.nbr[o++] = j;
+    }
+    __syncthreads();
+    if (t == 0) {
+        int ng = 0, mp = 0, r = 0;
+        while (r < U) {
+            const int L = w.ord[r];
+            if (w.flag[L]) {
+                ++r;
+                continue;
+            }
+            const int st = mp;
+            const bool sm = w.selfm[L] != 0;
+            bool placed = !sm;
+            for (int e = w.nbo[L]; e < w.nbo[L + 1]; ++e) {
+                const int j = w.nbr[e];
+                if (!placed && j > L) {
+                    w.mem[mp++] = L;
+                    placed = true;
+                }
+                if (!w.flag[j]) w.mem[mp++] = j;
+            }
+            if (!placed) w.mem[mp++] = L;
+            if (mp == st) break; /* empty group: zero weight, the reference stops */
+            for (int e = st; e < mp; ++e) w.flag[w.mem[e]] = 1;
+            a.glead[sbase + ng] = L;
+            a.gsta[sbase + ng] = st;
+            a.gend[sbase + ng] = mp;
+            ++ng;
+            if (sm) ++r; /* a leader outside its group is taken again, finds nothing, stops */
+        }
+        a.ngroups[p] = ng;
+        a.nmerged[p] = ng;
+    }
+    __syncthreads();
+    for (int i = t; i < U; i += NT) a.gmem[sbase + i] = w.mem[i];
+}
+
+__global__ __launch_bounds__(128) void k_merge_moments(MergeArgs a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.Stot) return;
+    const int p = upper_parent(a.survoff, a.Np, i);
+    const int64_t sbase = a.survoff[p];
+    const int gi = (int)(i - sbase);
+    if (gi >= a.ngroups[p]) return; /* also every slot of a parent left to the round loop */
+    const int st = a.gsta[sbase + gi], en = a.gend[sbase + gi];
+    double W = 0.0;
+    for (int e = st; e < en; ++e) W += a.sw[sbase + a.gmem[sbase + e]];
+    if (W == 0) atomicMin(&a.nmerged[p], gi);
+    double mu[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) mu[c] = 0.0;
+    for (int e = st; e < en; ++e) {
+        const int64_t si = sbase + a.gmem[sbase + e];
+        const double w = a.sw[si];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) mu[c] += w * a.smu[c * a.scap + si];
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) mu[c] = mu[c] / W;
+    double P[21];
+#pragma unroll
+    for (int q = 0; q < 21; ++q) P[q] = 0.0;
+    for (int e = st; e < en; ++e) {
+        const int64_t si = sbase + a.gmem[sbase + e];
+        const double w = a.sw[si];
+        const int g = a.sg[si];
+        double dl[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) dl[c] = mu[c] - a.smu[c * a.scap + si];
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+#pragma unroll
+            for (int c = 0; c <= r; ++c) {
+                const double Pe = g >= 0 ? a.pre[(int64_t)(PP_PN + pidx(r, c)) * a.Ng + g] : a.birth.P[pidx(r, c)];
+                P[pidx(r, c)] += w * (Pe + dl[c] * dl[r]);
+            }
+    }
+#pragma unroll
+    for (int q = 0; q < 21; ++q) P[q] = P[q] / W;
+    const int64_t o = i;
+    if (is_pd6(P)) {
+        a.mw[o] = W;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) a.mmu[c * a.scap + o] = mu[c];
+#pragma unroll
+        for (int q = 0; q < 21; ++q) a.mP[q * a.scap + o] = P[q];
+    } else {
+        const int64_t li = sbase + a.glead[sbase + gi];
+        const int g = a.sg[li];
+        a.mw[o] = a.sw[li];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) a.mmu[c * a.scap + o] = a.smu[c * a.scap + li];
+#pragma unroll
+        for (int q = 0; q < 21; ++q)
+            a.mP[q * a.scap + o] = g >= 0 ? a.pre[(int64_t)(PP_PN + q) * a.Ng + g] : a.birth.P[q];
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_merge_rounds(MergeArgs a) {
+    extern __shared__ char lds[];
+    __shared__ int s_lead, s_pos, s_nm;
+    __shared__ double s_W, s_lw, s_lmu[6], s_lP[21], s_mu[6], s_P[21];
+    const int p = blockIdx.x;
+    const int t = threadIdx.x;
+    if (a.ngroups[p] != -1) return; /* uniform */
+    const int64_t sbase = a.survoff[p];
+    const int U = a.survoff[p + 1] - a.survoff[p];
+    const bool big = U > a.lds_cap;
+    if ((a.big != 0) != big) return;
+    char* ws = big ? (a.scratch + sbase * MERGE_WS_PER_SLOT + (int64_t)p * MERGE_WS_PER_PARENT) : lds;
+    MergeWs w(ws, U);
+    const double thr = a.min_separation;
+    merge_load_sort(a, sbase, U, w);
+    for (int i = t; i < U; i += NT) w.selfm[i] = merge_self(a, sbase + i, thr) ? 1 : 0;
     if (t == 0) s_pos = 0;
     __syncthreads();
-
     const int lane = t & 63, wave = t >> 6;
-    int K = 0;
     const int64_t mbase = sbase;
+    int K = 0;
+    uint8_t* memb = (uint8_t*)w.nbr; /* round membership bits; the neighbour-list area is free here */
     for (;;) {
         if (wave == 0) {
             int found = -1;
-            for (int r0 = s_pos; r0 < U; r0 += 64) {
-                const int r = r0 + lane;
-                const bool c = r < U && !flag[ord[r]];
+            for (int r0w = s_pos; r0w < U; r0w += 64) {
+                const int r = r0w + lane;
+                const bool c = r < U && !w.flag[w.ord[r]];
                 const uint64_t b = __ballot(c);
                 if (b) {
-                    found = r0 + __ffsll((unsigned long long)b) - 1;
+                    found = r0w + __ffsll((unsigned long long)b) - 1;
                     break;
                 }
             }
             if (lane == 0) {
-                /* the next search restarts at the leader's rank: a leader
-                 * that is not its own member stays selectable, and the
-                 * following round then finds no members and stops, as the
-                 * reference's argmax loop does */
-                s_lead = found >= 0 ? ord[found] : -1;
+                /* restart at the leader's rank: a leader outside its own
+                 * group stays selectable and the next round stops */
+                s_lead = found >= 0 ? w.ord[found] : -1;
                 s_pos = found >= 0 ? found : U;
             }
         }
@@ -698,33 +949,20 @@ __global__ __launch_bounds__(NT) void k_merge(MergeArgs a) {
             s_lP[t] = g >= 0 ? a.pre[(int64_t)(PP_PN + t) * a.Ng + g] : a.birth.P[t];
         }
         __syncthreads();
-        const double lu = pf_u[lead];
-        const double lsu = pf_su[lead];
         for (int i = t; i < U; i += NT) {
             uint8_t mb = 0;
-            if (!flag[i]) {
-                const double du = pf_u[i] - lu;
-                const double su = (double)pf_su[i] + lsu;
-                /* d^2 >= du^2 / Sigma_uu: reject only with a margin */
-                if (!(du * du > 1.001 * thr * su)) {
-                    double mu[6], P[21];
-#pragma unroll
-                    for (int c = 0; c < 6; ++c) mu[c] = a.smu[c * a.scap + sbase + i];
-                    survivor_P(a, sbase + i, P);
-                    const double d = mahal6(s_lmu, s_lP, mu, P);
-                    mb = d < thr ? 1 : 0;
-                }
-            }
+            if (!w.flag[i])
+                mb = (i == lead ? w.selfm[i] != 0 : merge_pair(a, sbase, w.pf_u, w.pf_su, lead, i, thr)) ? 1 : 0;
             memb[i] = mb;
         }
         __syncthreads();
         if (wave == 0) {
             int nm = 0;
-            for (int r0 = 0; r0 < U; r0 += 64) {
-                const int i = r0 + lane;
+            for (int r0w = 0; r0w < U; r0w += 64) {
+                const int i = r0w + lane;
                 const bool mbit = i < U && memb[i];
                 const uint64_t b = __ballot(mbit);
-                if (mbit) mlist[nm + __popcll(b & ((1ull << lane) - 1ull))] = i;
+                if (mbit) w.mem[nm + __popcll(b & ((1ull << lane) - 1ull))] = i;
                 nm += __popcll(b);
             }
             if (lane == 0) s_nm = nm;
@@ -733,7 +971,7 @@ __global__ __launch_bounds__(NT) void k_merge(MergeArgs a) {
         const int nm = s_nm;
         if (t == 0) {
             double W = 0.0;
-            for (int j = 0; j < nm; ++j) W += a.sw[sbase + mlist[j]];
+            for (int j = 0; j < nm; ++j) W += a.sw[sbase + w.mem[j]];
             s_W = W;
         }
         __syncthreads();
@@ -742,7 +980,7 @@ __global__ __launch_bounds__(NT) void k_merge(MergeArgs a) {
         if (t < 6) {
             double acc = 0.0;
             for (int j = 0; j < nm; ++j) {
-                const int64_t si = sbase + mlist[j];
+                const int64_t si = sbase + w.mem[j];
                 acc += a.sw[si] * a.smu[t * a.scap + si];
             }
             s_mu[t] = acc / W;
@@ -754,7 +992,7 @@ __global__ __launch_bounds__(NT) void k_merge(MergeArgs a) {
             const int c = t - r * (r + 1) / 2;
             double acc = 0.0;
             for (int j = 0; j < nm; ++j) {
-                const int64_t si = sbase + mlist[j];
+                const int64_t si = sbase + w.mem[j];
                 const int g = a.sg[si];
                 const double Pe = g >= 0 ? a.pre[(int64_t)(PP_PN + t) * a.Ng + g] : a.birth.P[t];
                 const double dc = s_mu[c] - a.smu[c * a.scap + si];
@@ -771,7 +1009,7 @@ __global__ __launch_bounds__(NT) void k_merge(MergeArgs a) {
             for (int c = 0; c < 6; ++c) a.mmu[c * a.scap + o] = ok ? s_mu[c] : s_lmu[c];
             for (int e = 0; e < 21; ++e) a.mP[e * a.scap + o] = ok ? s_P[e] : s_lP[e];
         }
-        for (int j = t; j < nm; j += NT) flag[mlist[j]] = 1;
+        for (int j = t; j < nm; j += NT) w.flag[w.mem[j]] = 1;
         ++K;
         __syncthreads();
     }
@@ -1149,7 +1387,20 @@ hipError_t launch_survivors(hipStream_t s, const SurvArgs& a) {
 }
 
 hipError_t launch_merge(hipStream_t s, const MergeArgs& a, int lds_bytes) {
-    k_merge<<<a.Np, NT, a.big ? 0 : lds_bytes, s>>>(a);
+    MergeArgs b = a;
+    b.big = 0;
+    k_merge_groups<<<a.Np, NT, lds_bytes, s>>>(b);
+    if (a.scratch) {
+        b.big = 1;
+        k_merge_groups<<<a.Np, NT, 0, s>>>(b);
+    }
+    if (a.Stot > 0) k_merge_moments<<<nblocks(a.Stot, 128), 128, 0, s>>>(a);
+    b.big = 0;
+    k_merge_rounds<<<a.Np, NT, lds_bytes, s>>>(b);
+    if (a.scratch) {
+        b.big = 1;
+        k_merge_rounds<<<a.Np, NT, 0, s>>>(b);
+    }
     LAUNCH_CHECK();
 }
 

@@ -1,0 +1,116 @@
+"""The C ABI (include/scphd.h): exports, struct layouts, configuration
+parsing and argument checking -- everything that runs without a GPU."""
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+import scphd_amd
+import workloads
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, "include", "scphd.h")
+GOLDEN = os.path.join(REPO, "tests", "golden")
+
+
+def declared():
+    src = open(HEADER).read()
+    return sorted(set(re.findall(r"^[a-z_ ]*\**\s*(scphd_\w+)\s*\(", src, flags=re.M)))
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared()
+    assert len(names) >= 20
+    L = ctypes.CDLL(scphd_amd.LIB_PATH)
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert sorted(scphd_amd.EXPORTED) == names
+
+
+def test_params_layout_matches_oracle_mirror():
+    assert ctypes.sizeof(scphd_amd.Params) == oracle.lib().orc_params_size() == ctypes.sizeof(oracle.OracleParams)
+    for (n1, t1), (n2, t2) in zip(scphd_amd.Params._fields_, oracle.OracleParams._fields_):
+        assert n1 == n2
+        assert getattr(scphd_amd.Params, n1).offset == getattr(oracle.OracleParams, n2).offset
+
+
+def test_defaults_are_reference_config():
+    p = scphd_amd.Params()
+    assert scphd_amd.lib().scphd_params_init(ctypes.byref(p)) == 0
+    assert p.n_particles == 100 and p.particles_per_feature == 100
+    assert p.min_separation == 6.0 and p.min_weight == 1e-4 and p.ps == 0.9
+    assert p.fx == -895.6561 and p.v0 == 300.0 and p.lambda_ == 10.0001
+    assert p.birth_w0 == 1e-6 and p.birth_var_vd0 == 0.05
+
+
+def test_reference_config_file_parses_with_mean_std_mapping():
+    d = scphd_amd.load_params(os.path.join(GOLDEN, "calibration.cfg"))
+    assert d["n_particles"] == 100 and d["max_steps"] == 100 and d["verbosity"] == 0
+    assert d["slam"] == 0  # absent in the file: false
+    s3 = math.sqrt(3.0)
+    # prior.cartesian.z: mean -3, std 3 -> uniform [-3 - 3 sqrt 3, -3 + 3 sqrt 3]
+    assert d["prior_a"][2] == pytest.approx(-3 - 3 * s3) and d["prior_b"][2] == pytest.approx(-3 + 3 * s3)
+    assert d["prior_a"][1] == -50.0 and d["prior_b"][1] == -50.0
+    assert d["prior_a"][7] == pytest.approx(0.1 - 0.1 * s3)
+    assert d["cam_accel_std"] == [0.0, 0.0, 0.15, 0.0, 0.001, 0.0]
+    assert d["feat_accel_std"] == [0.05, 0.05, 0.1]
+    assert d["std_u"] == 5.0 and d["pd"] == 0.95 and d["lambda"] == 10.0001
+    assert d["birth_w0"] == 1e-6 and d["birth_d0"] == 8.0 and d["birth_var_vd0"] == 0.05
+    assert d["min_neff"] == 0.75 and d["min_range"] == 0.5 and d["max_velocity"] == 5.0
+
+
+def test_generated_config_round_trips(tmp_path):
+    p, _ = workloads.workload("c1")
+    path = tmp_path / "c1.cfg"
+    path.write_text(workloads.to_cfg(p))
+    d = scphd_amd.load_params(str(path))
+    for k, v in p.items():
+        if k == "storage":
+            continue
+        if isinstance(v, list):
+            np.testing.assert_allclose(d[k], v, rtol=1e-15, err_msg=k)
+        else:
+            assert d[k] == pytest.approx(v, rel=1e-15), k
+
+
+def test_config_errors_name_the_problem(tmp_path):
+    good = open(os.path.join(GOLDEN, "calibration.cfg")).read()
+    bad = tmp_path / "missing.cfg"
+    bad.write_text(good.replace("min_separation = 6.0", ""))
+    with pytest.raises(scphd_amd.ScphdError, match="min_separation"):
+        scphd_amd.load_params(str(bad))
+    broken = tmp_path / "broken.cfg"
+    broken.write_text("n_particles = = 3\n")
+    with pytest.raises(scphd_amd.ScphdError, match="CONFIG"):
+        scphd_amd.load_params(str(broken))
+    with pytest.raises(scphd_amd.ScphdError, match="cannot open"):
+        scphd_amd.load_params(str(tmp_path / "nope.cfg"))
+
+
+def test_config_grammar_variants(tmp_path):
+    """Comments of three kinds, ':' and '=', ',' and ';' terminators,
+    hex and L-suffixed integers, booleans in any case, @include."""
+    inc = tmp_path / "birth.cfg"
+    inc.write_text('birth : { w0 = 1e-6; d0 = 8.0, vu0 = 0; vv0 = 0\n vd0 = 0 var_d0 = 2.0 var_vu0 = 6 '
+                   'var_vv0 = 6 var_vd0 = 0.05 }\n')
+    body = open(os.path.join(GOLDEN, "calibration.cfg")).read()
+    body = re.sub(r"birth : \{.*?\}\s*$", '@include "birth.cfg"\n', body, flags=re.S)
+    body = body.replace("n_particles = 100", "n_particles = 0x40L /* hex */ // comment")
+    body = body.replace("save_all_maps = false", "save_all_maps = TRUE")
+    main = tmp_path / "main.cfg"
+    main.write_text(body)
+    d = scphd_amd.load_params(str(main))
+    assert d["n_particles"] == 64 and d["save_all_maps"] == 1 and d["birth_var_vu0"] == 6.0
+
+
+def test_create_rejects_bad_arguments():
+    p, _ = workloads.workload("c0")
+    for k, v in [("n_particles", 0), ("particles_per_feature", 0), ("particles_per_feature", 1000), ("storage", 7)]:
+        q = dict(p)
+        q[k] = v
+        with pytest.raises(scphd_amd.ScphdError, match="ARG"):
+            scphd_amd.Filter(q)
