@@ -92,6 +92,7 @@ def lib():
         "orc_get_fit_debug": (None, [_P, dp]),
         "orc_philox": (None, [up, up, up]),
         "orc_normals": (None, [_U64, _U32, _U64, _U32, _U32, _U32, _I32, dp]),
+        "orc_box_muller": (None, [_U32, _U32, dp]),
         "orc_wrap_angle": (_D, [_D]),
         "orc_safe_log": (_D, [_D]),
         "orc_lse2": (_D, [_D, _D]),
@@ -234,6 +235,12 @@ def philox(ctr, key):
     P = ctypes.POINTER(_U32)
     lib().orc_philox(c.ctypes.data_as(P), k.ctypes.data_as(P), o.ctypes.data_as(P))
     return o
+
+
+def box_muller(x0, x1):
+    z = np.zeros(2)
+    lib().orc_box_muller(x0, x1, _dp(z))
+    return z
 
 
 def normals(seed, stream, step, parent, gauss, elem, n):

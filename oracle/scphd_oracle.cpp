@@ -8,7 +8,8 @@
  *
  * Every function cites the reference file:line whose arithmetic it restates.
  * Deliberate, documented deviations (the same ones the GPU path follows):
- *   D1  noise: Philox4x32-10 counter streams + Box-Muller instead of thrust's
+ *   D1  noise: Philox4x32-10 counter streams + Box-Muller (box_muller,
+ *       single-precision polynomial log / sin / cos) instead of thrust's
  *       time-seeded default_random_engine (reference :92-127, :1062-1118,
  *       :640-690, :2663-2666, prior :912-979).
  *   D2  exact per-parent segmentation of normalisers, cardinalities and map
@@ -312,9 +313,68 @@ static inline double u01(uint32_t lo, uint32_t hi) {  /* [0, 1) */
     return (double)b * 0x1.0p-53;
 }
 
-/* n (<= 8) standard normals by Box-Muller, four per Philox block: the pair
- * (x0, x1) gives a = (x0 + 1) 2^-32 in (0, 1] and c = x1 2^-32 in [0, 1),
- * r = sqrt(-2 log a), z = (r cos 2 pi c, r sin 2 pi c); (x2, x3) likewise. */
+/* Box-Muller pair (Box and Muller 1958) from two Philox words, in single
+ * precision with explicit fused multiply-adds (bit-identical on any
+ * IEEE-754 machine):
+ *   a = (x0 + 1) 2^-32, truncated to 24 significant bits, written f 2^e with
+ *   f in [sqrt(1/2), sqrt(2)); log a = e ln2 + s P(s), s = f - 1, P the
+ *   degree-8 Chebyshev fit of log(1 + s)/s;
+ *   c = (x1 >> 8) 2^-24: octant o = top 3 bits, offset t in [0, 1] (from
+ *   the low 21 bits, mirrored in odd octants); sin(pi/4 t) = t S(t^2),
+ *   cos(pi/4 t) = C(t^2), Taylor to t^9 / t^8;
+ *   r = sqrt(-2 log a) (FP64); z0 = r cos 2 pi c, z1 = r sin 2 pi c. */
+static void box_muller(uint32_t x0, uint32_t x1, double& z0, double& z1) {
+    static const float PLOG[9] = {1.0f,          -0x1.fffff8p-2f, 0x1.555790p-2f, -0x1.0005a6p-2f, 0x1.98b80ap-3f,
+                                  -0x1.5329bep-3f, 0x1.322850p-3f, -0x1.26729ep-3f, 0x1.6626eap-4f};
+    static const float PSIN[5] = {0x1.921fb6p-1f, -0x1.4abbcep-4f, 0x1.466bc6p-9f, -0x1.32d2ccp-15f, 0x1.507834p-22f};
+    static const float PCOS[5] = {1.0f, -0x1.3bd3ccp-2f, 0x1.03c1f0p-6f, -0x1.55d3c8p-12f, 0x1.e1f506p-19f};
+    double la = 0.0;
+    const uint64_t n = (uint64_t)x0 + 1;  /* in [1, 2^32] */
+    if (n < (1ull << 32)) {
+        int k = 0;  /* n = m 2^(k - 31) with m in [2^31, 2^32) */
+        uint64_t m = n;
+        while (m < (1ull << 31)) {
+            m <<= 1;
+            --k;
+        }
+        float f = std::ldexp((float)(uint32_t)(m >> 8), -23);  /* [1, 2) */
+        int e = k - 1;  /* a = n 2^-32 = (m 2^-31) 2^(k - 1) */
+        if (f > 0x1.6a09e6p+0f) {
+            f = f / 2.0f;
+            e += 1;
+        }
+        const float s = f - 1.0f;
+        float p = PLOG[8];
+        for (int i = 7; i >= 0; --i) p = std::fmaf(p, s, PLOG[i]);
+        la = std::fma((double)e, 0x1.62e42fefa39efp-1, (double)(s * p));
+    }
+    const double r = std::sqrt(-2.0 * la);
+    const uint32_t c = x1 >> 8;
+    const uint32_t oct = c >> 21, off = c & 0x1FFFFF;
+    const bool odd = (oct & 1) != 0;
+    const float t = std::ldexp((float)(odd ? 0x200000 - off : off), -21);
+    const float t2 = t * t;
+    float ps = PSIN[4], pc = PCOS[4];
+    for (int i = 3; i >= 0; --i) {
+        ps = std::fmaf(ps, t2, PSIN[i]);
+        pc = std::fmaf(pc, t2, PCOS[i]);
+    }
+    const float st = t * ps;
+    /* angle within the quadrant, then the quadrant's rotation */
+    const float sq = odd ? pc : st, cq = odd ? st : pc;
+    float sn, cs;
+    switch (oct >> 1) {
+        case 0: sn = sq; cs = cq; break;
+        case 1: sn = cq; cs = -sq; break;
+        case 2: sn = -sq; cs = -cq; break;
+        default: sn = -cq; cs = sq; break;
+    }
+    z0 = r * (double)cs;
+    z1 = r * (double)sn;
+}
+
+/* n (<= 8) standard normals, four per Philox block: words (x0, x1) give the
+ * first pair, (x2, x3) the second. */
 static void normals(uint64_t seed, uint32_t stream, uint64_t step, uint32_t parent, uint32_t gauss,
                     uint32_t elem, int n, double* z) {
     for (int b = 0; 4 * b < n; ++b) {
@@ -323,15 +383,16 @@ static void normals(uint64_t seed, uint32_t stream, uint64_t step, uint32_t pare
         for (int h = 0; h < 2; ++h) {
             const int i = 4 * b + 2 * h;
             if (i >= n) break;
-            const double a = ((double)x[2 * h] + 1.0) * 0x1.0p-32;
-            const double c = (double)x[2 * h + 1] * 0x1.0p-32;
-            const double r = std::sqrt(-2.0 * std::log(a));
-            const double th = (2.0 * M_PI) * c;
-            z[i] = r * std::cos(th);
-            if (i + 1 < n) z[i + 1] = r * std::sin(th);
+            double z0, z1;
+            box_muller(x[2 * h], x[2 * h + 1], z0, z1);
+            z[i] = z0;
+            if (i + 1 < n) z[i + 1] = z1;
         }
     }
 }
+
+/* exported for the unit tests: one Box-Muller pair from raw words */
+extern "C" void orc_box_muller(uint32_t x0, uint32_t x1, double* z) { box_muller(x0, x1, z[0], z[1]); }
 
 /* ---------------------------------------------------------------- state */
 struct Comp {  /* one 6-D Gaussian: weight, mean, full column-major covariance */

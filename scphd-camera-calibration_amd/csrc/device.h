@@ -76,11 +76,73 @@ SCPHD_HD inline double uniform_oc(uint32_t lo, uint32_t hi) { /* (0, 1] */
     return (double)(((((uint64_t)hi << 32) | lo) >> 11) + 1) * 0x1.0p-53;
 }
 
-/* N (<= 8) standard normals by the Box-Muller transform, four per Philox
- * block: words (x0, x1) give a = (x0 + 1) 2^-32 in (0, 1], c = x1 2^-32 in
- * [0, 1), r = sqrt(-2 log a), (z0, z1) = r (cos 2 pi c, sin 2 pi c); words
- * (x2, x3) give (z2, z3).  cos/sin of 2 pi c are evaluated as sincospi(2c),
- * which needs no argument reduction. */
+/* One Box-Muller pair from two Philox words (Box and Muller 1958),
+ * evaluated with single-precision adds, multiplies and fused multiply-adds
+ * only, so that any IEEE-754 implementation returns the same bits:
+ *   a = (x0 + 1) 2^-32 in (0, 1], its leading 24 bits kept:
+ *       a = f 2^e with f in [sqrt(1/2), sqrt(2)), log a = e ln2 + s P(s),
+ *       s = f - 1, P the degree-8 Chebyshev fit of log(1 + s)/s on that
+ *       range (|error| < 2e-8);
+ *   c = (x1 >> 8) 2^-24 in [0, 1): the octant of 2 pi c and the offset t in
+ *       [0, 1] inside it come from the integer bits, sin(pi/4 t) = t S(t^2)
+ *       and cos(pi/4 t) = C(t^2) are Taylor polynomials to t^9 and t^8
+ *       (|error| < 4e-8);
+ *   r = sqrt(-2 log a) in double; z0 = r cos 2 pi c, z1 = r sin 2 pi c. */
+__device__ inline void box_muller(uint32_t x0, uint32_t x1, double& z0, double& z1) {
+#pragma clang fp contract(off)
+    const uint32_t n = x0 + 1u; /* 0 stands for a = 1 (log a = 0) */
+    const uint32_t nn = n != 0u ? n : 1u;
+    const int lz = __clz((int)nn);
+    const uint32_t m = nn << lz; /* [2^31, 2^32) */
+    float f = (float)(m >> 8) * 0x1.0p-23f;
+    int e = -1 - lz;
+    if (f > 0x1.6a09e6p+0f) {
+        f *= 0.5f;
+        e += 1;
+    }
+    const float sx = f - 1.0f;
+    float pl = 0x1.6626eap-4f;
+    pl = __builtin_fmaf(pl, sx, -0x1.26729ep-3f);
+    pl = __builtin_fmaf(pl, sx, 0x1.322850p-3f);
+    pl = __builtin_fmaf(pl, sx, -0x1.5329bep-3f);
+    pl = __builtin_fmaf(pl, sx, 0x1.98b80ap-3f);
+    pl = __builtin_fmaf(pl, sx, -0x1.0005a6p-2f);
+    pl = __builtin_fmaf(pl, sx, 0x1.555790p-2f);
+    pl = __builtin_fmaf(pl, sx, -0x1.fffff8p-2f);
+    pl = __builtin_fmaf(pl, sx, 1.0f);
+    const float lf = sx * pl;
+    const double L = n != 0u ? __builtin_fma((double)e, 0x1.62e42fefa39efp-1, (double)lf) : 0.0;
+    const double r = __builtin_sqrt(-2.0 * L);
+    const uint32_t c = x1 >> 8;
+    const uint32_t oct = c >> 21;
+    const uint32_t fr = c & 0x1FFFFFu;
+    const float t = (float)((oct & 1u) ? (0x200000u - fr) : fr) * 0x1.0p-21f;
+    const float t2 = t * t;
+    float ps = 0x1.507834p-22f;
+    ps = __builtin_fmaf(ps, t2, -0x1.32d2ccp-15f);
+    ps = __builtin_fmaf(ps, t2, 0x1.466bc6p-9f);
+    ps = __builtin_fmaf(ps, t2, -0x1.4abbcep-4f);
+    ps = __builtin_fmaf(ps, t2, 0x1.921fb6p-1f);
+    const float sv = t * ps;
+    float pc = 0x1.e1f506p-19f;
+    pc = __builtin_fmaf(pc, t2, -0x1.55d3c8p-12f);
+    pc = __builtin_fmaf(pc, t2, 0x1.03c1f0p-6f);
+    pc = __builtin_fmaf(pc, t2, -0x1.3bd3ccp-2f);
+    pc = __builtin_fmaf(pc, t2, 1.0f);
+    /* quadrant angle phi = 2 pi c - q pi/2: odd octants measure t from pi/2 */
+    const float sphi = (oct & 1u) ? pc : sv;
+    const float cphi = (oct & 1u) ? sv : pc;
+    const uint32_t q = oct >> 1;
+    float sn = (q & 1u) ? cphi : sphi;
+    float cs = (q & 1u) ? sphi : cphi;
+    if (q == 1u || q == 2u) cs = -cs;
+    if (q >= 2u) sn = -sn;
+    z0 = r * (double)cs;
+    z1 = r * (double)sn;
+}
+
+/* N (<= 8) standard normals, four per Philox block: words (x0, x1) give
+ * (z0, z1), words (x2, x3) give (z2, z3) (box_muller). */
 template <int N>
 __device__ inline void normals(NoiseKey key, uint32_t stream, uint64_t step, uint32_t parent, uint32_t gauss,
                                uint32_t elem, double z[N]) {
@@ -92,19 +154,15 @@ __device__ inline void normals(NoiseKey key, uint32_t stream, uint64_t step, uin
         for (int h = 0; h < 2; ++h) {
             const int i = 4 * b + 2 * h;
             if (i < N) {
-                const double a = ((double)x[2 * h] + 1.0) * 0x1.0p-32;
-                const double c = (double)x[2 * h + 1] * 0x1.0p-32;
-                const double r = sqrt(-2.0 * log(a));
-                double sn, cs;
-                sincospi(2.0 * c, &sn, &cs);
-                z[i] = r * cs;
-                if (i + 1 < N) z[i + 1] = r * sn;
+                double z0, z1;
+                box_muller(x[2 * h], x[2 * h + 1], z0, z1);
+                z[i] = z0;
+                if (i + 1 < N) z[i + 1] = z1;
             }
         }
     }
 }
 
-/* ------------------------------------------------------------------ math */
 SCPHD_HD inline double safe_log(double x) { return x <= 0 ? SCPHD_LOG0 : log(x); }
 
 SCPHD_HD constexpr int pidx(int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }

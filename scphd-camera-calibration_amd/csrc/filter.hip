@@ -116,7 +116,7 @@ struct scphd_filter {
     int cur = 0;
     /* per-Gaussian intermediates */
     DBuf<double> fit, pre;
-    DBuf<int32_t> pdok, keep, keepscan, ndlist;
+    DBuf<int32_t> pdok, keep, keepscan, ndlist, redo;
     /* measurements, normalisers, survivors, merge */
     DBuf<double> du, dv, eta;
     int capM = 0;
@@ -239,6 +239,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
         HIPTRY(f->pre.ensure((int64_t)PP_COUNT * Ng));
         HIPTRY(f->pdok.ensure(Ng));
         HIPTRY(f->keep.ensure(Ng));
+        HIPTRY(f->redo.ensure(Ng));
         HIPTRY(f->ndlist.ensure(Ng));
         FitArgs fa;
         fa.gblock = f->gblock.p;
@@ -257,6 +258,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
         fa.min_weight = P.min_weight;
         fa.fit = f->fit.p;
         fa.keep = f->keep.p;
+        fa.redo = f->redo.p;
         fa.nd_base = 0;
         record(f, 6);
         HIPTRY(launch_fit<T>(s, fa, f->gen[f->cur].planes<T>(), f->gen[nxt].planes<T>()));
@@ -537,7 +539,7 @@ void scphd_destroy(scphd_t* f) {
                            &f->dneff};
     for (auto* b : dbs) b->release();
     DBuf<int32_t>* ibs[] = {&f->anc, &f->newcnt, &f->goff, &f->gparent, &f->gblock, &f->goff2, &f->gparent2,
-                            &f->gblock2, &f->pdok, &f->keep, &f->keepscan, &f->ndlist, &f->nsurv, &f->survoff,
+                            &f->gblock2, &f->pdok, &f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->nsurv, &f->survoff,
                             &f->nmerged, &f->mergeoff, &f->outcnt, &f->scantmp, &f->sg, &f->sm, &f->dflag,
                             &f->ngroups, &f->glead, &f->gsta, &f->gend, &f->gmem};
     for (auto* b : ibs) b->release();

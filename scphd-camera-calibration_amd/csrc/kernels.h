@@ -86,6 +86,7 @@ struct FitArgs {
     double min_weight;
     double* fit;     /* FP_COUNT planes */
     int32_t* keep;   /* non-detection term kept */
+    int32_t* redo;   /* per Gaussian: bit 0 all-zero weights, bit 1 cloud still to write */
     int64_t nd_base; /* destination block of the non-detection cloud of entry g: nd_base + g */
 };
 

@@ -214,124 +214,246 @@ __global__ void k_materialize(Planes<T> src, Planes<T> dst, const int32_t* gbloc
 }
 
 /* ------------------------------------------------------- reproject + fit */
-/* One wavefront per Gaussian: deferred prediction, world -> disparity with
- * the parent's camera and the field-of-view Pd (worldToDisparityKernel,
+/* Sum over the 16 lanes of a row (four DPP steps: swap neighbours, swap
+ * pairs, half-row mirror, row mirror); every lane of the row gets the sum.
+ * The row must be fully active; rows are independent. */
+__device__ inline double row_sum16(double x) {
+    x += dpp_f64<0xB1>(x);
+    x += dpp_f64<0x4E>(x);
+    x += dpp_f64<0x141>(x);
+    x += dpp_f64<0x140>(x);
+    return x;
+}
+
+/* One sample of a Gaussian's cloud: the deferred prediction applied, its
+ * disparity-space point, its field-of-view flag and stored weight. */
+template <typename T>
+__device__ inline void load_sample(const Planes<T>& src, int64_t si, T r[7]) {
+#pragma unroll
+    for (int c = 0; c < 7; ++c) r[c] = src.p[c][si];
+}
+
+template <typename T>
+__device__ inline void eval_sample(const FitArgs& a, const T r[7], int p, int k, int s, const CameraPose& pose,
+                                   EuclideanPoint& x, DisparityPoint& d, bool& in, double& w) {
+    x = EuclideanPoint{(double)r[0], (double)r[1], (double)r[2], (double)r[3], (double)r[4], (double)r[5]};
+    w = (double)r[6];
+    if (a.pending) x = predict_sample(x, a.feat, a.dt, a.key, a.pstep, (uint32_t)p, (uint32_t)k, (uint32_t)s);
+    d = a.meas.computeMeasurement(x, pose);
+    in = a.meas.inRange(d);
+}
+
+template <typename T>
+__device__ inline void fit_sample(const FitArgs& a, const Planes<T>& src, int64_t si, int p, int k, int s,
+                                  const CameraPose& pose, EuclideanPoint& x, DisparityPoint& d, bool& in, double& w) {
+    T r[7];
+    load_sample(src, si, r);
+    eval_sample(a, r, p, k, s, pose, x, d, in, w);
+}
+
+template <typename T>
+__device__ inline void write_nd(const FitArgs& a, Planes<T>& dst, int64_t di, const EuclideanPoint& x, bool in) {
+    dst.p[0][di] = (T)x.x; dst.p[1][di] = (T)x.y; dst.p[2][di] = (T)x.z;
+    dst.p[3][di] = (T)x.vx; dst.p[4][di] = (T)x.vy; dst.p[5][di] = (T)x.vz;
+    dst.p[6][di] = (T)(1.0 - (in ? a.meas.pd() : 0.0));
+}
+
+__device__ inline void write_fit(const FitArgs& a, int64_t g, int lane, const double* mu, const double* P, double pdg) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int q = lane + 16 * h;
+        double v = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            if (q == FP_MU + c) v = mu[c];
+#pragma unroll
+        for (int e = 0; e < 21; ++e)
+            if (q == FP_P + e) v = P[e];
+        if (q == FP_PD) v = pdg;
+        if (q < FP_COUNT) a.fit[(int64_t)q * a.Ng + g] = v;
+    }
+}
+
+/* Sixteen lanes (one row of a wavefront) per Gaussian, four Gaussians per
+ * wavefront: deferred prediction, world -> disparity with the parent's
+ * camera and the field-of-view Pd (worldToDisparityKernel,
  * src/kernels.cu:6-47), per-Gaussian Pd (computePdKernel :82-99), sample
- * reweighting and normalisation (src/scphdcameracalibration.cu:1602-1608,
- * normalizeWeightsKernel src/kernels.cu:996-1018), weighted moment fit
- * (fitGaussiansKernel :102-438) and, when the non-detection term is kept
- * (w (1 - pd) >= min_weight, computeNonDetections
- * src/scphdcameracalibration.cu:1290-1420), the non-detection cloud
- * (disparity -> world round trip, weights 1 - pd_i) written to its own
- * block.  Samples stay in registers: one HBM read per sample. */
-template <typename T, int NS>
+ * reweighting by pd and normalisation (src/scphdcameracalibration.cu:1602-1608,
+ * normalizeWeightsKernel src/kernels.cu:996-1018), and the weighted moment
+ * fit (fitGaussiansKernel :102-438, covariance normaliser (W^2 + sum w^2) / W).
+ *
+ * Samples stream through once: each lane accumulates, with weights
+ * w' = w pd, the sums of w', w'^2, w' (q - K) and w' (q - K)(q - K)^T about
+ * the Gaussian's first sample K, from which the normalised mean and
+ * centred covariance follow; 30 row sums by DPP.
+ *
+ * The non-detection cloud (kept when w_g (1 - pd_g) >= min_weight,
+ * computeNonDetections src/scphdcameracalibration.cu:1290-1420; sample
+ * weights 1 - pd_i; the samples themselves are the predicted cloud, which
+ * the reference's world -> disparity -> world round trip reproduces to
+ * rounding) is written during the sweep when w_g alone settles the
+ * decision.  Two rare cases are left to k_fit_fix through redo[g]: a
+ * kept cloud whose decision needed pd_g (bit 1), and a cloud whose weights
+ * are all zero (bit 0), which the reference normalises to weight 1 per
+ * sample. */
+template <typename T>
 __global__ __launch_bounds__(256) void k_fit(FitArgs a, Planes<T> src, Planes<T> dst) {
-    const int lane = threadIdx.x & 63;
-    const int64_t g = (int64_t)blockIdx.x * WPB + (threadIdx.x >> 6);
-    if (g >= a.Ng) return; /* whole wavefront; no workgroup barriers below */
+    const int lane = threadIdx.x & 15;
+    const int64_t g = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+    if (g >= a.Ng) return; /* whole rows; rows never exchange data */
     const int p = a.gparent[g];
     const int k = (int)(g - a.goff[p]);
     const int64_t b0 = (int64_t)a.gblock[g] * a.S;
     const CameraPose pose = load_pose(a.pose, p);
     const int S = a.S;
+    const double wg = a.gw[g];
+    const double Pd = a.meas.pd();
+    /* keep decided before the sweep: certainly kept (1 - pd_g >= 1 - Pd up
+     * to rounding of the Pd sum) or certainly dropped (1 - pd_g <= 1) */
+    const bool sure_keep = wg * (1.0 - Pd) * (1.0 - 1e-12) >= a.min_weight;
+    const bool sure_drop = !(wg >= a.min_weight);
+    const int64_t d0 = (a.nd_base + g) * (int64_t)S;
 
-    double q[NS][6], wv[NS];
-    uint32_t inmask = 0; /* bit i: sample lane + 64 i is inside the field of view */
-    double spd = 0.0, sw = 0.0;
+    double spd = 0.0, sw = 0.0, sw2 = 0.0, m1[6], m2[21], K[6];
 #pragma unroll
-    for (int i = 0; i < NS; ++i) {
-        const int s = lane + 64 * i;
-        wv[i] = 0.0;
+    for (int c = 0; c < 6; ++c) m1[c] = 0.0;
 #pragma unroll
-        for (int c = 0; c < 6; ++c) q[i][c] = 0.0;
-        if (s < S) {
-            const int64_t si = b0 + s;
-            EuclideanPoint x{(double)src.p[0][si], (double)src.p[1][si], (double)src.p[2][si],
-                             (double)src.p[3][si], (double)src.p[4][si], (double)src.p[5][si]};
-            const double w = (double)src.p[6][si];
-            if (a.pending) x = predict_sample(x, a.feat, a.dt, a.key, a.pstep, (uint32_t)p, (uint32_t)k, (uint32_t)s);
-            const DisparityPoint d = a.meas.computeMeasurement(x, pose);
-            q[i][0] = d.u; q[i][1] = d.v; q[i][2] = d.d; q[i][3] = d.vu; q[i][4] = d.vv; q[i][5] = d.vd;
-            const bool in = a.meas.inRange(d);
-            const double pd = in ? a.meas.pd() : 0.0;
-            inmask |= in ? (1u << i) : 0u;
-            wv[i] = w * pd;
-            spd += pd;
-            sw += wv[i];
+    for (int e = 0; e < 21; ++e) m2[e] = 0.0;
+    /* loads run one step ahead of the arithmetic */
+    T cur[7], nxt[7];
+    if (lane < S) load_sample(src, b0 + lane, cur);
+    for (int s0 = 0; s0 < S; s0 += 16) {
+        const int s = s0 + lane;
+        const bool valid = s < S;
+        if (s + 16 < S) load_sample(src, b0 + s + 16, nxt);
+        EuclideanPoint x;
+        DisparityPoint d;
+        bool in = false;
+        double w = 0.0;
+        if (valid) eval_sample(a, cur, p, k, s, pose, x, d, in, w);
+#pragma unroll
+        for (int c = 0; c < 7; ++c) cur[c] = nxt[c];
+        const double q[6] = {d.u, d.v, d.d, d.vu, d.vv, d.vd};
+        if (s0 == 0) {
+            /* shift: the Gaussian's first sample, broadcast along the row */
+#pragma unroll
+            for (int c = 0; c < 6; ++c) K[c] = __shfl(q[c], 0, 16);
         }
-    }
-    spd = wave_sum_dpp(spd);
-    sw = wave_sum_dpp(sw);
-    const double pdg = spd / S;
-
-    /* normalised weights w / sum (1 everywhere for an all-zero cloud),
-     * weighted mean, W, sum of squares */
-    const double rsw = sw > 0 ? 1.0 / sw : 0.0;
-    double m[6] = {0, 0, 0, 0, 0, 0}, W = 0.0, W2 = 0.0;
-#pragma unroll
-    for (int i = 0; i < NS; ++i) {
-        const int s = lane + 64 * i;
-        double wn = 0.0;
-        if (s < S) wn = sw > 0 ? wv[i] * rsw : 1.0;
-        wv[i] = wn;
-#pragma unroll
-        for (int c = 0; c < 6; ++c) m[c] += q[i][c] * wn;
-        W += wn;
-        W2 += wn * wn;
-    }
-#pragma unroll
-    for (int c = 0; c < 6; ++c) m[c] = wave_sum_dpp(m[c]);
-    W = wave_sum_dpp(W);
-    W2 = wave_sum_dpp(W2);
-    const double cn = W / (W * W + W2);
-
-    /* centred second moments */
-    double P[21];
-#pragma unroll
-    for (int e = 0; e < 21; ++e) P[e] = 0.0;
-#pragma unroll
-    for (int i = 0; i < NS; ++i) {
+        const double pd = in ? Pd : 0.0;
+        const double wp = valid ? w * pd : 0.0;
+        spd += pd;
+        sw += wp;
+        sw2 += wp * wp;
         double dv[6];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) dv[c] = q[i][c] - m[c];
+        for (int c = 0; c < 6; ++c) dv[c] = valid ? q[c] - K[c] : 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) m1[c] += wp * dv[c];
 #pragma unroll
         for (int r = 0; r < 6; ++r)
 #pragma unroll
-            for (int c = 0; c <= r; ++c) P[pidx(r, c)] += dv[r] * dv[c] * wv[i];
+            for (int c = 0; c <= r; ++c) m2[pidx(r, c)] += wp * dv[r] * dv[c];
+        if (sure_keep && valid) write_nd(a, dst, d0 + s, x, in);
     }
+    spd = row_sum16(spd);
+    sw = row_sum16(sw);
+    sw2 = row_sum16(sw2);
 #pragma unroll
-    for (int e = 0; e < 21; ++e) P[e] = wave_sum_dpp(P[e]);
+    for (int c = 0; c < 6; ++c) m1[c] = row_sum16(m1[c]);
+#pragma unroll
+    for (int e = 0; e < 21; ++e) m2[e] = row_sum16(m2[e]);
+    const double pdg = spd / S;
 
-    /* lanes 0..27 write one output plane each */
-    {
-        double v = 0.0;
+    /* normalised weights w'_i / sum w': mean K + sum w (q - K); centred
+     * covariance sum w (q - K)(q - K)^T - (2 - W) dm dm^T with W = sum w */
+    const double r = sw > 0 ? 1.0 / sw : 0.0;
+    const double W = sw * r, W2 = sw2 * r * r;
+    double mu[6], dm[6], P[21];
 #pragma unroll
-        for (int c = 0; c < 6; ++c)
-            if (lane == FP_MU + c) v = m[c];
-#pragma unroll
-        for (int e = 0; e < 21; ++e)
-            if (lane == FP_P + e) v = P[e] / cn;
-        if (lane == FP_PD) v = pdg;
-        if (lane < FP_COUNT) a.fit[(int64_t)lane * a.Ng + g] = v;
+    for (int c = 0; c < 6; ++c) {
+        dm[c] = m1[c] * r;
+        mu[c] = K[c] + dm[c];
     }
+    const double scale = (W * W + W2) / W;
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) P[pidx(i, j)] = (m2[pidx(i, j)] * r - (2.0 - W) * dm[i] * dm[j]) * scale;
+    write_fit(a, g, lane, mu, P, pdg);
 
-    /* non-detection term */
-    const double wg = a.gw[g];
     const bool keep = wg * (1 - pdg) >= a.min_weight;
-    if (lane == 0) a.keep[g] = keep ? 1 : 0;
-    if (keep) {
-        const int64_t d0 = (a.nd_base + g) * (int64_t)S;
-#pragma unroll
-        for (int i = 0; i < NS; ++i) {
-            const int s = lane + 64 * i;
+    if (lane == 0) {
+        a.keep[g] = keep ? 1 : 0;
+        a.redo[g] = (sw > 0 ? 0 : 1) | ((keep && !sure_keep && !sure_drop) ? 2 : 0);
+    }
+}
+
+/* Rare cases of k_fit, one row per flagged Gaussian: bit 0, every weight
+ * zero: unit weights (mean = sum of the samples, as the reference's
+ * normalisation of an all-zero cloud gives), refitted by two sweeps;
+ * bit 1, the non-detection cloud of a kept Gaussian whose decision needed
+ * pd_g. */
+template <typename T>
+__global__ __launch_bounds__(256) void k_fit_fix(FitArgs a, Planes<T> src, Planes<T> dst) {
+    const int lane = threadIdx.x & 15;
+    const int64_t g = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+    if (g >= a.Ng) return;
+    const int redo = a.redo[g];
+    if (redo == 0) return; /* uniform along the row */
+    const int p = a.gparent[g];
+    const int k = (int)(g - a.goff[p]);
+    const int64_t b0 = (int64_t)a.gblock[g] * a.S;
+    const CameraPose pose = load_pose(a.pose, p);
+    const int S = a.S;
+    if (redo & 1) {
+        double sq[6] = {0, 0, 0, 0, 0, 0};
+        for (int s0 = 0; s0 < S; s0 += 16) {
+            const int s = s0 + lane;
             if (s < S) {
-                const DisparityPoint d{q[i][0], q[i][1], q[i][2], q[i][3], q[i][4], q[i][5]};
-                const EuclideanPoint x = a.meas.invertMeasurement(d, pose);
-                const int64_t di = d0 + s;
-                dst.p[0][di] = (T)x.x; dst.p[1][di] = (T)x.y; dst.p[2][di] = (T)x.z;
-                dst.p[3][di] = (T)x.vx; dst.p[4][di] = (T)x.vy; dst.p[5][di] = (T)x.vz;
-                dst.p[6][di] = (T)(1.0 - (((inmask >> i) & 1u) ? a.meas.pd() : 0.0));
+                EuclideanPoint x;
+                DisparityPoint d;
+                bool in;
+                double w;
+                fit_sample(a, src, b0 + s, p, k, s, pose, x, d, in, w);
+                sq[0] += d.u; sq[1] += d.v; sq[2] += d.d; sq[3] += d.vu; sq[4] += d.vv; sq[5] += d.vd;
             }
+        }
+        double mu[6], cc[21], P[21];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) mu[c] = row_sum16(sq[c]);
+#pragma unroll
+        for (int e = 0; e < 21; ++e) cc[e] = 0.0;
+        for (int s0 = 0; s0 < S; s0 += 16) {
+            const int s = s0 + lane;
+            if (s < S) {
+                EuclideanPoint x;
+                DisparityPoint d;
+                bool in;
+                double w;
+                fit_sample(a, src, b0 + s, p, k, s, pose, x, d, in, w);
+                const double dv[6] = {d.u - mu[0], d.v - mu[1], d.d - mu[2], d.vu - mu[3], d.vv - mu[4], d.vd - mu[5]};
+#pragma unroll
+                for (int r = 0; r < 6; ++r)
+#pragma unroll
+                    for (int c = 0; c <= r; ++c) cc[pidx(r, c)] += dv[r] * dv[c];
+            }
+        }
+        const double W = (double)S, W2 = (double)S;
+        const double scale = (W * W + W2) / W;
+#pragma unroll
+        for (int e = 0; e < 21; ++e) P[e] = row_sum16(cc[e]) * scale;
+        const double pdg = a.fit[(int64_t)FP_PD * a.Ng + g];
+        write_fit(a, g, lane, mu, P, pdg);
+    }
+    if (redo & 2) {
+        const int64_t d0 = (a.nd_base + g) * (int64_t)S;
+        for (int s = lane; s < S; s += 16) {
+            EuclideanPoint x;
+            DisparityPoint d;
+            bool in;
+            double w;
+            fit_sample(a, src, b0 + s, p, k, s, pose, x, d, in, w);
+            write_nd(a, dst, d0 + s, x, in);
         }
     }
 }
@@ -1354,18 +1476,8 @@ hipError_t launch_materialize(hipStream_t s, Planes<T> src, Planes<T> dst, const
 template <typename T>
 hipError_t launch_fit(hipStream_t s, const FitArgs& a, Planes<T> src, Planes<T> dst) {
     if (a.Ng <= 0) return hipSuccess;
-    const unsigned nb = nblocks(a.Ng, WPB);
-    const int ns = (a.S + 63) / 64;
-    if (ns <= 1)
-        k_fit<T, 1><<<nb, 256, 0, s>>>(a, src, dst);
-    else if (ns <= 2)
-        k_fit<T, 2><<<nb, 256, 0, s>>>(a, src, dst);
-    else if (ns <= 4)
-        k_fit<T, 4><<<nb, 256, 0, s>>>(a, src, dst);
-    else if (ns <= 8)
-        k_fit<T, 8><<<nb, 256, 0, s>>>(a, src, dst);
-    else
-        return hipErrorInvalidValue;
+    k_fit<T><<<nblocks(a.Ng, 16), 256, 0, s>>>(a, src, dst);
+    k_fit_fix<T><<<nblocks(a.Ng, 16), 256, 0, s>>>(a, src, dst);
     LAUNCH_CHECK();
 }
 

@@ -57,6 +57,42 @@ def test_normals_are_standard_normal_and_keyed():
     np.testing.assert_array_equal(oracle.normals(5, 1, 0, 0, 0, 0, 6)[:3], oracle.normals(5, 1, 0, 0, 0, 0, 3))
 
 
+def _box_muller_exact(x0, x1):
+    """Box-Muller with libm on the same inputs: a = (x0 + 1) 2^-32 to 24
+    significant bits (truncated), c = (x1 >> 8) 2^-24."""
+    n = x0 + 1
+    if n >= 1 << 32:
+        a = 1.0
+    else:
+        sh = max(n.bit_length() - 24, 0)
+        a = ((n >> sh) << sh) / 2.0**32
+    c = (x1 >> 8) / 2.0**24
+    r = math.sqrt(-2.0 * math.log(a))
+    return r * math.cos(2 * math.pi * c), r * math.sin(2 * math.pi * c)
+
+
+def test_box_muller_polynomials_match_libm():
+    rng = np.random.default_rng(7)
+    words = [(0, 0), (0xFFFFFFFF, 0xFFFFFFFF), (0xFFFFFFFE, 0x00FFFFFF), (1, 0x20000000), (0x7FFFFFFF, 0x40000000),
+             (0xB504F333, 0x60000000), (0xB504F334, 0x80000000), (12345, 0xA0000000), (99, 0xC0000000),
+             (0x80000000, 0xE0000000), (0x0000FFFF, 0x1FFFFFFF), (0x00FFFFFF, 0x3FFFFF00)]
+    words += [(int(a), int(b)) for a, b in rng.integers(0, 1 << 32, size=(4000, 2), dtype=np.uint64)]
+    for x0, x1 in words:
+        z = oracle.box_muller(x0, x1)
+        e = _box_muller_exact(x0, x1)
+        assert abs(z[0] - e[0]) < 4e-7 * (1 + abs(e[0])) and abs(z[1] - e[1]) < 4e-7 * (1 + abs(e[1])), (x0, x1, z, e)
+    # a = 1 gives r = 0; the smallest a gives the largest radius
+    assert np.all(oracle.box_muller(0xFFFFFFFF, 0x12345678) == 0)
+    assert abs(np.hypot(*oracle.box_muller(0, 0)) - math.sqrt(64 * math.log(2))) < 1e-6
+
+
+def test_normals_are_box_muller_of_philox_words():
+    w = oracle.philox([(7 << 2) | 1, 3, 2, (5 << 4) | 2], [11, 0])
+    z = oracle.normals(11, 2, 5, 2, 3, 7, 8)
+    np.testing.assert_array_equal(z[4:6], oracle.box_muller(int(w[0]), int(w[1])))
+    np.testing.assert_array_equal(z[6:8], oracle.box_muller(int(w[2]), int(w[3])))
+
+
 # ----------------------------------------------------------------- scalars
 def test_safe_log_and_wrap_angle():
     assert oracle.safe_log(0.0) == -np.finfo(np.float32).max
