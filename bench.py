@@ -104,9 +104,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=12)
     ap.add_argument("--workload", default="c1")
     ap.add_argument("--storage", default="fp32", choices=["fp32", "fp64"])
-    ap.add_argument("--cpu-parents", type=int, default=32)
-    ap.add_argument("--cpu-warm", type=int, default=10)
-    ap.add_argument("--cpu-timed", type=int, default=2)
+    ap.add_argument("--cpu-parents", type=int, default=512)
+    ap.add_argument("--cpu-warm", type=int, default=12)
+    ap.add_argument("--cpu-timed", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -206,6 +206,100 @@ int scphd_get_ancestors(scphd_t* f, int32_t* ancestors);
 int scphd_set_timing(scphd_t* f, int32_t enabled);
 int scphd_get_phase_ms(scphd_t* f, double* ms8);
 
+/* Calibration estimate from the current parents, computed on the device:
+ * mean12 = sum_p w_p x_p (w normalised), cov144 = sum_p w_p (x_p - mean)
+ * (x_p - mean)^T (12 x 12, row-major), *cardinality = sum of the Gaussian
+ * weights of the highest-weight parent's map (the map writeMat writes).
+ * Any pointer may be NULL.  Replaces the evaluation of the written results,
+ * load_disparity_results.m:27-34 (exp_traj, cov_traj, cardinality). */
+int scphd_get_estimate(scphd_t* f, double* mean12, double* cov144, double* cardinality);
+
+/* Device memory currently held by the handle, in bytes (buffers grow
+ * geometrically and are kept between steps).  No counterpart in the
+ * reference. */
+int scphd_get_memory(scphd_t* f, int64_t* bytes);
+
+/* ======================================================== batched / served
+ * Many rigs (independent calibration instances, each with its own
+ * parameters, parent count and measurement stream) share one GPU.  Each
+ * admitted rig keeps its filter state resident in device memory; frames are
+ * queued per rig and processed in order by a pool of host workers, each rig
+ * on its own HIP stream, so that the device work of different rigs
+ * overlaps.  A rig is admitted only while the sum of the reservations of the
+ * admitted rigs stays within the HBM budget; a refused rig can be offered
+ * again once others leave.  A rig's results are the same whether it runs
+ * alone or with others.  No counterpart in the reference, which runs one
+ * rig per process (src/main.cpp:67-96, batch_run.sh:15). */
+typedef struct scphd_batch scphd_batch_t;
+
+typedef struct scphd_batch_opts {
+    int64_t hbm_budget_bytes;  /* admission budget; 0: 90 % of the free device memory */
+    int32_t max_rigs;          /* rig slots; 0: 1024                               */
+    int32_t n_workers;         /* frames in flight on the device; 0: 8              */
+} scphd_batch_opts;
+
+typedef struct scphd_frame_result {
+    int32_t rig;
+    int32_t status;            /* SCPHD_OK or the error of the failing call       */
+    int64_t frame_id;          /* as submitted                                     */
+    double t_submit;           /* seconds on the batch clock (scphd_batch_now)     */
+    double t_start;
+    double t_end;
+    int64_t n_meas;
+    int64_t n_pairs;           /* component updates of the frame (N_g * M)         */
+    int64_t n_gaussians;       /* Gaussians after the frame                        */
+    int32_t resampled;
+    int32_t reserved0;
+    double estimate[12];       /* weighted parent mean after the frame             */
+    double cardinality;
+} scphd_frame_result;
+
+typedef struct scphd_batch_stats {
+    int64_t rigs;              /* admitted rigs                                    */
+    int64_t reserved_bytes;    /* sum of their reservations                        */
+    int64_t budget_bytes;
+    int64_t used_bytes;        /* device memory held by their handles              */
+    int64_t queued_frames;     /* submitted, not yet finished                      */
+    int64_t frames_done;       /* since creation                                   */
+    int64_t pairs_done;        /* component updates since creation                 */
+    int64_t refused;           /* admissions refused for lack of budget            */
+} scphd_batch_stats;
+
+int scphd_batch_opts_init(scphd_batch_opts* o);
+int scphd_batch_create(const scphd_batch_opts* o, scphd_batch_t** out);
+/* Waits for queued frames, then frees every rig. */
+void scphd_batch_destroy(scphd_batch_t* b);
+const char* scphd_batch_last_error(const scphd_batch_t* b);
+/* Seconds since the batch was created (the clock of scphd_frame_result). */
+double scphd_batch_now(const scphd_batch_t* b);
+
+/* Device memory a rig needs with about gaussians_per_parent Gaussians per
+ * parent (state, both sample generations and per-step work arrays, with the
+ * growth slack of the buffers). */
+int64_t scphd_rig_footprint(const scphd_params* p, int32_t gaussians_per_parent);
+
+/* Admit a rig with reservation reserve_bytes (0: scphd_rig_footprint with
+ * max_gaussians, or 1000 when that is 0).  SCPHD_ERR_CAPACITY when the
+ * budget or the slots are exhausted. */
+int scphd_batch_add_rig(scphd_batch_t* b, const scphd_params* p, int64_t reserve_bytes, int32_t* rig);
+/* Wait for the rig's frames in flight, drop its queued frames, free it. */
+int scphd_batch_remove_rig(scphd_batch_t* b, int32_t rig);
+
+/* Queue one frame of a rig: predict(dt), update(u, v, fixed_camera),
+ * checkStuff, resample -- one iteration of src/main.cpp:78-97.  u and v are
+ * copied; the call returns at once. */
+int scphd_batch_submit(scphd_batch_t* b, int32_t rig, int64_t frame_id, double dt, const double* u,
+                       const double* v, int32_t n_meas, int32_t fixed_camera);
+/* Block until every submitted frame has finished; *frames_done (may be NULL)
+ * receives the number finished since the previous call. */
+int scphd_batch_step(scphd_batch_t* b, int32_t* frames_done);
+/* Move up to cap finished-frame results into out (oldest first). */
+int scphd_batch_collect(scphd_batch_t* b, scphd_frame_result* out, int32_t cap, int32_t* n_out);
+int scphd_batch_get_stats(scphd_batch_t* b, scphd_batch_stats* s);
+/* The rig's filter handle, for read-back (scphd_get_parents, scphd_get_maps,
+ * ...) while none of its frames is queued or running; NULL for an unknown rig. */
+scphd_t* scphd_batch_filter(scphd_batch_t* b, int32_t rig);
+
 #ifdef __cplusplus
 }
 #endif

@@ -122,7 +122,7 @@ struct scphd_filter {
     int capM = 0;
     DBuf<int32_t> nsurv, survoff, nmerged, mergeoff, outcnt, scantmp, sg, sm, dflag;
     DBuf<int32_t> ngroups, glead, gsta, gend, gmem;
-    DBuf<double> sw, smu, mw, mmu, mP, dneff;
+    DBuf<double> sw, smu, mw, mmu, mP, dneff, est;
     DBuf<char> mscratch;
     int32_t* hbuf = nullptr; /* pinned read-back area */
     int64_t hbuf_n = 0;
@@ -263,6 +263,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
         record(f, 6);
         HIPTRY(launch_fit<T>(s, fa, f->gen[f->cur].planes<T>(), f->gen[nxt].planes<T>()));
         record(f, 7);
+        HIPTRY(launch_fit_fix<T>(s, fa, f->gen[f->cur].planes<T>(), f->gen[nxt].planes<T>()));
         HIPTRY(launch_precompute(s, f->fit.p, f->gw.p, Ng, P.std_u * P.std_u, P.std_v * P.std_v, f->pre.p,
                                  f->pdok.p));
     }
@@ -536,7 +537,7 @@ void scphd_destroy(scphd_t* f) {
     if (f->stream) (void)hipStreamSynchronize(f->stream);
     DBuf<double>* dbs[] = {&f->ext, &f->pw, &f->pose, &f->loglik, &f->cum, &f->ext_tmp, &f->gw, &f->gw2,
                            &f->fit, &f->pre, &f->du, &f->dv, &f->eta, &f->sw, &f->smu, &f->mw, &f->mmu, &f->mP,
-                           &f->dneff};
+                           &f->dneff, &f->est};
     for (auto* b : dbs) b->release();
     DBuf<int32_t>* ibs[] = {&f->anc, &f->newcnt, &f->goff, &f->gparent, &f->gblock, &f->goff2, &f->gparent2,
                             &f->gblock2, &f->pdok, &f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->nsurv, &f->survoff,
@@ -820,6 +821,38 @@ int scphd_set_timing(scphd_t* f, int32_t enabled) {
 int scphd_get_phase_ms(scphd_t* f, double* ms8) {
     if (check_handle(f) || !ms8) return SCPHD_ERR_ARG;
     for (int i = 0; i < 8; ++i) ms8[i] = f->phase_ms[i];
+    return SCPHD_OK;
+}
+
+int scphd_get_estimate(scphd_t* f, double* mean12, double* cov144, double* cardinality) {
+    if (check_handle(f)) return SCPHD_ERR_ARG;
+    HIPTRY(hipSetDevice(f->dev));
+    HIPTRY(f->est.ensure(157));
+    HIPTRY(launch_estimate(f->stream, f->ext.p, f->pw.p, f->goff.p, f->gw.p, f->Np, f->est.p));
+    double h[157];
+    HIPTRY(hipMemcpyAsync(h, f->est.p, sizeof h, hipMemcpyDeviceToHost, f->stream));
+    HIPTRY(hipStreamSynchronize(f->stream));
+    if (mean12) std::memcpy(mean12, h, 12 * sizeof(double));
+    if (cov144) std::memcpy(cov144, h + 12, 144 * sizeof(double));
+    if (cardinality) *cardinality = h[156];
+    return SCPHD_OK;
+}
+
+int scphd_get_memory(scphd_t* f, int64_t* bytes) {
+    if (check_handle(f) || !bytes) return SCPHD_ERR_ARG;
+    int64_t b = 0;
+    DBuf<double>* dbs[] = {&f->ext, &f->pw, &f->pose, &f->loglik, &f->cum, &f->ext_tmp, &f->gw, &f->gw2,
+                           &f->fit, &f->pre, &f->du, &f->dv, &f->eta, &f->sw, &f->smu, &f->mw, &f->mmu, &f->mP,
+                           &f->dneff, &f->est};
+    for (auto* d : dbs) b += d->n * (int64_t)sizeof(double);
+    DBuf<int32_t>* ibs[] = {&f->anc, &f->newcnt, &f->goff, &f->gparent, &f->gblock, &f->goff2, &f->gparent2,
+                            &f->gblock2, &f->pdok, &f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->nsurv, &f->survoff,
+                            &f->nmerged, &f->mergeoff, &f->outcnt, &f->scantmp, &f->sg, &f->sm, &f->dflag,
+                            &f->ngroups, &f->glead, &f->gsta, &f->gend, &f->gmem};
+    for (auto* d : ibs) b += d->n * (int64_t)sizeof(int32_t);
+    b += f->mscratch.n;
+    for (const auto& g : f->gen) b += g.cap_blocks * (int64_t)f->S * (int64_t)f->esz * 7;
+    *bytes = b;
     return SCPHD_OK;
 }
 

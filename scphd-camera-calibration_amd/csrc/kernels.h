@@ -90,8 +90,11 @@ struct FitArgs {
     int64_t nd_base; /* destination block of the non-detection cloud of entry g: nd_base + g */
 };
 
+/* k_fit: the streaming sweep; k_fit_fix: the rare cases it flags in redo[] */
 template <typename T>
 hipError_t launch_fit(hipStream_t s, const FitArgs& a, Planes<T> src, Planes<T> dst);
+template <typename T>
+hipError_t launch_fit_fix(hipStream_t s, const FitArgs& a, Planes<T> src, Planes<T> dst);
 
 hipError_t launch_precompute(hipStream_t s, const double* fit, const double* gw, int64_t Ng, double var_u,
                              double var_v, double* pre, int32_t* pdok);
@@ -163,6 +166,11 @@ int64_t merge_scratch_bytes(int64_t Stot, int Np);
 hipError_t launch_iota(hipStream_t s, int32_t* x, int64_t n);
 
 hipError_t launch_parent_weights(hipStream_t s, double* pw, const double* loglik, int Np);
+
+/* out[157]: weighted parent mean (12), covariance (144, row-major), summed
+ * Gaussian weight of the first highest-weight parent */
+hipError_t launch_estimate(hipStream_t s, const double* ext, const double* pw, const int32_t* goff,
+                           const double* gw, int Np, double* out);
 
 struct SampleArgs {
     const int32_t* mergeoff; /* Np+1 */

@@ -1154,6 +1154,100 @@ __global__ __launch_bounds__(1024) void k_parent_weights(double* pw, const doubl
     for (int p = threadIdx.x; p < Np; p += blockDim.x) pw[p] = exp(safe_log(pw[p]) + loglik[p] - mx) / s;
 }
 
+/* --------------------------------------------------------- estimate */
+/* Sum each of N per-thread values over a 256-thread block; every thread
+ * receives the totals (fixed order: wave sums, then waves ascending). */
+template <int N>
+__device__ inline void block_sum_n(double (&v)[N], double* sh) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const double t = wave_sum(v[i]);
+        if (lane == 0) sh[w * N + i] = t;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = ((sh[i] + sh[N + i]) + sh[2 * N + i]) + sh[3 * N + i];
+    __syncthreads();
+}
+
+/* One 256-thread workgroup: weighted mean and covariance of the parent
+ * extrinsics and the summed Gaussian weight of the first highest-weight
+ * parent's map (exp_traj, cov_traj and cardinality of
+ * load_disparity_results.m:27-34).  out: mean[12], cov[144] row-major,
+ * cardinality. */
+__global__ __launch_bounds__(256) void k_estimate(const double* ext, const double* pw, const int32_t* goff,
+                                                   const double* gw, int Np, double* out) {
+    __shared__ double sh[4 * 78];
+    __shared__ double bws[4];
+    __shared__ int bis[4];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    double s1[13];
+#pragma unroll
+    for (int c = 0; c < 13; ++c) s1[c] = 0.0;
+    double bw = -1.0;
+    int bi = 0x7fffffff;
+    for (int p = t; p < Np; p += 256) {
+        const double w = pw[p];
+        s1[12] += w;
+#pragma unroll
+        for (int c = 0; c < 12; ++c) s1[c] += w * ext[(int64_t)p * 12 + c];
+        if (w > bw) { /* p ascending within a thread: strict > keeps the first */
+            bw = w;
+            bi = p;
+        }
+    }
+    /* first maximum: larger weight, then smaller index */
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ow = __shfl_xor(bw, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ow > bw || (ow == bw && oi < bi)) {
+            bw = ow;
+            bi = oi;
+        }
+    }
+    if (lane == 0) {
+        bws[wv] = bw;
+        bis[wv] = bi;
+    }
+    block_sum_n<13>(s1, sh);
+    int best = bis[0];
+    double bestw = bws[0];
+    for (int k = 1; k < 4; ++k)
+        if (bws[k] > bestw || (bws[k] == bestw && bis[k] < best)) {
+            bestw = bws[k];
+            best = bis[k];
+        }
+    const double W = s1[12];
+    const double r = W > 0 ? 1.0 / W : 0.0;
+    double mu[12];
+#pragma unroll
+    for (int c = 0; c < 12; ++c) mu[c] = s1[c] * r;
+    /* covariance one row per pass (12 running sums per thread) */
+    for (int i = 0; i < 12; ++i) {
+        double row[12];
+#pragma unroll
+        for (int j = 0; j < 12; ++j) row[j] = 0.0;
+        for (int p = t; p < Np; p += 256) {
+            const double* x = ext + (int64_t)p * 12;
+            const double wdi = pw[p] * (x[i] - mu[i]);
+#pragma unroll
+            for (int j = 0; j < 12; ++j) row[j] += wdi * (x[j] - mu[j]);
+        }
+        block_sum_n<12>(row, sh);
+        if (t < 12) out[12 + i * 12 + t] = row[t] * r;
+    }
+    double card[1] = {0.0};
+    if (best < Np)
+        for (int64_t g = goff[best] + t; g < goff[best + 1]; g += 256) card[0] += gw[g];
+    block_sum_n<1>(card, sh);
+    if (t < 12) out[t] = mu[t];
+    if (t == 0) {
+        out[156] = card[0];
+    }
+}
+
 /* ---------------------------------------------------- posterior samples */
 /* One wavefront per merged Gaussian: S draws x = L z + mu with L the
  * Cholesky factor of the merged covariance (sample_disparity_gaussian,
@@ -1477,6 +1571,12 @@ template <typename T>
 hipError_t launch_fit(hipStream_t s, const FitArgs& a, Planes<T> src, Planes<T> dst) {
     if (a.Ng <= 0) return hipSuccess;
     k_fit<T><<<nblocks(a.Ng, 16), 256, 0, s>>>(a, src, dst);
+    LAUNCH_CHECK();
+}
+
+template <typename T>
+hipError_t launch_fit_fix(hipStream_t s, const FitArgs& a, Planes<T> src, Planes<T> dst) {
+    if (a.Ng <= 0) return hipSuccess;
     k_fit_fix<T><<<nblocks(a.Ng, 16), 256, 0, s>>>(a, src, dst);
     LAUNCH_CHECK();
 }
@@ -1518,6 +1618,13 @@ hipError_t launch_merge(hipStream_t s, const MergeArgs& a, int lds_bytes) {
 
 hipError_t launch_parent_weights(hipStream_t s, double* pw, const double* loglik, int Np) {
     k_parent_weights<<<1, 1024, 0, s>>>(pw, loglik, Np);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_estimate(hipStream_t s, const double* ext, const double* pw, const int32_t* goff,
+                           const double* gw, int Np, double* out) {
+    if (Np <= 0) return hipErrorInvalidValue;
+    k_estimate<<<1, 256, 0, s>>>(ext, pw, goff, gw, Np, out);
     LAUNCH_CHECK();
 }
 
@@ -1610,6 +1717,8 @@ template hipError_t launch_materialize<double>(hipStream_t, Planes<double>, Plan
                                                double, NoiseKey, uint64_t);
 template hipError_t launch_fit<float>(hipStream_t, const FitArgs&, Planes<float>, Planes<float>);
 template hipError_t launch_fit<double>(hipStream_t, const FitArgs&, Planes<double>, Planes<double>);
+template hipError_t launch_fit_fix<float>(hipStream_t, const FitArgs&, Planes<float>, Planes<float>);
+template hipError_t launch_fit_fix<double>(hipStream_t, const FitArgs&, Planes<double>, Planes<double>);
 template hipError_t launch_sample<float>(hipStream_t, const SampleArgs&, Planes<float>);
 template hipError_t launch_sample<double>(hipStream_t, const SampleArgs&, Planes<double>);
 template hipError_t launch_extract<float>(hipStream_t, Planes<float>, const int32_t*, int64_t, int, double*);

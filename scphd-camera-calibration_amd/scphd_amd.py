@@ -30,7 +30,11 @@ EXPORTED = [
     "scphd_predict", "scphd_update", "scphd_update_device", "scphd_resample", "scphd_check", "scphd_sync",
     "scphd_get_counts", "scphd_get_parents", "scphd_get_maps", "scphd_set_state", "scphd_get_update_stats",
     "scphd_get_update_debug", "scphd_get_ancestors", "scphd_set_timing", "scphd_get_phase_ms",
-    "scphd_get_fit_debug",
+    "scphd_get_fit_debug", "scphd_get_estimate", "scphd_get_memory",
+    "scphd_batch_opts_init", "scphd_batch_create", "scphd_batch_destroy", "scphd_batch_last_error",
+    "scphd_batch_now", "scphd_rig_footprint", "scphd_batch_add_rig", "scphd_batch_remove_rig",
+    "scphd_batch_submit", "scphd_batch_step", "scphd_batch_collect", "scphd_batch_get_stats",
+    "scphd_batch_filter",
 ]
 
 
@@ -88,6 +92,25 @@ class UpdateStats(ctypes.Structure):
                                     "cap_exceeded")]
 
 
+class BatchOpts(ctypes.Structure):
+    """scphd_batch_opts (include/scphd.h)."""
+
+    _fields_ = [("hbm_budget_bytes", _I64), ("max_rigs", _I32), ("n_workers", _I32)]
+
+
+class FrameResult(ctypes.Structure):
+    """scphd_frame_result (include/scphd.h)."""
+
+    _fields_ = [("rig", _I32), ("status", _I32), ("frame_id", _I64), ("t_submit", _D), ("t_start", _D),
+                ("t_end", _D), ("n_meas", _I64), ("n_pairs", _I64), ("n_gaussians", _I64), ("resampled", _I32),
+                ("reserved0", _I32), ("estimate", _D * 12), ("cardinality", _D)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [(n, _I64) for n in ("rigs", "reserved_bytes", "budget_bytes", "used_bytes", "queued_frames",
+                                    "frames_done", "pairs_done", "refused")]
+
+
 _lib = None
 
 
@@ -124,6 +147,21 @@ def lib():
         "scphd_set_timing": (_I32, [_P, _I32]),
         "scphd_get_phase_ms": (_I32, [_P, dp]),
         "scphd_get_fit_debug": (_I32, [_P, dp]),
+        "scphd_get_estimate": (_I32, [_P, dp, dp, dp]),
+        "scphd_get_memory": (_I32, [_P, ctypes.POINTER(_I64)]),
+        "scphd_batch_opts_init": (_I32, [ctypes.POINTER(BatchOpts)]),
+        "scphd_batch_create": (_I32, [ctypes.POINTER(BatchOpts), ctypes.POINTER(_P)]),
+        "scphd_batch_destroy": (None, [_P]),
+        "scphd_batch_last_error": (ctypes.c_char_p, [_P]),
+        "scphd_batch_now": (_D, [_P]),
+        "scphd_rig_footprint": (_I64, [pp, _I32]),
+        "scphd_batch_add_rig": (_I32, [_P, pp, _I64, ip]),
+        "scphd_batch_remove_rig": (_I32, [_P, _I32]),
+        "scphd_batch_submit": (_I32, [_P, _I32, _I64, _D, dp, dp, _I32, _I32]),
+        "scphd_batch_step": (_I32, [_P, ip]),
+        "scphd_batch_collect": (_I32, [_P, ctypes.POINTER(FrameResult), _I32, ip]),
+        "scphd_batch_get_stats": (_I32, [_P, ctypes.POINTER(BatchStats)]),
+        "scphd_batch_filter": (_P, [_P, _I32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -286,3 +324,131 @@ class Filter:
         out = np.zeros(8)
         self._check(lib().scphd_get_phase_ms(self.h, _dp(out)), "get_phase_ms")
         return out
+
+    def estimate(self):
+        """Weighted parent mean (12), covariance (12 x 12) and the highest-weight
+        parent's cardinality (scphd_get_estimate)."""
+        m = np.zeros(12)
+        c = np.zeros((12, 12))
+        card = np.zeros(1)
+        self._check(lib().scphd_get_estimate(self.h, _dp(m), _dp(c), _dp(card)), "get_estimate")
+        return m, c, float(card[0])
+
+    def memory(self):
+        b = _I64(0)
+        self._check(lib().scphd_get_memory(self.h, ctypes.byref(b)), "get_memory")
+        return b.value
+
+
+def rig_footprint(params, gaussians_per_parent):
+    """Device bytes a rig needs at about gaussians_per_parent Gaussians per parent."""
+    p = Params.from_dict(params) if isinstance(params, dict) else params
+    return lib().scphd_rig_footprint(ctypes.byref(p), int(gaussians_per_parent))
+
+
+class _Borrowed(Filter):
+    """A rig's filter handle owned by a Batch (read-back only; never destroyed here)."""
+
+    def __init__(self, h, params):
+        self.params = Params.from_dict(params) if isinstance(params, dict) else params
+        self.h = h
+        self.Np = self.params.n_particles
+        self.S = self.params.particles_per_feature
+
+    def close(self):
+        self.h = None
+
+
+class Batch:
+    """Many rigs on one GPU (scphd_batch_*): admission by HBM reservation,
+    per-rig frame queues, frames of different rigs overlapping on the device."""
+
+    def __init__(self, hbm_budget_bytes=0, max_rigs=0, n_workers=0):
+        L = lib()
+        o = BatchOpts()
+        L.scphd_batch_opts_init(ctypes.byref(o))
+        o.hbm_budget_bytes = int(hbm_budget_bytes)
+        if max_rigs:
+            o.max_rigs = int(max_rigs)
+        if n_workers:
+            o.n_workers = int(n_workers)
+        h = _P()
+        rc = L.scphd_batch_create(ctypes.byref(o), ctypes.byref(h))
+        if rc != SCPHD_OK:
+            raise ScphdError(rc, "scphd_batch_create failed")
+        self.h = h
+        self.params = {}
+
+    def close(self):
+        h = getattr(self, "h", None)
+        if h:
+            lib().scphd_batch_destroy(h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != SCPHD_OK:
+            msg = lib().scphd_batch_last_error(self.h)
+            raise ScphdError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def now(self):
+        return lib().scphd_batch_now(self.h)
+
+    def add_rig(self, params, reserve_bytes=0):
+        """Admit a rig; returns its id, or None when the HBM budget or the slots
+        are exhausted (offer it again later)."""
+        p = Params.from_dict(params) if isinstance(params, dict) else params
+        r = _I32(-1)
+        rc = lib().scphd_batch_add_rig(self.h, ctypes.byref(p), int(reserve_bytes), ctypes.byref(r))
+        if rc == -4:
+            return None
+        self._check(rc, "add_rig")
+        self.params[r.value] = p
+        return r.value
+
+    def remove_rig(self, rig):
+        self._check(lib().scphd_batch_remove_rig(self.h, int(rig)), "remove_rig")
+        self.params.pop(rig, None)
+
+    def submit(self, rig, frame_id, dt, u, v, fixed):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        if u.shape != v.shape:
+            raise ValueError("u and v differ in length")
+        self._check(lib().scphd_batch_submit(self.h, int(rig), int(frame_id), float(dt), _dp(u), _dp(v), len(u),
+                                             int(bool(fixed))), "submit")
+
+    def step(self):
+        """Wait for every submitted frame; returns the number finished."""
+        n = _I32(0)
+        self._check(lib().scphd_batch_step(self.h, ctypes.byref(n)), "step")
+        return n.value
+
+    def collect(self, cap=4096):
+        """Finished-frame results, oldest first, as a list of dicts."""
+        out = []
+        buf = (FrameResult * cap)()
+        while True:
+            n = _I32(0)
+            self._check(lib().scphd_batch_collect(self.h, buf, cap, ctypes.byref(n)), "collect")
+            for i in range(n.value):
+                r = buf[i]
+                d = {name: getattr(r, name) for name, _ in FrameResult._fields_ if name != "reserved0"}
+                d["estimate"] = np.array(list(r.estimate))
+                out.append(d)
+            if n.value < cap:
+                return out
+
+    def stats(self):
+        s = BatchStats()
+        self._check(lib().scphd_batch_get_stats(self.h, ctypes.byref(s)), "get_stats")
+        return {n: getattr(s, n) for n, _ in BatchStats._fields_}
+
+    def filter(self, rig):
+        """The rig's filter for read-back while none of its frames is queued or running."""
+        h = lib().scphd_batch_filter(self.h, int(rig))
+        if not h:
+            raise ScphdError(-1, f"unknown rig {rig}")
+        return _Borrowed(_P(h), self.params[rig])

@@ -1,0 +1,164 @@
+"""Served calibration: rigs arrive over time, are admitted by HBM reservation,
+stream frames at a fixed rate, and share one GPU through the batch API
+(scphd_batch_* in include/scphd.h).
+
+run_served() replays an arrival list in real time:
+  * an arriving rig waits in a FIFO until its reservation fits the HBM budget
+    (scphd_batch_add_rig); the wait is its admission delay;
+  * once admitted, its frame k is due at t_admit + k / fps and is submitted
+    when due (frames that come due while earlier ones are still queued or on
+    the GPU wait in the rig's queue: that wait is part of their latency);
+  * a rig whose frames are all done leaves (scphd_batch_remove_rig) and frees
+    its reservation for the rigs waiting to be admitted.
+Per-frame latency = completion time - due time.  Throughput = component
+updates (N_g * M per frame) / wall time.
+
+run_plain() runs the same rigs through the plain path: one rig at a time, its
+frames back to back on one Filter, for the throughput and per-frame latency
+of one rig per launch.
+
+Neither loads anything but the library; scene generation happens before the
+clock starts.
+"""
+import math
+import time
+
+import numpy as np
+
+import scene
+import scphd_amd
+
+
+def expected_gaussians_per_parent(s):
+    """Steady-state map size estimate: each visible landmark (about 90 % of
+    the box seen by a 640 x 480, f = 500 camera) carries one detection term
+    plus 2-3 non-detection clouds, and clutter births survive about one step:
+    3.5 * 0.95 * 0.9 * L + lambda."""
+    return int(3.5 * 0.95 * 0.9 * s["n_landmarks"] + s["lam"]) + 16
+
+
+def prepare(arrivals):
+    """Draw every rig's measurement sets (before any timing)."""
+    rigs = []
+    for t, p, s in arrivals:
+        sc = scene.generate(seed=p["seed"], **s)
+        frames = [scene.frame_measurements(sc, k) for k in range(s["n_frames"])]
+        rigs.append(dict(t=t, p=p, s=s, frames=frames))
+    return rigs
+
+
+def _pct(x, q):
+    return float(np.percentile(x, q)) if len(x) else None
+
+
+def run_served(rigs, hbm_budget_bytes=0, n_workers=8, reserve_factor=1.5, progress=None, max_wall=None):
+    """Replay prepared rigs (see prepare) in real time; returns a summary dict."""
+    b = scphd_amd.Batch(hbm_budget_bytes=hbm_budget_bytes, n_workers=n_workers)
+    try:
+        waiting = []          # indices of arrived, not yet admitted rigs (FIFO)
+        active = {}           # rig id -> state
+        nxt = 0               # next arrival
+        lat, svc, adm, pairs = [], [], [], 0
+        done_rigs, peak_res, peak_used, peak_active, n_frames = 0, 0, 0, 0, 0
+        t0 = b.now()
+        last_print = t0
+        while nxt < len(rigs) or waiting or active:
+            now = b.now() - t0
+            if max_wall is not None and now > max_wall:
+                break
+            while nxt < len(rigs) and rigs[nxt]["t"] <= now:
+                waiting.append(nxt)
+                nxt += 1
+            while waiting:
+                r = rigs[waiting[0]]
+                res = int(reserve_factor * scphd_amd.rig_footprint(r["p"], expected_gaussians_per_parent(r["s"])))
+                rid = b.add_rig(r["p"], res)
+                if rid is None:
+                    break
+                waiting.pop(0)
+                adm.append(now - r["t"])
+                active[rid] = dict(r=r, t_admit=now, sent=0, done=0, due={})
+            for rid, a in active.items():
+                fps_dt = a["r"]["s"]["dt"]
+                fr = a["r"]["frames"]
+                while a["sent"] < len(fr) and a["t_admit"] + a["sent"] * fps_dt <= now:
+                    k = a["sent"]
+                    u, v, fixed = fr[k]
+                    a["due"][k] = t0 + a["t_admit"] + k * fps_dt
+                    b.submit(rid, k, fps_dt, u, v, fixed)
+                    a["sent"] += 1
+            for x in b.collect():
+                a = active[x["rig"]]
+                if x["status"] != 0:
+                    raise RuntimeError(f"rig {x['rig']} frame {x['frame_id']} failed: status {x['status']}")
+                lat.append(x["t_end"] - a["due"].pop(x["frame_id"]))
+                svc.append(x["t_end"] - x["t_start"])
+                pairs += x["n_pairs"]
+                n_frames += 1
+                a["done"] += 1
+            for rid in [rid for rid, a in active.items() if a["done"] == len(a["r"]["frames"])]:
+                b.remove_rig(rid)
+                del active[rid]
+                done_rigs += 1
+            st = b.stats()
+            peak_res = max(peak_res, st["reserved_bytes"])
+            peak_used = max(peak_used, st["used_bytes"])
+            peak_active = max(peak_active, len(active))
+            if progress and now - (last_print - t0) >= 5.0:
+                last_print = b.now()
+                progress(f"served t={now:.1f}s rigs done {done_rigs}/{len(rigs)} active {len(active)} "
+                         f"waiting {len(waiting)} frames {n_frames} queued {st['queued_frames']}")
+            # next event: a due frame, an arrival, or a completion
+            time.sleep(0.0005)
+        wall = b.now() - t0
+        st = b.stats()
+    finally:
+        b.close()
+    lat = np.array(lat) * 1e3
+    svc = np.array(svc) * 1e3
+    adm = np.array(adm)
+    return dict(rigs=len(rigs), rigs_done=done_rigs, frames=n_frames, wall_s=wall,
+                updates_per_s=pairs / wall if wall > 0 else 0.0, pairs=pairs,
+                latency_ms=dict(p50=_pct(lat, 50), p99=_pct(lat, 99), max=float(lat.max()) if len(lat) else None),
+                service_ms=dict(p50=_pct(svc, 50), p99=_pct(svc, 99)),
+                admission_delay_s=dict(p50=_pct(adm, 50), p99=_pct(adm, 99), max=float(adm.max()) if len(adm) else None),
+                peak_active_rigs=peak_active, peak_reserved_gb=peak_res / 1e9, peak_used_gb=peak_used / 1e9,
+                refused_admissions=st["refused"], budget_gb=st["budget_bytes"] / 1e9, n_workers=n_workers)
+
+
+def run_plain(rigs, progress=None):
+    """The same rigs one at a time through one Filter each, frames back to back."""
+    lat, pairs, t_all = [], 0, 0.0
+    for i, r in enumerate(rigs):
+        f = scphd_amd.Filter(r["p"])
+        dt = r["s"]["dt"]
+        for u, v, fixed in r["frames"]:
+            t0 = time.perf_counter()
+            f.predict(dt)
+            f.update(u, v, fixed)
+            st = f.update_stats()
+            if f.check() != 0:
+                raise RuntimeError("nan weight")
+            f.resample()
+            f.estimate()
+            t1 = time.perf_counter()
+            lat.append(t1 - t0)
+            pairs += st["n_pairs"]
+            t_all += t1 - t0
+        f.close()
+        if progress and (i + 1) % 10 == 0:
+            progress(f"plain rigs {i + 1}/{len(rigs)}")
+    lat = np.array(lat) * 1e3
+    return dict(rigs=len(rigs), frames=len(lat), wall_s=t_all, updates_per_s=pairs / t_all if t_all > 0 else 0.0,
+                pairs=pairs, latency_ms=dict(p50=_pct(lat, 50), p99=_pct(lat, 99), max=float(lat.max())))
+
+
+def offered_load(rigs, step_ms_per_gaussian_sample):
+    """Rough device time the arrivals ask for per second of wall time, from a
+    measured cost per (Gaussian x sample) of one step."""
+    if not rigs:
+        return 0.0
+    horizon = max(r["t"] + len(r["frames"]) * r["s"]["dt"] for r in rigs)
+    work = sum(len(r["frames"]) * r["p"]["n_particles"] * expected_gaussians_per_parent(r["s"]) *
+               r["p"]["particles_per_feature"] for r in rigs)
+    return work * step_ms_per_gaussian_sample * 1e-3 / horizon if horizon > 0 else math.inf

@@ -85,6 +85,63 @@ def workload(name, **overrides):
     return p, s
 
 
+def rig(seed, n_particles, n_landmarks, lam, f=500.0, baseline=0.2, samples=100, max_gaussians=0,
+        n_frames=100, dt=0.5, scene_rig=0):
+    """One rig of a batched or served workload: filter parameters keyed by
+    `seed` and the settings of its synthetic scene (scene.generate with
+    seed=seed, rig=scene_rig)."""
+    p = base_params(n_particles, samples, max_gaussians, lam, f=f, baseline=baseline, seed=seed)
+    s = dict(n_landmarks=n_landmarks, lam=float(lam), f=float(f), baseline=float(baseline), n_frames=n_frames,
+             dt=dt, rig=scene_rig)
+    return p, s
+
+
+def _splitmix64(master, index):
+    x = (master + 0x9E3779B97F4A7C15 * (index + 1)) & 0xFFFFFFFFFFFFFFFF
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return x ^ (x >> 31)
+
+
+def c2_rigs(master=1, n_rigs=64, samples=100, n_frames=100):
+    """Batched ragged workload: n_rigs independent rigs, each with parents
+    uniform in {256, 512, 1024, 2048, 4096}, landmarks uniform in [20, 300],
+    clutter lambda uniform in [5, 80], baseline uniform in [0.1, 0.5] m and
+    f uniform in [400, 800] px; rig r's filter seed and scene draw come from
+    SplitMix64(master, r).  S (unspecified by the workload) = `samples`."""
+    out = []
+    for r in range(n_rigs):
+        sd = _splitmix64(master, r)
+        u = [((_splitmix64(sd, j) >> 11) * 2.0 ** -53) for j in range(5)]
+        n_p = [256, 512, 1024, 2048, 4096][min(int(u[0] * 5), 4)]
+        out.append(rig(seed=sd, n_particles=n_p, n_landmarks=20 + int(u[1] * 281), lam=5.0 + 75.0 * u[2],
+                       baseline=0.1 + 0.4 * u[3], f=400.0 + 400.0 * u[4], samples=samples, n_frames=n_frames,
+                       scene_rig=r))
+    return out
+
+
+def c4_arrivals(master=1, rate=20.0, horizon=30.0, samples=100, fps=30.0, dur=(2.0, 10.0), parents=(256, 8192),
+                landmarks=(20, 400), lam=(5.0, 100.0)):
+    """Served workload: rigs arriving as a Poisson process of `rate` rigs/s
+    over `horizon` seconds; parents log-uniform in `parents`, landmarks
+    uniform in `landmarks`, clutter lambda uniform in `lam`, frames at `fps`
+    for a duration uniform in `dur`.  Returns [(arrival_s, params, scene)]."""
+    out, t, r = [], 0.0, 0
+    while True:
+        sd = _splitmix64(master, r)
+        u = [((_splitmix64(sd, j) >> 11) * 2.0 ** -53) for j in range(6)]
+        t += -math.log(1.0 - u[0]) / rate
+        if t > horizon:
+            return out
+        n_p = int(round(math.exp(math.log(parents[0]) + u[1] * (math.log(parents[1]) - math.log(parents[0])))))
+        n_frames = max(1, int((dur[0] + (dur[1] - dur[0]) * u[2]) * fps))
+        p, s = rig(seed=sd, n_particles=n_p, n_landmarks=landmarks[0] + int(u[3] * (landmarks[1] - landmarks[0] + 1)),
+                   lam=lam[0] + (lam[1] - lam[0]) * u[4], samples=samples, n_frames=n_frames, dt=1.0 / fps,
+                   scene_rig=r)
+        out.append((t, p, s))
+        r += 1
+
+
 def to_cfg(p):
     """Render parameters as a libconfig file in the schema the reference code
     reads (prior.{cartesian,angular}.{x..vz}.{a,b})."""

@@ -19,7 +19,7 @@ GOLDEN = os.path.join(REPO, "tests", "golden")
 
 def declared():
     src = open(HEADER).read()
-    return sorted(set(re.findall(r"^[a-z_ ]*\**\s*(scphd_\w+)\s*\(", src, flags=re.M)))
+    return sorted(set(re.findall(r"^[a-z0-9_ ]*\**\s*(scphd_\w+)\s*\(", src, flags=re.M)))
 
 
 def test_every_declared_symbol_is_exported():
@@ -114,3 +114,45 @@ def test_create_rejects_bad_arguments():
         q[k] = v
         with pytest.raises(scphd_amd.ScphdError, match="ARG"):
             scphd_amd.Filter(q)
+
+
+# ------------------------------------------------------------ batch / served
+def test_batch_options_and_argument_checks():
+    o = scphd_amd.BatchOpts()
+    assert scphd_amd.lib().scphd_batch_opts_init(ctypes.byref(o)) == 0
+    assert (o.hbm_budget_bytes, o.max_rigs, o.n_workers) == (0, 1024, 8)
+    bad = scphd_amd.BatchOpts(-1, 0, 0)
+    h = ctypes.c_void_p()
+    assert scphd_amd.lib().scphd_batch_create(ctypes.byref(bad), ctypes.byref(h)) == -1
+    bad = scphd_amd.BatchOpts(1 << 30, 0, 65)
+    assert scphd_amd.lib().scphd_batch_create(ctypes.byref(bad), ctypes.byref(h)) == -1
+
+
+def test_rig_footprint_scales_with_the_state():
+    p, _ = workloads.workload("c0")
+    base = scphd_amd.rig_footprint(p, 100)
+    assert scphd_amd.rig_footprint(dict(p, n_particles=2 * p["n_particles"]), 100) > 1.9 * base - (128 << 20)
+    assert scphd_amd.rig_footprint(p, 200) > base
+    assert scphd_amd.rig_footprint(dict(p, particles_per_feature=2 * p["particles_per_feature"]), 100) > base
+    assert scphd_amd.rig_footprint(dict(p, storage=1), 100) > base
+    # both sample generations at FP32: 2 * N_p * G * S * 28 B is a lower bound
+    assert base >= 2 * p["n_particles"] * 100 * p["particles_per_feature"] * 28
+    assert scphd_amd.rig_footprint(dict(p, n_particles=0), 100) == 0
+
+
+def test_batch_admission_is_refused_beyond_the_budget():
+    """Runs without a GPU: a reservation above the budget is refused before any
+    device work, and the refusal is counted."""
+    p, _ = workloads.workload("c0")
+    need = scphd_amd.rig_footprint(p, 100)
+    b = scphd_amd.Batch(hbm_budget_bytes=need - 1, max_rigs=4, n_workers=2)
+    try:
+        assert b.add_rig(p, reserve_bytes=need) is None
+        s = b.stats()
+        assert s["refused"] == 1 and s["rigs"] == 0 and s["reserved_bytes"] == 0 and s["budget_bytes"] == need - 1
+        with pytest.raises(scphd_amd.ScphdError, match="ARG"):
+            b.submit(0, 0, 0.5, [1.0], [2.0], True)
+        assert b.step() == 0
+        assert b.collect() == []
+    finally:
+        b.close()

@@ -1,0 +1,135 @@
+"""Batched / served operation (scphd_batch_*) on the GPU.
+
+* A rig's results are the same alone and in a mix: each rig is run alone
+  through a plain Filter, then together with other rigs through one Batch
+  with its frames interleaved with theirs; parents, weights, maps and the
+  per-frame estimates must be bit-identical.
+* Admission by HBM reservation, removal, and re-admission.
+* The device-side estimate (scphd_get_estimate) against numpy on the
+  parents read back.
+"""
+import numpy as np
+import pytest
+
+import scene
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+RIGS = [dict(seed=11, n_particles=64, n_landmarks=25, lam=8.0, f=500.0, baseline=0.2, samples=64),
+        dict(seed=12, n_particles=96, n_landmarks=40, lam=15.0, f=650.0, baseline=0.3, samples=50),
+        dict(seed=13, n_particles=48, n_landmarks=15, lam=4.0, f=420.0, baseline=0.15, samples=80)]
+FRAMES = 6
+
+
+def make(i, frames=FRAMES):
+    p, s = workloads.rig(n_frames=frames, scene_rig=i, **RIGS[i])
+    sc = scene.generate(seed=p["seed"], **s)
+    return p, s, [scene.frame_measurements(sc, k) for k in range(frames)]
+
+
+def alone(i):
+    import scphd_amd
+
+    p, s, sets = make(i)
+    f = scphd_amd.Filter(p)
+    est = []
+    for u, v, fixed in sets:
+        f.predict(s["dt"])
+        f.update(u, v, fixed)
+        assert f.check() == 0
+        f.resample()
+        est.append(f.estimate()[0])
+    out = f.state(), np.array(est)
+    f.close()
+    return out
+
+
+def test_rig_results_are_identical_alone_and_in_a_mix():
+    import scphd_amd
+
+    ref = [alone(i) for i in range(len(RIGS))]
+    b = scphd_amd.Batch(n_workers=3)
+    try:
+        rigs, sets = [], []
+        for i in range(len(RIGS)):
+            p, s, fs = make(i)
+            rigs.append(b.add_rig(p))
+            sets.append((s, fs))
+        assert None not in rigs
+        # interleave: frame k of every rig before frame k+1 of any
+        for k in range(FRAMES):
+            for r, (s, fs) in zip(rigs, sets):
+                u, v, fixed = fs[k]
+                b.submit(r, k, s["dt"], u, v, fixed)
+        assert b.step() == FRAMES * len(RIGS)
+        res = b.collect()
+        assert len(res) == FRAMES * len(RIGS)
+        assert all(x["status"] == 0 for x in res)
+        for i, r in enumerate(rigs):
+            mine = sorted((x for x in res if x["rig"] == r), key=lambda x: x["frame_id"])
+            assert [x["frame_id"] for x in mine] == list(range(FRAMES))
+            # a rig's frames run in order
+            assert all(mine[j]["t_start"] >= mine[j - 1]["t_end"] for j in range(1, FRAMES))
+            np.testing.assert_array_equal(np.array([x["estimate"] for x in mine]), ref[i][1])
+            st = b.filter(r).state()
+            for key in ("states", "weights", "gcount", "wg", "samples"):
+                np.testing.assert_array_equal(st[key], ref[i][0][key], err_msg=f"rig {i} {key}")
+        s = b.stats()
+        assert s["frames_done"] == FRAMES * len(RIGS) and s["queued_frames"] == 0 and s["rigs"] == len(RIGS)
+        assert s["pairs_done"] == sum(x["n_pairs"] for x in res) > 0
+        assert 0 < s["used_bytes"] <= s["reserved_bytes"]
+    finally:
+        b.close()
+
+
+def test_admission_removal_and_readmission():
+    import scphd_amd
+
+    p0, _, _ = make(0)
+    need = scphd_amd.rig_footprint(p0, 200)
+    b = scphd_amd.Batch(hbm_budget_bytes=2 * need + need // 2, n_workers=2)
+    try:
+        r0 = b.add_rig(p0, need)
+        r1 = b.add_rig(make(1)[0], need)
+        assert r0 is not None and r1 is not None
+        assert b.add_rig(make(2)[0], need) is None
+        assert b.stats()["refused"] == 1
+        b.remove_rig(r0)
+        r2 = b.add_rig(make(2)[0], need)
+        assert r2 is not None
+        s = b.stats()
+        assert s["rigs"] == 2 and s["reserved_bytes"] == 2 * need
+        # the re-admitted rig runs
+        p2, s2, fs = make(2)
+        for k, (u, v, fixed) in enumerate(fs[:2]):
+            b.submit(r2, k, s2["dt"], u, v, fixed)
+        assert b.step() == 2
+        assert [x["status"] for x in b.collect()] == [0, 0]
+        with pytest.raises(scphd_amd.ScphdError):
+            b.submit(r0, 0, 0.5, [1.0], [1.0], True)
+    finally:
+        b.close()
+
+
+def test_estimate_matches_parents_read_back():
+    import scphd_amd
+
+    p, s, sets = make(1)
+    f = scphd_amd.Filter(p)
+    for u, v, fixed in sets[:4]:
+        f.predict(s["dt"])
+        f.update(u, v, fixed)
+        f.resample()
+    m, c, card = f.estimate()
+    st, w = f.parents()
+    wn = w / w.sum()
+    mu = wn @ st
+    d = st - mu
+    np.testing.assert_allclose(m, mu, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(c, (d * wn[:, None]).T @ d, rtol=1e-10, atol=1e-18)
+    gc, wg, _ = f.maps()
+    best = int(np.argmax(w))
+    off = np.concatenate([[0], np.cumsum(gc)])
+    assert abs(card - wg[off[best]:off[best + 1]].sum()) <= 1e-12 * max(1.0, card)
+    f.close()
