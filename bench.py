@@ -97,6 +97,49 @@ def cpu_baseline(p, s, parents, warm, timed, threads):
     return upd / sec if sec > 0 else 0.0, sec, orc.counts()["n_gaussians"] / parents
 
 
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "k_fit_c1_traffic.txt")
+
+
+def measured_traffic():
+    """HBM bytes per k_fit launch from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel factor), or None."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    vals = {}
+    for line in open(TRAFFIC_FILE):
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return float(vals["hbm_bytes_per_launch"]), float(vals["algorithmic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def served_leg(args, seed):
+    """Served calibration (scaled version of the 20 rigs/s workload): rigs
+    arrive as a Poisson process, are admitted by HBM reservation, stream
+    30 Hz frames; compared with the plain path running the first rigs one at
+    a time."""
+    import serve  # noqa: E402
+
+    arr = workloads.c4_arrivals(master=seed, rate=args.served_rate, horizon=args.served_horizon,
+                                parents=tuple(args.served_parents), landmarks=tuple(args.served_landmarks),
+                                lam=tuple(args.served_lambda), dur=tuple(args.served_duration))
+    rigs = serve.prepare(arr)
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    sv = serve.run_served(rigs, n_workers=args.served_workers, progress=log)
+    pl = serve.run_plain(rigs[:args.served_plain_rigs], progress=log)
+    return {
+        "workload": f"Poisson {args.served_rate:g} rigs/s over {args.served_horizon:g} s, parents log-uniform "
+                    f"{args.served_parents}, landmarks {args.served_landmarks}, lambda {args.served_lambda}, "
+                    f"30 Hz for {args.served_duration} s, S = 100, FP32 storage",
+        "served": sv,
+        "plain_one_rig_at_a_time": pl,
+        "throughput_ratio_served_over_plain": sv["updates_per_s"] / pl["updates_per_s"] if pl["updates_per_s"] else None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +151,15 @@ def main():
     ap.add_argument("--cpu-warm", type=int, default=12)
     ap.add_argument("--cpu-timed", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--served-rate", type=float, default=2.0, help="rig arrivals per second")
+    ap.add_argument("--served-horizon", type=float, default=12.0, help="seconds of arrivals")
+    ap.add_argument("--served-parents", type=int, nargs=2, default=[256, 2048])
+    ap.add_argument("--served-landmarks", type=int, nargs=2, default=[20, 200])
+    ap.add_argument("--served-lambda", type=float, nargs=2, default=[5.0, 60.0])
+    ap.add_argument("--served-duration", type=float, nargs=2, default=[2.0, 6.0])
+    ap.add_argument("--served-workers", type=int, default=8)
+    ap.add_argument("--served-plain-rigs", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X and the filter step "
@@ -215,11 +267,16 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": fit_gbs / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": (measured_traffic() or (None,))[0],
+            "traffic_note": "HBM bytes per k_fit launch measured by rocprofv3 FETCH_SIZE + WRITE_SIZE (copy-kernel "
+                            "scaled) at the same workload, profiles/k_fit_c1_traffic.txt; algorithmic bytes per "
+                            f"launch there: {(measured_traffic() or (None, None))[1]}",
             "algorithmic_bytes": f"7*{esz} B per input sample read + 7*{esz} B per kept non-detection sample written",
             "k_sample_achieved_gbs": smp_gbs,
         },
     }
+    if not args.no_served:
+        out["served"] = served_leg(args, p["seed"])
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cpu_v, cpu_s, cpu_gp = cpu_baseline(p, s, args.cpu_parents, args.cpu_warm, args.cpu_timed, threads)

@@ -219,6 +219,23 @@ int scphd_get_estimate(scphd_t* f, double* mean12, double* cov144, double* cardi
  * reference. */
 int scphd_get_memory(scphd_t* f, int64_t* bytes);
 
+/* Read a measurement sequence from a MAT-file (level 5): variable varname
+ * is a cell array whose cell k is a 2 x n_k double matrix (rows u, v).
+ * First call with counts == NULL: sets *n_steps and *n_total.  Second call
+ * with *n_steps / *n_total holding the capacities of counts[n_steps] and
+ * u, v[n_total]: fills them (frame k's measurements follow frame k-1's).
+ * SCPHD_ERR_CONFIG for a missing variable or a malformed file.  Replaces
+ * readMatfile, src/main.cpp:19-49. */
+int scphd_mat_read_measurements(const char* path, const char* varname, int32_t* n_steps, int64_t* n_total,
+                                int32_t* counts, double* u, double* v);
+
+/* Write the parents and the highest-weight parent's map to a MAT-file
+ * (level 5): particles (12 x N_p), weights (1 x N_p), map_weights (1 x G),
+ * map_particles (6 x G*S), map_particle_weights (1 x G*S).  Replaces
+ * writeMat, src/scphdcameracalibration.cu:2200-2373 (which writes MAT 7.3
+ * through HDF5). */
+int scphd_write_mat(scphd_t* f, const char* path);
+
 /* ======================================================== batched / served
  * Many rigs (independent calibration instances, each with its own
  * parameters, parent count and measurement stream) share one GPU.  Each
@@ -285,9 +302,15 @@ int scphd_batch_add_rig(scphd_batch_t* b, const scphd_params* p, int64_t reserve
 /* Wait for the rig's frames in flight, drop its queued frames, free it. */
 int scphd_batch_remove_rig(scphd_batch_t* b, int32_t rig);
 
+/* Write <dir>/<frame_id>.mat (scphd_write_mat) after each of the rig's
+ * frames, as the reference main writes k.mat (src/main.cpp:91-93); NULL or
+ * "" stops it.  Only while none of the rig's frames is queued or running. */
+int scphd_batch_set_output(scphd_batch_t* b, int32_t rig, const char* dir);
+
 /* Queue one frame of a rig: predict(dt), update(u, v, fixed_camera),
- * checkStuff, resample -- one iteration of src/main.cpp:78-97.  u and v are
- * copied; the call returns at once. */
+ * writeMat (when an output directory is set), checkStuff, resample -- one
+ * iteration of src/main.cpp:78-97.  u and v are copied; the call returns at
+ * once. */
 int scphd_batch_submit(scphd_batch_t* b, int32_t rig, int64_t frame_id, double dt, const double* u,
                        const double* v, int32_t n_meas, int32_t fixed_camera);
 /* Block until every submitted frame has finished; *frames_done (may be NULL)

@@ -8,7 +8,8 @@
  * in a per-rig FIFO.  A pool of worker threads takes frames, oldest
  * submission first, from rigs that have no frame in flight, and runs one
  * iteration of the reference main loop (src/main.cpp:78-97: predict,
- * update, checkStuff, resample) plus the calibration estimate.  One frame
+ * update, writeMat when an output directory is set, checkStuff,
+ * resample) plus the calibration estimate.  One frame
  * per rig is in flight at a time, so a rig's frames keep their order, while
  * the frames of different rigs overlap on the device (up to n_workers at
  * once).  A rig's computation never depends on the other rigs: its noise is
@@ -52,6 +53,7 @@ struct Rig {
     bool busy = false;     /* a frame is running (or the handle is being created) */
     bool removing = false; /* scphd_batch_remove_rig is waiting for it */
     int64_t mem = 0;       /* device bytes held, refreshed after each frame */
+    std::string outdir;    /* when set: <outdir>/<frame_id>.mat after each frame */
 };
 
 using Clock = std::chrono::steady_clock;
@@ -86,8 +88,10 @@ void run_frame(Rig* rig, const Frame& fr, scphd_frame_result& r) {
         rc = scphd_update(f, fr.u.data(), fr.v.data(), (int32_t)fr.u.size(), fr.fixed);
     scphd_update_stats st{};
     if (rc == SCPHD_OK) rc = scphd_get_update_stats(f, &st);
-    if (rc == SCPHD_OK) rc = scphd_check(f);
     int32_t resampled = 0;
+    if (rc == SCPHD_OK && !rig->outdir.empty())
+        rc = scphd_write_mat(f, (rig->outdir + "/" + std::to_string(fr.id) + ".mat").c_str());
+    if (rc == SCPHD_OK) rc = scphd_check(f);
     if (rc == SCPHD_OK) rc = scphd_resample(f, &resampled);
     if (rc == SCPHD_OK) rc = scphd_get_estimate(f, r.estimate, nullptr, &r.cardinality);
     scphd_counts c{};
@@ -284,6 +288,18 @@ int scphd_batch_remove_rig(scphd_batch_t* b, int32_t id) {
         b->cv_idle.notify_all();
     }
     scphd_destroy(gone->f);
+    return SCPHD_OK;
+}
+
+int scphd_batch_set_output(scphd_batch_t* b, int32_t id, const char* dir) {
+    if (!b) return SCPHD_ERR_ARG;
+    std::lock_guard<std::mutex> lk(b->mu);
+    Rig* r = find(b, id);
+    if (!r || !r->f || r->removing || r->busy || !r->q.empty()) {
+        b->err = "unknown rig, or frames of it are queued or running";
+        return SCPHD_ERR_ARG;
+    }
+    r->outdir = dir ? dir : "";
     return SCPHD_OK;
 }
 

@@ -254,6 +254,50 @@ SCPHD_HD inline double mahal6(const double* ma, const double* Pa, const double* 
     return dist;
 }
 
+/* 1/x within about one ulp: the hardware reciprocal estimate refined by two
+ * Newton steps (finite, normal x). */
+__device__ inline double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
+/* DisparityMeasurementModel::computeMeasurement of a world point seen by
+ * camera c, with the three quotients fx/z, fy/z, 1/z^2 taken from one
+ * reciprocal 1/z (within a few ulps of the divisions). */
+__device__ inline DisparityPoint disparity_fast(const DisparityMeasurementModel& m, const CameraPose& c,
+                                                const EuclideanPoint& pw) {
+    const EuclideanPoint p = c.worldToCamera(pw);
+    const double iz = rcp_nr(p.z);
+    const double fz = m.fx() * iz, gz = m.fy() * iz, iz2 = iz * iz;
+    DisparityPoint q;
+    q.u = m.u0() - fz * p.x;
+    q.v = m.v0() - gz * p.y;
+    q.d = -fz;
+    q.vu = -m.fx() * (p.vx * p.z - p.vz * p.x) * iz2;
+    q.vv = -m.fy() * (p.vy * p.z - p.vz * p.y) * iz2;
+    q.vd = m.fx() * p.vz * iz2;
+    return q;
+}
+
+/* DisparityMeasurementModel::invertMeasurement into the world frame of
+ * camera c, with 1/d and 1/d^2 taken from one reciprocal. */
+__device__ inline EuclideanPoint world_from_disparity_fast(const DisparityMeasurementModel& m, const CameraPose& c,
+                                                           const DisparityPoint& q) {
+    const double id = rcp_nr(q.d), id2 = id * id, r = m.fx() / m.fy();
+    const double du = q.u - m.u0(), dv = q.v - m.v0();
+    EuclideanPoint e;
+    e.x = du * id;
+    e.y = r * dv * id;
+    e.z = -m.fx() * id;
+    e.vx = (q.vu * q.d - q.vd * du) * id2;
+    e.vy = r * (q.vv * q.d - q.vd * dv) * id2;
+    e.vz = m.fx() * q.vd * id2;
+    return c.cameraToWorld(e);
+}
+
 /* Measurement-independent part of the 2-D Kalman detection update with
  * H = [I2 0], R = diag(var_u, var_v) (computeDetectionsKernel,
  * src/kernels.cu:492-595): symmetrised innovation covariance and its

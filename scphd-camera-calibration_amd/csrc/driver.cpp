@@ -89,37 +89,7 @@ void SCPHDCameraCalibration::setMeasurement_model(const DisparityMeasurementMode
 }
 
 void SCPHDCameraCalibration::writeMat(const char* filename) {
-    const int Np = params_.n_particles, S = params_.particles_per_feature;
-    scphd_counts c;
-    check(scphd_get_counts(h_, &c), "get_counts");
-    std::vector<double> st(12 * (size_t)Np), w(Np);
-    check(scphd_get_parents(h_, st.data(), w.data()), "get_parents");
-    std::vector<int32_t> gc(Np);
-    std::vector<double> wg((size_t)c.n_gaussians), smp((size_t)c.n_samples * 7);
-    check(scphd_get_maps(h_, gc.data(), wg.data(), smp.data()), "get_maps");
-    int best = 0;
-    for (int p = 1; p < Np; ++p)
-        if (w[p] > w[best]) best = p;
-    int64_t g0 = 0;
-    for (int p = 0; p < best; ++p) g0 += gc[p];
-    const int64_t G = gc[best], n = c.n_samples;
-    std::vector<double> particles(12 * (size_t)Np);
-    for (int p = 0; p < Np; ++p)
-        for (int k = 0; k < 12; ++k) particles[(size_t)p * 12 + k] = st[(size_t)p * 12 + k]; /* 12 x Np col-major */
-    std::vector<double> mw(wg.begin() + g0, wg.begin() + g0 + G);
-    std::vector<double> mp((size_t)G * S * 6), mpw((size_t)G * S);
-    for (int64_t i = 0; i < G * S; ++i) {
-        for (int k = 0; k < 6; ++k) mp[(size_t)i * 6 + k] = smp[(size_t)k * n + g0 * S + i];
-        mpw[(size_t)i] = smp[(size_t)6 * n + g0 * S + i];
-    }
-    MatWriter mat;
-    mat.add_matrix("particles", 12, Np, particles);
-    mat.add_matrix("weights", 1, Np, w);
-    mat.add_matrix("map_weights", 1, G, mw);
-    mat.add_matrix("map_particles", 6, G * S, mp);
-    mat.add_matrix("map_particle_weights", 1, G * S, mpw);
-    std::string err;
-    if (!mat.write(filename, &err)) throw std::runtime_error("writeMat: " + err);
+    check(scphd_write_mat(h_, filename), "writeMat");
 }
 
 void SCPHDCameraCalibration::transformTest() {

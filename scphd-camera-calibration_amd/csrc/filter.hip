@@ -120,7 +120,7 @@ struct scphd_filter {
     /* measurements, normalisers, survivors, merge */
     DBuf<double> du, dv, eta;
     int capM = 0;
-    DBuf<int32_t> nsurv, survoff, nmerged, mergeoff, outcnt, scantmp, sg, sm, dflag;
+    DBuf<int32_t> nsurv, survoff, nmerged, mergeoff, outcnt, scantmp, sg, sm, dflag, mcnt;
     DBuf<int32_t> ngroups, glead, gsta, gend, gmem;
     DBuf<double> sw, smu, mw, mmu, mP, dneff, est;
     DBuf<char> mscratch;
@@ -260,6 +260,8 @@ int update_t(scphd_filter* f, int M, int fixed) {
         fa.keep = f->keep.p;
         fa.redo = f->redo.p;
         fa.nd_base = 0;
+        fa.off32 = std::max(f->gen[0].cap_blocks, f->gen[1].cap_blocks) * (int64_t)S * (int64_t)f->esz <
+                   (int64_t)0xFFFFFFFFll - 64 * (int64_t)f->esz;
         record(f, 6);
         HIPTRY(launch_fit<T>(s, fa, f->gen[f->cur].planes<T>(), f->gen[nxt].planes<T>()));
         record(f, 7);
@@ -295,6 +297,9 @@ int update_t(scphd_filter* f, int M, int fixed) {
     ea.eta = f->eta.p;
     ea.loglik = f->loglik.p;
     ea.nsurv = f->nsurv.p;
+    HIPTRY(f->mcnt.ensure((int64_t)Np * std::max(M, 1)));
+    ea.mcnt = f->mcnt.p;
+    ea.lw_gate = P.min_weight > 0 ? std::log(P.min_weight) - 1e-3 : -INFINITY;
     HIPTRY(launch_eta(s, ea));
     HIPTRY(launch_scan(s, f->nsurv.p, f->survoff.p, Np, f->scantmp.p));
     if (ensure_hbuf(f, 2 * (int64_t)Np + 16) != SCPHD_OK) return SCPHD_ERR_HIP;
@@ -404,6 +409,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
     pa.key = f->key;
     pa.ustep = (uint64_t)f->n_update;
     pa.post_base = Ng;
+    pa.off32 = f->gen[nxt].cap_blocks * (int64_t)S * (int64_t)f->esz < (int64_t)0xFFFFFFFFll - 64 * (int64_t)f->esz;
     record(f, 8);
     HIPTRY(launch_sample<T>(s, pa, f->gen[nxt].planes<T>()));
     record(f, 9);
@@ -541,7 +547,7 @@ void scphd_destroy(scphd_t* f) {
     for (auto* b : dbs) b->release();
     DBuf<int32_t>* ibs[] = {&f->anc, &f->newcnt, &f->goff, &f->gparent, &f->gblock, &f->goff2, &f->gparent2,
                             &f->gblock2, &f->pdok, &f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->nsurv, &f->survoff,
-                            &f->nmerged, &f->mergeoff, &f->outcnt, &f->scantmp, &f->sg, &f->sm, &f->dflag,
+                            &f->nmerged, &f->mergeoff, &f->outcnt, &f->scantmp, &f->sg, &f->sm, &f->dflag, &f->mcnt,
                             &f->ngroups, &f->glead, &f->gsta, &f->gend, &f->gmem};
     for (auto* b : ibs) b->release();
     f->mscratch.release();
@@ -847,7 +853,7 @@ int scphd_get_memory(scphd_t* f, int64_t* bytes) {
     for (auto* d : dbs) b += d->n * (int64_t)sizeof(double);
     DBuf<int32_t>* ibs[] = {&f->anc, &f->newcnt, &f->goff, &f->gparent, &f->gblock, &f->goff2, &f->gparent2,
                             &f->gblock2, &f->pdok, &f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->nsurv, &f->survoff,
-                            &f->nmerged, &f->mergeoff, &f->outcnt, &f->scantmp, &f->sg, &f->sm, &f->dflag,
+                            &f->nmerged, &f->mergeoff, &f->outcnt, &f->scantmp, &f->sg, &f->sm, &f->dflag, &f->mcnt,
                             &f->ngroups, &f->glead, &f->gsta, &f->gend, &f->gmem};
     for (auto* d : ibs) b += d->n * (int64_t)sizeof(int32_t);
     b += f->mscratch.n;

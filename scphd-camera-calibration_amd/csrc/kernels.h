@@ -88,6 +88,7 @@ struct FitArgs {
     int32_t* keep;   /* non-detection term kept */
     int32_t* redo;   /* per Gaussian: bit 0 all-zero weights, bit 1 cloud still to write */
     int64_t nd_base; /* destination block of the non-detection cloud of entry g: nd_base + g */
+    int off32;       /* every source and destination plane is below 4 GiB: 32-bit byte offsets */
 };
 
 /* k_fit: the streaming sweep; k_fit_fix: the rare cases it flags in redo[] */
@@ -115,6 +116,8 @@ struct EtaArgs {
     double* eta;       /* Np*M */
     double* loglik;    /* Np   */
     int32_t* nsurv;    /* Np   */
+    int32_t* mcnt;     /* Np*M detection terms kept per (parent, measurement) */
+    double lw_gate;    /* log(min_weight) - 1e-3: below it exp() < min_weight for certain */
 };
 hipError_t launch_eta(hipStream_t s, const EtaArgs& a);
 
@@ -186,6 +189,7 @@ struct SampleArgs {
     NoiseKey key;
     uint64_t ustep;
     int64_t post_base;
+    int off32; /* every destination plane is below 4 GiB: 32-bit byte offsets */
 };
 template <typename T>
 hipError_t launch_sample(hipStream_t s, const SampleArgs& a, Planes<T> dst);

@@ -227,6 +227,32 @@ __device__ inline double row_sum16(double x) {
 
 /* One sample of a Gaussian's cloud: the deferred prediction applied, its
  * disparity-space point, its field-of-view flag and stored weight. */
+/* Plane element at byte offset `off` from a uniform plane base.  With a
+ * 32-bit offset the address is a scalar base plus one per-lane offset
+ * register (the global_load saddr form); 64-bit offsets serve planes of
+ * 4 GiB and more. */
+template <typename T, typename Off>
+__device__ inline T& at(T* base, Off off) {
+    return *(T*)((char*)base + off);
+}
+
+template <typename T, typename Off>
+__device__ inline void load_sample_off(const Planes<T>& src, Off off, T r[7]) {
+#pragma unroll
+    for (int c = 0; c < 7; ++c) r[c] = at(src.p[c], off);
+}
+
+template <typename T, typename Off>
+__device__ inline void write_nd_off(const FitArgs& a, Planes<T>& dst, Off off, const EuclideanPoint& x, bool in) {
+    at(dst.p[0], off) = (T)x.x;
+    at(dst.p[1], off) = (T)x.y;
+    at(dst.p[2], off) = (T)x.z;
+    at(dst.p[3], off) = (T)x.vx;
+    at(dst.p[4], off) = (T)x.vy;
+    at(dst.p[5], off) = (T)x.vz;
+    at(dst.p[6], off) = (T)(1.0 - (in ? a.meas.pd() : 0.0));
+}
+
 template <typename T>
 __device__ inline void load_sample(const Planes<T>& src, int64_t si, T r[7]) {
 #pragma unroll
@@ -239,7 +265,7 @@ __device__ inline void eval_sample(const FitArgs& a, const T r[7], int p, int k,
     x = EuclideanPoint{(double)r[0], (double)r[1], (double)r[2], (double)r[3], (double)r[4], (double)r[5]};
     w = (double)r[6];
     if (a.pending) x = predict_sample(x, a.feat, a.dt, a.key, a.pstep, (uint32_t)p, (uint32_t)k, (uint32_t)s);
-    d = a.meas.computeMeasurement(x, pose);
+    d = disparity_fast(a.meas, pose, x);
     in = a.meas.inRange(d);
 }
 
@@ -296,15 +322,21 @@ __device__ inline void write_fit(const FitArgs& a, int64_t g, int lane, const do
  * kept cloud whose decision needed pd_g (bit 1), and a cloud whose weights
  * are all zero (bit 0), which the reference normalises to weight 1 per
  * sample. */
-template <typename T>
-__global__ __launch_bounds__(256) void k_fit(FitArgs a, Planes<T> src, Planes<T> dst) {
-    const int lane = threadIdx.x & 15;
-    const int64_t g = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
-    if (g >= a.Ng) return; /* whole rows; rows never exchange data */
-    const int p = a.gparent[g];
+template <typename T, typename Off>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_fit(FitArgs a, Planes<T> src, Planes<T> dst) {
+    /* the row's camera pose lives in LDS and is re-read per sample, which
+     * keeps 24 registers free for occupancy */
+    __shared__ double s_pose[16 * 12];
+    const int lane = threadIdx.x & 15, row = threadIdx.x >> 4;
+    const int64_t g = (int64_t)blockIdx.x * 16 + row;
+    const bool act = g < a.Ng;
+    const int p = act ? a.gparent[g] : 0;
+    if (act && lane < 12) s_pose[row * 12 + lane] = a.pose[(int64_t)p * 12 + lane];
+    __syncthreads();
+    if (!act) return; /* whole rows; rows never exchange data */
+    const volatile double* sp = s_pose + row * 12;
     const int k = (int)(g - a.goff[p]);
     const int64_t b0 = (int64_t)a.gblock[g] * a.S;
-    const CameraPose pose = load_pose(a.pose, p);
     const int S = a.S;
     const double wg = a.gw[g];
     const double Pd = a.meas.pd();
@@ -321,16 +353,27 @@ __global__ __launch_bounds__(256) void k_fit(FitArgs a, Planes<T> src, Planes<T>
     for (int e = 0; e < 21; ++e) m2[e] = 0.0;
     /* loads run one step ahead of the arithmetic */
     T cur[7], nxt[7];
-    if (lane < S) load_sample(src, b0 + lane, cur);
+    const Off so = (Off)((b0 + lane) * (int64_t)sizeof(T)); /* byte offsets, advanced 16 samples a step */
+    Off dof = (Off)((d0 + lane) * (int64_t)sizeof(T));
+    Off sof = so;
+    if (lane < S) load_sample_off(src, sof, cur);
     for (int s0 = 0; s0 < S; s0 += 16) {
         const int s = s0 + lane;
         const bool valid = s < S;
-        if (s + 16 < S) load_sample(src, b0 + s + 16, nxt);
+        sof += (Off)(16 * sizeof(T));
+        if (s + 16 < S) load_sample_off(src, sof, nxt);
         EuclideanPoint x;
         DisparityPoint d;
         bool in = false;
         double w = 0.0;
-        if (valid) eval_sample(a, cur, p, k, s, pose, x, d, in, w);
+        if (valid) {
+            CameraPose pose;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) pose.R[i] = sp[i];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) pose.t[i] = sp[9 + i];
+            eval_sample(a, cur, p, k, s, pose, x, d, in, w);
+        }
 #pragma unroll
         for (int c = 0; c < 7; ++c) cur[c] = nxt[c];
         const double q[6] = {d.u, d.v, d.d, d.vu, d.vv, d.vd};
@@ -353,7 +396,8 @@ __global__ __launch_bounds__(256) void k_fit(FitArgs a, Planes<T> src, Planes<T>
         for (int r = 0; r < 6; ++r)
 #pragma unroll
             for (int c = 0; c <= r; ++c) m2[pidx(r, c)] += wp * dv[r] * dv[c];
-        if (sure_keep && valid) write_nd(a, dst, d0 + s, x, in);
+        if (sure_keep && valid) write_nd_off(a, dst, dof, x, in);
+        dof += (Off)(16 * sizeof(T));
     }
     spd = row_sum16(spd);
     sw = row_sum16(sw);
@@ -562,6 +606,7 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
                 }
         }
         double T = 0.0;
+        int mcnt = 0;
         for (int c0 = 0; c0 < G; c0 += ETA_CH) {
             const int n = min(ETA_CH, G - c0);
             if (!single) {
@@ -570,8 +615,11 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
                 __syncthreads();
             }
             if (valid)
-                for (int j = 0; j < n; ++j)
-                    if (c0 + j != am) T += exp(det_lw(ch, j, zu, zv) - mx);
+                for (int j = 0; j < n; ++j) {
+                    const double x = det_lw(ch, j, zu, zv) - mx;
+                    /* exp(x) is exactly +0 for x <= -746: skipping it changes no bit */
+                    if (c0 + j != am && x > -746.0) T += exp(x);
+                }
         }
         if (am >= 0) T += exp(a.bpc - mx);
         const double eta = mx + log1p(T);
@@ -588,12 +636,14 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
             }
             if (valid) {
                 for (int j = 0; j < n; ++j) {
-                    const double w = exp(det_lw(ch, j, zu, zv) - eta);
-                    cnt += (w >= a.min_weight && ch.ok[j]) ? 1 : 0;
+                    const double x = det_lw(ch, j, zu, zv) - eta;
+                    if (x > a.lw_gate && ch.ok[j]) mcnt += exp(x) >= a.min_weight ? 1 : 0;
                 }
             }
         }
         if (valid) {
+            a.mcnt[(int64_t)p * a.M + m] = mcnt;
+            cnt += mcnt;
             const double w = exp(a.birth.logw0 - eta);
             cnt += (w >= a.min_weight && a.birth.pd_ok) ? 1 : 0;
         }
@@ -630,31 +680,22 @@ __global__ __launch_bounds__(NT) void k_survivors(SurvArgs a) {
         const bool valid = m < e.M;
         const double zu = valid ? e.zu[m] : 0.0, zv = valid ? e.zv[m] : 0.0;
         const double eta = valid ? e.eta[(int64_t)p * e.M + m] : 0.0;
-        int cnt = 0;
-        for (int c0 = 0; c0 < G; c0 += ETA_CH) {
-            const int n = min(ETA_CH, G - c0);
-            __syncthreads();
-            stage_chunk(ch, e, g0 + c0, n);
-            __syncthreads();
-            if (valid)
-                for (int j = 0; j < n; ++j) {
-                    const double w = exp(det_lw(ch, j, zu, zv) - eta);
-                    cnt += (w >= e.min_weight && ch.ok[j]) ? 1 : 0;
-                }
-        }
+        const int cnt = valid ? e.mcnt[(int64_t)p * e.M + m] : 0; /* detection terms kept, from k_eta */
         int tot;
         int o = run + block_exscan(cnt, sh, &tot);
         for (int c0 = 0; c0 < G; c0 += ETA_CH) {
             const int n = min(ETA_CH, G - c0);
-            if (G > ETA_CH) {
+            if (G > ETA_CH || r == 0) { /* a single chunk is staged once and stays */
                 __syncthreads();
                 stage_chunk(ch, e, g0 + c0, n);
                 __syncthreads();
             }
             if (valid)
                 for (int j = 0; j < n; ++j) {
-                    const double w = exp(det_lw(ch, j, zu, zv) - eta);
-                    if (w >= e.min_weight && ch.ok[j]) {
+                    const double x = det_lw(ch, j, zu, zv) - eta;
+                    if (!(x > e.lw_gate && ch.ok[j])) continue; /* exp(x) < min_weight for sure */
+                    const double w = exp(x);
+                    if (w >= e.min_weight) {
                         if (o < cap) {
                             const int64_t gi = g0 + c0 + j;
                             const int64_t so = base + o;
@@ -700,12 +741,14 @@ __global__ __launch_bounds__(NT) void k_survivors(SurvArgs a) {
 
 /* -------------------------------------------------------------- merging */
 /* Workspace of one parent's merge, U survivor slots (Up = next power of two):
- *   key_w double[Up]  sort keys;            ord  int[Up]   (weight desc, index asc) order
- *   pf_u  double[U]   mean u coordinate;    pf_su float[U] its variance
- *   flag  uint8[U]    merged;               selfm uint8[U] member of its own group
- *   nbo   int[U+1]    neighbour offsets;    mem  int[U]    group members
- *   glead int[U]      group leaders;        gsta int[U+1]  group starts in mem
- *   nbr   int[ecap]   neighbour lists, ecap = 4 U + 64                        */
+ *   key_w double[Up]  sort keys;            ord   int[Up]   (weight desc, index asc) order
+ *   bu, bv double[U]  mean u, v;            bs    double[3U] (u,v) covariance block uu uv vv
+ *   flag  uint8[U]    merged;               selfm uint8[U]  member of its own group
+ *   nbo   int[U+1]    neighbour offsets;    cnt   int[U]    neighbour counts / fill cursors
+ *   mem   int[U]      group members;
+ *   nbr   int[ecap]   neighbour lists, ecap = 4 U + 64
+ *   cand  uint32[ccap] candidate pairs i << 16 | j (i < j), ccap = 8 U + 64
+ *   res   uint8[ccap] their membership                                         */
 __host__ __device__ inline int next_pow2(int x) {
     int v = 1;
     while (v < x) v <<= 1;
@@ -713,27 +756,30 @@ __host__ __device__ inline int next_pow2(int x) {
 }
 
 struct MergeLayout {
-    int64_t ord, pf_u, pf_su, flag, selfm, nbo, mem, glead, gsta, nbr, ecap, bytes;
+    int64_t ord, bu, bv, bs, flag, selfm, nbo, cnt, mem, nbr, ecap, cand, ccap, res, bytes;
     __host__ __device__ explicit MergeLayout(int U) {
         const int64_t Up = next_pow2(U > 0 ? U : 1);
         ord = Up * 8;
-        pf_u = (ord + Up * 4 + 7) & ~(int64_t)7;
-        pf_su = pf_u + (int64_t)U * 8;
-        flag = pf_su + (int64_t)U * 4;
+        bu = (ord + Up * 4 + 7) & ~(int64_t)7;
+        bv = bu + (int64_t)U * 8;
+        bs = bv + (int64_t)U * 8;
+        flag = bs + (int64_t)U * 24;
         selfm = flag + U;
         nbo = (selfm + U + 7) & ~(int64_t)7;
-        mem = nbo + ((int64_t)U + 1) * 4;
-        glead = mem + (int64_t)U * 4;
-        gsta = glead + (int64_t)U * 4;
-        nbr = gsta + ((int64_t)U + 1) * 4;
+        cnt = nbo + ((int64_t)U + 1) * 4;
+        mem = cnt + (int64_t)U * 4;
+        nbr = mem + (int64_t)U * 4;
         ecap = 4 * (int64_t)U + 64;
-        bytes = nbr + ecap * 4;
+        cand = nbr + ecap * 4;
+        ccap = 8 * (int64_t)U + 64;
+        res = cand + ccap * 4;
+        bytes = res + ccap;
     }
 };
 
 /* global scratch for parents that do not fit in LDS: MergeLayout(U).bytes
- * < 66 U + 300 since Up < 2 U */
-static constexpr int64_t MERGE_WS_PER_SLOT = 72, MERGE_WS_PER_PARENT = 512;
+ * <= 134 U + 600 since Up < 2 U */
+static constexpr int64_t MERGE_WS_PER_SLOT = 176, MERGE_WS_PER_PARENT = 1024;
 
 int merge_lds_bytes(int cap) { return (int)MergeLayout(cap).bytes; }
 
@@ -757,16 +803,64 @@ __device__ inline void survivor_mu(const MergeArgs& a, int64_t si, double* mu) {
     for (int c = 0; c < 6; ++c) mu[c] = a.smu[c * a.scap + si];
 }
 
+/* Workspace pointers of one parent's merge. */
+struct MergeWs {
+    double* key_w;
+    int* ord;
+    double* bu;
+    double* bv;
+    double* bs;
+    uint8_t* flag;
+    uint8_t* selfm;
+    int* nbo;
+    int* cnt;
+    int* mem;
+    int* nbr;
+    uint32_t* cand;
+    uint8_t* res;
+    int64_t ecap, ccap;
+    __device__ MergeWs(char* ws, int U) {
+        const MergeLayout ly(U);
+        key_w = (double*)ws;
+        ord = (int*)(ws + ly.ord);
+        bu = (double*)(ws + ly.bu);
+        bv = (double*)(ws + ly.bv);
+        bs = (double*)(ws + ly.bs);
+        flag = (uint8_t*)(ws + ly.flag);
+        selfm = (uint8_t*)(ws + ly.selfm);
+        nbo = (int*)(ws + ly.nbo);
+        cnt = (int*)(ws + ly.cnt);
+        mem = (int*)(ws + ly.mem);
+        nbr = (int*)(ws + ly.nbr);
+        cand = (uint32_t*)(ws + ly.cand);
+        res = (uint8_t*)(ws + ly.res);
+        ecap = ly.ecap;
+        ccap = ly.ccap;
+    }
+};
+
+/* Can survivors i and j be merge partners?  The (u, v) marginal of the
+ * squared Mahalanobis distance with Sigma = P_i + P_j is a lower bound of
+ * the full 6-D distance for positive-definite Sigma (and a non-PD Sigma
+ * never merges), so a pair is rejected when the marginal exceeds the
+ * threshold by 0.1 %; an ill-conditioned (u, v) block is never rejected. */
+__device__ inline bool merge_bound(const MergeWs& w, int i, int j, double thr) {
+    const double du = w.bu[j] - w.bu[i], dv = w.bv[j] - w.bv[i];
+    const double suu = w.bs[3 * i] + w.bs[3 * j];
+    const double suv = w.bs[3 * i + 1] + w.bs[3 * j + 1];
+    const double svv = w.bs[3 * i + 2] + w.bs[3 * j + 2];
+    const double det = suu * svv - suv * suv;
+    if (!(det > 1e-6 * suu * svv)) return true;
+    const double q = svv * du * du - 2.0 * suv * du * dv + suu * dv * dv;
+    return !(q > 1.001 * thr * det);
+}
+
 /* Merge membership of survivors i and j: squared Mahalanobis distance with
  * Sigma = P_i + P_j below min_separation (computeMahalDist,
- * src/device_math.cu:374-453; the computed value is symmetric in i and j
- * to the last bit).  A one-dimensional bound skips the 6x6 evaluation:
- * d^2 >= du^2 / Sigma_uu, rejected only beyond a 0.1 % margin. */
-__device__ inline bool merge_pair(const MergeArgs& a, int64_t sbase, const double* pf_u, const float* pf_su, int i,
-                                  int j, double thr) {
-    const double du = pf_u[j] - pf_u[i];
-    const double su = (double)pf_su[j] + (double)pf_su[i];
-    if (du * du > 1.001 * thr * su) return false;
+ * src/device_math.cu:374-453).  The computed value is symmetric in i and j
+ * to the last bit (the sum P_i + P_j commutes and negating the difference
+ * negates every forward-substitution term). */
+__device__ inline bool merge_full(const MergeArgs& a, int64_t sbase, int i, int j, double thr) {
     double mi[6], mj[6], Pi[21], Pj[21];
     survivor_mu(a, sbase + i, mi);
     survivor_mu(a, sbase + j, mj);
@@ -775,32 +869,9 @@ __device__ inline bool merge_pair(const MergeArgs& a, int64_t sbase, const doubl
     return mahal6(mi, Pi, mj, Pj) < thr;
 }
 
-/* Workspace pointers of one parent's merge. */
-struct MergeWs {
-    double* key_w;
-    int* ord;
-    double* pf_u;
-    float* pf_su;
-    uint8_t* flag;
-    uint8_t* selfm;
-    int* nbo;
-    int* mem;
-    int* nbr;
-    int64_t ecap;
-    __device__ MergeWs(char* ws, int U) {
-        const MergeLayout ly(U);
-        key_w = (double*)ws;
-        ord = (int*)(ws + ly.ord);
-        pf_u = (double*)(ws + ly.pf_u);
-        pf_su = (float*)(ws + ly.pf_su);
-        flag = (uint8_t*)(ws + ly.flag);
-        selfm = (uint8_t*)(ws + ly.selfm);
-        nbo = (int*)(ws + ly.nbo);
-        mem = (int*)(ws + ly.mem);
-        nbr = (int*)(ws + ly.nbr);
-        ecap = ly.ecap;
-    }
-};
+__device__ inline bool merge_pair(const MergeArgs& a, int64_t sbase, const MergeWs& w, int i, int j, double thr) {
+    return merge_bound(w, i, j, thr) && merge_full(a, sbase, i, j, thr);
+}
 
 /* Load one parent's survivors into the workspace and sort them by
  * (weight desc, index asc), the order in which the reference's argmax
@@ -812,9 +883,12 @@ __device__ inline void merge_load_sort(const MergeArgs& a, int64_t sbase, int U,
         if (i < U) {
             w.key_w[i] = a.sw[sbase + i];
             w.ord[i] = i;
-            w.pf_u[i] = a.smu[0 * a.scap + sbase + i];
+            w.bu[i] = a.smu[0 * a.scap + sbase + i];
+            w.bv[i] = a.smu[1 * a.scap + sbase + i];
             const int g = a.sg[sbase + i];
-            w.pf_su[i] = (float)(g >= 0 ? a.pre[(int64_t)(PP_PN + pidx(0, 0)) * a.Ng + g] : a.birth.P[pidx(0, 0)]);
+            w.bs[3 * i] = g >= 0 ? a.pre[(int64_t)(PP_PN + pidx(0, 0)) * a.Ng + g] : a.birth.P[pidx(0, 0)];
+            w.bs[3 * i + 1] = g >= 0 ? a.pre[(int64_t)(PP_PN + pidx(1, 0)) * a.Ng + g] : a.birth.P[pidx(1, 0)];
+            w.bs[3 * i + 2] = g >= 0 ? a.pre[(int64_t)(PP_PN + pidx(1, 1)) * a.Ng + g] : a.birth.P[pidx(1, 1)];
             w.flag[i] = 0;
         } else {
             w.key_w[i] = -INFINITY;
@@ -858,16 +932,21 @@ __device__ inline bool merge_self(const MergeArgs& a, int64_t si, double thr) {
  * leader, stop when nothing is left or a group's weight is zero.
  *
  * Evaluated in phases instead of the reference's barrier-bound rounds.
- * k_merge_groups (one workgroup per parent): sort; all pairs' membership
- * into neighbour lists; one lane walks the sorted order forming each
- * leader's group from its unmerged neighbours -- the same groups as the
- * round loop, since membership depends only on the pair.  k_merge_moments
- * (one thread per group): moments with members summed in ascending index
- * order.  k_merge_rounds: the round loop, for parents whose neighbour
- * lists overflow the workspace. */
+ * k_merge_groups (one workgroup per parent):
+ *   1. sort by (weight desc, index asc);
+ *   2. candidate pairs i < j that pass merge_bound, compacted in LDS;
+ *   3. the full 6x6 test, one lane per candidate pair (and per self pair),
+ *      so that lanes stay busy however sparse the candidates are;
+ *   4. neighbour lists from the members, each sorted ascending;
+ *   5. one lane walks the sorted order forming each leader's group from its
+ *      unmerged neighbours -- the same groups as the round loop, since
+ *      membership depends only on the pair.
+ * k_merge_moments (one thread per group): moments with members summed in
+ * ascending index order.  k_merge_rounds: the round loop, for parents whose
+ * candidate or neighbour lists overflow the workspace. */
 __global__ __launch_bounds__(NT) void k_merge_groups(MergeArgs a) {
     extern __shared__ char lds[];
-    __shared__ int s_tot;
+    __shared__ int s_nc, s_tot;
     __shared__ int sh[8];
     const int p = blockIdx.x;
     const int t = threadIdx.x;
@@ -882,29 +961,64 @@ __global__ __launch_bounds__(NT) void k_merge_groups(MergeArgs a) {
         }
         return;
     }
+    if (U >= 65536) { /* candidate pairs pack two 16-bit indices: the round loop takes it */
+        if (t == 0) {
+            a.ngroups[p] = -1;
+            a.nmerged[p] = 0;
+        }
+        return;
+    }
     char* ws = big ? (a.scratch + sbase * MERGE_WS_PER_SLOT + (int64_t)p * MERGE_WS_PER_PARENT) : lds;
     MergeWs w(ws, U);
     const double thr = a.min_separation;
-    merge_load_sort(a, sbase, U, w);
+    if (t == 0) s_nc = 0;
+    merge_load_sort(a, sbase, U, w); /* ends with a barrier */
 
-    /* neighbour counts, rows [r0, r1) per thread */
+    /* 2. candidates: rows strided over threads, row i scans j > i */
+    for (int i = t; i < U; i += NT) {
+        w.cnt[i] = 0;
+        for (int j = i + 1; j < U; ++j)
+            if (merge_bound(w, i, j, thr)) {
+                const int c = atomicAdd(&s_nc, 1);
+                if (c < w.ccap) w.cand[c] = ((uint32_t)i << 16) | (uint32_t)j;
+            }
+    }
+    __syncthreads();
+    const int nc = s_nc;
+    if (nc > w.ccap) {
+        if (t == 0) {
+            a.ngroups[p] = -1;
+            a.nmerged[p] = 0;
+        }
+        return;
+    }
+    /* 3. full tests: candidates, then every survivor against itself */
+    for (int c = t; c < nc + U; c += NT) {
+        if (c < nc) {
+            const int i = (int)(w.cand[c] >> 16), j = (int)(w.cand[c] & 0xFFFFu);
+            const bool m = merge_full(a, sbase, i, j, thr);
+            w.res[c] = m ? 1 : 0;
+            if (m) {
+                atomicAdd(&w.cnt[i], 1);
+                atomicAdd(&w.cnt[j], 1);
+            }
+        } else {
+            const int i = c - nc;
+            w.selfm[i] = merge_self(a, sbase + i, thr) ? 1 : 0;
+        }
+    }
+    __syncthreads();
+    /* 4. neighbour lists: offsets (rows [r0, r1) per thread), fill, sort */
     const int rc = (U + NT - 1) / NT;
     const int r0 = min(U, t * rc), r1 = min(U, r0 + rc);
     int mine = 0;
-    for (int i = r0; i < r1; ++i) {
-        int ci = 0;
-        for (int j = 0; j < U; ++j)
-            if (j != i && merge_pair(a, sbase, w.pf_u, w.pf_su, i, j, thr)) ++ci;
-        w.selfm[i] = merge_self(a, sbase + i, thr) ? 1 : 0;
-        w.nbo[i] = ci;
-        mine += ci;
-    }
+    for (int i = r0; i < r1; ++i) mine += w.cnt[i];
     int tot;
     int base = block_exscan(mine, sh, &tot);
     for (int i = r0; i < r1; ++i) {
-        const int ci = w.nbo[i];
         w.nbo[i] = base;
-        base += ci;
+        base += w.cnt[i];
+        w.cnt[i] = 0;
     }
     if (t == 0) {
         w.nbo[U] = tot;
@@ -918,12 +1032,27 @@ __global__ __launch_bounds__(NT) void k_merge_groups(MergeArgs a) {
         }
         return;
     }
-    for (int i = r0; i < r1; ++i) {
-        int o = w.nbo[i];
-        for (int j = 0; j < U; ++j)
-            if (j != i && merge_pair(a, sbase, w.pf_u, w.pf_su, i, j, thr)) w.nbr[o++] = j;
+    for (int c = t; c < nc; c += NT)
+        if (w.res[c]) {
+            const int i = (int)(w.cand[c] >> 16), j = (int)(w.cand[c] & 0xFFFFu);
+            w.nbr[w.nbo[i] + atomicAdd(&w.cnt[i], 1)] = j;
+            w.nbr[w.nbo[j] + atomicAdd(&w.cnt[j], 1)] = i;
+        }
+    __syncthreads();
+    for (int i = t; i < U; i += NT) { /* insertion sort: lists are short */
+        const int e0 = w.nbo[i], e1 = w.nbo[i + 1];
+        for (int e = e0 + 1; e < e1; ++e) {
+            const int x = w.nbr[e];
+            int f = e - 1;
+            while (f >= e0 && w.nbr[f] > x) {
+                w.nbr[f + 1] = w.nbr[f];
+                --f;
+            }
+            w.nbr[f + 1] = x;
+        }
     }
     __syncthreads();
+    /* 5. the greedy walk */
     if (t == 0) {
         int ng = 0, mp = 0, r = 0;
         while (r < U) {
@@ -1074,7 +1203,7 @@ __global__ __launch_bounds__(NT) void k_merge_rounds(MergeArgs a) {
         for (int i = t; i < U; i += NT) {
             uint8_t mb = 0;
             if (!w.flag[i])
-                mb = (i == lead ? w.selfm[i] != 0 : merge_pair(a, sbase, w.pf_u, w.pf_su, lead, i, thr)) ? 1 : 0;
+                mb = (i == lead ? w.selfm[i] != 0 : merge_pair(a, sbase, w, lead, i, thr)) ? 1 : 0;
             memb[i] = mb;
         }
         __syncthreads();
@@ -1249,16 +1378,19 @@ __global__ __launch_bounds__(256) void k_estimate(const double* ext, const doubl
 }
 
 /* ---------------------------------------------------- posterior samples */
-/* One wavefront per merged Gaussian: S draws x = L z + mu with L the
- * Cholesky factor of the merged covariance (sample_disparity_gaussian,
- * src/scphdcameracalibration.cu:640-690) mapped to the world by the
- * parent's camera (disparityToWorldKernel, src/kernels.cu:49-80), sample
- * weight 1/S (:2009-2010). */
-template <typename T, int NS>
+/* Sixteen lanes (one row of a wavefront) per merged Gaussian, four per
+ * wavefront: S draws x = L z + mu with L the Cholesky factor of the merged
+ * covariance (sample_disparity_gaussian, src/scphdcameracalibration.cu:640-690),
+ * factored once per lane and used for every sample the lane draws, mapped
+ * to the world by the parent's camera (disparityToWorldKernel,
+ * src/kernels.cu:49-80), sample weight 1/S (:2009-2010).  Destination
+ * addresses are byte offsets from the plane bases (32-bit when every plane
+ * is below 4 GiB). */
+template <typename T, typename Off>
 __global__ __launch_bounds__(256) void k_sample(SampleArgs a, Planes<T> dst) {
-    const int lane = threadIdx.x & 63;
-    const int64_t gm = (int64_t)blockIdx.x * WPB + (threadIdx.x >> 6);
-    if (gm >= a.Kt) return;
+    const int lane = threadIdx.x & 15;
+    const int64_t gm = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+    if (gm >= a.Kt) return; /* whole rows */
     const int p = upper_parent(a.mergeoff, a.Np, gm);
     const int k = (int)(gm - a.mergeoff[p]);
     const int64_t rec = (int64_t)a.survoff[p] + k;
@@ -1269,29 +1401,29 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a, Planes<T> dst) {
     for (int e = 0; e < 21; ++e) P[e] = a.mP[e * a.scap + rec];
     chol6(P, L);
     const CameraPose pose = load_pose(a.pose, p);
-    const int64_t d0 = (a.post_base + gm) * (int64_t)a.S;
-    const double w = 1.0 / a.S;
+    const T w = (T)(1.0 / a.S);
+    Off off = (Off)(((a.post_base + gm) * (int64_t)a.S + lane) * (int64_t)sizeof(T));
+    for (int s = lane; s < a.S; s += 16) {
+        double z[6];
+        normals<6>(a.key, NS_POSTERIOR, a.ustep, (uint32_t)p, (uint32_t)k, (uint32_t)s, z);
+        double q[6];
 #pragma unroll
-    for (int i = 0; i < NS; ++i) {
-        const int s = lane + 64 * i;
-        if (s < a.S) {
-            double z[6];
-            normals<6>(a.key, NS_POSTERIOR, a.ustep, (uint32_t)p, (uint32_t)k, (uint32_t)s, z);
-            double q[6];
+        for (int r = 0; r < 6; ++r) {
+            double acc = 0.0;
 #pragma unroll
-            for (int r = 0; r < 6; ++r) {
-                double acc = 0.0;
-#pragma unroll
-                for (int c = 0; c <= r; ++c) acc += L[pidx(r, c)] * z[c];
-                q[r] = acc + mu[r];
-            }
-            const DisparityPoint d{q[0], q[1], q[2], q[3], q[4], q[5]};
-            const EuclideanPoint x = a.meas.invertMeasurement(d, pose);
-            const int64_t di = d0 + s;
-            dst.p[0][di] = (T)x.x; dst.p[1][di] = (T)x.y; dst.p[2][di] = (T)x.z;
-            dst.p[3][di] = (T)x.vx; dst.p[4][di] = (T)x.vy; dst.p[5][di] = (T)x.vz;
-            dst.p[6][di] = (T)w;
+            for (int c = 0; c <= r; ++c) acc += L[pidx(r, c)] * z[c];
+            q[r] = acc + mu[r];
         }
+        const DisparityPoint d{q[0], q[1], q[2], q[3], q[4], q[5]};
+        const EuclideanPoint x = world_from_disparity_fast(a.meas, pose, d);
+        at(dst.p[0], off) = (T)x.x;
+        at(dst.p[1], off) = (T)x.y;
+        at(dst.p[2], off) = (T)x.z;
+        at(dst.p[3], off) = (T)x.vx;
+        at(dst.p[4], off) = (T)x.vy;
+        at(dst.p[5], off) = (T)x.vz;
+        at(dst.p[6], off) = w;
+        off += (Off)(16 * sizeof(T));
     }
 }
 
@@ -1570,7 +1702,10 @@ hipError_t launch_materialize(hipStream_t s, Planes<T> src, Planes<T> dst, const
 template <typename T>
 hipError_t launch_fit(hipStream_t s, const FitArgs& a, Planes<T> src, Planes<T> dst) {
     if (a.Ng <= 0) return hipSuccess;
-    k_fit<T><<<nblocks(a.Ng, 16), 256, 0, s>>>(a, src, dst);
+    if (a.off32)
+        k_fit<T, uint32_t><<<nblocks(a.Ng, 16), 256, 0, s>>>(a, src, dst);
+    else
+        k_fit<T, uint64_t><<<nblocks(a.Ng, 16), 256, 0, s>>>(a, src, dst);
     LAUNCH_CHECK();
 }
 
@@ -1631,18 +1766,11 @@ hipError_t launch_estimate(hipStream_t s, const double* ext, const double* pw, c
 template <typename T>
 hipError_t launch_sample(hipStream_t s, const SampleArgs& a, Planes<T> dst) {
     if (a.Kt <= 0) return hipSuccess;
-    const unsigned nb = nblocks(a.Kt, WPB);
-    const int ns = (a.S + 63) / 64;
-    if (ns <= 1)
-        k_sample<T, 1><<<nb, 256, 0, s>>>(a, dst);
-    else if (ns <= 2)
-        k_sample<T, 2><<<nb, 256, 0, s>>>(a, dst);
-    else if (ns <= 4)
-        k_sample<T, 4><<<nb, 256, 0, s>>>(a, dst);
-    else if (ns <= 8)
-        k_sample<T, 8><<<nb, 256, 0, s>>>(a, dst);
+    const unsigned nb = nblocks(a.Kt, 16);
+    if (a.off32)
+        k_sample<T, uint32_t><<<nb, 256, 0, s>>>(a, dst);
     else
-        return hipErrorInvalidValue;
+        k_sample<T, uint64_t><<<nb, 256, 0, s>>>(a, dst);
     LAUNCH_CHECK();
 }
 

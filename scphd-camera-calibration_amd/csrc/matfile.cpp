@@ -87,6 +87,13 @@ bool parse_matrix(const uint8_t* d, uint32_t nbytes, MatVar& v, std::string& nam
     uint32_t flags;
     std::memcpy(&flags, q, 4);
     v.mclass = flags & 0xff;
+    /* only cells and numeric arrays are read; other classes (char, struct,
+     * object, sparse, opaque, ...) are skipped whole */
+    if (v.mclass != mxCELL && !(v.mclass >= 6 && v.mclass <= 15)) {
+        v.mclass = 0;
+        name.clear();
+        return true;
+    }
     if (!read_tag(c, t, nb, q) || t != miINT32) return false; /* dimensions */
     v.dims.resize(nb / 4);
     for (size_t j = 0; j < v.dims.size(); ++j) {
@@ -113,7 +120,7 @@ bool parse_matrix(const uint8_t* d, uint32_t nbytes, MatVar& v, std::string& nam
         if (!to_doubles(t, q, nb, v.data)) return false;
         return (int64_t)v.data.size() == v.numel();
     }
-    return true; /* other classes: kept with dims only */
+    return true;
 }
 
 bool parse_elements(const uint8_t* d, size_t n, std::map<std::string, MatVar>* vars, std::string* err) {
@@ -126,6 +133,7 @@ bool parse_elements(const uint8_t* d, size_t n, std::map<std::string, MatVar>* v
             return false;
         }
         if (t == miCOMPRESSED) {
+            c.i = (size_t)(q - c.p) + nb; /* compressed elements carry no padding */
             std::vector<uint8_t> out;
             z_stream zs;
             std::memset(&zs, 0, sizeof zs);
@@ -157,7 +165,7 @@ bool parse_elements(const uint8_t* d, size_t n, std::map<std::string, MatVar>* v
                 *err = "malformed matrix element";
                 return false;
             }
-            (*vars)[name] = std::move(v);
+            if (v.mclass != 0) (*vars)[name] = std::move(v);
         }
     }
     return true;

@@ -34,7 +34,7 @@ EXPORTED = [
     "scphd_batch_opts_init", "scphd_batch_create", "scphd_batch_destroy", "scphd_batch_last_error",
     "scphd_batch_now", "scphd_rig_footprint", "scphd_batch_add_rig", "scphd_batch_remove_rig",
     "scphd_batch_submit", "scphd_batch_step", "scphd_batch_collect", "scphd_batch_get_stats",
-    "scphd_batch_filter",
+    "scphd_batch_filter", "scphd_mat_read_measurements", "scphd_write_mat", "scphd_batch_set_output",
 ]
 
 
@@ -162,6 +162,10 @@ def lib():
         "scphd_batch_collect": (_I32, [_P, ctypes.POINTER(FrameResult), _I32, ip]),
         "scphd_batch_get_stats": (_I32, [_P, ctypes.POINTER(BatchStats)]),
         "scphd_batch_filter": (_P, [_P, _I32]),
+        "scphd_mat_read_measurements": (_I32, [ctypes.c_char_p, ctypes.c_char_p, ip, ctypes.POINTER(_I64), ip, dp,
+                                               dp]),
+        "scphd_write_mat": (_I32, [_P, ctypes.c_char_p]),
+        "scphd_batch_set_output": (_I32, [_P, _I32, ctypes.c_char_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -183,6 +187,27 @@ class ScphdError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
         self.code = code
+
+
+def read_measurements(path, varname):
+    """Measurement sequence of a MAT-file cell array (scphd_mat_read_measurements):
+    a list of (u, v) arrays, one per frame."""
+    L = lib()
+    n = _I32(0)
+    tot = _I64(0)
+    rc = L.scphd_mat_read_measurements(path.encode(), varname.encode(), ctypes.byref(n), ctypes.byref(tot), None,
+                                       None, None)
+    if rc != SCPHD_OK:
+        raise ScphdError(rc, f"cannot read {varname} from {path}")
+    cnt = np.zeros(max(n.value, 1), dtype=np.int32)
+    u = np.zeros(max(tot.value, 1))
+    v = np.zeros(max(tot.value, 1))
+    rc = L.scphd_mat_read_measurements(path.encode(), varname.encode(), ctypes.byref(n), ctypes.byref(tot), _ip(cnt),
+                                       _dp(u), _dp(v))
+    if rc != SCPHD_OK:
+        raise ScphdError(rc, f"cannot read {varname} from {path}")
+    off = np.concatenate([[0], np.cumsum(cnt[:n.value])])
+    return [(u[off[k]:off[k + 1]].copy(), v[off[k]:off[k + 1]].copy()) for k in range(n.value)]
 
 
 def load_params(path):
@@ -334,6 +359,10 @@ class Filter:
         self._check(lib().scphd_get_estimate(self.h, _dp(m), _dp(c), _dp(card)), "get_estimate")
         return m, c, float(card[0])
 
+    def write_mat(self, path):
+        """writeMat: parents and the highest-weight parent's map (scphd_write_mat)."""
+        self._check(lib().scphd_write_mat(self.h, path.encode()), "write_mat")
+
     def memory(self):
         b = _I64(0)
         self._check(lib().scphd_get_memory(self.h, ctypes.byref(b)), "get_memory")
@@ -411,6 +440,11 @@ class Batch:
     def remove_rig(self, rig):
         self._check(lib().scphd_batch_remove_rig(self.h, int(rig)), "remove_rig")
         self.params.pop(rig, None)
+
+    def set_output(self, rig, directory):
+        """Write <directory>/<frame_id>.mat after each of the rig's frames (None stops)."""
+        d = directory.encode() if directory else None
+        self._check(lib().scphd_batch_set_output(self.h, int(rig), d), "set_output")
 
     def submit(self, rig, frame_id, dt, u, v, fixed):
         u = np.ascontiguousarray(u, dtype=np.float64)

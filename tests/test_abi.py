@@ -156,3 +156,112 @@ def test_batch_admission_is_refused_beyond_the_budget():
         assert b.collect() == []
     finally:
         b.close()
+
+
+# ------------------------------------------------------------------ MAT I/O
+def _cells(frames):
+    c = np.empty((1, len(frames)), dtype=object)
+    for k, (u, v) in enumerate(frames):
+        c[0, k] = np.vstack([u, v]) if len(u) else np.zeros((2, 0))
+    return c
+
+
+@pytest.mark.parametrize("compress", [False, True])
+def test_mat_measurement_reader_matches_scipy(tmp_path, compress):
+    import scipy.io
+
+    rng = np.random.default_rng(5)
+    z1 = [(rng.uniform(0, 640, n), rng.uniform(0, 480, n)) for n in (3, 0, 7, 1)]
+    z2 = [(rng.uniform(0, 640, n), rng.uniform(0, 480, n)) for n in (2, 5, 0, 4)]
+    path = str(tmp_path / "seq.mat")
+    scipy.io.savemat(path, {"Z1": _cells(z1), "Z2": _cells(z2), "other": np.eye(3)}, do_compression=compress)
+    for name, ref in (("Z1", z1), ("Z2", z2)):
+        got = scphd_amd.read_measurements(path, name)
+        assert len(got) == len(ref)
+        for (gu, gv), (ru, rv) in zip(got, ref):
+            np.testing.assert_array_equal(gu, ru)
+            np.testing.assert_array_equal(gv, rv)
+    with pytest.raises(scphd_amd.ScphdError, match="CONFIG"):
+        scphd_amd.read_measurements(path, "Z3")
+    with pytest.raises(scphd_amd.ScphdError, match="CONFIG"):
+        scphd_amd.read_measurements(path, "other")  # not a cell array
+
+
+REF_DATA = "/root/reference/data"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_DATA), reason="the reference's data files are not present")
+def test_mat_reader_reads_the_reference_sequences():
+    """The reference's own input sequences (data/*.mat) read as scipy reads them."""
+    import scipy.io
+
+    n = 0
+    for fn in sorted(os.listdir(REF_DATA)):
+        path = os.path.join(REF_DATA, fn)
+        try:
+            ref = scipy.io.loadmat(path)
+        except (NotImplementedError, ValueError):
+            continue  # not MAT level 5
+        for name in ("Z1", "Z2"):
+            if name not in ref:
+                continue
+            got = scphd_amd.read_measurements(path, name)
+            cells = ref[name].ravel(order="F")
+            assert len(got) == len(cells)
+            for (gu, gv), c in zip(got, cells):
+                c = np.asarray(c, dtype=np.float64).reshape(2, -1) if c.size else np.zeros((2, 0))
+                np.testing.assert_array_equal(gu, c[0])
+                np.testing.assert_array_equal(gv, c[1])
+            n += 1
+    assert n > 0
+
+
+# --------------------------------------------------------------- evaluation
+def test_ospa_matches_its_definition_on_known_cases():
+    import evaluate
+
+    X = np.array([[0.0, 0, 0], [1, 0, 0]])
+    assert evaluate.ospa(X, X, c=2, p=1) == 0
+    assert evaluate.ospa([], [], c=2) == 0
+    assert evaluate.ospa(X, np.zeros((0, 3)), c=2, p=2) == 2  # cardinality error only
+    # one missing point: ((0 + c^p) / 2)^(1/p)
+    assert abs(evaluate.ospa(X[:1], X, c=3, p=2) - math.sqrt(9 / 2)) < 1e-12
+    # assignment is optimal, not greedy: swapped order costs nothing
+    assert evaluate.ospa(X, X[::-1], c=5) == 0
+    # cut-off: a far point contributes c
+    Y = np.array([[0.0, 0, 0], [100, 0, 0]])
+    assert abs(evaluate.ospa(X, Y, c=2, p=1) - (0 + 2) / 2) < 1e-12
+    # symmetric and brute-force equal on random sets
+    rng = np.random.default_rng(3)
+    from itertools import permutations
+    for _ in range(20):
+        A, B = rng.normal(size=(3, 2)), rng.normal(size=(4, 2))
+        c, p = 1.5, 2.0
+        best = min(sum(min(c, np.linalg.norm(A[i] - B[j])) ** p for i, j in enumerate(perm))
+                   for perm in permutations(range(4), 3))
+        ref = ((best + c ** p * 1) / 4) ** (1 / p)
+        assert abs(evaluate.ospa(A, B, c, p) - ref) < 1e-12
+        assert abs(evaluate.ospa(B, A, c, p) - ref) < 1e-12
+
+
+def test_calibration_error_wraps_angles():
+    import evaluate
+
+    est = np.zeros(12)
+    est[[0, 1, 2]] = [0.21, 0.0, -0.01]
+    est[[6, 7, 8]] = [math.pi - 0.01, 0.0, -0.02]
+    err = evaluate.calibration_error(est, [0.2, 0.0, 0.0, -math.pi + 0.01, 0.0, 0.0])
+    np.testing.assert_allclose(err, [0.01, 0.0, -0.01, -0.02, 0.0, -0.02], atol=1e-12)
+
+
+def test_map_points_keep_heavy_gaussians():
+    import evaluate
+
+    S = 4
+    gc = np.array([1, 2])
+    wg = np.array([0.9, 0.4, 0.7])
+    smp = np.zeros((7, 3 * S))
+    for g in range(3):
+        smp[:3, g * S:(g + 1) * S] = g + 1
+    np.testing.assert_array_equal(evaluate.map_points(gc, wg, smp, 1, S), [[3.0, 3.0, 3.0]])
+    np.testing.assert_array_equal(evaluate.map_points(gc, wg, smp, 0, S), [[1.0, 1.0, 1.0]])
