@@ -206,6 +206,16 @@ int scphd_get_ancestors(scphd_t* f, int32_t* ancestors);
 int scphd_set_timing(scphd_t* f, int32_t enabled);
 int scphd_get_phase_ms(scphd_t* f, double* ms8);
 
+/* Noise source of the handle.  mode 0: the counter-based Philox4x32-10
+ * streams, keyed by seed from the next call on (the same key as
+ * scphd_params.seed).  Injected noise buffers (the reference's parity
+ * protocol for an oracle that cannot reproduce the device generator) are not
+ * needed here -- the FP64 oracle reproduces these streams bit for bit -- and
+ * any other mode, or a non-NULL injected, is SCPHD_ERR_ARG.  No counterpart
+ * in the reference, whose engines are seeded from time()
+ * (src/scphdcameracalibration.cu:1047-1118). */
+int scphd_set_noise(scphd_t* f, int32_t mode, uint64_t seed, const double* injected, size_t n);
+
 /* Calibration estimate from the current parents, computed on the device:
  * mean12 = sum_p w_p x_p (w normalised), cov144 = sum_p w_p (x_p - mean)
  * (x_p - mean)^T (12 x 12, row-major), *cardinality = sum of the Gaussian

@@ -830,6 +830,15 @@ int scphd_get_phase_ms(scphd_t* f, double* ms8) {
     return SCPHD_OK;
 }
 
+int scphd_set_noise(scphd_t* f, int32_t mode, uint64_t seed, const double* injected, size_t n) {
+    if (check_handle(f)) return SCPHD_ERR_ARG;
+    if (mode != 0 || injected || n)
+        return fail(f, SCPHD_ERR_ARG, "only mode 0 (Philox streams keyed by seed) is provided");
+    f->prm.seed = seed;
+    f->key = NoiseKey::from_seed(seed);
+    return SCPHD_OK;
+}
+
 int scphd_get_estimate(scphd_t* f, double* mean12, double* cov144, double* cardinality) {
     if (check_handle(f)) return SCPHD_ERR_ARG;
     HIPTRY(hipSetDevice(f->dev));

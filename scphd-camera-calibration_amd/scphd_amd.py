@@ -30,7 +30,7 @@ EXPORTED = [
     "scphd_predict", "scphd_update", "scphd_update_device", "scphd_resample", "scphd_check", "scphd_sync",
     "scphd_get_counts", "scphd_get_parents", "scphd_get_maps", "scphd_set_state", "scphd_get_update_stats",
     "scphd_get_update_debug", "scphd_get_ancestors", "scphd_set_timing", "scphd_get_phase_ms",
-    "scphd_get_fit_debug", "scphd_get_estimate", "scphd_get_memory",
+    "scphd_get_fit_debug", "scphd_get_estimate", "scphd_get_memory", "scphd_set_noise",
     "scphd_batch_opts_init", "scphd_batch_create", "scphd_batch_destroy", "scphd_batch_last_error",
     "scphd_batch_now", "scphd_rig_footprint", "scphd_batch_add_rig", "scphd_batch_remove_rig",
     "scphd_batch_submit", "scphd_batch_step", "scphd_batch_collect", "scphd_batch_get_stats",
@@ -148,6 +148,7 @@ def lib():
         "scphd_get_phase_ms": (_I32, [_P, dp]),
         "scphd_get_fit_debug": (_I32, [_P, dp]),
         "scphd_get_estimate": (_I32, [_P, dp, dp, dp]),
+        "scphd_set_noise": (_I32, [_P, _I32, _U64, dp, ctypes.c_size_t]),
         "scphd_get_memory": (_I32, [_P, ctypes.POINTER(_I64)]),
         "scphd_batch_opts_init": (_I32, [ctypes.POINTER(BatchOpts)]),
         "scphd_batch_create": (_I32, [ctypes.POINTER(BatchOpts), ctypes.POINTER(_P)]),
@@ -358,6 +359,10 @@ class Filter:
         card = np.zeros(1)
         self._check(lib().scphd_get_estimate(self.h, _dp(m), _dp(c), _dp(card)), "get_estimate")
         return m, c, float(card[0])
+
+    def set_noise(self, seed, mode=0):
+        """Re-key the Philox noise streams (scphd_set_noise)."""
+        self._check(lib().scphd_set_noise(self.h, int(mode), int(seed) & 0xFFFFFFFFFFFFFFFF, None, 0), "set_noise")
 
     def write_mat(self, path):
         """writeMat: parents and the highest-weight parent's map (scphd_write_mat)."""

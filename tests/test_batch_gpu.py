@@ -133,3 +133,24 @@ def test_estimate_matches_parents_read_back():
     off = np.concatenate([[0], np.cumsum(gc)])
     assert abs(card - wg[off[best]:off[best + 1]].sum()) <= 1e-12 * max(1.0, card)
     f.close()
+
+
+def test_set_noise_rekeys_the_streams():
+    import scphd_amd
+
+    p, s, sets = make(2, frames=2)
+    runs = []
+    for reseed in (None, 9, 9):
+        f = scphd_amd.Filter(p)
+        if reseed is not None:
+            f.set_noise(reseed)
+        for u, v, fixed in sets:
+            f.predict(s["dt"])
+            f.update(u, v, fixed)
+        runs.append(f.state())
+        if reseed is not None:
+            with pytest.raises(scphd_amd.ScphdError, match="ARG"):
+                f.set_noise(1, mode=1)
+        f.close()
+    np.testing.assert_array_equal(runs[1]["samples"], runs[2]["samples"])
+    assert not np.array_equal(runs[0]["states"], runs[1]["states"])
