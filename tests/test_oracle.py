@@ -331,6 +331,64 @@ def test_merge_greedy_order_and_ties():
     np.testing.assert_allclose(om[0], m0 + 100 * step)
 
 
+def _walk_groups(w, m, P, thr):
+    """Greedy merge groups formed by one pass over the (weight desc, index
+    asc) order with precomputed pairwise membership -- the phased form the
+    GPU merge uses: a leader takes its unmerged members; a leader that is not
+    a member of its own group (non-PD covariance) stays selectable, is taken
+    again with nothing left and ends the loop."""
+    n = len(w)
+    mem = [[oracle.mahal(m[i], P[i], m[j], P[j]) < thr for j in range(n)] for i in range(n)]
+    order = sorted(range(n), key=lambda i: (-w[i], i))
+    flag = [False] * n
+    groups, r = [], 0
+    while r < n:
+        L = order[r]
+        if flag[L]:
+            r += 1
+            continue
+        g = [j for j in range(n) if not flag[j] and mem[L][j]]
+        if not g:
+            break
+        for j in g:
+            flag[j] = True
+        groups.append((L, g))
+        if mem[L][L]:
+            r += 1
+    return groups
+
+
+def test_merge_groups_from_a_single_walk_equal_the_round_loop():
+    """The oracle's merge is the reference's round loop (re-evaluating every
+    unmerged term against each new leader); the walk over pairwise
+    membership must give the same groups: same count, same weights, same
+    moment-matched means.  Mixtures with clusters, chains, weight ties and a
+    non-PD covariance."""
+    P0 = params(min_separation=4.0)
+    rng = np.random.default_rng(12)
+    for trial in range(12):
+        n = int(rng.integers(2, 14))
+        centres = rng.normal(0, 3.0, size=(3, 6))
+        ms = centres[rng.integers(0, 3, n)] + rng.normal(0, 1.0, size=(n, 6))
+        Ps = [spd(rng) for _ in range(n)]
+        ws = rng.choice([0.1, 0.2, 0.3], size=n)  # ties are common
+        if trial % 3 == 0:
+            Ps[int(rng.integers(0, n))] = -np.eye(6)  # not PD: never a member of anything
+        ow, om, _ = oracle.merge(P0, ws, ms, Ps)
+        groups = _walk_groups(ws, ms, Ps, 4.0)
+        assert len(ow) == len(groups), trial
+        for k, (L, g) in enumerate(groups):
+            W = sum(ws[j] for j in g)
+            mu = sum(ws[j] * ms[j] for j in g) / W
+            C = sum(ws[j] * (Ps[j] + np.outer(mu - ms[j], mu - ms[j])) for j in g) / W
+            if np.all(np.linalg.eigvalsh((C + C.T) / 2) > 0):
+                assert ow[k] == pytest.approx(W, rel=1e-12), (trial, k)
+                np.testing.assert_allclose(om[k], mu, rtol=1e-9, atol=1e-12)
+            else:  # non-PD merge: the leader is emitted
+                assert ow[k] == ws[L]
+                np.testing.assert_array_equal(om[k], ms[L])
+
+
 # ------------------------------------------------------------- resampling
 def test_systematic_resampling_definition_and_counts():
     rng = np.random.default_rng(11)
