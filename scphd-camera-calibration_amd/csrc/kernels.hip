@@ -1052,37 +1052,70 @@ __global__ __launch_bounds__(NT) void k_merge_groups(MergeArgs a) {
         }
     }
     __syncthreads();
-    /* 5. the greedy walk */
-    if (t == 0) {
+    /* 5. the greedy walk, by wavefront 0: the next unmerged leader by a
+     * ballot over the sorted ranks, its unmerged neighbours (ascending, with
+     * the leader inserted at its index when it is its own member) compacted
+     * by ballots over its neighbour list */
+    if (t < 64) {
+        const int lane = t;
         int ng = 0, mp = 0, r = 0;
-        while (r < U) {
-            const int L = w.ord[r];
-            if (w.flag[L]) {
-                ++r;
-                continue;
-            }
-            const int st = mp;
-            const bool sm = w.selfm[L] != 0;
-            bool placed = !sm;
-            for (int e = w.nbo[L]; e < w.nbo[L + 1]; ++e) {
-                const int j = w.nbr[e];
-                if (!placed && j > L) {
-                    w.mem[mp++] = L;
-                    placed = true;
+        for (;;) {
+            int L = -1;
+            for (; r < U; r += 64) {
+                const int rr = r + lane;
+                const bool c = rr < U && !w.flag[w.ord[rr]];
+                const uint64_t bm = __ballot(c);
+                if (bm) {
+                    r += __ffsll((unsigned long long)bm) - 1;
+                    L = w.ord[r];
+                    break;
                 }
-                if (!w.flag[j]) w.mem[mp++] = j;
             }
-            if (!placed) w.mem[mp++] = L;
-            if (mp == st) break; /* empty group: zero weight, the reference stops */
-            for (int e = st; e < mp; ++e) w.flag[w.mem[e]] = 1;
-            a.glead[sbase + ng] = L;
-            a.gsta[sbase + ng] = st;
-            a.gend[sbase + ng] = mp;
+            if (L < 0) break;
+            const bool sm = w.selfm[L] != 0;
+            const int e0 = w.nbo[L], e1 = w.nbo[L + 1];
+            /* taken neighbours below L: the leader's slot when it is a member */
+            int nlt = 0, ntk = 0;
+            for (int e = e0; e < e1; e += 64) {
+                const int ei = e + lane;
+                const int j = ei < e1 ? w.nbr[ei] : 0x7fffffff;
+                const bool tk = ei < e1 && !w.flag[j];
+                nlt += __popcll(__ballot(tk && j < L));
+                ntk += __popcll(__ballot(tk));
+            }
+            const int gsz = ntk + (sm ? 1 : 0);
+            if (gsz == 0) break; /* empty group: zero weight, the reference stops */
+            const int st = mp;
+            int seen = 0;
+            for (int e = e0; e < e1; e += 64) {
+                const int ei = e + lane;
+                const int j = ei < e1 ? w.nbr[ei] : 0x7fffffff;
+                const bool tk = ei < e1 && !w.flag[j];
+                const uint64_t bm = __ballot(tk);
+                if (tk) {
+                    const int k = seen + __popcll(bm & ((1ull << lane) - 1ull));
+                    w.mem[st + k + ((sm && k >= nlt) ? 1 : 0)] = j;
+                }
+                seen += __popcll(bm);
+            }
+            if (lane == 0 && sm) w.mem[st + nlt] = L;
+            mp += gsz;
+            __builtin_amdgcn_wave_barrier();
+            __threadfence_block();
+            for (int e = st + lane; e < mp; e += 64) w.flag[w.mem[e]] = 1;
+            __threadfence_block();
+            if (lane == 0) {
+                a.glead[sbase + ng] = L;
+                a.gsta[sbase + ng] = st;
+                a.gend[sbase + ng] = mp;
+            }
             ++ng;
             if (sm) ++r; /* a leader outside its group is taken again, finds nothing, stops */
         }
-        a.ngroups[p] = ng;
-        a.nmerged[p] = ng;
+        if (lane == 0) {
+            a.ngroups[p] = ng;
+            a.nmerged[p] = ng;
+        }
     }
     __syncthreads();
     for (int i = t; i < U; i += NT) a.gmem[sbase + i] = w.mem[i];

@@ -58,6 +58,7 @@ def run_served(rigs, hbm_budget_bytes=0, n_workers=8, reserve_factor=1.5, progre
         waiting = []          # indices of arrived, not yet admitted rigs (FIFO)
         active = {}           # rig id -> state
         nxt = 0               # next arrival
+        blocked = False       # the head of the admission queue was refused and nothing has left since
         lat, svc, adm, pairs = [], [], [], 0
         done_rigs, peak_res, peak_used, peak_active, n_frames = 0, 0, 0, 0, 0
         t0 = b.now()
@@ -69,11 +70,12 @@ def run_served(rigs, hbm_budget_bytes=0, n_workers=8, reserve_factor=1.5, progre
             while nxt < len(rigs) and rigs[nxt]["t"] <= now:
                 waiting.append(nxt)
                 nxt += 1
-            while waiting:
+            while waiting and not blocked:
                 r = rigs[waiting[0]]
                 res = int(reserve_factor * scphd_amd.rig_footprint(r["p"], expected_gaussians_per_parent(r["s"])))
                 rid = b.add_rig(r["p"], res)
                 if rid is None:
+                    blocked = True  # offered again once a rig leaves
                     break
                 waiting.pop(0)
                 adm.append(now - r["t"])
@@ -100,6 +102,7 @@ def run_served(rigs, hbm_budget_bytes=0, n_workers=8, reserve_factor=1.5, progre
                 b.remove_rig(rid)
                 del active[rid]
                 done_rigs += 1
+                blocked = False
             st = b.stats()
             peak_res = max(peak_res, st["reserved_bytes"])
             peak_used = max(peak_used, st["used_bytes"])
