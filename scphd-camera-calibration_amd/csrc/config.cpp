@@ -37,6 +37,8 @@ namespace {
 struct Value {
     enum Kind { NUM, BOOL, STR, GROUP, LIST } kind = NUM;
     double num = 0.0;
+    bool is_int = false; /* an integer literal: exact value in u (two's complement when negative) */
+    uint64_t u = 0;
     bool b = false;
     std::string str;
     std::vector<Value> items;
@@ -134,13 +136,26 @@ struct Parser {
         while (!t.empty() && (t.back() == 'L' || t.back() == 'l')) t.pop_back();
         char* end = nullptr;
         double x;
-        if (t.size() > 2 && t[0] == '0' && (t[1] == 'x' || t[1] == 'X'))
-            x = (double)strtoull(t.c_str(), &end, 16);
-        else
+        bool integer = false;
+        uint64_t ui = 0;
+        if (t.size() > 2 && t[0] == '0' && (t[1] == 'x' || t[1] == 'X')) {
+            ui = strtoull(t.c_str(), &end, 16);
+            x = (double)ui;
+            integer = true;
+        } else {
+            size_t d = (t[0] == '-' || t[0] == '+') ? 1 : 0;
+            integer = d < t.size() && t.find_first_not_of("0123456789", d) == std::string::npos;
+            if (integer) {
+                char* e2 = nullptr;
+                ui = t[0] == '-' ? (uint64_t)strtoll(t.c_str(), &e2, 10) : strtoull(t.c_str(), &e2, 10);
+            }
             x = strtod(t.c_str(), &end);
+        }
         if (!end || *end != '\0') return fail("malformed number '" + tok + "'");
         v.kind = Value::NUM;
         v.num = x;
+        v.is_int = integer;
+        v.u = ui;
         return true;
     }
     bool value(Value& v, const std::string& path) {
@@ -248,6 +263,25 @@ struct Lookup {
             return false;
         }
         x = it->second.num;
+        return true;
+    }
+    /* an unsigned 64-bit setting, exact for integer literals */
+    bool u64(const std::string& k, uint64_t& x) {
+        auto it = m.find(k);
+        if (it == m.end() || it->second.kind != Value::NUM) {
+            if (err.empty()) err = "setting is not a number: " + k;
+            return false;
+        }
+        const Value& v = it->second;
+        if (v.is_int) {
+            x = v.u;
+            return true;
+        }
+        if (v.num != std::floor(v.num) || v.num < 0 || v.num >= 9007199254740992.0) {
+            if (err.empty()) err = "setting is not an integer: " + k;
+            return false;
+        }
+        x = (uint64_t)v.num;
         return true;
     }
     bool i32(const std::string& k, int32_t& x) {
@@ -402,11 +436,7 @@ int scphd_params_load(const char* cfg_path, scphd_params* p, char* errbuf, size_
         q.slam = sl != 0;
     }
     L.opt_i32("max_gaussians", q.max_gaussians);
-    if (L.has("seed")) {
-        double sd = 0;
-        ok &= L.num("seed", sd);
-        q.seed = (uint64_t)sd;
-    }
+    if (L.has("seed")) ok &= L.u64("seed", q.seed);
     if (L.has("storage")) {
         const Value& v = ps.out.at("storage");
         if (v.kind == Value::STR)

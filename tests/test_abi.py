@@ -265,3 +265,14 @@ def test_map_points_keep_heavy_gaussians():
         smp[:3, g * S:(g + 1) * S] = g + 1
     np.testing.assert_array_equal(evaluate.map_points(gc, wg, smp, 1, S), [[3.0, 3.0, 3.0]])
     np.testing.assert_array_equal(evaluate.map_points(gc, wg, smp, 0, S), [[1.0, 1.0, 1.0]])
+
+
+def test_seed_is_parsed_exactly_to_64_bits(tmp_path):
+    p, _ = workloads.workload("c0")
+    for sd in (0, 2 ** 53 + 1, 2 ** 64 - 59):
+        cfg = tmp_path / f"s{sd}.cfg"
+        cfg.write_text(workloads.to_cfg(dict(p, seed=sd)))
+        assert scphd_amd.load_params(str(cfg))["seed"] == sd
+    cfg = tmp_path / "hex.cfg"
+    cfg.write_text(workloads.to_cfg(p).replace(f"seed = {p['seed']}", "seed = 0xFFFFFFFFFFFFFFC5L"))
+    assert scphd_amd.load_params(str(cfg))["seed"] == 2 ** 64 - 59
