@@ -148,8 +148,9 @@ def main():
     ap.add_argument("--workload", default="c1")
     ap.add_argument("--storage", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--cpu-parents", type=int, default=512)
-    ap.add_argument("--cpu-warm", type=int, default=12)
-    ap.add_argument("--cpu-timed", type=int, default=16)
+    ap.add_argument("--cpu-warm", type=int, default=3, help="untimed oracle frames (maps reach their "
+                                                             "steady size after 2)")
+    ap.add_argument("--cpu-timed", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--served-rate", type=float, default=2.0, help="rig arrivals per second")
