@@ -485,6 +485,24 @@ class Batch:
         self._check(lib().scphd_batch_get_stats(self.h, ctypes.byref(s)), "get_stats")
         return {n: getattr(s, n) for n, _ in BatchStats._fields_}
 
+    def snapshot(self, rig):
+        """Evict a rig: its complete state (parents, maps, noise counters) as
+        host arrays, then the rig is removed and its reservation freed.  Call
+        while none of its frames is queued or running."""
+        state = self.filter(rig).state()
+        params = self.params[rig]
+        self.remove_rig(rig)
+        return dict(params=params, state=state)
+
+    def resume(self, snap, reserve_bytes=0):
+        """Re-admit an evicted rig with its state; returns the new rig id, or
+        None when the budget does not admit it yet.  The rig continues
+        exactly where it stopped."""
+        rig = self.add_rig(snap["params"], reserve_bytes)
+        if rig is not None:
+            self.filter(rig).set_state(snap["state"])
+        return rig
+
     def filter(self, rig):
         """The rig's filter for read-back while none of its frames is queued or running."""
         h = lib().scphd_batch_filter(self.h, int(rig))
