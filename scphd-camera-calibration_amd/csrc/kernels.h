@@ -86,7 +86,7 @@ struct FitArgs {
     double min_weight;
     double* fit;     /* FP_COUNT planes */
     int32_t* keep;   /* non-detection term kept */
-    int32_t* redo;   /* per Gaussian: bit 0 all-zero weights, bit 1 cloud still to write */
+    int32_t* redo;   /* per Gaussian: 1 when every sample weight was zero (refit by k_fit_fix) */
     int64_t nd_base; /* destination block of the non-detection cloud of entry g: nd_base + g */
     int off32;       /* every source and destination plane is below 4 GiB: 32-bit byte offsets */
 };

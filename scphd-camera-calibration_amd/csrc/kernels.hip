@@ -277,13 +277,6 @@ __device__ inline void fit_sample(const FitArgs& a, const Planes<T>& src, int64_
     eval_sample(a, r, p, k, s, pose, x, d, in, w);
 }
 
-template <typename T>
-__device__ inline void write_nd(const FitArgs& a, Planes<T>& dst, int64_t di, const EuclideanPoint& x, bool in) {
-    dst.p[0][di] = (T)x.x; dst.p[1][di] = (T)x.y; dst.p[2][di] = (T)x.z;
-    dst.p[3][di] = (T)x.vx; dst.p[4][di] = (T)x.vy; dst.p[5][di] = (T)x.vz;
-    dst.p[6][di] = (T)(1.0 - (in ? a.meas.pd() : 0.0));
-}
-
 __device__ inline void write_fit(const FitArgs& a, int64_t g, int lane, const double* mu, const double* P, double pdg) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -317,13 +310,17 @@ __device__ inline void write_fit(const FitArgs& a, int64_t g, int lane, const do
  * computeNonDetections src/scphdcameracalibration.cu:1290-1420; sample
  * weights 1 - pd_i; the samples themselves are the predicted cloud, which
  * the reference's world -> disparity -> world round trip reproduces to
- * rounding) is written during the sweep when w_g alone settles the
- * decision.  Two rare cases are left to k_fit_fix through redo[g]: a
- * kept cloud whose decision needed pd_g (bit 1), and a cloud whose weights
- * are all zero (bit 0), which the reference normalises to weight 1 per
- * sample. */
+ * rounding) is written during the sweep for every term not certainly
+ * dropped.  A cloud whose weights are all zero, which the reference
+ * normalises to weight 1 per sample, is refitted by k_fit_fix (redo[g]). */
+/* waves per SIMD the fit kernel is compiled for (register budget 512 / n) */
+#ifndef SCPHD_FIT_WAVES
+#define SCPHD_FIT_WAVES 2
+#endif
+
 template <typename T, typename Off>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_fit(FitArgs a, Planes<T> src, Planes<T> dst) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_WAVES, SCPHD_FIT_WAVES))) void k_fit(
+    FitArgs a, Planes<T> src, Planes<T> dst) {
     /* the row's camera pose lives in LDS and is re-read per sample, which
      * keeps 24 registers free for occupancy */
     __shared__ double s_pose[16 * 12];
@@ -340,10 +337,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     const int S = a.S;
     const double wg = a.gw[g];
     const double Pd = a.meas.pd();
-    /* keep decided before the sweep: certainly kept (1 - pd_g >= 1 - Pd up
-     * to rounding of the Pd sum) or certainly dropped (1 - pd_g <= 1) */
-    const bool sure_keep = wg * (1.0 - Pd) * (1.0 - 1e-12) >= a.min_weight;
-    const bool sure_drop = !(wg >= a.min_weight);
+    /* the non-detection cloud is written during the sweep unless the term is
+     * certainly dropped (1 - pd_g <= 1); whether it is kept is decided after
+     * the sweep, and an unkept cloud is simply not referenced by the next
+     * map table (a write is cheaper than a second sweep) */
+    const bool nd_write = wg >= a.min_weight;
     const int64_t d0 = (a.nd_base + g) * (int64_t)S;
 
     double spd = 0.0, sw = 0.0, sw2 = 0.0, m1[6], m2[21], K[6];
@@ -396,7 +394,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         for (int r = 0; r < 6; ++r)
 #pragma unroll
             for (int c = 0; c <= r; ++c) m2[pidx(r, c)] += wp * dv[r] * dv[c];
-        if (sure_keep && valid) write_nd_off(a, dst, dof, x, in);
+        if (nd_write && valid) write_nd_off(a, dst, dof, x, in);
         dof += (Off)(16 * sizeof(T));
     }
     spd = row_sum16(spd);
@@ -428,15 +426,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     const bool keep = wg * (1 - pdg) >= a.min_weight;
     if (lane == 0) {
         a.keep[g] = keep ? 1 : 0;
-        a.redo[g] = (sw > 0 ? 0 : 1) | ((keep && !sure_keep && !sure_drop) ? 2 : 0);
+        a.redo[g] = sw > 0 ? 0 : 1;
     }
 }
 
-/* Rare cases of k_fit, one row per flagged Gaussian: bit 0, every weight
- * zero: unit weights (mean = sum of the samples, as the reference's
- * normalisation of an all-zero cloud gives), refitted by two sweeps;
- * bit 1, the non-detection cloud of a kept Gaussian whose decision needed
- * pd_g. */
+/* The rare case of k_fit, one row per flagged Gaussian: every weight zero.
+ * Refitted by two sweeps with unit weights (mean = sum of the samples, as
+ * the reference's normalisation of an all-zero cloud gives). */
 template <typename T>
 __global__ __launch_bounds__(256) void k_fit_fix(FitArgs a, Planes<T> src, Planes<T> dst) {
     const int lane = threadIdx.x & 15;
@@ -488,17 +484,6 @@ __global__ __launch_bounds__(256) void k_fit_fix(FitArgs a, Planes<T> src, Plane
         for (int e = 0; e < 21; ++e) P[e] = row_sum16(cc[e]) * scale;
         const double pdg = a.fit[(int64_t)FP_PD * a.Ng + g];
         write_fit(a, g, lane, mu, P, pdg);
-    }
-    if (redo & 2) {
-        const int64_t d0 = (a.nd_base + g) * (int64_t)S;
-        for (int s = lane; s < S; s += 16) {
-            EuclideanPoint x;
-            DisparityPoint d;
-            bool in;
-            double w;
-            fit_sample(a, src, b0 + s, p, k, s, pose, x, d, in, w);
-            write_nd(a, dst, d0 + s, x, in);
-        }
     }
 }
 

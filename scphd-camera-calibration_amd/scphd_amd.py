@@ -13,7 +13,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libscphd_hip.so")
+# SCPHD_LIBRARY selects another build of the library (e.g. a variant under test)
+LIB_PATH = os.environ.get("SCPHD_LIBRARY") or os.path.join(HERE, "libscphd_hip.so")
 
 SCPHD_OK = 0
 ERRORS = {-1: "SCPHD_ERR_ARG", -2: "SCPHD_ERR_HIP", -3: "SCPHD_ERR_NAN", -4: "SCPHD_ERR_CAPACITY",

@@ -51,8 +51,12 @@ def _pct(x, q):
     return float(np.percentile(x, q)) if len(x) else None
 
 
-def run_served(rigs, hbm_budget_bytes=0, n_workers=8, reserve_factor=1.5, progress=None, max_wall=None):
-    """Replay prepared rigs (see prepare) in real time; returns a summary dict."""
+def run_served(rigs, hbm_budget_bytes=0, n_workers=8, reserve_factor=1.5, progress=None, max_wall=None,
+               paced=True):
+    """Replay prepared rigs (see prepare) in real time; returns a summary
+    dict.  paced=False submits all of a rig's frames when it is admitted and
+    admits every rig at once (as the budget allows): the saturated
+    throughput of the served form."""
     b = scphd_amd.Batch(hbm_budget_bytes=hbm_budget_bytes, n_workers=n_workers)
     try:
         waiting = []          # indices of arrived, not yet admitted rigs (FIFO)
@@ -67,7 +71,7 @@ def run_served(rigs, hbm_budget_bytes=0, n_workers=8, reserve_factor=1.5, progre
             now = b.now() - t0
             if max_wall is not None and now > max_wall:
                 break
-            while nxt < len(rigs) and rigs[nxt]["t"] <= now:
+            while nxt < len(rigs) and (rigs[nxt]["t"] <= now or not paced):
                 waiting.append(nxt)
                 nxt += 1
             while waiting and not blocked:
@@ -78,15 +82,15 @@ def run_served(rigs, hbm_budget_bytes=0, n_workers=8, reserve_factor=1.5, progre
                     blocked = True  # offered again once a rig leaves
                     break
                 waiting.pop(0)
-                adm.append(now - r["t"])
+                adm.append(now - r["t"] if paced else 0.0)
                 active[rid] = dict(r=r, t_admit=now, sent=0, done=0, due={})
             for rid, a in active.items():
                 fps_dt = a["r"]["s"]["dt"]
                 fr = a["r"]["frames"]
-                while a["sent"] < len(fr) and a["t_admit"] + a["sent"] * fps_dt <= now:
+                while a["sent"] < len(fr) and (a["t_admit"] + a["sent"] * fps_dt <= now or not paced):
                     k = a["sent"]
                     u, v, fixed = fr[k]
-                    a["due"][k] = t0 + a["t_admit"] + k * fps_dt
+                    a["due"][k] = t0 + (a["t_admit"] + k * fps_dt if paced else now)
                     b.submit(rid, k, fps_dt, u, v, fixed)
                     a["sent"] += 1
             for x in b.collect():
@@ -126,7 +130,8 @@ def run_served(rigs, hbm_budget_bytes=0, n_workers=8, reserve_factor=1.5, progre
                 service_ms=dict(p50=_pct(svc, 50), p99=_pct(svc, 99)),
                 admission_delay_s=dict(p50=_pct(adm, 50), p99=_pct(adm, 99), max=float(adm.max()) if len(adm) else None),
                 peak_active_rigs=peak_active, peak_reserved_gb=peak_res / 1e9, peak_used_gb=peak_used / 1e9,
-                refused_admissions=st["refused"], budget_gb=st["budget_bytes"] / 1e9, n_workers=n_workers)
+                refused_admissions=st["refused"], budget_gb=st["budget_bytes"] / 1e9, n_workers=n_workers,
+                paced=paced)
 
 
 def run_plain(rigs, progress=None):

@@ -96,8 +96,10 @@ def test_admission_removal_and_readmission():
         assert b.add_rig(make(2)[0], need) is None
         assert b.stats()["refused"] == 1
         b.remove_rig(r0)
+        with pytest.raises(scphd_amd.ScphdError):  # a removed rig takes no frames
+            b.submit(r0, 0, 0.5, [1.0], [1.0], True)
         r2 = b.add_rig(make(2)[0], need)
-        assert r2 is not None
+        assert r2 == r0  # the freed slot is reused
         s = b.stats()
         assert s["rigs"] == 2 and s["reserved_bytes"] == 2 * need
         # the re-admitted rig runs
@@ -107,7 +109,7 @@ def test_admission_removal_and_readmission():
         assert b.step() == 2
         assert [x["status"] for x in b.collect()] == [0, 0]
         with pytest.raises(scphd_amd.ScphdError):
-            b.submit(r0, 0, 0.5, [1.0], [1.0], True)
+            b.submit(999, 0, 0.5, [1.0], [1.0], True)
     finally:
         b.close()
 
