@@ -162,7 +162,7 @@ def main():
     ap.add_argument("--cpu-parents", type=int, default=512)
     ap.add_argument("--cpu-warm", type=int, default=3, help="untimed oracle frames (maps reach their "
                                                              "steady size after 2)")
-    ap.add_argument("--cpu-timed", type=int, default=8)
+    ap.add_argument("--cpu-timed", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--served-rate", type=float, default=2.0, help="rig arrivals per second")

@@ -35,6 +35,9 @@
  *   D9  positive-definiteness: relative Cholesky pivot > 1e-9 instead of
  *       "no NaN in the factor" (isPosDef, src/device_math.cu:800-812); see
  *       is_pos_def6.
+ *   D13 the fit of a cloud with pd_g = 0 (entirely out of view) is its first
+ *       sample with zero covariance; the reference refits it with unit
+ *       weights, but its detection terms have weight 0 either way.
  * With storage32 set, the sample cloud (positions, velocities, weights) is
  * rounded to float at the end of every update, as the GPU's FP32 storage
  * does; all arithmetic stays FP64.
@@ -677,11 +680,19 @@ static int update(State& st, const double* zu, const double* zv, int M, int fixe
                 w[s] = st.smp[6][i] * pd;
             }
             pdg[k] = pdsum / S;
-            normalise_weights(w.data(), S);
-            /* fitGaussiansKernel, src/kernels.cu:102-438 */
             Comp& f = pred[k];
             f.w = st.wg[g];
             const double* q0 = &dsp[(size_t)k * S * 6];
+            if (pdsum == 0.0) {
+                /* D13: a cloud entirely out of view (pd_g = 0) has detection
+                 * terms of weight 0, so its fit cannot reach any output; it
+                 * is defined as the first sample with zero covariance */
+                for (int c = 0; c < 6; ++c) f.m[c] = q0[c];
+                for (int c = 0; c < 36; ++c) f.P[c] = 0.0;
+                continue;
+            }
+            normalise_weights(w.data(), S);
+            /* fitGaussiansKernel, src/kernels.cu:102-438 */
             for (int c = 0; c < 6; ++c) {
                 double a = 0.0;
                 for (int s = 0; s < S; ++s) a += q0[s * 6 + c] * w[s];

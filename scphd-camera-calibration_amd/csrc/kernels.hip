@@ -312,7 +312,9 @@ __device__ inline void write_fit(const FitArgs& a, int64_t g, int lane, const do
  * the reference's world -> disparity -> world round trip reproduces to
  * rounding) is written during the sweep for every term not certainly
  * dropped.  A cloud whose weights are all zero, which the reference
- * normalises to weight 1 per sample, is refitted by k_fit_fix (redo[g]). */
+ * normalises to weight 1 per sample, is refitted by k_fit_fix (redo[g]) when
+ * pd_g > 0; with pd_g = 0 its fit cannot reach any output and is the first
+ * sample with zero covariance. */
 /* waves per SIMD the fit kernel is compiled for (register budget 512 / n) */
 #ifndef SCPHD_FIT_WAVES
 #define SCPHD_FIT_WAVES 2
@@ -416,7 +418,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
         dm[c] = m1[c] * r;
         mu[c] = K[c] + dm[c];
     }
-    const double scale = (W * W + W2) / W;
+    /* W = 0: with pd_g = 0 the fit is the first sample with zero covariance
+     * (its detection terms have weight 0); with pd_g > 0 (possible only when
+     * Pd = 1) k_fit_fix refits it with unit weights, as the reference does */
+    const double scale = W > 0 ? (W * W + W2) / W : 0.0;
 #pragma unroll
     for (int i = 0; i < 6; ++i)
 #pragma unroll
@@ -426,7 +431,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
     const bool keep = wg * (1 - pdg) >= a.min_weight;
     if (lane == 0) {
         a.keep[g] = keep ? 1 : 0;
-        a.redo[g] = sw > 0 ? 0 : 1;
+        a.redo[g] = (sw > 0 || spd == 0.0) ? 0 : 1;
     }
 }
 

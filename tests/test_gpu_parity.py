@@ -119,6 +119,16 @@ def test_lockstep_samples_per_gaussian_200():
     run_lockstep(p, s, frames=4)
 
 
+def test_lockstep_pd_one_refits_all_zero_clouds():
+    """Pd = 1: a kept non-detection cloud weighs 0 wherever it is in view, so
+    in the next update every w_i pd_i is 0 while pd_g > 0 -- the cloud is
+    refitted with unit weights (k_fit_fix), as the reference normalises an
+    all-zero cloud to weight 1 per sample."""
+    p, s = small(1, n_particles=24, pd=1.0)
+    rep = run_lockstep(p, s, frames=5)
+    assert rep[-1][1] > 0
+
+
 def test_empty_measurement_set():
     """M = 0: no births, no detection terms; only non-detection terms and
     the cardinality term of the parent likelihood remain."""
