@@ -59,27 +59,37 @@ struct Generation {
         }
         cap_blocks = 0;
     }
-    /* grow to at least need blocks keeping the first keep blocks */
+    /* grow to at least need blocks keeping the first keep blocks.  Planes
+     * are replaced one at a time (allocate, copy the kept blocks, free the
+     * old plane), so the transient is one plane, not a whole generation; if
+     * an allocation fails, every plane still holds the kept blocks and at
+     * least the old capacity, which stays the recorded one. */
     hipError_t ensure(int64_t need, int64_t keep, int S, size_t esz, hipStream_t s) {
         if (need <= cap_blocks) return hipSuccess;
         const int64_t cap = std::max<int64_t>(need + need / 4, 16);
-        void* np[7] = {};
+        if (keep <= 0) {
+            release(); /* nothing to keep: free first, then allocate */
+        }
         for (int c = 0; c < 7; ++c) {
-            hipError_t e = hipMalloc(&np[c], (size_t)cap * S * esz);
+            void* np = nullptr;
+            hipError_t e = hipMalloc(&np, (size_t)cap * S * esz);
             if (e != hipSuccess) {
-                for (int d = 0; d < c; ++d) (void)hipFree(np[d]);
+                if (keep <= 0) release();
                 return e;
             }
             if (keep > 0 && pl[c]) {
-                e = hipMemcpyAsync(np[c], pl[c], (size_t)keep * S * esz, hipMemcpyDeviceToDevice, s);
-                if (e != hipSuccess) return e;
+                e = hipMemcpyAsync(np, pl[c], (size_t)keep * S * esz, hipMemcpyDeviceToDevice, s);
+                if (e == hipSuccess) e = hipStreamSynchronize(s);
+                if (e != hipSuccess) {
+                    (void)hipFree(np);
+                    return e;
+                }
             }
+            if (pl[c]) (void)hipFree(pl[c]);
+            pl[c] = np;
         }
-        hipError_t e = hipStreamSynchronize(s);
-        release();
-        for (int c = 0; c < 7; ++c) pl[c] = np[c];
         cap_blocks = cap;
-        return e;
+        return hipSuccess;
     }
     template <typename T>
     Planes<T> planes() const {

@@ -6,7 +6,15 @@ and compared frame by frame.  At this size the FP64 CPU oracle is out of
 reach; tests/test_precision_gpu.py compares both modes against it on the
 same scene at 128 parents.
 
-    python run_c3.py [--parents 16384] [--frames 200]
+    python run_c3.py [--parents 16384] [--frames 30] [--fp32-only]
+
+Maps are uncapped (max_gaussians = 0, as in the reference).  They grow
+through the run -- landmarks that leave the view keep their non-detection
+clouds for ~180 frames (weight x ps per frame) -- from ~6 M Gaussians at
+frame 0 to ~25 M at frame 80 at 16384 parents, so with FP64 storage
+(56 B per sample, two generations, and a third while a generation grows)
+the comparison fits at 8192 parents; --fp32-only runs the plain path alone
+(e.g. all 200 frames at 16384 parents) and reports its throughput.
 
 Per frame (JSON lines on stdout): extrinsic estimate of each mode and their
 relative difference (x, y, z, theta, phi, psi), map sizes, and whether the
@@ -51,12 +59,18 @@ def run(p, s, sc, frames, log):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--parents", type=int, default=16384)
-    ap.add_argument("--frames", type=int, default=200)
+    ap.add_argument("--frames", type=int, default=30)
+    ap.add_argument("--fp32-only", action="store_true")
     args = ap.parse_args(argv)
-    p, s = workloads.workload("c3", n_particles=args.parents, n_frames=args.frames)
+    p, s = workloads.workload("c3", n_particles=args.parents, n_frames=args.frames, max_gaussians=0)
     sc = scene.generate(seed=p["seed"], **s)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     r32, t32 = run(dict(p, storage=0), s, sc, args.frames, log)
+    if args.fp32_only:
+        print(json.dumps(dict(summary=True, parents=args.parents, frames=args.frames, seconds_fp32=t32,
+                              ms_per_frame=t32 / args.frames * 1e3, final_gaussians=r32[-1]["ng"],
+                              final_estimate_fp32=list(r32[-1]["mean"][POSE]))))
+        return 0
     r64, t64 = run(dict(p, storage=1), s, sc, args.frames, log)
     worst, same_anc, same_ng = 0.0, 0, 0
     for k, (a, b) in enumerate(zip(r32, r64)):
