@@ -216,6 +216,14 @@ int scphd_get_phase_ms(scphd_t* f, double* ms8);
  * (src/scphdcameracalibration.cu:1047-1118). */
 int scphd_set_noise(scphd_t* f, int32_t mode, uint64_t seed, const double* injected, size_t n);
 
+/* Six standard normals per element elem = 0..n_elem-1 of one noise stream
+ * (counter: elem << 2 | block, gauss, parent, step << 4 | stream; key =
+ * seed), drawn on the device exactly as the filter kernels draw them, into
+ * out[6 * n_elem].  For verifying the device noise against a host
+ * restatement; no counterpart in the reference. */
+int scphd_debug_normals(uint64_t seed, uint32_t stream, uint64_t step, uint32_t parent, uint32_t gauss,
+                        int32_t n_elem, double* out);
+
 /* Calibration estimate from the current parents, computed on the device:
  * mean12 = sum_p w_p x_p (w normalised), cov144 = sum_p w_p (x_p - mean)
  * (x_p - mean)^T (12 x 12, row-major), *cardinality = sum of the Gaussian

@@ -325,13 +325,14 @@ static inline double u01(uint32_t lo, uint32_t hi) {  /* [0, 1) */
  *   c = (x1 >> 8) 2^-24: octant o = top 3 bits, offset t in [0, 1] (from
  *   the low 21 bits, mirrored in odd octants); sin(pi/4 t) = t S(t^2),
  *   cos(pi/4 t) = C(t^2), Taylor to t^9 / t^8;
- *   r = sqrt(-2 log a) (FP64); z0 = r cos 2 pi c, z1 = r sin 2 pi c. */
+ *   r = sqrt(-2 log a); z0 = r cos 2 pi c, z1 = r sin 2 pi c, all in single
+ *   precision (correctly rounded sqrt), widened to double. */
 static void box_muller(uint32_t x0, uint32_t x1, double& z0, double& z1) {
     static const float PLOG[9] = {1.0f,          -0x1.fffff8p-2f, 0x1.555790p-2f, -0x1.0005a6p-2f, 0x1.98b80ap-3f,
                                   -0x1.5329bep-3f, 0x1.322850p-3f, -0x1.26729ep-3f, 0x1.6626eap-4f};
     static const float PSIN[5] = {0x1.921fb6p-1f, -0x1.4abbcep-4f, 0x1.466bc6p-9f, -0x1.32d2ccp-15f, 0x1.507834p-22f};
     static const float PCOS[5] = {1.0f, -0x1.3bd3ccp-2f, 0x1.03c1f0p-6f, -0x1.55d3c8p-12f, 0x1.e1f506p-19f};
-    double la = 0.0;
+    float la = 0.0f;
     const uint64_t n = (uint64_t)x0 + 1;  /* in [1, 2^32] */
     if (n < (1ull << 32)) {
         int k = 0;  /* n = m 2^(k - 31) with m in [2^31, 2^32) */
@@ -349,9 +350,9 @@ static void box_muller(uint32_t x0, uint32_t x1, double& z0, double& z1) {
         const float s = f - 1.0f;
         float p = PLOG[8];
         for (int i = 7; i >= 0; --i) p = std::fmaf(p, s, PLOG[i]);
-        la = std::fma((double)e, 0x1.62e42fefa39efp-1, (double)(s * p));
+        la = std::fmaf((float)e, 0x1.62e430p-1f, s * p);
     }
-    const double r = std::sqrt(-2.0 * la);
+    const float r = std::sqrt(-2.0f * la);  /* float overload: correctly rounded */
     const uint32_t c = x1 >> 8;
     const uint32_t oct = c >> 21, off = c & 0x1FFFFF;
     const bool odd = (oct & 1) != 0;
@@ -372,8 +373,8 @@ static void box_muller(uint32_t x0, uint32_t x1, double& z0, double& z1) {
         case 2: sn = -sq; cs = -cq; break;
         default: sn = -cq; cs = sq; break;
     }
-    z0 = r * (double)cs;
-    z1 = r * (double)sn;
+    z0 = (double)(r * cs);
+    z1 = (double)(r * sn);
 }
 
 /* n (<= 8) standard normals, four per Philox block: words (x0, x1) give the

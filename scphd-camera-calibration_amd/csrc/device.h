@@ -77,8 +77,9 @@ SCPHD_HD inline double uniform_oc(uint32_t lo, uint32_t hi) { /* (0, 1] */
 }
 
 /* One Box-Muller pair from two Philox words (Box and Muller 1958),
- * evaluated with single-precision adds, multiplies and fused multiply-adds
- * only, so that any IEEE-754 implementation returns the same bits:
+ * evaluated with single-precision adds, multiplies, fused multiply-adds and
+ * the correctly rounded square root only, so that any IEEE-754
+ * implementation returns the same bits:
  *   a = (x0 + 1) 2^-32 in (0, 1], its leading 24 bits kept:
  *       a = f 2^e with f in [sqrt(1/2), sqrt(2)), log a = e ln2 + s P(s),
  *       s = f - 1, P the degree-8 Chebyshev fit of log(1 + s)/s on that
@@ -87,7 +88,8 @@ SCPHD_HD inline double uniform_oc(uint32_t lo, uint32_t hi) { /* (0, 1] */
  *       [0, 1] inside it come from the integer bits, sin(pi/4 t) = t S(t^2)
  *       and cos(pi/4 t) = C(t^2) are Taylor polynomials to t^9 and t^8
  *       (|error| < 4e-8);
- *   r = sqrt(-2 log a) in double; z0 = r cos 2 pi c, z1 = r sin 2 pi c. */
+ *   r = sqrt(-2 log a); z0 = r cos 2 pi c, z1 = r sin 2 pi c (single
+ *   precision, widened to double). */
 __device__ inline void box_muller(uint32_t x0, uint32_t x1, double& z0, double& z1) {
 #pragma clang fp contract(off)
     const uint32_t n = x0 + 1u; /* 0 stands for a = 1 (log a = 0) */
@@ -111,8 +113,8 @@ __device__ inline void box_muller(uint32_t x0, uint32_t x1, double& z0, double& 
     pl = __builtin_fmaf(pl, sx, -0x1.fffff8p-2f);
     pl = __builtin_fmaf(pl, sx, 1.0f);
     const float lf = sx * pl;
-    const double L = n != 0u ? __builtin_fma((double)e, 0x1.62e42fefa39efp-1, (double)lf) : 0.0;
-    const double r = __builtin_sqrt(-2.0 * L);
+    const float L = n != 0u ? __builtin_fmaf((float)e, 0x1.62e430p-1f, lf) : 0.0f;
+    const float r = __builtin_sqrtf(-2.0f * L); /* correctly rounded (IEEE sqrt) */
     const uint32_t c = x1 >> 8;
     const uint32_t oct = c >> 21;
     const uint32_t fr = c & 0x1FFFFFu;
@@ -137,8 +139,8 @@ __device__ inline void box_muller(uint32_t x0, uint32_t x1, double& z0, double& 
     float cs = (q & 1u) ? sphi : cphi;
     if (q == 1u || q == 2u) cs = -cs;
     if (q >= 2u) sn = -sn;
-    z0 = r * (double)cs;
-    z1 = r * (double)sn;
+    z0 = (double)(r * cs);
+    z1 = (double)(r * sn);
 }
 
 /* N (<= 8) standard normals, four per Philox block: words (x0, x1) give

@@ -840,6 +840,21 @@ int scphd_get_phase_ms(scphd_t* f, double* ms8) {
     return SCPHD_OK;
 }
 
+int scphd_debug_normals(uint64_t seed, uint32_t stream, uint64_t step, uint32_t parent, uint32_t gauss,
+                        int32_t n_elem, double* out) {
+    if (n_elem < 0 || n_elem > (1 << 24) || (n_elem > 0 && !out)) return SCPHD_ERR_ARG;
+    if (n_elem == 0) return SCPHD_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SCPHD_ERR_NODEVICE;
+    double* d = nullptr;
+    const size_t bytes = (size_t)n_elem * 6 * sizeof(double);
+    if (hipMalloc((void**)&d, bytes) != hipSuccess) return SCPHD_ERR_HIP;
+    hipError_t e = launch_debug_normals(nullptr, NoiseKey::from_seed(seed), stream, step, parent, gauss, n_elem, d);
+    if (e == hipSuccess) e = hipMemcpy(out, d, bytes, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    return e == hipSuccess ? SCPHD_OK : SCPHD_ERR_HIP;
+}
+
 int scphd_set_noise(scphd_t* f, int32_t mode, uint64_t seed, const double* injected, size_t n) {
     if (check_handle(f)) return SCPHD_ERR_ARG;
     if (mode != 0 || injected || n)

@@ -170,6 +170,10 @@ hipError_t launch_iota(hipStream_t s, int32_t* x, int64_t n);
 
 hipError_t launch_parent_weights(hipStream_t s, double* pw, const double* loglik, int Np);
 
+/* out[6 n]: six normals of every element 0..n-1 of one noise stream */
+hipError_t launch_debug_normals(hipStream_t s, NoiseKey key, uint32_t stream, uint64_t step, uint32_t parent,
+                                uint32_t gauss, int n, double* out);
+
 /* out[157]: weighted parent mean (12), covariance (144, row-major), summed
  * Gaussian weight of the first highest-weight parent */
 hipError_t launch_estimate(hipStream_t s, const double* ext, const double* pw, const int32_t* goff,

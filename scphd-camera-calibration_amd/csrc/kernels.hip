@@ -1306,6 +1306,19 @@ __global__ __launch_bounds__(1024) void k_parent_weights(double* pw, const doubl
     for (int p = threadIdx.x; p < Np; p += blockDim.x) pw[p] = exp(safe_log(pw[p]) + loglik[p] - mx) / s;
 }
 
+/* ------------------------------------------------------ noise check */
+/* Six normals per element elem = 0..n-1 of one counter stream, exactly as
+ * the filter kernels draw them, for comparison with the CPU restatement. */
+__global__ void k_debug_normals(NoiseKey key, uint32_t stream, uint64_t step, uint32_t parent, uint32_t gauss, int n,
+                                double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double z[6];
+    normals<6>(key, stream, step, parent, gauss, (uint32_t)i, z);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) out[(int64_t)i * 6 + c] = z[c];
+}
+
 /* --------------------------------------------------------- estimate */
 /* Sum each of N per-thread values over a 256-thread block; every thread
  * receives the totals (fixed order: wave sums, then waves ascending). */
@@ -1776,6 +1789,13 @@ hipError_t launch_merge(hipStream_t s, const MergeArgs& a, int lds_bytes) {
 
 hipError_t launch_parent_weights(hipStream_t s, double* pw, const double* loglik, int Np) {
     k_parent_weights<<<1, 1024, 0, s>>>(pw, loglik, Np);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_debug_normals(hipStream_t s, NoiseKey key, uint32_t stream, uint64_t step, uint32_t parent,
+                                uint32_t gauss, int n, double* out) {
+    if (n <= 0) return hipSuccess;
+    k_debug_normals<<<nblocks(n, 256), 256, 0, s>>>(key, stream, step, parent, gauss, n, out);
     LAUNCH_CHECK();
 }
 

@@ -32,6 +32,7 @@ EXPORTED = [
     "scphd_get_counts", "scphd_get_parents", "scphd_get_maps", "scphd_set_state", "scphd_get_update_stats",
     "scphd_get_update_debug", "scphd_get_ancestors", "scphd_set_timing", "scphd_get_phase_ms",
     "scphd_get_fit_debug", "scphd_get_estimate", "scphd_get_memory", "scphd_set_noise",
+    "scphd_debug_normals",
     "scphd_batch_opts_init", "scphd_batch_create", "scphd_batch_destroy", "scphd_batch_last_error",
     "scphd_batch_now", "scphd_rig_footprint", "scphd_batch_add_rig", "scphd_batch_remove_rig",
     "scphd_batch_submit", "scphd_batch_step", "scphd_batch_collect", "scphd_batch_get_stats",
@@ -150,6 +151,7 @@ def lib():
         "scphd_get_fit_debug": (_I32, [_P, dp]),
         "scphd_get_estimate": (_I32, [_P, dp, dp, dp]),
         "scphd_set_noise": (_I32, [_P, _I32, _U64, dp, ctypes.c_size_t]),
+        "scphd_debug_normals": (_I32, [_U64, ctypes.c_uint32, _U64, ctypes.c_uint32, ctypes.c_uint32, _I32, dp]),
         "scphd_get_memory": (_I32, [_P, ctypes.POINTER(_I64)]),
         "scphd_batch_opts_init": (_I32, [ctypes.POINTER(BatchOpts)]),
         "scphd_batch_create": (_I32, [ctypes.POINTER(BatchOpts), ctypes.POINTER(_P)]),
@@ -210,6 +212,15 @@ def read_measurements(path, varname):
         raise ScphdError(rc, f"cannot read {varname} from {path}")
     off = np.concatenate([[0], np.cumsum(cnt[:n.value])])
     return [(u[off[k]:off[k + 1]].copy(), v[off[k]:off[k + 1]].copy()) for k in range(n.value)]
+
+
+def device_normals(seed, stream, step, parent, gauss, n_elem):
+    """Six normals per element 0..n_elem-1 drawn on the GPU (scphd_debug_normals)."""
+    out = np.zeros(6 * max(n_elem, 1))
+    rc = lib().scphd_debug_normals(seed, stream, step, parent, gauss, n_elem, _dp(out))
+    if rc != SCPHD_OK:
+        raise ScphdError(rc, "scphd_debug_normals failed")
+    return out[:6 * n_elem].reshape(n_elem, 6)
 
 
 def load_params(path):
