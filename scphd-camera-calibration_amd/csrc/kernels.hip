@@ -964,14 +964,48 @@ __global__ __launch_bounds__(NT) void k_merge_groups(MergeArgs a) {
     if (t == 0) s_nc = 0;
     merge_load_sort(a, sbase, U, w); /* ends with a barrier */
 
-    /* 2. candidates: rows strided over threads, row i scans j > i */
-    for (int i = t; i < U; i += NT) {
-        w.cnt[i] = 0;
-        for (int j = i + 1; j < U; ++j)
-            if (merge_bound(w, i, j, thr)) {
-                const int c = atomicAdd(&s_nc, 1);
-                if (c < w.ccap) w.cand[c] = ((uint32_t)i << 16) | (uint32_t)j;
+    /* 2. candidates: the U (U - 1) / 2 pairs i < j in row-major order, split
+     * into NT contiguous stretches (row i starts at i U - i (i + 1) / 2);
+     * survivor i's bound data stays in registers along its row */
+    for (int i = t; i < U; i += NT) w.cnt[i] = 0;
+    {
+        const int64_t npairs = (int64_t)U * (U - 1) / 2;
+        const int64_t per = (npairs + NT - 1) / NT;
+        const int64_t q0 = (int64_t)t * per, q1 = min(npairs, q0 + per);
+        if (q0 < q1) {
+            auto row_start = [U](int64_t r) { return r * U - r * (r + 1) / 2; };
+            const double b = 2.0 * U - 1.0;
+            int64_t i = (int64_t)((b - sqrt(fmax(b * b - 8.0 * (double)q0, 0.0))) * 0.5);
+            i = max<int64_t>(0, min<int64_t>(i, U - 2));
+            while (i > 0 && row_start(i) > q0) --i;
+            while (i + 1 < U - 1 && row_start(i + 1) <= q0) ++i;
+            int j = (int)(i + 1 + (q0 - row_start(i)));
+            double ui = w.bu[i], vi = w.bv[i], ai = w.bs[3 * i], bi = w.bs[3 * i + 1], ci = w.bs[3 * i + 2];
+            for (int64_t q = q0; q < q1; ++q) {
+                const double du = w.bu[j] - ui, dv = w.bv[j] - vi;
+                const double suu = ai + w.bs[3 * j], suv = bi + w.bs[3 * j + 1], svv = ci + w.bs[3 * j + 2];
+                const double det = suu * svv - suv * suv;
+                /* merge_bound: reject only a well-conditioned (u, v) block whose
+                 * marginal distance exceeds the threshold by 0.1 % */
+                const bool cand = !(det > 1e-6 * suu * svv) ||
+                                  !(svv * du * du - 2.0 * suv * du * dv + suu * dv * dv > 1.001 * thr * det);
+                if (cand) {
+                    const int c = atomicAdd(&s_nc, 1);
+                    if (c < w.ccap) w.cand[c] = ((uint32_t)i << 16) | (uint32_t)j;
+                }
+                if (++j == U) {
+                    ++i;
+                    j = (int)i + 1;
+                    if (i < U - 1) {
+                        ui = w.bu[i];
+                        vi = w.bv[i];
+                        ai = w.bs[3 * i];
+                        bi = w.bs[3 * i + 1];
+                        ci = w.bs[3 * i + 2];
+                    }
+                }
             }
+        }
     }
     __syncthreads();
     const int nc = s_nc;
@@ -1049,51 +1083,83 @@ __global__ __launch_bounds__(NT) void k_merge_groups(MergeArgs a) {
     if (t < 64) {
         const int lane = t;
         int ng = 0, mp = 0, r = 0;
+        int chunk = -1, ordv = 0; /* ord[chunk + lane], cached while the scan stays in one chunk */
         for (;;) {
             int L = -1;
-            for (; r < U; r += 64) {
-                const int rr = r + lane;
-                const bool c = rr < U && !w.flag[w.ord[rr]];
+            while (r < U) {
+                const int c0 = r & ~63;
+                if (c0 != chunk) {
+                    chunk = c0;
+                    ordv = c0 + lane < U ? w.ord[c0 + lane] : 0;
+                }
+                const int rr = c0 + lane;
+                const bool c = rr >= r && rr < U && !w.flag[ordv];
                 const uint64_t bm = __ballot(c);
                 if (bm) {
-                    r += __ffsll((unsigned long long)bm) - 1;
-                    L = w.ord[r];
+                    const int k = __ffsll((unsigned long long)bm) - 1;
+                    r = c0 + k;
+                    L = __builtin_amdgcn_readlane(ordv, k);
                     break;
                 }
+                r = c0 + 64;
             }
             if (L < 0) break;
             const bool sm = w.selfm[L] != 0;
             const int e0 = w.nbo[L], e1 = w.nbo[L + 1];
-            /* taken neighbours below L: the leader's slot when it is a member */
-            int nlt = 0, ntk = 0;
-            for (int e = e0; e < e1; e += 64) {
-                const int ei = e + lane;
-                const int j = ei < e1 ? w.nbr[ei] : 0x7fffffff;
-                const bool tk = ei < e1 && !w.flag[j];
-                nlt += __popcll(__ballot(tk && j < L));
-                ntk += __popcll(__ballot(tk));
-            }
-            const int gsz = ntk + (sm ? 1 : 0);
-            if (gsz == 0) break; /* empty group: zero weight, the reference stops */
             const int st = mp;
-            int seen = 0;
-            for (int e = e0; e < e1; e += 64) {
-                const int ei = e + lane;
+            int gsz;
+            if (e1 - e0 <= 64) {
+                /* one pass: members and their flags written at once (a
+                 * neighbour appears once in a list, and never the leader) */
+                const int ei = e0 + lane;
                 const int j = ei < e1 ? w.nbr[ei] : 0x7fffffff;
                 const bool tk = ei < e1 && !w.flag[j];
                 const uint64_t bm = __ballot(tk);
+                const int nlt = __popcll(__ballot(tk && j < L));
+                gsz = __popcll(bm) + (sm ? 1 : 0);
+                if (gsz == 0) break; /* empty group: zero weight, the reference stops */
                 if (tk) {
-                    const int k = seen + __popcll(bm & ((1ull << lane) - 1ull));
+                    const int k = __popcll(bm & ((1ull << lane) - 1ull));
                     w.mem[st + k + ((sm && k >= nlt) ? 1 : 0)] = j;
+                    w.flag[j] = 1;
                 }
-                seen += __popcll(bm);
+                if (lane == 0 && sm) {
+                    w.mem[st + nlt] = L;
+                    w.flag[L] = 1;
+                }
+            } else {
+                /* taken neighbours below L: the leader's slot when it is a member */
+                int nlt = 0, ntk = 0;
+                for (int e = e0; e < e1; e += 64) {
+                    const int ei = e + lane;
+                    const int j = ei < e1 ? w.nbr[ei] : 0x7fffffff;
+                    const bool tk = ei < e1 && !w.flag[j];
+                    nlt += __popcll(__ballot(tk && j < L));
+                    ntk += __popcll(__ballot(tk));
+                }
+                gsz = ntk + (sm ? 1 : 0);
+                if (gsz == 0) break;
+                int seen = 0;
+                for (int e = e0; e < e1; e += 64) {
+                    const int ei = e + lane;
+                    const int j = ei < e1 ? w.nbr[ei] : 0x7fffffff;
+                    const bool tk = ei < e1 && !w.flag[j];
+                    const uint64_t bm = __ballot(tk);
+                    if (tk) {
+                        const int k = seen + __popcll(bm & ((1ull << lane) - 1ull));
+                        w.mem[st + k + ((sm && k >= nlt) ? 1 : 0)] = j;
+                    }
+                    seen += __popcll(bm);
+                }
+                if (lane == 0 && sm) w.mem[st + nlt] = L;
+                __builtin_amdgcn_wave_barrier();
+                for (int e = st + lane; e < st + gsz; e += 64) w.flag[w.mem[e]] = 1;
             }
-            if (lane == 0 && sm) w.mem[st + nlt] = L;
             mp += gsz;
+            /* LDS accesses of one wavefront complete in order: the next scan
+             * sees these flags without waiting (the global stores below are
+             * not waited for) */
             __builtin_amdgcn_wave_barrier();
-            __threadfence_block();
-            for (int e = st + lane; e < mp; e += 64) w.flag[w.mem[e]] = 1;
-            __threadfence_block();
             if (lane == 0) {
                 a.glead[sbase + ng] = L;
                 a.gsta[sbase + ng] = st;
