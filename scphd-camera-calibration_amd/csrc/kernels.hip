@@ -1076,37 +1076,43 @@ __global__ __launch_bounds__(NT) void k_merge_groups(MergeArgs a) {
         }
     }
     __syncthreads();
-    /* 5. the greedy walk, by wavefront 0: the next unmerged leader by a
-     * ballot over the sorted ranks, its unmerged neighbours (ascending, with
-     * the leader inserted at its index when it is its own member) compacted
-     * by ballots over its neighbour list */
-    if (t < 64) {
-        const int lane = t;
-        int ng = 0, mp = 0, r = 0;
-        int chunk = -1, ordv = 0; /* ord[chunk + lane], cached while the scan stays in one chunk */
-        for (;;) {
-            int L = -1;
-            while (r < U) {
-                const int c0 = r & ~63;
-                if (c0 != chunk) {
-                    chunk = c0;
-                    ordv = c0 + lane < U ? w.ord[c0 + lane] : 0;
-                }
-                const int rr = c0 + lane;
-                const bool c = rr >= r && rr < U && !w.flag[ordv];
-                const uint64_t bm = __ballot(c);
-                if (bm) {
-                    const int k = __ffsll((unsigned long long)bm) - 1;
-                    r = c0 + k;
-                    L = __builtin_amdgcn_readlane(ordv, k);
-                    break;
-                }
-                r = c0 + 64;
+    /* 5. the greedy walk.  A survivor with an empty neighbour list is
+     * isolated: no leader takes it, so when the walk reaches it, it forms
+     * the group {itself}, or stops the walk when it is not its own member.
+     * The sequential walk therefore visits only the connected survivors, in
+     * rank order; every rank's outcome (group size, stop) is then placed in
+     * rank order by two scans.  Per rank r: rinfo[2 r] = group size | STOP,
+     * rinfo[2 r + 1] = start of the members in tmp (connected leaders). */
+    constexpr int STOP = 1 << 30;
+    int* rinfo = (int*)w.bu; /* free after the candidates: 8 bytes per slot */
+    int* nl = w.cnt;         /* connected ranks, ascending; free after the lists */
+    int* tmp = (int*)w.cand; /* members of connected groups; ccap >= U */
+    {
+        int mine = 0;
+        for (int r = r0; r < r1; ++r) mine += w.nbo[w.ord[r] + 1] != w.nbo[w.ord[r]];
+        int nn;
+        int k = block_exscan(mine, sh, &nn);
+        for (int r = r0; r < r1; ++r) {
+            const int L = w.ord[r];
+            if (w.nbo[L + 1] != w.nbo[L]) {
+                nl[k++] = r;
+                rinfo[2 * r] = 0;
+            } else {
+                rinfo[2 * r] = w.selfm[L] ? 1 : STOP;
             }
-            if (L < 0) break;
+        }
+        if (t == 0) s_tot = nn;
+    }
+    __syncthreads();
+    if (t < 64) {
+        const int lane = t, nn = s_tot;
+        int tp = 0;
+        for (int k = 0; k < nn; ++k) {
+            const int r = nl[k], L = w.ord[r];
+            if (w.flag[L]) continue; /* taken by an earlier leader */
             const bool sm = w.selfm[L] != 0;
             const int e0 = w.nbo[L], e1 = w.nbo[L + 1];
-            const int st = mp;
+            const int st = tp;
             int gsz;
             if (e1 - e0 <= 64) {
                 /* one pass: members and their flags written at once (a
@@ -1117,14 +1123,13 @@ __global__ __launch_bounds__(NT) void k_merge_groups(MergeArgs a) {
                 const uint64_t bm = __ballot(tk);
                 const int nlt = __popcll(__ballot(tk && j < L));
                 gsz = __popcll(bm) + (sm ? 1 : 0);
-                if (gsz == 0) break; /* empty group: zero weight, the reference stops */
                 if (tk) {
-                    const int k = __popcll(bm & ((1ull << lane) - 1ull));
-                    w.mem[st + k + ((sm && k >= nlt) ? 1 : 0)] = j;
+                    const int q = __popcll(bm & ((1ull << lane) - 1ull));
+                    tmp[st + q + ((sm && q >= nlt) ? 1 : 0)] = j;
                     w.flag[j] = 1;
                 }
                 if (lane == 0 && sm) {
-                    w.mem[st + nlt] = L;
+                    tmp[st + nlt] = L;
                     w.flag[L] = 1;
                 }
             } else {
@@ -1138,37 +1143,74 @@ __global__ __launch_bounds__(NT) void k_merge_groups(MergeArgs a) {
                     ntk += __popcll(__ballot(tk));
                 }
                 gsz = ntk + (sm ? 1 : 0);
-                if (gsz == 0) break;
-                int seen = 0;
-                for (int e = e0; e < e1; e += 64) {
-                    const int ei = e + lane;
-                    const int j = ei < e1 ? w.nbr[ei] : 0x7fffffff;
-                    const bool tk = ei < e1 && !w.flag[j];
-                    const uint64_t bm = __ballot(tk);
-                    if (tk) {
-                        const int k = seen + __popcll(bm & ((1ull << lane) - 1ull));
-                        w.mem[st + k + ((sm && k >= nlt) ? 1 : 0)] = j;
+                if (gsz > 0) {
+                    int seen = 0;
+                    for (int e = e0; e < e1; e += 64) {
+                        const int ei = e + lane;
+                        const int j = ei < e1 ? w.nbr[ei] : 0x7fffffff;
+                        const bool tk = ei < e1 && !w.flag[j];
+                        const uint64_t bm = __ballot(tk);
+                        if (tk) {
+                            const int q = seen + __popcll(bm & ((1ull << lane) - 1ull));
+                            tmp[st + q + ((sm && q >= nlt) ? 1 : 0)] = j;
+                        }
+                        seen += __popcll(bm);
                     }
-                    seen += __popcll(bm);
+                    if (lane == 0 && sm) tmp[st + nlt] = L;
+                    __builtin_amdgcn_wave_barrier();
+                    for (int e = st + lane; e < st + gsz; e += 64) w.flag[tmp[e]] = 1;
                 }
-                if (lane == 0 && sm) w.mem[st + nlt] = L;
-                __builtin_amdgcn_wave_barrier();
-                for (int e = st + lane; e < st + gsz; e += 64) w.flag[w.mem[e]] = 1;
             }
-            mp += gsz;
-            /* LDS accesses of one wavefront complete in order: the next scan
-             * sees these flags without waiting (the global stores below are
-             * not waited for) */
-            __builtin_amdgcn_wave_barrier();
+            /* empty group: zero weight, the reference stops; a leader outside
+             * its group is taken again, finds nothing and stops */
             if (lane == 0) {
-                a.glead[sbase + ng] = L;
-                a.gsta[sbase + ng] = st;
-                a.gend[sbase + ng] = mp;
+                rinfo[2 * r] = gsz | (sm ? 0 : STOP);
+                rinfo[2 * r + 1] = st;
             }
-            ++ng;
-            if (sm) ++r; /* a leader outside its group is taken again, finds nothing, stops */
+            tp += gsz;
+            /* LDS accesses of one wavefront complete in order: the next
+             * iteration sees these flags without waiting */
+            __builtin_amdgcn_wave_barrier();
+            if (!sm || gsz == 0) break;
         }
-        if (lane == 0) {
+    }
+    if (t == 0) s_nc = U; /* first stopping rank */
+    __syncthreads();
+    for (int r = r0; r < r1; ++r)
+        if (rinfo[2 * r] & STOP) {
+            atomicMin(&s_nc, r);
+            break;
+        }
+    __syncthreads();
+    {
+        const int rstop = s_nc; /* ranks after it form no group */
+        int mg = 0, mm = 0;
+        for (int r = r0; r < min(r1, rstop + 1); ++r) {
+            const int n = rinfo[2 * r] & (STOP - 1);
+            mg += n > 0;
+            mm += n;
+        }
+        int ng;
+        int gi = block_exscan(mg, sh, &ng);
+        int mtot;
+        int mp = block_exscan(mm, sh, &mtot);
+        for (int r = r0; r < min(r1, rstop + 1); ++r) {
+            const int n = rinfo[2 * r] & (STOP - 1);
+            if (n == 0) continue;
+            const int L = w.ord[r];
+            a.glead[sbase + gi] = L;
+            a.gsta[sbase + gi] = mp;
+            a.gend[sbase + gi] = mp + n;
+            if (w.nbo[L + 1] == w.nbo[L]) {
+                w.mem[mp] = L;
+            } else {
+                const int s0 = rinfo[2 * r + 1];
+                for (int q = 0; q < n; ++q) w.mem[mp + q] = tmp[s0 + q];
+            }
+            ++gi;
+            mp += n;
+        }
+        if (t == 0) {
             a.ngroups[p] = ng;
             a.nmerged[p] = ng;
         }
