@@ -373,7 +373,6 @@ int update_t(scphd_filter* f, int M, int fixed) {
     ma.gend = f->gend.p;
     ma.gmem = f->gmem.p;
     ma.Stot = Stot;
-    ma.big = 0;
     if (maxU > f->lds_cap) {
         HIPTRY(f->mscratch.ensure(merge_scratch_bytes(Stot, Np)));
         ma.scratch = f->mscratch.p;

@@ -147,7 +147,6 @@ struct MergeArgs {
     double min_separation;
     int Np;
     int32_t lds_cap;   /* parents with more survivors use global scratch */
-    int32_t big;       /* 0: parents with <= lds_cap survivors, 1: the others */
     char* scratch;     /* global scratch for big parents: merge_scratch_bytes(Stot, Np) */
     double* mw;        /* merged outputs at survoff[p] + k */
     double* mmu;       /* 6 planes, stride scap */

@@ -323,8 +323,8 @@ __device__ inline void write_fit(const FitArgs& a, int64_t g, int lane, const do
 template <typename T, typename Off>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_WAVES, SCPHD_FIT_WAVES))) void k_fit(
     FitArgs a, Planes<T> src, Planes<T> dst) {
-    /* the row's camera pose lives in LDS and is re-read per sample, which
-     * keeps 24 registers free for occupancy */
+    /* the rows' camera poses are staged through LDS once and then held in
+     * registers for the whole sweep */
     __shared__ double s_pose[16 * 12];
     const int lane = threadIdx.x & 15, row = threadIdx.x >> 4;
     const int64_t g = (int64_t)blockIdx.x * 16 + row;
@@ -333,7 +333,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
     if (act && lane < 12) s_pose[row * 12 + lane] = a.pose[(int64_t)p * 12 + lane];
     __syncthreads();
     if (!act) return; /* whole rows; rows never exchange data */
-    const volatile double* sp = s_pose + row * 12;
+    CameraPose pose;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) pose.R[i] = s_pose[row * 12 + i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) pose.t[i] = s_pose[row * 12 + 9 + i];
     const int k = (int)(g - a.goff[p]);
     const int64_t b0 = (int64_t)a.gblock[g] * a.S;
     const int S = a.S;
@@ -351,31 +355,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
     for (int c = 0; c < 6; ++c) m1[c] = 0.0;
 #pragma unroll
     for (int e = 0; e < 21; ++e) m2[e] = 0.0;
-    /* loads run one step ahead of the arithmetic */
-    T cur[7], nxt[7];
-    const Off so = (Off)((b0 + lane) * (int64_t)sizeof(T)); /* byte offsets, advanced 16 samples a step */
+    /* loads run three steps (48 samples) ahead of the arithmetic: the
+     * sweep is bound by memory latency unless a row keeps several steps of
+     * loads in flight */
+    T cur[7], nx1[7], nx2[7], nx3[7];
+    Off sof = (Off)((b0 + lane) * (int64_t)sizeof(T)); /* byte offsets, advanced 16 samples a step */
     Off dof = (Off)((d0 + lane) * (int64_t)sizeof(T));
-    Off sof = so;
     if (lane < S) load_sample_off(src, sof, cur);
+    sof += (Off)(16 * sizeof(T));
+    if (lane + 16 < S) load_sample_off(src, sof, nx1);
+    sof += (Off)(16 * sizeof(T));
+    if (lane + 32 < S) load_sample_off(src, sof, nx2);
     for (int s0 = 0; s0 < S; s0 += 16) {
         const int s = s0 + lane;
         const bool valid = s < S;
         sof += (Off)(16 * sizeof(T));
-        if (s + 16 < S) load_sample_off(src, sof, nxt);
+        if (s + 48 < S) load_sample_off(src, sof, nx3);
         EuclideanPoint x;
         DisparityPoint d;
         bool in = false;
         double w = 0.0;
-        if (valid) {
-            CameraPose pose;
+        if (valid) eval_sample(a, cur, p, k, s, pose, x, d, in, w);
 #pragma unroll
-            for (int i = 0; i < 9; ++i) pose.R[i] = sp[i];
-#pragma unroll
-            for (int i = 0; i < 3; ++i) pose.t[i] = sp[9 + i];
-            eval_sample(a, cur, p, k, s, pose, x, d, in, w);
+        for (int c = 0; c < 7; ++c) {
+            cur[c] = nx1[c];
+            nx1[c] = nx2[c];
+            nx2[c] = nx3[c];
         }
-#pragma unroll
-        for (int c = 0; c < 7; ++c) cur[c] = nxt[c];
         const double q[6] = {d.u, d.v, d.d, d.vu, d.vv, d.vd};
         if (s0 == 0) {
             /* shift: the Gaussian's first sample, broadcast along the row */
@@ -777,15 +783,21 @@ int64_t merge_scratch_bytes(int64_t Stot, int Np) {
     return Stot * MERGE_WS_PER_SLOT + (int64_t)Np * MERGE_WS_PER_PARENT + 256;
 }
 
+/* Packed covariance entry e of a survivor's term: the updated covariance of
+ * table entry g, or the birth covariance (g < 0).  The birth entry passes
+ * through an empty asm so that it is a value, not a load: a choice between
+ * two loads is otherwise folded into one load from a chosen address, global
+ * or kernel-argument memory, which compiles to a flat load. */
+__device__ inline double term_P(const MergeArgs& a, int g, int e) {
+    double b = a.birth.P[e];
+    asm volatile("" : "+v"(b));
+    return g >= 0 ? a.pre[(int64_t)(PP_PN + e) * a.Ng + g] : b;
+}
+
 __device__ inline void survivor_P(const MergeArgs& a, int64_t si, double* P) {
     const int g = a.sg[si];
-    if (g >= 0) {
 #pragma unroll
-        for (int e = 0; e < 21; ++e) P[e] = a.pre[(int64_t)(PP_PN + e) * a.Ng + g];
-    } else {
-#pragma unroll
-        for (int e = 0; e < 21; ++e) P[e] = a.birth.P[e];
-    }
+    for (int e = 0; e < 21; ++e) P[e] = term_P(a, g, e);
 }
 
 __device__ inline void survivor_mu(const MergeArgs& a, int64_t si, double* mu) {
@@ -876,9 +888,9 @@ __device__ inline void merge_load_sort(const MergeArgs& a, int64_t sbase, int U,
             w.bu[i] = a.smu[0 * a.scap + sbase + i];
             w.bv[i] = a.smu[1 * a.scap + sbase + i];
             const int g = a.sg[sbase + i];
-            w.bs[3 * i] = g >= 0 ? a.pre[(int64_t)(PP_PN + pidx(0, 0)) * a.Ng + g] : a.birth.P[pidx(0, 0)];
-            w.bs[3 * i + 1] = g >= 0 ? a.pre[(int64_t)(PP_PN + pidx(1, 0)) * a.Ng + g] : a.birth.P[pidx(1, 0)];
-            w.bs[3 * i + 2] = g >= 0 ? a.pre[(int64_t)(PP_PN + pidx(1, 1)) * a.Ng + g] : a.birth.P[pidx(1, 1)];
+            w.bs[3 * i] = term_P(a, g, pidx(0, 0));
+            w.bs[3 * i + 1] = term_P(a, g, pidx(1, 0));
+            w.bs[3 * i + 2] = term_P(a, g, pidx(1, 1));
             w.flag[i] = 0;
         } else {
             w.key_w[i] = -INFINITY;
@@ -934,6 +946,7 @@ __device__ inline bool merge_self(const MergeArgs& a, int64_t si, double thr) {
  * k_merge_moments (one thread per group): moments with members summed in
  * ascending index order.  k_merge_rounds: the round loop, for parents whose
  * candidate or neighbour lists overflow the workspace. */
+template <bool BIG>
 __global__ __launch_bounds__(NT) void k_merge_groups(MergeArgs a) {
     extern __shared__ char lds[];
     __shared__ int s_nc, s_tot;
@@ -942,8 +955,7 @@ __global__ __launch_bounds__(NT) void k_merge_groups(MergeArgs a) {
     const int t = threadIdx.x;
     const int64_t sbase = a.survoff[p];
     const int U = a.survoff[p + 1] - a.survoff[p];
-    const bool big = U > a.lds_cap;
-    if ((a.big != 0) != big) return; /* the other launch handles this parent (uniform) */
+    if (BIG != (U > a.lds_cap)) return; /* the other launch handles this parent (uniform) */
     if (U == 0) {
         if (t == 0) {
             a.ngroups[p] = 0;
@@ -958,7 +970,9 @@ __global__ __launch_bounds__(NT) void k_merge_groups(MergeArgs a) {
         }
         return;
     }
-    char* ws = big ? (a.scratch + sbase * MERGE_WS_PER_SLOT + (int64_t)p * MERGE_WS_PER_PARENT) : lds;
+    /* a compile-time choice of workspace, so that the LDS instantiation
+     * addresses it with ds_* instructions rather than flat ones */
+    char* ws = BIG ? (a.scratch + sbase * MERGE_WS_PER_SLOT + (int64_t)p * MERGE_WS_PER_PARENT) : lds;
     MergeWs w(ws, U);
     const double thr = a.min_separation;
     if (t == 0) s_nc = 0;
@@ -1255,7 +1269,7 @@ __global__ __launch_bounds__(128) void k_merge_moments(MergeArgs a) {
         for (int r = 0; r < 6; ++r)
 #pragma unroll
             for (int c = 0; c <= r; ++c) {
-                const double Pe = g >= 0 ? a.pre[(int64_t)(PP_PN + pidx(r, c)) * a.Ng + g] : a.birth.P[pidx(r, c)];
+                const double Pe = term_P(a, g, pidx(r, c));
                 P[pidx(r, c)] += w * (Pe + dl[c] * dl[r]);
             }
     }
@@ -1276,10 +1290,11 @@ __global__ __launch_bounds__(128) void k_merge_moments(MergeArgs a) {
         for (int c = 0; c < 6; ++c) a.mmu[c * a.scap + o] = a.smu[c * a.scap + li];
 #pragma unroll
         for (int q = 0; q < 21; ++q)
-            a.mP[q * a.scap + o] = g >= 0 ? a.pre[(int64_t)(PP_PN + q) * a.Ng + g] : a.birth.P[q];
+            a.mP[q * a.scap + o] = term_P(a, g, q);
     }
 }
 
+template <bool BIG>
 __global__ __launch_bounds__(NT) void k_merge_rounds(MergeArgs a) {
     extern __shared__ char lds[];
     __shared__ int s_lead, s_pos, s_nm;
@@ -1289,9 +1304,8 @@ __global__ __launch_bounds__(NT) void k_merge_rounds(MergeArgs a) {
     if (a.ngroups[p] != -1) return; /* uniform */
     const int64_t sbase = a.survoff[p];
     const int U = a.survoff[p + 1] - a.survoff[p];
-    const bool big = U > a.lds_cap;
-    if ((a.big != 0) != big) return;
-    char* ws = big ? (a.scratch + sbase * MERGE_WS_PER_SLOT + (int64_t)p * MERGE_WS_PER_PARENT) : lds;
+    if (BIG != (U > a.lds_cap)) return;
+    char* ws = BIG ? (a.scratch + sbase * MERGE_WS_PER_SLOT + (int64_t)p * MERGE_WS_PER_PARENT) : lds;
     MergeWs w(ws, U);
     const double thr = a.min_separation;
     merge_load_sort(a, sbase, U, w);
@@ -1328,7 +1342,7 @@ __global__ __launch_bounds__(NT) void k_merge_rounds(MergeArgs a) {
         if (t == 0) s_lw = a.sw[sbase + lead];
         if (t < 21) {
             const int g = a.sg[sbase + lead];
-            s_lP[t] = g >= 0 ? a.pre[(int64_t)(PP_PN + t) * a.Ng + g] : a.birth.P[t];
+            s_lP[t] = term_P(a, g, t);
         }
         __syncthreads();
         for (int i = t; i < U; i += NT) {
@@ -1376,7 +1390,7 @@ __global__ __launch_bounds__(NT) void k_merge_rounds(MergeArgs a) {
             for (int j = 0; j < nm; ++j) {
                 const int64_t si = sbase + w.mem[j];
                 const int g = a.sg[si];
-                const double Pe = g >= 0 ? a.pre[(int64_t)(PP_PN + t) * a.Ng + g] : a.birth.P[t];
+                const double Pe = term_P(a, g, t);
                 const double dc = s_mu[c] - a.smu[c * a.scap + si];
                 const double dr = s_mu[r] - a.smu[r * a.scap + si];
                 acc += a.sw[si] * (Pe + dc * dr);
@@ -1878,20 +1892,11 @@ hipError_t launch_survivors(hipStream_t s, const SurvArgs& a) {
 }
 
 hipError_t launch_merge(hipStream_t s, const MergeArgs& a, int lds_bytes) {
-    MergeArgs b = a;
-    b.big = 0;
-    k_merge_groups<<<a.Np, NT, lds_bytes, s>>>(b);
-    if (a.scratch) {
-        b.big = 1;
-        k_merge_groups<<<a.Np, NT, 0, s>>>(b);
-    }
+    k_merge_groups<false><<<a.Np, NT, lds_bytes, s>>>(a);
+    if (a.scratch) k_merge_groups<true><<<a.Np, NT, 0, s>>>(a);
     if (a.Stot > 0) k_merge_moments<<<nblocks(a.Stot, 128), 128, 0, s>>>(a);
-    b.big = 0;
-    k_merge_rounds<<<a.Np, NT, lds_bytes, s>>>(b);
-    if (a.scratch) {
-        b.big = 1;
-        k_merge_rounds<<<a.Np, NT, 0, s>>>(b);
-    }
+    k_merge_rounds<false><<<a.Np, NT, lds_bytes, s>>>(a);
+    if (a.scratch) k_merge_rounds<true><<<a.Np, NT, 0, s>>>(a);
     LAUNCH_CHECK();
 }
 
