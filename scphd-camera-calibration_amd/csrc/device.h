@@ -19,6 +19,19 @@
 namespace scphd {
 
 /* ----------------------------------------------------------------- noise */
+/* x ^ y ^ k for a key word k that is the same in every lane (it is read
+ * from a scalar register): one three-input bitwise instruction, truth
+ * table 0x96, on the device */
+SCPHD_HD inline uint32_t xor3_key(uint32_t x, uint32_t y, uint32_t k) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t r;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(x), "v"(y), "s"(k));
+    return r;
+#else
+    return x ^ y ^ k;
+#endif
+}
+
 /* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
  * easy as 1, 2, 3", SC'11): 10 rounds of two 32x32->64 multiplies, key
  * bumped by the Weyl constants between rounds. */
@@ -28,8 +41,8 @@ SCPHD_HD inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32
     for (int r = 0; r < 10; ++r) {
         const uint64_t a = (uint64_t)0xD2511F53u * c0;
         const uint64_t b = (uint64_t)0xCD9E8D57u * c2;
-        const uint32_t n0 = (uint32_t)(b >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = (uint32_t)(a >> 32) ^ c3 ^ k1;
+        const uint32_t n0 = xor3_key((uint32_t)(b >> 32), c1, k0);
+        const uint32_t n2 = xor3_key((uint32_t)(a >> 32), c3, k1);
         c1 = (uint32_t)b;
         c3 = (uint32_t)a;
         c0 = n0;

@@ -12,6 +12,8 @@
 
 #include <float.h>
 
+#include <type_traits>
+
 namespace scphd {
 
 static constexpr int NT = 256; /* threads per workgroup for per-parent kernels */
@@ -348,6 +350,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
      * the sweep, and an unkept cloud is simply not referenced by the next
      * map table (a write is cheaper than a second sweep) */
     const bool nd_write = wg >= a.min_weight;
+    /* ... and the row stops writing it once its in-view count proves the
+     * term dropped: the computed pd sum is at least n_in Pd (1 - 1e-9), so
+     * n_in > S (1 - min_weight / wg) / (Pd (1 - 1e-9)) + 1 gives
+     * wg (1 - pd_g) < min_weight */
+    const double nin_max = (Pd > 0 && a.min_weight > 0 && nd_write)
+                               ? (1.0 - a.min_weight / wg) * S / (Pd * (1.0 - 1e-9)) + 1.0
+                               : INFINITY;
+    bool ndw = nd_write;
+    int nin = 0;
     const int64_t d0 = (a.nd_base + g) * (int64_t)S;
 
     double spd = 0.0, sw = 0.0, sw2 = 0.0, m1[6], m2[21], K[6];
@@ -355,22 +366,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
     for (int c = 0; c < 6; ++c) m1[c] = 0.0;
 #pragma unroll
     for (int e = 0; e < 21; ++e) m2[e] = 0.0;
-    /* loads run three steps (48 samples) ahead of the arithmetic: the
-     * sweep is bound by memory latency unless a row keeps several steps of
-     * loads in flight */
+    /* loads are issued three steps (48 samples) ahead of the arithmetic
+     * (measured faster than one step ahead); the buffer rotation at the end
+     * of a step waits for that step's load.  Full steps carry no per-lane
+     * sample guard; only a partial last step does. */
+    const int nin_lim = nin_max < 1e9 ? (int)floor(nin_max) : 0x7fffffff; /* n_in <= nin_max */
     T cur[7], nx1[7], nx2[7], nx3[7];
     Off sof = (Off)((b0 + lane) * (int64_t)sizeof(T)); /* byte offsets, advanced 16 samples a step */
     Off dof = (Off)((d0 + lane) * (int64_t)sizeof(T));
+    const Off last = (Off)((b0 + S - 1) * (int64_t)sizeof(T)); /* the cloud's last sample */
     if (lane < S) load_sample_off(src, sof, cur);
     sof += (Off)(16 * sizeof(T));
     if (lane + 16 < S) load_sample_off(src, sof, nx1);
     sof += (Off)(16 * sizeof(T));
     if (lane + 32 < S) load_sample_off(src, sof, nx2);
-    for (int s0 = 0; s0 < S; s0 += 16) {
+    /* one step of 16 samples, s0 .. s0 + 15; FULL: all of them exist.  The
+     * load of the step three ahead is unconditional (no branch in the loop):
+     * past the end of the cloud it reads the last sample again. */
+    auto step = [&](auto full, int s0) {
+        constexpr bool FULL = decltype(full)::value;
         const int s = s0 + lane;
-        const bool valid = s < S;
+        const bool valid = FULL || s < S;
         sof += (Off)(16 * sizeof(T));
-        if (s + 48 < S) load_sample_off(src, sof, nx3);
+        load_sample_off(src, s + 48 < S ? sof : last, nx3);
         EuclideanPoint x;
         DisparityPoint d;
         bool in = false;
@@ -402,9 +420,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
         for (int r = 0; r < 6; ++r)
 #pragma unroll
             for (int c = 0; c <= r; ++c) m2[pidx(r, c)] += wp * dv[r] * dv[c];
-        if (nd_write && valid) write_nd_off(a, dst, dof, x, in);
+        if (ndw && valid) write_nd_off(a, dst, dof, x, in);
         dof += (Off)(16 * sizeof(T));
-    }
+        const uint64_t bin = __ballot(valid && in);
+        nin += __popcll((bin >> (threadIdx.x & 48)) & 0xFFFFull); /* this row's 16 lanes */
+        ndw = ndw && nin <= nin_lim;
+    };
+    const std::integral_constant<bool, true> full;
+    const std::integral_constant<bool, false> guarded;
+    int s0 = 0;
+    for (; s0 + 16 <= S; s0 += 16) step(full, s0);
+    if (s0 < S) step(guarded, s0);
     spd = row_sum16(spd);
     sw = row_sum16(sw);
     sw2 = row_sum16(sw2);
