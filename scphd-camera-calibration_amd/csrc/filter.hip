@@ -125,8 +125,8 @@ struct scphd_filter {
     Generation gen[2];
     int cur = 0;
     /* per-Gaussian intermediates */
-    DBuf<double> fit, pre;
-    DBuf<int32_t> pdok, keep, keepscan, ndlist, redo;
+    DBuf<double> fit, pre, lik;
+    DBuf<int32_t> keep, keepscan, ndlist, redo;
     /* measurements, normalisers, survivors, merge */
     DBuf<double> du, dv, eta;
     int capM = 0;
@@ -247,7 +247,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
         HIPTRY(f->gen[nxt].ensure(Ng, 0, S, f->esz, s));
         HIPTRY(f->fit.ensure((int64_t)FP_COUNT * Ng));
         HIPTRY(f->pre.ensure((int64_t)PP_COUNT * Ng));
-        HIPTRY(f->pdok.ensure(Ng));
+        HIPTRY(f->lik.ensure((int64_t)LK_COUNT * (Ng + 3))); /* 3 records of padding: read four at a time */
         HIPTRY(f->keep.ensure(Ng));
         HIPTRY(f->redo.ensure(Ng));
         HIPTRY(f->ndlist.ensure(Ng));
@@ -277,7 +277,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
         record(f, 7);
         HIPTRY(launch_fit_fix<T>(s, fa, f->gen[f->cur].planes<T>(), f->gen[nxt].planes<T>()));
         HIPTRY(launch_precompute(s, f->fit.p, f->gw.p, Ng, P.std_u * P.std_u, P.std_v * P.std_v, f->pre.p,
-                                 f->pdok.p));
+                                 f->lik.p));
     }
     f->pending = false;
     record(f, 1);
@@ -295,7 +295,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
     ea.goff = f->goff.p;
     ea.gw = f->gw.p;
     ea.pre = f->pre.p;
-    ea.pdok = f->pdok.p;
+    ea.lik = f->lik.p;
     ea.Ng = Ng;
     ea.zu = f->du.p;
     ea.zv = f->dv.p;
@@ -551,11 +551,11 @@ void scphd_destroy(scphd_t* f) {
     (void)hipSetDevice(f->dev);
     if (f->stream) (void)hipStreamSynchronize(f->stream);
     DBuf<double>* dbs[] = {&f->ext, &f->pw, &f->pose, &f->loglik, &f->cum, &f->ext_tmp, &f->gw, &f->gw2,
-                           &f->fit, &f->pre, &f->du, &f->dv, &f->eta, &f->sw, &f->smu, &f->mw, &f->mmu, &f->mP,
+                           &f->fit, &f->pre, &f->lik, &f->du, &f->dv, &f->eta, &f->sw, &f->smu, &f->mw, &f->mmu, &f->mP,
                            &f->dneff, &f->est};
     for (auto* b : dbs) b->release();
     DBuf<int32_t>* ibs[] = {&f->anc, &f->newcnt, &f->goff, &f->gparent, &f->gblock, &f->goff2, &f->gparent2,
-                            &f->gblock2, &f->pdok, &f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->nsurv, &f->survoff,
+                            &f->gblock2, &f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->nsurv, &f->survoff,
                             &f->nmerged, &f->mergeoff, &f->outcnt, &f->scantmp, &f->sg, &f->sm, &f->dflag, &f->mcnt,
                             &f->ngroups, &f->glead, &f->gsta, &f->gend, &f->gmem};
     for (auto* b : ibs) b->release();
@@ -881,11 +881,11 @@ int scphd_get_memory(scphd_t* f, int64_t* bytes) {
     if (check_handle(f) || !bytes) return SCPHD_ERR_ARG;
     int64_t b = 0;
     DBuf<double>* dbs[] = {&f->ext, &f->pw, &f->pose, &f->loglik, &f->cum, &f->ext_tmp, &f->gw, &f->gw2,
-                           &f->fit, &f->pre, &f->du, &f->dv, &f->eta, &f->sw, &f->smu, &f->mw, &f->mmu, &f->mP,
+                           &f->fit, &f->pre, &f->lik, &f->du, &f->dv, &f->eta, &f->sw, &f->smu, &f->mw, &f->mmu, &f->mP,
                            &f->dneff, &f->est};
     for (auto* d : dbs) b += d->n * (int64_t)sizeof(double);
     DBuf<int32_t>* ibs[] = {&f->anc, &f->newcnt, &f->goff, &f->gparent, &f->gblock, &f->goff2, &f->gparent2,
-                            &f->gblock2, &f->pdok, &f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->nsurv, &f->survoff,
+                            &f->gblock2, &f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->nsurv, &f->survoff,
                             &f->nmerged, &f->mergeoff, &f->outcnt, &f->scantmp, &f->sg, &f->sm, &f->dflag, &f->mcnt,
                             &f->ngroups, &f->glead, &f->gsta, &f->gend, &f->gmem};
     for (auto* d : ibs) b += d->n * (int64_t)sizeof(int32_t);

@@ -29,11 +29,14 @@ namespace scphd {
 enum FitPlane { FP_MU = 0, FP_P = 6, FP_PD = 27, FP_COUNT = 28 };
 /* per-Gaussian detection terms, stride Ng */
 enum PrePlane {
-    PP_MUU = 0, PP_MUV, PP_S0, PP_S12, PP_S3, PP_C, PP_H, /* likelihood record */
-    PP_K = 7,                                           /* gain, 12 */
-    PP_PN = 19,                                         /* updated covariance, 21 packed */
-    PP_COUNT = 40
+    PP_K = 0,   /* gain, 12 */
+    PP_PN = 12, /* updated covariance, 21 packed */
+    PP_COUNT = 33
 };
+/* per-Gaussian likelihood record, LK_COUNT consecutive doubles (64 B):
+ * predicted measurement, S^-1 entries (s0, s1 + s2, s3), log(pd_g w_g),
+ * log(det S) / 2 and 1 when the updated covariance is PD, else 0 */
+enum LikField { LK_MUU = 0, LK_MUV, LK_S0, LK_S12, LK_S3, LK_C, LK_H, LK_OK, LK_COUNT = 8 };
 
 template <typename T>
 struct Planes {
@@ -98,14 +101,14 @@ template <typename T>
 hipError_t launch_fit_fix(hipStream_t s, const FitArgs& a, Planes<T> src, Planes<T> dst);
 
 hipError_t launch_precompute(hipStream_t s, const double* fit, const double* gw, int64_t Ng, double var_u,
-                             double var_v, double* pre, int32_t* pdok);
+                             double var_v, double* pre, double* lik);
 
 struct EtaArgs {
     const int32_t* goff;
     const double* gw;
-    const double* pre;
-    const int32_t* pdok;
-    int64_t Ng; /* plane stride */
+    const double* pre; /* PP_COUNT planes */
+    const double* lik; /* LK_COUNT doubles per Gaussian */
+    int64_t Ng;        /* plane stride */
     const double* zu;
     const double* zv;
     int M;

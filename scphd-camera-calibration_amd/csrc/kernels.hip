@@ -528,7 +528,7 @@ __global__ __launch_bounds__(256) void k_fit_fix(FitArgs a, Planes<T> src, Plane
  * (computeDetectionsKernel, src/kernels.cu:492-598): innovation covariance,
  * gain, Joseph covariance and its PD test, log(pd) + log(w), log det. */
 __global__ void k_precompute(const double* fit, const double* gw, int64_t Ng, double var_u, double var_v,
-                             double* pre, int32_t* pdok) {
+                             double* pre, double* lik) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= Ng) return;
     double P[21];
@@ -538,45 +538,53 @@ __global__ void k_precompute(const double* fit, const double* gw, int64_t Ng, do
     detection_terms(P, var_u, var_v, r);
     const bool ok = is_pd6(r.Pn);
     const double pdg = fit[FP_PD * Ng + g];
-    pre[PP_MUU * Ng + g] = fit[(FP_MU + 0) * Ng + g];
-    pre[PP_MUV * Ng + g] = fit[(FP_MU + 1) * Ng + g];
-    pre[PP_S0 * Ng + g] = r.s0;
-    pre[PP_S12 * Ng + g] = r.s1 + r.s2;
-    pre[PP_S3 * Ng + g] = r.s3;
-    pre[PP_C * Ng + g] = safe_log(pdg) + safe_log(gw[g]);
-    pre[PP_H * Ng + g] = 0.5 * safe_log(r.det);
+    double* rec = lik + g * LK_COUNT;
+    rec[LK_MUU] = fit[(FP_MU + 0) * Ng + g];
+    rec[LK_MUV] = fit[(FP_MU + 1) * Ng + g];
+    rec[LK_S0] = r.s0;
+    rec[LK_S12] = r.s1 + r.s2;
+    rec[LK_S3] = r.s3;
+    rec[LK_C] = safe_log(pdg) + safe_log(gw[g]);
+    rec[LK_H] = 0.5 * safe_log(r.det);
+    rec[LK_OK] = ok ? 1.0 : 0.0;
 #pragma unroll
     for (int i = 0; i < 12; ++i) pre[(PP_K + i) * Ng + g] = r.K[i];
 #pragma unroll
     for (int e = 0; e < 21; ++e) pre[(PP_PN + e) * Ng + g] = r.Pn[e];
-    pdok[g] = ok ? 1 : 0;
 }
 
 /* ------------------------------------------- normalisers and survivors */
-static constexpr int ETA_CH = 512; /* Gaussians staged in LDS per chunk */
+/* Likelihood records read through the constant address space: their
+ * address is the same in every lane (one Gaussian, all measurements), so
+ * the reads are scalar loads into SGPRs, with no LDS staging and no
+ * per-lane copies.  Scalar loads return out of order, so every use waits
+ * for all of them: records come four at a time, one wait per four (the
+ * record buffer holds three records of padding past the last Gaussian). */
+typedef __attribute__((address_space(4))) const double lik_t;
 
-struct LikChunk {
-    double v[7][ETA_CH];
-    int ok[ETA_CH];
+struct Lik4 {
+    double v[4][LK_COUNT];
 };
 
-__device__ inline void stage_chunk(LikChunk& c, const EtaArgs& a, int64_t g0, int n) {
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+__device__ inline Lik4 load_lik4(const double* lik, int64_t g) {
+    lik_t* r = (lik_t*)(lik + g * LK_COUNT);
+    Lik4 x;
 #pragma unroll
-        for (int q = 0; q < 7; ++q) c.v[q][j] = a.pre[(int64_t)q * a.Ng + g0 + j];
-        c.ok[j] = a.pdok[g0 + j];
-    }
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int q = 0; q < LK_COUNT; ++q) x.v[k][q] = r[k * LK_COUNT + q];
+    return x;
 }
 
-/* log weight of the detection term of staged Gaussian j and measurement
+/* log weight of the detection term of Gaussian record r and measurement
  * (zu, zv): log pd + log w - dist/2 - log 2 pi - log det / 2
  * (src/kernels.cu:527-539); log(0) when the updated covariance is not PD
  * (:597-598). */
-__device__ inline double det_lw(const LikChunk& c, int j, double zu, double zv) {
-    const double i0 = zu - c.v[PP_MUU][j], i1 = zv - c.v[PP_MUV][j];
-    const double dist = i0 * i0 * c.v[PP_S0][j] + i0 * i1 * c.v[PP_S12][j] + i1 * i1 * c.v[PP_S3][j];
-    const double lw = c.v[PP_C][j] - 0.5 * dist - 1.8378770664093453 - c.v[PP_H][j];
-    return c.ok[j] ? lw : SCPHD_LOG0;
+__device__ inline double det_lw(const double* r, double zu, double zv) {
+    const double i0 = zu - r[LK_MUU], i1 = zv - r[LK_MUV];
+    const double dist = i0 * i0 * r[LK_S0] + i0 * i1 * r[LK_S12] + i1 * i1 * r[LK_S3];
+    const double lw = r[LK_C] - 0.5 * dist - 1.8378770664093453 - r[LK_H];
+    return r[LK_OK] != 0.0 ? lw : SCPHD_LOG0;
 }
 
 /* Per parent: eta_{p,m} = log( sum_g exp(lw_{g,m}) + M w0 + kappa ) for
@@ -585,10 +593,9 @@ __device__ inline double det_lw(const LikChunk& c, int j, double zu, double zv) 
  * single-cluster log-likelihood sum_m eta + sum_g w_g (:1813-1849) and the
  * number of detection/birth terms whose normalised weight reaches
  * min_weight (updateKernel's prune flag, src/kernels.cu:608-659).
- * One workgroup per parent, one thread per measurement, Gaussians staged
- * through LDS. */
+ * One workgroup per parent, one thread per measurement, the Gaussians'
+ * likelihood records read by scalar loads. */
 __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
-    __shared__ LikChunk ch;
     __shared__ double red[16];
     const int p = blockIdx.x;
     const int t = threadIdx.x;
@@ -597,11 +604,6 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
     double eta_sum = 0.0;
     int cnt = 0;
     const int rounds = (a.M + NT - 1) / NT;
-    const bool single = G <= ETA_CH; /* whole map staged once */
-    if (single && G > 0) {
-        stage_chunk(ch, a, g0, G);
-        __syncthreads();
-    }
     for (int r = 0; r < rounds; ++r) {
         const int m = r * NT + t;
         const bool valid = m < a.M;
@@ -611,59 +613,43 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
          * bpc: the normaliser definition the oracle uses bit for bit */
         double mx = a.bpc;
         int am = -1;
-        for (int c0 = 0; c0 < G; c0 += ETA_CH) {
-            const int n = min(ETA_CH, G - c0);
-            if (!single) {
-                __syncthreads();
-                stage_chunk(ch, a, g0 + c0, n);
-                __syncthreads();
-            }
-            if (valid)
-                for (int j = 0; j < n; ++j) {
-                    const double l = det_lw(ch, j, zu, zv);
-                    if (l > mx) {
-                        mx = l;
-                        am = c0 + j;
-                    }
-                }
-        }
         double T = 0.0;
         int mcnt = 0;
-        for (int c0 = 0; c0 < G; c0 += ETA_CH) {
-            const int n = min(ETA_CH, G - c0);
-            if (!single) {
-                __syncthreads();
-                stage_chunk(ch, a, g0 + c0, n);
-                __syncthreads();
-            }
-            if (valid)
-                for (int j = 0; j < n; ++j) {
-                    const double x = det_lw(ch, j, zu, zv) - mx;
-                    /* exp(x) is exactly +0 for x <= -746: skipping it changes no bit */
-                    if (c0 + j != am && x > -746.0) T += exp(x);
+        if (valid) {
+            for (int j0 = 0; j0 < G; j0 += 4) {
+                const Lik4 R = load_lik4(a.lik, g0 + j0);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const double l = det_lw(R.v[q], zu, zv);
+                    if (j0 + q < G && l > mx) {
+                        mx = l;
+                        am = j0 + q;
+                    }
                 }
+            }
+            for (int j0 = 0; j0 < G; j0 += 4) {
+                const Lik4 R = load_lik4(a.lik, g0 + j0);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const double x = det_lw(R.v[q], zu, zv) - mx;
+                    /* exp(x) is exactly +0 for x <= -746: skipping it changes no bit */
+                    if (j0 + q < G && j0 + q != am && x > -746.0) T += exp(x);
+                }
+            }
         }
         if (am >= 0) T += exp(a.bpc - mx);
         const double eta = mx + log1p(T);
         if (valid) {
             a.eta[(int64_t)p * a.M + m] = eta;
             eta_sum += eta;
-        }
-        for (int c0 = 0; c0 < G; c0 += ETA_CH) {
-            const int n = min(ETA_CH, G - c0);
-            if (!single) {
-                __syncthreads();
-                stage_chunk(ch, a, g0 + c0, n);
-                __syncthreads();
-            }
-            if (valid) {
-                for (int j = 0; j < n; ++j) {
-                    const double x = det_lw(ch, j, zu, zv) - eta;
-                    if (x > a.lw_gate && ch.ok[j]) mcnt += exp(x) >= a.min_weight ? 1 : 0;
+            for (int j0 = 0; j0 < G; j0 += 4) {
+                const Lik4 R = load_lik4(a.lik, g0 + j0);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const double x = det_lw(R.v[q], zu, zv) - eta;
+                    if (j0 + q < G && x > a.lw_gate && R.v[q][LK_OK] != 0.0) mcnt += exp(x) >= a.min_weight ? 1 : 0;
                 }
             }
-        }
-        if (valid) {
             a.mcnt[(int64_t)p * a.M + m] = mcnt;
             cnt += mcnt;
             const double w = exp(a.birth.logw0 - eta);
@@ -686,7 +672,6 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
  * reference's pack order (updateKernel, src/kernels.cu:608-659: detection
  * terms measurement-major, then births). */
 __global__ __launch_bounds__(NT) void k_survivors(SurvArgs a) {
-    __shared__ LikChunk ch;
     __shared__ int sh[8];
     const EtaArgs& e = a.e;
     const int p = blockIdx.x;
@@ -705,37 +690,35 @@ __global__ __launch_bounds__(NT) void k_survivors(SurvArgs a) {
         const int cnt = valid ? e.mcnt[(int64_t)p * e.M + m] : 0; /* detection terms kept, from k_eta */
         int tot;
         int o = run + block_exscan(cnt, sh, &tot);
-        for (int c0 = 0; c0 < G; c0 += ETA_CH) {
-            const int n = min(ETA_CH, G - c0);
-            if (G > ETA_CH || r == 0) { /* a single chunk is staged once and stays */
-                __syncthreads();
-                stage_chunk(ch, e, g0 + c0, n);
-                __syncthreads();
-            }
-            if (valid)
-                for (int j = 0; j < n; ++j) {
-                    const double x = det_lw(ch, j, zu, zv) - eta;
-                    if (!(x > e.lw_gate && ch.ok[j])) continue; /* exp(x) < min_weight for sure */
-                    const double w = exp(x);
-                    if (w >= e.min_weight) {
-                        if (o < cap) {
-                            const int64_t gi = g0 + c0 + j;
-                            const int64_t so = base + o;
-                            const double i0 = zu - ch.v[PP_MUU][j], i1 = zv - ch.v[PP_MUV][j];
-                            a.sw[so] = w;
-                            a.sg[so] = (int32_t)gi;
-                            a.sm[so] = m;
+        if (valid && cnt > 0)
+            for (int j0 = 0; j0 < G; j0 += 4) {
+                const Lik4 R = load_lik4(e.lik, g0 + j0);
 #pragma unroll
-                            for (int c = 0; c < 6; ++c) {
-                                const double mu = a.fit[(FP_MU + c) * e.Ng + gi];
-                                a.smu[c * a.cap + so] = mu + i0 * e.pre[(PP_K + c) * e.Ng + gi] +
-                                                        i1 * e.pre[(PP_K + 6 + c) * e.Ng + gi];
-                            }
+                for (int q = 0; q < 4; ++q) {
+                const double* rec = R.v[q];
+                const double x = det_lw(rec, zu, zv) - eta;
+                /* exp(x) < min_weight for sure unless x passes the gate */
+                if (!(j0 + q < G && x > e.lw_gate && rec[LK_OK] != 0.0)) continue;
+                const double w = exp(x);
+                if (w >= e.min_weight) {
+                    if (o < cap) {
+                        const int64_t gi = g0 + j0 + q;
+                        const int64_t so = base + o;
+                        const double i0 = zu - rec[LK_MUU], i1 = zv - rec[LK_MUV];
+                        a.sw[so] = w;
+                        a.sg[so] = (int32_t)gi;
+                        a.sm[so] = m;
+#pragma unroll
+                        for (int c = 0; c < 6; ++c) {
+                            const double mu = a.fit[(FP_MU + c) * e.Ng + gi];
+                            a.smu[c * a.cap + so] = mu + i0 * e.pre[(PP_K + c) * e.Ng + gi] +
+                                                    i1 * e.pre[(PP_K + 6 + c) * e.Ng + gi];
                         }
-                        ++o;
                     }
+                    ++o;
                 }
-        }
+                }
+            }
         run += tot;
     }
     for (int r = 0; r < rounds; ++r) {
@@ -1901,9 +1884,9 @@ hipError_t launch_fit_fix(hipStream_t s, const FitArgs& a, Planes<T> src, Planes
 }
 
 hipError_t launch_precompute(hipStream_t s, const double* fit, const double* gw, int64_t Ng, double var_u,
-                             double var_v, double* pre, int32_t* pdok) {
+                             double var_v, double* pre, double* lik) {
     if (Ng <= 0) return hipSuccess;
-    k_precompute<<<nblocks(Ng, 128), 128, 0, s>>>(fit, gw, Ng, var_u, var_v, pre, pdok);
+    k_precompute<<<nblocks(Ng, 128), 128, 0, s>>>(fit, gw, Ng, var_u, var_v, pre, lik);
     LAUNCH_CHECK();
 }
 
