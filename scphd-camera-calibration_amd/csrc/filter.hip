@@ -130,7 +130,7 @@ struct scphd_filter {
     /* measurements, normalisers, survivors, merge */
     DBuf<double> du, dv, eta;
     int capM = 0;
-    DBuf<int32_t> nsurv, survoff, nmerged, mergeoff, outcnt, scantmp, sg, sm, dflag, mcnt;
+    DBuf<int32_t> nsurv, survoff, nmerged, mergeoff, outcnt, scantmp, sg, sm, dflag, mcnt, mncand, mcand;
     DBuf<int32_t> ngroups, glead, gsta, gend, gmem;
     DBuf<double> sw, smu, mw, mmu, mP, dneff, est;
     DBuf<char> mscratch;
@@ -308,7 +308,11 @@ int update_t(scphd_filter* f, int M, int fixed) {
     ea.loglik = f->loglik.p;
     ea.nsurv = f->nsurv.p;
     HIPTRY(f->mcnt.ensure((int64_t)Np * std::max(M, 1)));
+    HIPTRY(f->mncand.ensure((int64_t)Np * std::max(M, 1)));
+    HIPTRY(f->mcand.ensure((int64_t)Np * std::max(M, 1) * ETA_NCAND));
     ea.mcnt = f->mcnt.p;
+    ea.mncand = f->mncand.p;
+    ea.mcand = f->mcand.p;
     ea.lw_gate = P.min_weight > 0 ? std::log(P.min_weight) - 1e-3 : -INFINITY;
     HIPTRY(launch_eta(s, ea));
     HIPTRY(launch_scan(s, f->nsurv.p, f->survoff.p, Np, f->scantmp.p));
@@ -556,7 +560,7 @@ void scphd_destroy(scphd_t* f) {
     for (auto* b : dbs) b->release();
     DBuf<int32_t>* ibs[] = {&f->anc, &f->newcnt, &f->goff, &f->gparent, &f->gblock, &f->goff2, &f->gparent2,
                             &f->gblock2, &f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->nsurv, &f->survoff,
-                            &f->nmerged, &f->mergeoff, &f->outcnt, &f->scantmp, &f->sg, &f->sm, &f->dflag, &f->mcnt,
+                            &f->nmerged, &f->mergeoff, &f->outcnt, &f->scantmp, &f->sg, &f->sm, &f->dflag, &f->mcnt, &f->mncand, &f->mcand,
                             &f->ngroups, &f->glead, &f->gsta, &f->gend, &f->gmem};
     for (auto* b : ibs) b->release();
     f->mscratch.release();
@@ -886,7 +890,7 @@ int scphd_get_memory(scphd_t* f, int64_t* bytes) {
     for (auto* d : dbs) b += d->n * (int64_t)sizeof(double);
     DBuf<int32_t>* ibs[] = {&f->anc, &f->newcnt, &f->goff, &f->gparent, &f->gblock, &f->goff2, &f->gparent2,
                             &f->gblock2, &f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->nsurv, &f->survoff,
-                            &f->nmerged, &f->mergeoff, &f->outcnt, &f->scantmp, &f->sg, &f->sm, &f->dflag, &f->mcnt,
+                            &f->nmerged, &f->mergeoff, &f->outcnt, &f->scantmp, &f->sg, &f->sm, &f->dflag, &f->mcnt, &f->mncand, &f->mcand,
                             &f->ngroups, &f->glead, &f->gsta, &f->gend, &f->gmem};
     for (auto* d : ibs) b += d->n * (int64_t)sizeof(int32_t);
     b += f->mscratch.n;

@@ -120,8 +120,11 @@ struct EtaArgs {
     double* loglik;    /* Np   */
     int32_t* nsurv;    /* Np   */
     int32_t* mcnt;     /* Np*M detection terms kept per (parent, measurement) */
+    int32_t* mncand;   /* Np*M candidates per (parent, measurement): terms that may be kept */
+    int32_t* mcand;    /* Np*M*ETA_NCAND their map indices (ascending), when at most ETA_NCAND */
     double lw_gate;    /* log(min_weight) - 1e-3: below it exp() < min_weight for certain */
 };
+static constexpr int ETA_NCAND = 4;
 hipError_t launch_eta(hipStream_t s, const EtaArgs& a);
 
 struct SurvArgs {

@@ -615,6 +615,10 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
         int am = -1;
         double T = 0.0;
         int mcnt = 0;
+        /* candidates: terms with l - mx above the gate (less 1e-6), the only
+         * ones that can pass it against eta >= mx; the first ETA_NCAND kept */
+        int nc = 0, c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+        const double cgate = a.lw_gate - 1e-6;
         if (valid) {
             for (int j0 = 0; j0 < G; j0 += 4) {
                 const Lik4 R = load_lik4(a.lik, g0 + j0);
@@ -634,23 +638,44 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
                     const double x = det_lw(R.v[q], zu, zv) - mx;
                     /* exp(x) is exactly +0 for x <= -746: skipping it changes no bit */
                     if (j0 + q < G && j0 + q != am && x > -746.0) T += exp(x);
+                    if (j0 + q < G && x > cgate && R.v[q][LK_OK] != 0.0) {
+                        const int j = j0 + q;
+                        c0 = nc == 0 ? j : c0;
+                        c1 = nc == 1 ? j : c1;
+                        c2 = nc == 2 ? j : c2;
+                        c3 = nc == 3 ? j : c3;
+                        ++nc;
+                    }
                 }
             }
         }
         if (am >= 0) T += exp(a.bpc - mx);
         const double eta = mx + log1p(T);
         if (valid) {
-            a.eta[(int64_t)p * a.M + m] = eta;
+            const int64_t pm = (int64_t)p * a.M + m;
+            a.eta[pm] = eta;
             eta_sum += eta;
-            for (int j0 = 0; j0 < G; j0 += 4) {
-                const Lik4 R = load_lik4(a.lik, g0 + j0);
+            if (nc <= ETA_NCAND) { /* the candidates alone */
+                for (int k = 0; k < nc; ++k) {
+                    const int j = k == 0 ? c0 : k == 1 ? c1 : k == 2 ? c2 : c3;
+                    const double* rec = a.lik + (g0 + j) * LK_COUNT;
+                    const double x = det_lw(rec, zu, zv) - eta;
+                    if (x > a.lw_gate && rec[LK_OK] != 0.0) mcnt += exp(x) >= a.min_weight ? 1 : 0;
+                    a.mcand[pm * ETA_NCAND + k] = j;
+                }
+            } else {
+                for (int j0 = 0; j0 < G; j0 += 4) {
+                    const Lik4 R = load_lik4(a.lik, g0 + j0);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const double x = det_lw(R.v[q], zu, zv) - eta;
-                    if (j0 + q < G && x > a.lw_gate && R.v[q][LK_OK] != 0.0) mcnt += exp(x) >= a.min_weight ? 1 : 0;
+                    for (int q = 0; q < 4; ++q) {
+                        const double x = det_lw(R.v[q], zu, zv) - eta;
+                        if (j0 + q < G && x > a.lw_gate && R.v[q][LK_OK] != 0.0)
+                            mcnt += exp(x) >= a.min_weight ? 1 : 0;
+                    }
                 }
             }
-            a.mcnt[(int64_t)p * a.M + m] = mcnt;
+            a.mncand[pm] = nc;
+            a.mcnt[pm] = mcnt;
             cnt += mcnt;
             const double w = exp(a.birth.logw0 - eta);
             cnt += (w >= a.min_weight && a.birth.pd_ok) ? 1 : 0;
@@ -690,35 +715,46 @@ __global__ __launch_bounds__(NT) void k_survivors(SurvArgs a) {
         const int cnt = valid ? e.mcnt[(int64_t)p * e.M + m] : 0; /* detection terms kept, from k_eta */
         int tot;
         int o = run + block_exscan(cnt, sh, &tot);
-        if (valid && cnt > 0)
-            for (int j0 = 0; j0 < G; j0 += 4) {
-                const Lik4 R = load_lik4(e.lik, g0 + j0);
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                const double* rec = R.v[q];
+        if (valid && cnt > 0) {
+            const int64_t pm = (int64_t)p * e.M + m;
+            /* detection term of map entry j: kept when its normalised weight
+             * reaches min_weight (the gate first: exp(x) < min_weight below it) */
+            auto term = [&](int j, const double* rec) {
                 const double x = det_lw(rec, zu, zv) - eta;
-                /* exp(x) < min_weight for sure unless x passes the gate */
-                if (!(j0 + q < G && x > e.lw_gate && rec[LK_OK] != 0.0)) continue;
+                if (!(x > e.lw_gate && rec[LK_OK] != 0.0)) return;
                 const double w = exp(x);
-                if (w >= e.min_weight) {
-                    if (o < cap) {
-                        const int64_t gi = g0 + j0 + q;
-                        const int64_t so = base + o;
-                        const double i0 = zu - rec[LK_MUU], i1 = zv - rec[LK_MUV];
-                        a.sw[so] = w;
-                        a.sg[so] = (int32_t)gi;
-                        a.sm[so] = m;
+                if (w < e.min_weight) return;
+                if (o < cap) {
+                    const int64_t gi = g0 + j;
+                    const int64_t so = base + o;
+                    const double i0 = zu - rec[LK_MUU], i1 = zv - rec[LK_MUV];
+                    a.sw[so] = w;
+                    a.sg[so] = (int32_t)gi;
+                    a.sm[so] = m;
 #pragma unroll
-                        for (int c = 0; c < 6; ++c) {
-                            const double mu = a.fit[(FP_MU + c) * e.Ng + gi];
-                            a.smu[c * a.cap + so] = mu + i0 * e.pre[(PP_K + c) * e.Ng + gi] +
-                                                    i1 * e.pre[(PP_K + 6 + c) * e.Ng + gi];
-                        }
+                    for (int c = 0; c < 6; ++c) {
+                        const double mu = a.fit[(FP_MU + c) * e.Ng + gi];
+                        a.smu[c * a.cap + so] =
+                            mu + i0 * e.pre[(PP_K + c) * e.Ng + gi] + i1 * e.pre[(PP_K + 6 + c) * e.Ng + gi];
                     }
-                    ++o;
                 }
+                ++o;
+            };
+            const int nc = e.mncand[pm];
+            if (nc <= ETA_NCAND) { /* the candidates k_eta listed, ascending */
+                for (int k = 0; k < nc; ++k) {
+                    const int j = e.mcand[pm * ETA_NCAND + k];
+                    term(j, e.lik + (g0 + j) * LK_COUNT);
+                }
+            } else {
+                for (int j0 = 0; j0 < G; j0 += 4) {
+                    const Lik4 R = load_lik4(e.lik, g0 + j0);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        if (j0 + q < G) term(j0 + q, R.v[q]);
                 }
             }
+        }
         run += tot;
     }
     for (int r = 0; r < rounds; ++r) {
