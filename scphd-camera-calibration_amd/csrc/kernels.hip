@@ -631,13 +631,19 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
                     }
                 }
             }
+            /* terms that cannot change T are skipped, which changes no bit:
+             * exp(x) is +0 for x <= -746, and for T in [2^(e-1), 2^e) a term
+             * below 2^(e-55), a quarter ulp, leaves T + exp(x) == T */
+            double thr = -746.0;
             for (int j0 = 0; j0 < G; j0 += 4) {
                 const Lik4 R = load_lik4(a.lik, g0 + j0);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const double x = det_lw(R.v[q], zu, zv) - mx;
-                    /* exp(x) is exactly +0 for x <= -746: skipping it changes no bit */
-                    if (j0 + q < G && j0 + q != am && x > -746.0) T += exp(x);
+                    if (j0 + q < G && j0 + q != am && x > thr) {
+                        T += exp(x);
+                        thr = fmax(-746.0, (double)(__builtin_amdgcn_frexp_exp(T) - 55) * 0.6931471805599453);
+                    }
                     if (j0 + q < G && x > cgate && R.v[q][LK_OK] != 0.0) {
                         const int j = j0 + q;
                         c0 = nc == 0 ? j : c0;
