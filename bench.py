@@ -287,6 +287,8 @@ def main():
                                  f"written + 248 B per Gaussian (fit planes, table, flags); last launch "
                                  f"{fit_bytes[-1]} B",
             "k_sample_achieved_gbs": smp_gbs,
+            "limiter_note": "hbm is the roofline the bytes are priced against; the kernel's limiter is VALU "
+                            "issue (SQ counters in profiles/k_fit_c1_counters.txt), with HBM close behind",
         },
     }
     if not args.no_served:
