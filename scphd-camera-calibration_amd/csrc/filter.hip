@@ -126,7 +126,7 @@ struct scphd_filter {
     int cur = 0;
     /* per-Gaussian intermediates */
     DBuf<double> fit, pre, lik;
-    DBuf<int32_t> keep, keepscan, ndlist, redo;
+    DBuf<int32_t> keep, keepscan, ndlist, redo, ndfill;
     /* measurements, normalisers, survivors, merge */
     DBuf<double> du, dv, eta;
     int capM = 0;
@@ -250,6 +250,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
         HIPTRY(f->lik.ensure((int64_t)LK_COUNT * (Ng + 3))); /* 3 records of padding: read four at a time */
         HIPTRY(f->keep.ensure(Ng));
         HIPTRY(f->redo.ensure(Ng));
+        HIPTRY(f->ndfill.ensure(Ng + 1)); /* [Ng]: the list length */
         HIPTRY(f->ndlist.ensure(Ng));
         FitArgs fa;
         fa.gblock = f->gblock.p;
@@ -269,6 +270,9 @@ int update_t(scphd_filter* f, int M, int fixed) {
         fa.fit = f->fit.p;
         fa.keep = f->keep.p;
         fa.redo = f->redo.p;
+        fa.ndfill = f->ndfill.p;
+        fa.ndfill_n = f->ndfill.p + Ng;
+        HIPTRY(hipMemsetAsync(fa.ndfill_n, 0, sizeof(int32_t), s));
         fa.nd_base = 0;
         fa.off32 = std::max(f->gen[0].cap_blocks, f->gen[1].cap_blocks) * (int64_t)S * (int64_t)f->esz <
                    (int64_t)0xFFFFFFFFll - 64 * (int64_t)f->esz;
@@ -276,6 +280,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
         HIPTRY(launch_fit<T>(s, fa, f->gen[f->cur].planes<T>(), f->gen[nxt].planes<T>()));
         record(f, 7);
         HIPTRY(launch_fit_fix<T>(s, fa, f->gen[f->cur].planes<T>(), f->gen[nxt].planes<T>()));
+        HIPTRY(launch_nd_fill<T>(s, fa, f->gen[f->cur].planes<T>(), f->gen[nxt].planes<T>()));
         HIPTRY(launch_precompute(s, f->fit.p, f->gw.p, Ng, P.std_u * P.std_u, P.std_v * P.std_v, f->pre.p,
                                  f->lik.p));
     }
@@ -559,7 +564,7 @@ void scphd_destroy(scphd_t* f) {
                            &f->dneff, &f->est};
     for (auto* b : dbs) b->release();
     DBuf<int32_t>* ibs[] = {&f->anc, &f->newcnt, &f->goff, &f->gparent, &f->gblock, &f->goff2, &f->gparent2,
-                            &f->gblock2, &f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->nsurv, &f->survoff,
+                            &f->gblock2, &f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->ndfill, &f->nsurv, &f->survoff,
                             &f->nmerged, &f->mergeoff, &f->outcnt, &f->scantmp, &f->sg, &f->sm, &f->dflag, &f->mcnt, &f->mncand, &f->mcand,
                             &f->ngroups, &f->glead, &f->gsta, &f->gend, &f->gmem};
     for (auto* b : ibs) b->release();
@@ -889,7 +894,7 @@ int scphd_get_memory(scphd_t* f, int64_t* bytes) {
                            &f->dneff, &f->est};
     for (auto* d : dbs) b += d->n * (int64_t)sizeof(double);
     DBuf<int32_t>* ibs[] = {&f->anc, &f->newcnt, &f->goff, &f->gparent, &f->gblock, &f->goff2, &f->gparent2,
-                            &f->gblock2, &f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->nsurv, &f->survoff,
+                            &f->gblock2, &f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->ndfill, &f->nsurv, &f->survoff,
                             &f->nmerged, &f->mergeoff, &f->outcnt, &f->scantmp, &f->sg, &f->sm, &f->dflag, &f->mcnt, &f->mncand, &f->mcand,
                             &f->ngroups, &f->glead, &f->gsta, &f->gend, &f->gmem};
     for (auto* d : ibs) b += d->n * (int64_t)sizeof(int32_t);

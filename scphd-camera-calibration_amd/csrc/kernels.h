@@ -90,15 +90,20 @@ struct FitArgs {
     double* fit;     /* FP_COUNT planes */
     int32_t* keep;   /* non-detection term kept */
     int32_t* redo;   /* per Gaussian: 1 when every sample weight was zero (refit by k_fit_fix) */
+    int32_t* ndfill;   /* kept terms whose clouds k_nd_fill writes (decided after the sweep) */
+    int32_t* ndfill_n; /* their number; zero before k_fit */
     int64_t nd_base; /* destination block of the non-detection cloud of entry g: nd_base + g */
     int off32;       /* every source and destination plane is below 4 GiB: 32-bit byte offsets */
 };
 
-/* k_fit: the streaming sweep; k_fit_fix: the rare cases it flags in redo[] */
+/* k_fit: the streaming sweep; k_fit_fix: the rare cases it flags in redo[];
+ * k_nd_fill: the non-detection clouds it lists in ndfill[] */
 template <typename T>
 hipError_t launch_fit(hipStream_t s, const FitArgs& a, Planes<T> src, Planes<T> dst);
 template <typename T>
 hipError_t launch_fit_fix(hipStream_t s, const FitArgs& a, Planes<T> src, Planes<T> dst);
+template <typename T>
+hipError_t launch_nd_fill(hipStream_t s, const FitArgs& a, Planes<T> src, Planes<T> dst);
 
 hipError_t launch_precompute(hipStream_t s, const double* fit, const double* gw, int64_t Ng, double var_u,
                              double var_v, double* pre, double* lik);

@@ -12,6 +12,8 @@
 
 #include <float.h>
 
+#include <algorithm>
+
 #include <type_traits>
 
 namespace scphd {
@@ -312,8 +314,11 @@ __device__ inline void write_fit(const FitArgs& a, int64_t g, int lane, const do
  * computeNonDetections src/scphdcameracalibration.cu:1290-1420; sample
  * weights 1 - pd_i; the samples themselves are the predicted cloud, which
  * the reference's world -> disparity -> world round trip reproduces to
- * rounding) is written during the sweep for every term not certainly
- * dropped.  A cloud whose weights are all zero, which the reference
+ * rounding) is written during the sweep when the term is kept for certain,
+ * w_g (1 - Pd) >= min_weight (pd_g <= Pd).  A term that may go either way
+ * is decided after the sweep and, when kept, listed for k_nd_fill, which
+ * writes its cloud; most such terms are dropped, and their clouds are never
+ * written.  A cloud whose weights are all zero, which the reference
  * normalises to weight 1 per sample, is refitted by k_fit_fix (redo[g]) when
  * pd_g > 0; with pd_g = 0 its fit cannot reach any output and is the first
  * sample with zero covariance. */
@@ -345,20 +350,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
     const int S = a.S;
     const double wg = a.gw[g];
     const double Pd = a.meas.pd();
-    /* the non-detection cloud is written during the sweep unless the term is
-     * certainly dropped (1 - pd_g <= 1); whether it is kept is decided after
-     * the sweep, and an unkept cloud is simply not referenced by the next
-     * map table (a write is cheaper than a second sweep) */
-    const bool nd_write = wg >= a.min_weight;
-    /* ... and the row stops writing it once its in-view count proves the
-     * term dropped: the computed pd sum is at least n_in Pd (1 - 1e-9), so
-     * n_in > S (1 - min_weight / wg) / (Pd (1 - 1e-9)) + 1 gives
-     * wg (1 - pd_g) < min_weight */
-    const double nin_max = (Pd > 0 && a.min_weight > 0 && nd_write)
-                               ? (1.0 - a.min_weight / wg) * S / (Pd * (1.0 - 1e-9)) + 1.0
-                               : INFINITY;
-    bool ndw = nd_write;
-    int nin = 0;
+    const bool nd_sure = wg * (1.0 - Pd) >= a.min_weight; /* kept for certain: cloud written in the sweep */
     const int64_t d0 = (a.nd_base + g) * (int64_t)S;
 
     double spd = 0.0, sw = 0.0, sw2 = 0.0, m1[6], m2[21], K[6];
@@ -370,7 +362,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
      * (measured faster than one step ahead); the buffer rotation at the end
      * of a step waits for that step's load.  Full steps carry no per-lane
      * sample guard; only a partial last step does. */
-    const int nin_lim = nin_max < 1e9 ? (int)floor(nin_max) : 0x7fffffff; /* n_in <= nin_max */
     T cur[7], nx1[7], nx2[7], nx3[7];
     Off sof = (Off)((b0 + lane) * (int64_t)sizeof(T)); /* byte offsets, advanced 16 samples a step */
     Off dof = (Off)((d0 + lane) * (int64_t)sizeof(T));
@@ -420,11 +411,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
         for (int r = 0; r < 6; ++r)
 #pragma unroll
             for (int c = 0; c <= r; ++c) m2[pidx(r, c)] += wp * dv[r] * dv[c];
-        if (ndw && valid) write_nd_off(a, dst, dof, x, in);
+        if (nd_sure && valid) write_nd_off(a, dst, dof, x, in);
         dof += (Off)(16 * sizeof(T));
-        const uint64_t bin = __ballot(valid && in);
-        nin += __popcll((bin >> (threadIdx.x & 48)) & 0xFFFFull); /* this row's 16 lanes */
-        ndw = ndw && nin <= nin_lim;
     };
     const std::integral_constant<bool, true> full;
     const std::integral_constant<bool, false> guarded;
@@ -464,6 +452,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
     if (lane == 0) {
         a.keep[g] = keep ? 1 : 0;
         a.redo[g] = (sw > 0 || spd == 0.0) ? 0 : 1;
+        if (keep && !nd_sure) a.ndfill[atomicAdd(a.ndfill_n, 1)] = (int32_t)g;
+    }
+}
+
+/* The non-detection clouds of the terms k_fit decided only after its sweep
+ * and kept, listed in ndfill[0, *ndfill_n): the predicted samples and the
+ * weights 1 - pd_i, computed by the same code as in the sweep.  One 16-lane
+ * row per listed term; workgroups stride over the list (its length is read
+ * on the device). */
+template <typename T, typename Off>
+__global__ __launch_bounds__(256) void k_nd_fill(FitArgs a, Planes<T> src, Planes<T> dst) {
+    const int lane = threadIdx.x & 15;
+    const int n = *a.ndfill_n;
+    const int stride = gridDim.x * 16;
+    for (int i = blockIdx.x * 16 + (threadIdx.x >> 4); i < n; i += stride) {
+        const int64_t g = a.ndfill[i];
+        const int p = a.gparent[g];
+        const int k = (int)(g - a.goff[p]);
+        const int64_t b0 = (int64_t)a.gblock[g] * a.S;
+        const int64_t d0 = (a.nd_base + g) * (int64_t)a.S;
+        const CameraPose pose = load_pose(a.pose, p);
+        for (int s = lane; s < a.S; s += 16) {
+            T r[7];
+            load_sample_off(src, (Off)((b0 + s) * (int64_t)sizeof(T)), r);
+            EuclideanPoint x;
+            DisparityPoint d;
+            bool in;
+            double w;
+            eval_sample(a, r, p, k, s, pose, x, d, in, w);
+            write_nd_off(a, dst, (Off)((d0 + s) * (int64_t)sizeof(T)), x, in);
+        }
     }
 }
 
@@ -1925,6 +1944,17 @@ hipError_t launch_fit_fix(hipStream_t s, const FitArgs& a, Planes<T> src, Planes
     LAUNCH_CHECK();
 }
 
+template <typename T>
+hipError_t launch_nd_fill(hipStream_t s, const FitArgs& a, Planes<T> src, Planes<T> dst) {
+    if (a.Ng <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>(nblocks(a.Ng, 16), 2048); /* the list is at most Ng long */
+    if (a.off32)
+        k_nd_fill<T, uint32_t><<<blocks, 256, 0, s>>>(a, src, dst);
+    else
+        k_nd_fill<T, uint64_t><<<blocks, 256, 0, s>>>(a, src, dst);
+    LAUNCH_CHECK();
+}
+
 hipError_t launch_precompute(hipStream_t s, const double* fit, const double* gw, int64_t Ng, double var_u,
                              double var_v, double* pre, double* lik) {
     if (Ng <= 0) return hipSuccess;
@@ -2054,6 +2084,8 @@ template hipError_t launch_fit<float>(hipStream_t, const FitArgs&, Planes<float>
 template hipError_t launch_fit<double>(hipStream_t, const FitArgs&, Planes<double>, Planes<double>);
 template hipError_t launch_fit_fix<float>(hipStream_t, const FitArgs&, Planes<float>, Planes<float>);
 template hipError_t launch_fit_fix<double>(hipStream_t, const FitArgs&, Planes<double>, Planes<double>);
+template hipError_t launch_nd_fill<float>(hipStream_t, const FitArgs&, Planes<float>, Planes<float>);
+template hipError_t launch_nd_fill<double>(hipStream_t, const FitArgs&, Planes<double>, Planes<double>);
 template hipError_t launch_sample<float>(hipStream_t, const SampleArgs&, Planes<float>);
 template hipError_t launch_sample<double>(hipStream_t, const SampleArgs&, Planes<double>);
 template hipError_t launch_extract<float>(hipStream_t, Planes<float>, const int32_t*, int64_t, int, double*);
