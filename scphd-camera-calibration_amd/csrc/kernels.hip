@@ -358,39 +358,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
     for (int c = 0; c < 6; ++c) m1[c] = 0.0;
 #pragma unroll
     for (int e = 0; e < 21; ++e) m2[e] = 0.0;
-    /* loads are issued three steps (48 samples) ahead of the arithmetic
-     * (measured faster than one step ahead); the buffer rotation at the end
-     * of a step waits for that step's load.  Full steps carry no per-lane
-     * sample guard; only a partial last step does. */
-    T cur[7], nx1[7], nx2[7], nx3[7];
+    /* three sample buffers: a step consumes one and loads the step two
+     * ahead into the one its predecessor consumed.  Full steps go in groups
+     * of three with the buffers in place, so no register copy waits on a
+     * load in flight; the last steps (fewer than three, or a partial one)
+     * rotate the buffers.  Full steps carry no per-lane sample guard. */
+    T bA[7], bB[7], bC[7];
     Off sof = (Off)((b0 + lane) * (int64_t)sizeof(T)); /* byte offsets, advanced 16 samples a step */
     Off dof = (Off)((d0 + lane) * (int64_t)sizeof(T));
     const Off last = (Off)((b0 + S - 1) * (int64_t)sizeof(T)); /* the cloud's last sample */
-    if (lane < S) load_sample_off(src, sof, cur);
+    if (lane < S) load_sample_off(src, sof, bA);
     sof += (Off)(16 * sizeof(T));
-    if (lane + 16 < S) load_sample_off(src, sof, nx1);
-    sof += (Off)(16 * sizeof(T));
-    if (lane + 32 < S) load_sample_off(src, sof, nx2);
-    /* one step of 16 samples, s0 .. s0 + 15; FULL: all of them exist.  The
-     * load of the step three ahead is unconditional (no branch in the loop):
-     * past the end of the cloud it reads the last sample again. */
-    auto step = [&](auto full, int s0) {
+    if (lane + 16 < S) load_sample_off(src, sof, bB);
+    /* one step of 16 samples, s0 .. s0 + 15, from `cur`, loading the step
+     * two ahead into `fill`; FULL: all of them exist.  That load is
+     * unconditional (no branch in the loop): past the end of the cloud it
+     * reads the last sample again. */
+    auto step = [&](auto full, const T(&cur)[7], T(&fill)[7], int s0) {
         constexpr bool FULL = decltype(full)::value;
         const int s = s0 + lane;
         const bool valid = FULL || s < S;
         sof += (Off)(16 * sizeof(T));
-        load_sample_off(src, s + 48 < S ? sof : last, nx3);
+        load_sample_off(src, s + 32 < S ? sof : last, fill);
         EuclideanPoint x;
         DisparityPoint d;
         bool in = false;
         double w = 0.0;
         if (valid) eval_sample(a, cur, p, k, s, pose, x, d, in, w);
-#pragma unroll
-        for (int c = 0; c < 7; ++c) {
-            cur[c] = nx1[c];
-            nx1[c] = nx2[c];
-            nx2[c] = nx3[c];
-        }
         const double q[6] = {d.u, d.v, d.d, d.vu, d.vv, d.vd};
         if (s0 == 0) {
             /* shift: the Gaussian's first sample, broadcast along the row */
@@ -417,8 +411,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
     const std::integral_constant<bool, true> full;
     const std::integral_constant<bool, false> guarded;
     int s0 = 0;
-    for (; s0 + 16 <= S; s0 += 16) step(full, s0);
-    if (s0 < S) step(guarded, s0);
+    for (; s0 + 48 <= S; s0 += 48) { /* three full steps, buffers in place */
+        step(full, bA, bC, s0);
+        step(full, bB, bA, s0 + 16);
+        step(full, bC, bB, s0 + 32);
+    }
+    for (; s0 < S; s0 += 16) { /* the last steps: buffers rotate */
+        step(guarded, bA, bC, s0);
+#pragma unroll
+        for (int c = 0; c < 7; ++c) {
+            bA[c] = bB[c];
+            bB[c] = bC[c];
+        }
+    }
     spd = row_sum16(spd);
     sw = row_sum16(sw);
     sw2 = row_sum16(sw2);
