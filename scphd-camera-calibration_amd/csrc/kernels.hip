@@ -478,15 +478,26 @@ __global__ __launch_bounds__(256) void k_nd_fill(FitArgs a, Planes<T> src, Plane
         const int64_t b0 = (int64_t)a.gblock[g] * a.S;
         const int64_t d0 = (a.nd_base + g) * (int64_t)a.S;
         const CameraPose pose = load_pose(a.pose, p);
-        for (int s = lane; s < a.S; s += 16) {
-            T r[7];
-            load_sample_off(src, (Off)((b0 + s) * (int64_t)sizeof(T)), r);
-            EuclideanPoint x;
-            DisparityPoint d;
-            bool in;
-            double w;
-            eval_sample(a, r, p, k, s, pose, x, d, in, w);
-            write_nd_off(a, dst, (Off)((d0 + s) * (int64_t)sizeof(T)), x, in);
+        /* four steps' loads issued before their arithmetic */
+        for (int s0 = 0; s0 < a.S; s0 += 64) {
+            T r[4][7];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int s = s0 + 16 * j + lane;
+                if (s < a.S) load_sample_off(src, (Off)((b0 + s) * (int64_t)sizeof(T)), r[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int s = s0 + 16 * j + lane;
+                if (s < a.S) {
+                    EuclideanPoint x;
+                    DisparityPoint d;
+                    bool in;
+                    double w;
+                    eval_sample(a, r[j], p, k, s, pose, x, d, in, w);
+                    write_nd_off(a, dst, (Off)((d0 + s) * (int64_t)sizeof(T)), x, in);
+                }
+            }
         }
     }
 }
@@ -600,6 +611,18 @@ __device__ inline Lik4 load_lik4(const double* lik, int64_t g) {
     return x;
 }
 
+__device__ inline double det_lw(const double* r, double zu, double zv);
+
+/* One record through the scalar path (its index the same in every lane). */
+__device__ inline double det_lw(lik_t* r, double zu, double zv) {
+    double v[LK_COUNT];
+#pragma unroll
+    for (int q = 0; q < LK_COUNT; ++q) v[q] = r[q];
+    return det_lw((const double*)v, zu, zv);
+}
+
+__device__ inline lik_t* lik_rec(const double* lik, int64_t g) { return (lik_t*)(lik + g * LK_COUNT); }
+
 /* log weight of the detection term of Gaussian record r and measurement
  * (zu, zv): log pd + log w - dist/2 - log 2 pi - log det / 2
  * (src/kernels.cu:527-539); log(0) when the updated covariance is not PD
@@ -619,8 +642,12 @@ __device__ inline double det_lw(const double* r, double zu, double zv) {
  * min_weight (updateKernel's prune flag, src/kernels.cu:608-659).
  * One workgroup per parent, one thread per measurement, the Gaussians'
  * likelihood records read by scalar loads. */
+static constexpr int ETA_HEAVY = 2048; /* LDS list of the Gaussians that can exceed bpc */
+
 __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
     __shared__ double red[16];
+    __shared__ int sh[8];
+    __shared__ int s_heavy[ETA_HEAVY];
     const int p = blockIdx.x;
     const int t = threadIdx.x;
     const int64_t g0 = a.goff[p];
@@ -628,6 +655,28 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
     double eta_sum = 0.0;
     int cnt = 0;
     const int rounds = (a.M + NT - 1) / NT;
+    /* Only a Gaussian with l > bpc can move the maximum off bpc, and
+     * l <= C - log 2 pi - H (the value at dist = 0; the computed quadratic
+     * form is never below -1e-9 of its scale), so the max pass visits only
+     * those whose bound exceeds bpc (with a 1e-6 relative margin), in
+     * ascending order: the same first maximum.  More than ETA_HEAVY of them:
+     * the whole map. */
+    int nheavy = 0;
+    for (int j0 = 0; j0 < G; j0 += NT) {
+        const int j = j0 + t;
+        bool h = false;
+        if (j < G) {
+            const double* rec = a.lik + (g0 + j) * LK_COUNT;
+            const double ub = rec[LK_C] - 1.8378770664093453 - rec[LK_H];
+            h = rec[LK_OK] != 0.0 && ub + 1e-6 * (1.0 + fabs(ub)) > a.bpc;
+        }
+        int tot;
+        const int pos = nheavy + block_exscan(h ? 1 : 0, sh, &tot);
+        if (h && pos < ETA_HEAVY) s_heavy[pos] = j;
+        nheavy += tot;
+    }
+    __syncthreads();
+    const bool listed = nheavy <= ETA_HEAVY;
     for (int r = 0; r < rounds; ++r) {
         const int m = r * NT + t;
         const bool valid = m < a.M;
@@ -644,14 +693,25 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
         int nc = 0, c0 = 0, c1 = 0, c2 = 0, c3 = 0;
         const double cgate = a.lw_gate - 1e-6;
         if (valid) {
-            for (int j0 = 0; j0 < G; j0 += 4) {
-                const Lik4 R = load_lik4(a.lik, g0 + j0);
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const double l = det_lw(R.v[q], zu, zv);
-                    if (j0 + q < G && l > mx) {
+            if (listed) {
+                for (int q = 0; q < nheavy; ++q) {
+                    const int j = __builtin_amdgcn_readfirstlane(s_heavy[q]);
+                    const double l = det_lw(lik_rec(a.lik, g0 + j), zu, zv);
+                    if (l > mx) {
                         mx = l;
-                        am = j0 + q;
+                        am = j;
+                    }
+                }
+            } else {
+                for (int j0 = 0; j0 < G; j0 += 4) {
+                    const Lik4 R = load_lik4(a.lik, g0 + j0);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const double l = det_lw(R.v[q], zu, zv);
+                        if (j0 + q < G && l > mx) {
+                            mx = l;
+                            am = j0 + q;
+                        }
                     }
                 }
             }

@@ -119,6 +119,16 @@ def test_lockstep_samples_per_gaussian_200():
     run_lockstep(p, s, frames=4)
 
 
+def test_lockstep_birth_term_dominates():
+    """Birth weight 0.05: log(M w0 + kappa) exceeds every detection term's
+    log weight (a 2-D density is at most 1 / (2 pi) with S >= R = I), so the
+    normaliser's maximum is the birth/clutter term for every measurement and
+    the max pass visits no Gaussian."""
+    p, s = small(1, n_particles=24, birth_w0=0.05)
+    rep = run_lockstep(p, s, frames=4)
+    assert rep[-1][1] > 0
+
+
 def test_lockstep_pd_one_refits_all_zero_clouds():
     """Pd = 1: a kept non-detection cloud weighs 0 wherever it is in view, so
     in the next update every w_i pd_i is 0 while pd_g > 0 -- the cloud is
