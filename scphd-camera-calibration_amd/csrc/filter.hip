@@ -134,8 +134,8 @@ struct scphd_filter {
     DBuf<int32_t> ngroups, glead, gsta, gend, gmem;
     DBuf<double> sw, smu, mw, mmu, mP, dneff, est;
     DBuf<char> mscratch;
-    int32_t* hbuf = nullptr; /* pinned read-back area */
-    int64_t hbuf_n = 0;
+    int32_t* hmap = nullptr; /* mapped host memory the device writes scalars into (k_readout) */
+    int32_t* dmap = nullptr; /* its device address */
     std::vector<int32_t> hoff;
 
     /* counters and deferred prediction */
@@ -168,16 +168,6 @@ int hipfail(scphd_filter* f, hipError_t e, const char* what) {
         hipError_t e__ = (expr);                                  \
         if (e__ != hipSuccess) return hipfail(f, e__, #expr);     \
     } while (0)
-
-int ensure_hbuf(scphd_filter* f, int64_t n) {
-    if (n <= f->hbuf_n) return SCPHD_OK;
-    if (f->hbuf) (void)hipHostFree(f->hbuf);
-    f->hbuf = nullptr;
-    const int64_t cap = std::max<int64_t>(n + n / 4, 1024);
-    HIPTRY(hipHostMalloc((void**)&f->hbuf, (size_t)cap * sizeof(int32_t), hipHostMallocDefault));
-    f->hbuf_n = cap;
-    return SCPHD_OK;
-}
 
 void set_models(scphd_filter* f) {
     const scphd_params& p = f->prm;
@@ -321,12 +311,10 @@ int update_t(scphd_filter* f, int M, int fixed) {
     ea.lw_gate = P.min_weight > 0 ? std::log(P.min_weight) - 1e-3 : -INFINITY;
     HIPTRY(launch_eta(s, ea));
     HIPTRY(launch_scan(s, f->nsurv.p, f->survoff.p, Np, f->scantmp.p));
-    if (ensure_hbuf(f, 2 * (int64_t)Np + 16) != SCPHD_OK) return SCPHD_ERR_HIP;
-    HIPTRY(hipMemcpyAsync(f->hbuf, f->survoff.p, (size_t)(Np + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPTRY(launch_readout(s, f->survoff.p, Np, 0, nullptr, nullptr, f->dmap));
     HIPTRY(hipStreamSynchronize(s));
-    const int64_t Stot = f->hbuf[Np];
-    int maxU = 0;
-    for (int p = 0; p < Np; ++p) maxU = std::max(maxU, f->hbuf[p + 1] - f->hbuf[p]);
+    const int64_t Stot = f->hmap[0];
+    const int maxU = f->hmap[1];
 
     HIPTRY(f->sw.ensure(Stot + 1));
     HIPTRY(f->sg.ensure(Stot + 1));
@@ -398,16 +386,12 @@ int update_t(scphd_filter* f, int M, int fixed) {
         HIPTRY(hipMemsetAsync(f->keepscan.p, 0, sizeof(int32_t), s));
     HIPTRY(launch_outcount(s, f->nmerged.p, f->goff.p, f->keepscan.p, Np, f->outcnt.p));
     HIPTRY(launch_scan(s, f->outcnt.p, f->goff2.p, Np, f->scantmp.p));
-    HIPTRY(hipMemcpyAsync(f->hbuf, f->goff2.p, (size_t)(Np + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    HIPTRY(hipMemcpyAsync(f->hbuf + Np + 1, f->mergeoff.p + Np, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    HIPTRY(hipMemcpyAsync(f->hbuf + Np + 2, f->dflag.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPTRY(launch_readout(s, f->goff2.p, Np, P.max_gaussians, f->mergeoff.p + Np, f->dflag.p, f->dmap));
     HIPTRY(hipStreamSynchronize(s));
-    const int64_t Nout = f->hbuf[Np];
-    const int64_t Kt = f->hbuf[Np + 1];
-    if (f->hbuf[Np + 2] != 0) return fail(f, SCPHD_ERR_STATE, "survivor count mismatch between passes");
-    int64_t capx = 0;
-    for (int p = 0; p < Np; ++p)
-        if (P.max_gaussians > 0 && f->hbuf[p + 1] - f->hbuf[p] > P.max_gaussians) ++capx;
+    const int64_t Nout = f->hmap[0];
+    const int64_t Kt = f->hmap[3];
+    if (f->hmap[4] != 0) return fail(f, SCPHD_ERR_STATE, "survivor count mismatch between passes");
+    const int64_t capx = f->hmap[2];
     f->stats.cap_exceeded = capx;
     if (capx > 0) return fail(f, SCPHD_ERR_CAPACITY, "a parent map exceeds max_gaussians");
 
@@ -545,6 +529,9 @@ int scphd_create(const scphd_params* p, scphd_t** out) {
         f->newcnt.ensure(Np + 1) || f->goff.ensure(Np + 1) || f->goff2.ensure(Np + 1) || f->dneff.ensure(1) ||
         f->dflag.ensure(4) || f->gparent.ensure(1) || f->gblock.ensure(1) || f->gw.ensure(1))
         return bail(SCPHD_ERR_HIP);
+    if (hipHostMalloc((void**)&f->hmap, 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&f->dmap, f->hmap, 0) != hipSuccess)
+        return bail(SCPHD_ERR_HIP);
     if (launch_prior(f->stream, f->ext.p, f->pw.p, Np, p->prior_a, p->prior_b, f->key) != hipSuccess)
         return bail(SCPHD_ERR_HIP);
     if (hipMemsetAsync(f->goff.p, 0, (size_t)(Np + 1) * sizeof(int32_t), f->stream) != hipSuccess)
@@ -571,7 +558,7 @@ void scphd_destroy(scphd_t* f) {
     f->mscratch.release();
     f->gen[0].release();
     f->gen[1].release();
-    if (f->hbuf) (void)hipHostFree(f->hbuf);
+    if (f->hmap) (void)hipHostFree(f->hmap);
     for (auto& e : f->ev)
         if (e) (void)hipEventDestroy(e);
     if (f->stream) (void)hipStreamDestroy(f->stream);
@@ -643,15 +630,13 @@ int scphd_resample(scphd_t* f, int32_t* resampled) {
     HIPTRY(launch_resample(s, ra));
     HIPTRY(f->scantmp.ensure(scan_tmp_size(Np + 1) + 16));
     HIPTRY(launch_scan(s, f->newcnt.p, f->goff2.p, Np, f->scantmp.p));
-    if (ensure_hbuf(f, 8) != SCPHD_OK) return SCPHD_ERR_HIP;
-    HIPTRY(hipMemcpyAsync(f->hbuf, f->dflag.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    HIPTRY(hipMemcpyAsync(f->hbuf + 1, f->goff2.p + Np, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPTRY(launch_readout(s, f->goff2.p, Np, 0, f->dflag.p, nullptr, f->dmap));
     HIPTRY(hipStreamSynchronize(s));
     f->n_resample++;
-    const int go = f->hbuf[0];
+    const int go = f->hmap[3];
     if (resampled) *resampled = go;
     if (!go) return SCPHD_OK;
-    const int64_t Nnew = f->hbuf[1];
+    const int64_t Nnew = f->hmap[0];
     HIPTRY(f->gparent2.ensure(Nnew + 1));
     HIPTRY(f->gblock2.ensure(Nnew + 1));
     HIPTRY(f->gw2.ensure(Nnew + 1));

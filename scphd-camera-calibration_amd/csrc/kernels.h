@@ -177,6 +177,10 @@ hipError_t launch_merge(hipStream_t s, const MergeArgs& a, int lds_bytes);
 int merge_lds_bytes(int cap);
 int64_t merge_scratch_bytes(int64_t Stot, int Np);
 hipError_t launch_iota(hipStream_t s, int32_t* x, int64_t n);
+/* out[0..4] (device-visible host memory): off[n], the largest segment of
+ * off, the number of segments above limit, *x0, *x1 */
+hipError_t launch_readout(hipStream_t s, const int32_t* off, int n, int32_t limit, const int32_t* x0,
+                          const int32_t* x1, int32_t* out);
 
 hipError_t launch_parent_weights(hipStream_t s, double* pw, const double* loglik, int Np);
 

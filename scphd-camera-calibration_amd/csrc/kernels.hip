@@ -2099,6 +2099,49 @@ hipError_t launch_resample(hipStream_t s, const ResampleArgs& a) {
     LAUNCH_CHECK();
 }
 
+/* Scalars the host needs after a scan, written by the device straight into
+ * mapped host memory, so that a read-back costs no copy-engine transfer:
+ * out[0] = off[n], out[1] = the largest segment off[p + 1] - off[p],
+ * out[2] = the number of segments above `limit` (0 when limit <= 0),
+ * out[3] = *x0 and out[4] = *x1 (0 for a null pointer).  One workgroup. */
+__global__ __launch_bounds__(1024) void k_readout(const int32_t* off, int n, int32_t limit, const int32_t* x0,
+                                                  const int32_t* x1, volatile int32_t* out) {
+    __shared__ int smax[16], scnt[16];
+    int mx = 0, c = 0;
+    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+        const int seg = off[p + 1] - off[p];
+        mx = max(mx, seg);
+        c += (limit > 0 && seg > limit) ? 1 : 0;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mx = max(mx, __shfl_xor(mx, o, 64));
+        c += __shfl_xor(c, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        smax[threadIdx.x >> 6] = mx;
+        scnt[threadIdx.x >> 6] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+            mx = max(mx, smax[w]);
+            c += scnt[w];
+        }
+        out[0] = off[n];
+        out[1] = mx;
+        out[2] = c;
+        out[3] = x0 ? *x0 : 0;
+        out[4] = x1 ? *x1 : 0;
+    }
+}
+
+hipError_t launch_readout(hipStream_t s, const int32_t* off, int n, int32_t limit, const int32_t* x0,
+                          const int32_t* x1, int32_t* out) {
+    k_readout<<<1, 1024, 0, s>>>(off, n, limit, x0, x1, out);
+    LAUNCH_CHECK();
+}
+
 hipError_t launch_iota(hipStream_t s, int32_t* x, int64_t n) {
     if (n <= 0) return hipSuccess;
     k_iota<<<nblocks(n, 256), 256, 0, s>>>(x, n);
