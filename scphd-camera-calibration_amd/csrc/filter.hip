@@ -101,6 +101,8 @@ struct Generation {
 
 }  // namespace
 
+static constexpr int MAP_EST = 256, MAP_BYTES = 4096; /* mapped read-back layout */
+
 struct scphd_filter {
     scphd_params prm;
     int dev = 0;
@@ -132,9 +134,12 @@ struct scphd_filter {
     int capM = 0;
     DBuf<int32_t> nsurv, survoff, nmerged, mergeoff, outcnt, scantmp, sg, sm, dflag, mcnt, mncand, mcand;
     DBuf<int32_t> ngroups, glead, gsta, gend, gmem;
-    DBuf<double> sw, smu, mw, mmu, mP, dneff, est;
+    DBuf<double> sw, smu, mw, mmu, mP, dneff;
     DBuf<char> mscratch;
-    int32_t* hmap = nullptr; /* mapped host memory the device writes scalars into (k_readout) */
+    /* mapped host memory the device writes read-back values into: int32
+     * [0, 16) for k_readout and k_nan_count, FP64 estimate from byte
+     * MAP_EST (157 doubles) */
+    int32_t* hmap = nullptr;
     int32_t* dmap = nullptr; /* its device address */
     std::vector<int32_t> hoff;
 
@@ -529,7 +534,7 @@ int scphd_create(const scphd_params* p, scphd_t** out) {
         f->newcnt.ensure(Np + 1) || f->goff.ensure(Np + 1) || f->goff2.ensure(Np + 1) || f->dneff.ensure(1) ||
         f->dflag.ensure(4) || f->gparent.ensure(1) || f->gblock.ensure(1) || f->gw.ensure(1))
         return bail(SCPHD_ERR_HIP);
-    if (hipHostMalloc((void**)&f->hmap, 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+    if (hipHostMalloc((void**)&f->hmap, MAP_BYTES, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
         hipHostGetDevicePointer((void**)&f->dmap, f->hmap, 0) != hipSuccess)
         return bail(SCPHD_ERR_HIP);
     if (launch_prior(f->stream, f->ext.p, f->pw.p, Np, p->prior_a, p->prior_b, f->key) != hipSuccess)
@@ -548,7 +553,7 @@ void scphd_destroy(scphd_t* f) {
     if (f->stream) (void)hipStreamSynchronize(f->stream);
     DBuf<double>* dbs[] = {&f->ext, &f->pw, &f->pose, &f->loglik, &f->cum, &f->ext_tmp, &f->gw, &f->gw2,
                            &f->fit, &f->pre, &f->lik, &f->du, &f->dv, &f->eta, &f->sw, &f->smu, &f->mw, &f->mmu, &f->mP,
-                           &f->dneff, &f->est};
+                           &f->dneff};
     for (auto* b : dbs) b->release();
     DBuf<int32_t>* ibs[] = {&f->anc, &f->newcnt, &f->goff, &f->gparent, &f->gblock, &f->goff2, &f->gparent2,
                             &f->gblock2, &f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->ndfill, &f->nsurv, &f->survoff,
@@ -653,11 +658,9 @@ int scphd_resample(scphd_t* f, int32_t* resampled) {
 int scphd_check(scphd_t* f) {
     if (check_handle(f)) return SCPHD_ERR_ARG;
     HIPTRY(hipSetDevice(f->dev));
-    std::vector<double> w(f->Np);
-    HIPTRY(hipMemcpyAsync(w.data(), f->pw.p, f->Np * sizeof(double), hipMemcpyDeviceToHost, f->stream));
+    HIPTRY(launch_nan_count(f->stream, f->pw.p, f->Np, f->dmap + 8));
     HIPTRY(hipStreamSynchronize(f->stream));
-    for (double x : w)
-        if (x != x) return fail(f, SCPHD_ERR_NAN, "nan weight detected");
+    if (f->hmap[8] != 0) return fail(f, SCPHD_ERR_NAN, "nan weight detected");
     return SCPHD_OK;
 }
 
@@ -860,11 +863,10 @@ int scphd_set_noise(scphd_t* f, int32_t mode, uint64_t seed, const double* injec
 int scphd_get_estimate(scphd_t* f, double* mean12, double* cov144, double* cardinality) {
     if (check_handle(f)) return SCPHD_ERR_ARG;
     HIPTRY(hipSetDevice(f->dev));
-    HIPTRY(f->est.ensure(157));
-    HIPTRY(launch_estimate(f->stream, f->ext.p, f->pw.p, f->goff.p, f->gw.p, f->Np, f->est.p));
-    double h[157];
-    HIPTRY(hipMemcpyAsync(h, f->est.p, sizeof h, hipMemcpyDeviceToHost, f->stream));
+    HIPTRY(launch_estimate(f->stream, f->ext.p, f->pw.p, f->goff.p, f->gw.p, f->Np,
+                           (double*)((char*)f->dmap + MAP_EST)));
     HIPTRY(hipStreamSynchronize(f->stream));
+    const double* h = (const double*)((const char*)f->hmap + MAP_EST);
     if (mean12) std::memcpy(mean12, h, 12 * sizeof(double));
     if (cov144) std::memcpy(cov144, h + 12, 144 * sizeof(double));
     if (cardinality) *cardinality = h[156];
@@ -876,7 +878,7 @@ int scphd_get_memory(scphd_t* f, int64_t* bytes) {
     int64_t b = 0;
     DBuf<double>* dbs[] = {&f->ext, &f->pw, &f->pose, &f->loglik, &f->cum, &f->ext_tmp, &f->gw, &f->gw2,
                            &f->fit, &f->pre, &f->lik, &f->du, &f->dv, &f->eta, &f->sw, &f->smu, &f->mw, &f->mmu, &f->mP,
-                           &f->dneff, &f->est};
+                           &f->dneff};
     for (auto* d : dbs) b += d->n * (int64_t)sizeof(double);
     DBuf<int32_t>* ibs[] = {&f->anc, &f->newcnt, &f->goff, &f->gparent, &f->gblock, &f->goff2, &f->gparent2,
                             &f->gblock2, &f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->ndfill, &f->nsurv, &f->survoff,

@@ -181,6 +181,8 @@ hipError_t launch_iota(hipStream_t s, int32_t* x, int64_t n);
  * off, the number of segments above limit, *x0, *x1 */
 hipError_t launch_readout(hipStream_t s, const int32_t* off, int n, int32_t limit, const int32_t* x0,
                           const int32_t* x1, int32_t* out);
+/* *out = the number of NaN entries of w[0, n) */
+hipError_t launch_nan_count(hipStream_t s, const double* w, int n, int32_t* out);
 
 hipError_t launch_parent_weights(hipStream_t s, double* pw, const double* loglik, int Np);
 

@@ -2136,6 +2136,28 @@ __global__ __launch_bounds__(1024) void k_readout(const int32_t* off, int n, int
     }
 }
 
+/* out = the number of NaN entries of w[0, n) (checkStuff,
+ * src/scphdcameracalibration.cu:812-821, without the host copy).  One
+ * workgroup. */
+__global__ __launch_bounds__(1024) void k_nan_count(const double* w, int n, volatile int32_t* out) {
+    __shared__ int sc[16];
+    int c = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) c += w[i] != w[i] ? 1 : 0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if ((threadIdx.x & 63) == 0) sc[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) c += sc[k];
+        *out = c;
+    }
+}
+
+hipError_t launch_nan_count(hipStream_t s, const double* w, int n, int32_t* out) {
+    k_nan_count<<<1, 1024, 0, s>>>(w, n, out);
+    LAUNCH_CHECK();
+}
+
 hipError_t launch_readout(hipStream_t s, const int32_t* off, int n, int32_t limit, const int32_t* x0,
                           const int32_t* x1, int32_t* out) {
     k_readout<<<1, 1024, 0, s>>>(off, n, limit, x0, x1, out);

@@ -184,3 +184,27 @@ def test_evicted_rig_resumes_exactly_where_it_stopped():
             np.testing.assert_array_equal(st[key], ref[0][key], err_msg=key)
     finally:
         b.close()
+
+
+def test_check_reports_a_nan_parent_weight():
+    """checkStuff (src/scphdcameracalibration.cu:812-821): a NaN parent
+    weight is an error code, counted on the device."""
+    import scphd_amd
+
+    p, s, sets = make(0)
+    f = scphd_amd.Filter(p)
+    u, v, fixed = sets[0]
+    f.predict(s["dt"])
+    f.update(u, v, fixed)
+    assert f.check() == 0
+    st = f.state()
+    w = st["weights"].copy()
+    w[len(w) // 2] = np.nan
+    st["weights"] = w
+    f.set_state(st)
+    assert scphd_amd.ERRORS.get(f.check()) == "SCPHD_ERR_NAN"
+    w[len(w) // 2] = 1.0 / len(w)
+    st["weights"] = w
+    f.set_state(st)
+    assert f.check() == 0
+    f.close()
