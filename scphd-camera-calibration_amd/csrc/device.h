@@ -109,7 +109,7 @@ __device__ inline void box_muller(uint32_t x0, uint32_t x1, double& z0, double& 
     const uint32_t nn = n != 0u ? n : 1u;
     const int lz = __clz((int)nn);
     const uint32_t m = nn << lz; /* [2^31, 2^32) */
-    float f = (float)(m >> 8) * 0x1.0p-23f;
+    float f = __uint_as_float(0x3F800000u | ((m >> 8) & 0x7FFFFFu)); /* (m >> 8) 2^-23, in [1, 2) */
     int e = -1 - lz;
     if (f > 0x1.6a09e6p+0f) {
         f *= 0.5f;
@@ -144,14 +144,15 @@ __device__ inline void box_muller(uint32_t x0, uint32_t x1, double& z0, double& 
     pc = __builtin_fmaf(pc, t2, 0x1.03c1f0p-6f);
     pc = __builtin_fmaf(pc, t2, -0x1.3bd3ccp-2f);
     pc = __builtin_fmaf(pc, t2, 1.0f);
-    /* quadrant angle phi = 2 pi c - q pi/2: odd octants measure t from pi/2 */
-    const float sphi = (oct & 1u) ? pc : sv;
-    const float cphi = (oct & 1u) ? sv : pc;
+    /* quadrant angle phi = 2 pi c - q pi/2: odd octants measure t from pi/2
+     * (sin and cos exchanged), odd quadrants exchange them again, so one
+     * exchange when exactly one applies; the quadrant's signs are sign-bit
+     * flips: -cos for q = 1, 2 and -sin for q = 2, 3 */
     const uint32_t q = oct >> 1;
-    float sn = (q & 1u) ? cphi : sphi;
-    float cs = (q & 1u) ? sphi : cphi;
-    if (q == 1u || q == 2u) cs = -cs;
-    if (q >= 2u) sn = -sn;
+    const bool swap = ((oct ^ q) & 1u) != 0u;
+    const float sn0 = swap ? pc : sv, cs0 = swap ? sv : pc;
+    const float sn = __uint_as_float(__float_as_uint(sn0) ^ ((q & 2u) << 30));
+    const float cs = __uint_as_float(__float_as_uint(cs0) ^ (((q + 1u) & 2u) << 30));
     z0 = (double)(r * cs);
     z1 = (double)(r * sn);
 }
