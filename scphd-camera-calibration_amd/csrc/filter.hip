@@ -107,6 +107,10 @@ struct scphd_filter {
     scphd_params prm;
     int dev = 0;
     hipStream_t stream = nullptr;
+    /* second stream: non-detection clouds (k_nd_fill) overlap the
+     * normalisers; joined before the next generation can grow */
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fit = nullptr, ev_nd = nullptr;
     std::string err;
     FilterModels models;
     BirthModel birth;
@@ -244,7 +248,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
         HIPTRY(f->pre.ensure((int64_t)PP_COUNT * Ng));
         HIPTRY(f->lik.ensure((int64_t)LK_COUNT * (Ng + 3))); /* 3 records of padding: read four at a time */
         HIPTRY(f->keep.ensure(Ng));
-        HIPTRY(f->redo.ensure(Ng));
+        HIPTRY(f->redo.ensure(Ng + 1)); /* [Ng]: the list length */
         HIPTRY(f->ndfill.ensure(Ng + 1)); /* [Ng]: the list length */
         HIPTRY(f->ndlist.ensure(Ng));
         FitArgs fa;
@@ -265,6 +269,8 @@ int update_t(scphd_filter* f, int M, int fixed) {
         fa.fit = f->fit.p;
         fa.keep = f->keep.p;
         fa.redo = f->redo.p;
+        fa.redo_n = f->redo.p + Ng;
+        HIPTRY(hipMemsetAsync(fa.redo_n, 0, sizeof(int32_t), s));
         fa.ndfill = f->ndfill.p;
         fa.ndfill_n = f->ndfill.p + Ng;
         HIPTRY(hipMemsetAsync(fa.ndfill_n, 0, sizeof(int32_t), s));
@@ -274,8 +280,11 @@ int update_t(scphd_filter* f, int M, int fixed) {
         record(f, 6);
         HIPTRY(launch_fit<T>(s, fa, f->gen[f->cur].planes<T>(), f->gen[nxt].planes<T>()));
         record(f, 7);
+        HIPTRY(hipEventRecord(f->ev_fit, s));
+        HIPTRY(hipStreamWaitEvent(f->side, f->ev_fit, 0));
+        HIPTRY(launch_nd_fill<T>(f->side, fa, f->gen[f->cur].planes<T>(), f->gen[nxt].planes<T>()));
+        HIPTRY(hipEventRecord(f->ev_nd, f->side));
         HIPTRY(launch_fit_fix<T>(s, fa, f->gen[f->cur].planes<T>(), f->gen[nxt].planes<T>()));
-        HIPTRY(launch_nd_fill<T>(s, fa, f->gen[f->cur].planes<T>(), f->gen[nxt].planes<T>()));
         HIPTRY(launch_precompute(s, f->fit.p, f->gw.p, Ng, P.std_u * P.std_u, P.std_v * P.std_v, f->pre.p,
                                  f->lik.p));
     }
@@ -391,6 +400,10 @@ int update_t(scphd_filter* f, int M, int fixed) {
         HIPTRY(hipMemsetAsync(f->keepscan.p, 0, sizeof(int32_t), s));
     HIPTRY(launch_outcount(s, f->nmerged.p, f->goff.p, f->keepscan.p, Np, f->outcnt.p));
     HIPTRY(launch_scan(s, f->outcnt.p, f->goff2.p, Np, f->scantmp.p));
+    /* the non-detection clouds in [0, Ng) of the next generation are
+     * complete before any return or growth of the generation (which copies
+     * them) */
+    HIPTRY(hipStreamWaitEvent(s, f->ev_nd, 0));
     HIPTRY(launch_readout(s, f->goff2.p, Np, P.max_gaussians, f->mergeoff.p + Np, f->dflag.p, f->dmap));
     HIPTRY(hipStreamSynchronize(s));
     const int64_t Nout = f->hmap[0];
@@ -525,7 +538,11 @@ int scphd_create(const scphd_params* p, scphd_t** out) {
         return code;
     };
     if (hipSetDevice(f->dev) != hipSuccess) return bail(SCPHD_ERR_NODEVICE);
-    if (hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking) != hipSuccess) return bail(SCPHD_ERR_HIP);
+    if (hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&f->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&f->ev_fit, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&f->ev_nd, hipEventDisableTiming) != hipSuccess)
+        return bail(SCPHD_ERR_HIP);
     for (auto& e : f->ev)
         if (hipEventCreate(&e) != hipSuccess) return bail(SCPHD_ERR_HIP);
     const int Np = f->Np;
@@ -551,6 +568,7 @@ void scphd_destroy(scphd_t* f) {
     if (!f) return;
     (void)hipSetDevice(f->dev);
     if (f->stream) (void)hipStreamSynchronize(f->stream);
+    if (f->side) (void)hipStreamSynchronize(f->side);
     DBuf<double>* dbs[] = {&f->ext, &f->pw, &f->pose, &f->loglik, &f->cum, &f->ext_tmp, &f->gw, &f->gw2,
                            &f->fit, &f->pre, &f->lik, &f->du, &f->dv, &f->eta, &f->sw, &f->smu, &f->mw, &f->mmu, &f->mP,
                            &f->dneff};
@@ -566,6 +584,9 @@ void scphd_destroy(scphd_t* f) {
     if (f->hmap) (void)hipHostFree(f->hmap);
     for (auto& e : f->ev)
         if (e) (void)hipEventDestroy(e);
+    if (f->ev_fit) (void)hipEventDestroy(f->ev_fit);
+    if (f->ev_nd) (void)hipEventDestroy(f->ev_nd);
+    if (f->side) (void)hipStreamDestroy(f->side);
     if (f->stream) (void)hipStreamDestroy(f->stream);
     delete f;
 }

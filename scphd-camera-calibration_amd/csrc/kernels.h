@@ -89,14 +89,15 @@ struct FitArgs {
     double min_weight;
     double* fit;     /* FP_COUNT planes */
     int32_t* keep;   /* non-detection term kept */
-    int32_t* redo;   /* per Gaussian: 1 when every sample weight was zero (refit by k_fit_fix) */
+    int32_t* redo;     /* Gaussians whose sample weights were all zero, refitted by k_fit_fix */
+    int32_t* redo_n;   /* their number; zero before k_fit */
     int32_t* ndfill;   /* kept terms whose clouds k_nd_fill writes (decided after the sweep) */
     int32_t* ndfill_n; /* their number; zero before k_fit */
     int64_t nd_base; /* destination block of the non-detection cloud of entry g: nd_base + g */
     int off32;       /* every source and destination plane is below 4 GiB: 32-bit byte offsets */
 };
 
-/* k_fit: the streaming sweep; k_fit_fix: the rare cases it flags in redo[];
+/* k_fit: the streaming sweep; k_fit_fix: the rare cases it lists in redo[];
  * k_nd_fill: the non-detection clouds it lists in ndfill[] */
 template <typename T>
 hipError_t launch_fit(hipStream_t s, const FitArgs& a, Planes<T> src, Planes<T> dst);

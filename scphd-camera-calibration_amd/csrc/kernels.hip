@@ -319,9 +319,9 @@ __device__ inline void write_fit(const FitArgs& a, int64_t g, int lane, const do
  * is decided after the sweep and, when kept, listed for k_nd_fill, which
  * writes its cloud; most such terms are dropped, and their clouds are never
  * written.  A cloud whose weights are all zero, which the reference
- * normalises to weight 1 per sample, is refitted by k_fit_fix (redo[g]) when
- * pd_g > 0; with pd_g = 0 its fit cannot reach any output and is the first
- * sample with zero covariance. */
+ * normalises to weight 1 per sample, is listed in redo[] and refitted by
+ * k_fit_fix when pd_g > 0; with pd_g = 0 its fit cannot reach any output and
+ * is the first sample with zero covariance. */
 /* waves per SIMD the fit kernel is compiled for (register budget 512 / n) */
 #ifndef SCPHD_FIT_WAVES
 #define SCPHD_FIT_WAVES 2
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
     const bool keep = wg * (1 - pdg) >= a.min_weight;
     if (lane == 0) {
         a.keep[g] = keep ? 1 : 0;
-        a.redo[g] = (sw > 0 || spd == 0.0) ? 0 : 1;
+        if (sw == 0 && spd != 0.0) a.redo[atomicAdd(a.redo_n, 1)] = (int32_t)g;
         if (keep && !nd_sure) a.ndfill[atomicAdd(a.ndfill_n, 1)] = (int32_t)g;
     }
 }
@@ -502,22 +502,22 @@ __global__ __launch_bounds__(256) void k_nd_fill(FitArgs a, Planes<T> src, Plane
     }
 }
 
-/* The rare case of k_fit, one row per flagged Gaussian: every weight zero.
- * Refitted by two sweeps with unit weights (mean = sum of the samples, as
- * the reference's normalisation of an all-zero cloud gives). */
+/* The rare case of k_fit, one row per listed Gaussian (redo[0, *redo_n)):
+ * every weight zero.  Refitted by two sweeps with unit weights (mean = sum
+ * of the samples, as the reference's normalisation of an all-zero cloud
+ * gives).  Workgroups stride over the list (its length is read on the
+ * device). */
 template <typename T>
 __global__ __launch_bounds__(256) void k_fit_fix(FitArgs a, Planes<T> src, Planes<T> dst) {
     const int lane = threadIdx.x & 15;
-    const int64_t g = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
-    if (g >= a.Ng) return;
-    const int redo = a.redo[g];
-    if (redo == 0) return; /* uniform along the row */
-    const int p = a.gparent[g];
-    const int k = (int)(g - a.goff[p]);
-    const int64_t b0 = (int64_t)a.gblock[g] * a.S;
-    const CameraPose pose = load_pose(a.pose, p);
+    const int nredo = *a.redo_n;
     const int S = a.S;
-    if (redo & 1) {
+    for (int i = blockIdx.x * 16 + (threadIdx.x >> 4); i < nredo; i += gridDim.x * 16) {
+        const int64_t g = a.redo[i];
+        const int p = a.gparent[g];
+        const int k = (int)(g - a.goff[p]);
+        const int64_t b0 = (int64_t)a.gblock[g] * S;
+        const CameraPose pose = load_pose(a.pose, p);
         double sq[6] = {0, 0, 0, 0, 0, 0};
         for (int s0 = 0; s0 < S; s0 += 16) {
             const int s = s0 + lane;
@@ -2005,7 +2005,7 @@ hipError_t launch_fit(hipStream_t s, const FitArgs& a, Planes<T> src, Planes<T> 
 template <typename T>
 hipError_t launch_fit_fix(hipStream_t s, const FitArgs& a, Planes<T> src, Planes<T> dst) {
     if (a.Ng <= 0) return hipSuccess;
-    k_fit_fix<T><<<nblocks(a.Ng, 16), 256, 0, s>>>(a, src, dst);
+    k_fit_fix<T><<<std::min<int64_t>(nblocks(a.Ng, 16), 256), 256, 0, s>>>(a, src, dst);
     LAUNCH_CHECK();
 }
 
