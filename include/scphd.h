@@ -232,6 +232,37 @@ int scphd_debug_normals(uint64_t seed, uint32_t stream, uint64_t step, uint32_t 
  * load_disparity_results.m:27-34 (exp_traj, cov_traj, cardinality). */
 int scphd_get_estimate(scphd_t* f, double* mean12, double* cov144, double* cardinality);
 
+/* ---- ensembles: many rigs of one configuration in one filter ----
+ * No counterpart in the reference, which runs one rig per process
+ * (src/main.cu:52-263).  An ensemble of n_rigs rigs, each of
+ * p->n_particles parents, is stepped by the same kernel launches as one
+ * filter of n_rigs * n_particles parents: rig r draws every noise stream
+ * from the key of seeds[r] with its parents numbered from 0, normalises and
+ * resamples its own parents, and reads its own measurement set, so each rig
+ * evolves exactly as a plain filter created with seed seeds[r] and fed the
+ * same measurements.  Parents of rig r are [r n_particles, (r + 1)
+ * n_particles) in scphd_get_parents, scphd_get_maps and
+ * scphd_set_state.  scphd_predict, scphd_resample (*resampled: any rig),
+ * scphd_check and the getters take an ensemble handle; scphd_update,
+ * scphd_update_device and scphd_set_noise refuse it. */
+int scphd_create_ensemble(const scphd_params* p, int32_t n_rigs, const uint64_t* seeds, scphd_t** out);
+
+/* Update every rig: rig r's measurements are u[moff[r] .. moff[r + 1]) and
+ * v[...] (host arrays; moff has n_rigs + 1 entries, moff[0] = 0, never
+ * decreasing). */
+int scphd_update_ensemble(scphd_t* f, const double* u, const double* v, const int32_t* moff,
+                          int32_t fixed_camera);
+
+/* scphd_get_estimate of every rig over its own parents, one launch:
+ * mean12[12 r ..], cov144[144 r ..], cardinality[r] for rig r (any may be
+ * NULL; a plain filter is one rig). */
+int scphd_get_rig_estimates(scphd_t* f, double* mean12, double* cov144, double* cardinality);
+
+/* Per rig, from the last scphd_resample: resampled[r] = 1 when rig r was
+ * resampled, neff[r] its effective sample size (either may be NULL;
+ * n_rigs entries each; a plain filter is one rig). */
+int scphd_get_rig_resample(scphd_t* f, int32_t* resampled, double* neff);
+
 /* Device memory currently held by the handle, in bytes (buffers grow
  * geometrically and are kept between steps).  No counterpart in the
  * reference. */

@@ -101,7 +101,12 @@ struct Generation {
 
 }  // namespace
 
-static constexpr int MAP_EST = 256, MAP_BYTES = 4096; /* mapped read-back layout */
+/* mapped read-back layout: int32 slots, an FP64 pair count at byte
+ * MAP_PAIRS, EST_DOUBLES per rig from byte MAP_EST */
+static constexpr int MAP_PAIRS = 128, MAP_EST = 256;
+static size_t map_bytes(int n_rigs) {
+    return (MAP_EST + (size_t)n_rigs * EST_DOUBLES * sizeof(double) + 4095) / 4096 * 4096;
+}
 
 struct scphd_filter {
     scphd_params prm;
@@ -116,6 +121,17 @@ struct scphd_filter {
     BirthModel birth;
     NoiseKey key;
     int Np = 0, S = 0;
+    /* ensemble: n_rigs rigs of one configuration with npr parents each
+     * (Np = n_rigs npr), stepped by the same launches; rig r draws its
+     * noise from dkeys[r] and reads its measurements at [moff[r], moff[r+1]) */
+    bool ensemble = false;
+    int n_rigs = 1, npr = 0;
+    DBuf<NoiseKey> dkeys;
+    DBuf<int32_t> dmoff, drflag;
+    DBuf<double> dbpcs;
+    std::vector<int32_t> hmoff;
+    std::vector<double> hbpcs;
+    int64_t ens_meas = 0;
     size_t esz = 4;
 
     /* parents */
@@ -141,8 +157,8 @@ struct scphd_filter {
     DBuf<double> sw, smu, mw, mmu, mP, dneff;
     DBuf<char> mscratch;
     /* mapped host memory the device writes read-back values into: int32
-     * [0, 16) for k_readout and k_nan_count, FP64 estimate from byte
-     * MAP_EST (157 doubles) */
+     * [0, 16) for k_readout and k_nan_count, an FP64 pair count at byte
+     * MAP_PAIRS, the FP64 estimates from byte MAP_EST (EST_DOUBLES a rig) */
     int32_t* hmap = nullptr;
     int32_t* dmap = nullptr; /* its device address */
     std::vector<int32_t> hoff;
@@ -170,6 +186,15 @@ int fail(scphd_filter* f, int code, const std::string& msg) {
 
 int hipfail(scphd_filter* f, hipError_t e, const char* what) {
     return fail(f, SCPHD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+/* the noise keys the kernels draw with: one, or one per rig */
+RigKeys rig_keys(const scphd_filter* f) {
+    RigKeys r;
+    r.key = f->key;
+    r.keys = f->ensemble ? f->dkeys.p : nullptr;
+    r.npr = f->npr;
+    return r;
 }
 
 #define HIPTRY(expr)                                              \
@@ -216,7 +241,7 @@ int materialize_t(scphd_filter* f) {
     const int nxt = 1 - f->cur;
     HIPTRY(f->gen[nxt].ensure(f->Ng, 0, f->S, f->esz, f->stream));
     HIPTRY(launch_materialize<T>(f->stream, f->gen[f->cur].planes<T>(), f->gen[nxt].planes<T>(), f->gblock.p,
-                                 f->gparent.p, f->goff.p, f->Ng, f->S, f->models.feat, f->pdt, f->key, f->pstep));
+                                 f->gparent.p, f->goff.p, f->Ng, f->S, f->models.feat, f->pdt, rig_keys(f), f->pstep));
     HIPTRY(launch_iota(f->stream, f->gblock.p, f->Ng));
     f->cur = nxt;
     return SCPHD_OK;
@@ -263,7 +288,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
         fa.feat = f->models.feat;
         fa.pending = f->pending ? 1 : 0;
         fa.dt = f->pdt;
-        fa.key = f->key;
+        fa.key = rig_keys(f);
         fa.pstep = f->pstep;
         fa.min_weight = P.min_weight;
         fa.fit = f->fit.p;
@@ -311,6 +336,12 @@ int update_t(scphd_filter* f, int M, int fixed) {
     ea.M = M;
     ea.Np = Np;
     ea.bpc = safe_log(M * P.birth_w0 + f->models.meas.kappa());
+    if (f->ensemble) { /* M is the largest rig's count, the stride of eta, mcnt, mncand and mcand */
+        ea.moff = f->dmoff.p;
+        ea.bpcs = f->dbpcs.p;
+        ea.npr = f->npr;
+        HIPTRY(launch_pair_count(s, f->goff.p, f->dmoff.p, f->npr, f->n_rigs, (double*)((char*)f->dmap + MAP_PAIRS)));
+    }
     ea.min_weight = P.min_weight;
     ea.birth = f->birth;
     ea.eta = f->eta.p;
@@ -389,7 +420,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
         ma.scratch = f->mscratch.p;
     }
     HIPTRY(launch_merge(s, ma, merge_lds_bytes(std::min(std::max(maxU, 1), f->lds_cap))));
-    if (!fixed) HIPTRY(launch_parent_weights(s, f->pw.p, f->loglik.p, Np));
+    if (!fixed) HIPTRY(launch_parent_weights(s, f->pw.p, f->loglik.p, f->npr, f->n_rigs));
     record(f, 3);
 
     /* output counts */
@@ -426,7 +457,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
     pa.Kt = Kt;
     pa.S = S;
     pa.meas = f->models.meas;
-    pa.key = f->key;
+    pa.key = rig_keys(f);
     pa.ustep = (uint64_t)f->n_update;
     pa.post_base = Ng;
     pa.off32 = f->gen[nxt].cap_blocks * (int64_t)S * (int64_t)f->esz < (int64_t)0xFFFFFFFFll - 64 * (int64_t)f->esz;
@@ -468,9 +499,9 @@ int update_t(scphd_filter* f, int M, int fixed) {
     f->Ng = Nout;
     f->n_update++;
 
-    f->stats.n_meas = M;
+    f->stats.n_meas = f->ensemble ? f->ens_meas : M;
     f->stats.n_gaussians_in = Ng;
-    f->stats.n_pairs = Ng * M;
+    f->stats.n_pairs = f->ensemble ? (int64_t)*(const double*)((const char*)f->hmap + MAP_PAIRS) : Ng * M;
     f->stats.n_survivors = Stot;
     f->stats.n_merged = Kt;
     f->stats.n_nondetect = Nout - Kt;
@@ -509,11 +540,17 @@ int check_handle(scphd_filter* f) { return f ? SCPHD_OK : SCPHD_ERR_ARG; }
 /* ================================================================== ABI */
 extern "C" {
 
-int scphd_create(const scphd_params* p, scphd_t** out) {
+}  /* extern "C" */
+
+namespace {
+
+/* One filter of n_rigs rigs (n_rigs = 1 and no seeds: the plain filter). */
+int create_filter(const scphd_params* p, int32_t n_rigs, const uint64_t* seeds, scphd_t** out) {
     if (!p || !out) return SCPHD_ERR_ARG;
     *out = nullptr;
     if (p->n_particles <= 0 || p->particles_per_feature <= 0 || p->particles_per_feature > 512)
         return SCPHD_ERR_ARG;
+    if (n_rigs <= 0 || (int64_t)n_rigs * p->n_particles > (int64_t)1 << 30) return SCPHD_ERR_ARG;
     if (p->storage != SCPHD_STORAGE_FP32 && p->storage != SCPHD_STORAGE_FP64) return SCPHD_ERR_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SCPHD_ERR_NODEVICE;
@@ -529,7 +566,10 @@ int scphd_create(const scphd_params* p, scphd_t** out) {
     scphd_filter* f = new scphd_filter();
     f->prm = *p;
     f->dev = best;
-    f->Np = p->n_particles;
+    f->ensemble = seeds != nullptr;
+    f->n_rigs = n_rigs;
+    f->npr = p->n_particles;
+    f->Np = n_rigs * p->n_particles;
     f->S = p->particles_per_feature;
     f->esz = p->storage == SCPHD_STORAGE_FP64 ? 8 : 4;
     set_models(f);
@@ -548,13 +588,21 @@ int scphd_create(const scphd_params* p, scphd_t** out) {
     const int Np = f->Np;
     if (f->ext.ensure(12 * (int64_t)Np) || f->ext_tmp.ensure(12 * (int64_t)Np) || f->pw.ensure(Np) ||
         f->pose.ensure(12 * (int64_t)Np) || f->loglik.ensure(Np) || f->cum.ensure(Np) || f->anc.ensure(Np) ||
-        f->newcnt.ensure(Np + 1) || f->goff.ensure(Np + 1) || f->goff2.ensure(Np + 1) || f->dneff.ensure(1) ||
-        f->dflag.ensure(4) || f->gparent.ensure(1) || f->gblock.ensure(1) || f->gw.ensure(1))
+        f->newcnt.ensure(Np + 1) || f->goff.ensure(Np + 1) || f->goff2.ensure(Np + 1) || f->dneff.ensure(n_rigs) ||
+        f->drflag.ensure(n_rigs) || f->dflag.ensure(4) || f->gparent.ensure(1) || f->gblock.ensure(1) ||
+        f->gw.ensure(1))
         return bail(SCPHD_ERR_HIP);
-    if (hipHostMalloc((void**)&f->hmap, MAP_BYTES, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+    if (f->ensemble) {
+        std::vector<NoiseKey> keys(n_rigs);
+        for (int r = 0; r < n_rigs; ++r) keys[r] = NoiseKey::from_seed(seeds[r]);
+        if (f->dkeys.ensure(n_rigs) || f->dmoff.ensure(n_rigs + 1) || f->dbpcs.ensure(n_rigs) ||
+            hipMemcpy(f->dkeys.p, keys.data(), n_rigs * sizeof(NoiseKey), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(SCPHD_ERR_HIP);
+    }
+    if (hipHostMalloc((void**)&f->hmap, map_bytes(n_rigs), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
         hipHostGetDevicePointer((void**)&f->dmap, f->hmap, 0) != hipSuccess)
         return bail(SCPHD_ERR_HIP);
-    if (launch_prior(f->stream, f->ext.p, f->pw.p, Np, p->prior_a, p->prior_b, f->key) != hipSuccess)
+    if (launch_prior(f->stream, f->ext.p, f->pw.p, Np, p->prior_a, p->prior_b, rig_keys(f)) != hipSuccess)
         return bail(SCPHD_ERR_HIP);
     if (hipMemsetAsync(f->goff.p, 0, (size_t)(Np + 1) * sizeof(int32_t), f->stream) != hipSuccess)
         return bail(SCPHD_ERR_HIP);
@@ -564,6 +612,17 @@ int scphd_create(const scphd_params* p, scphd_t** out) {
     return SCPHD_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int scphd_create(const scphd_params* p, scphd_t** out) { return create_filter(p, 1, nullptr, out); }
+
+int scphd_create_ensemble(const scphd_params* p, int32_t n_rigs, const uint64_t* seeds, scphd_t** out) {
+    if (!seeds) return SCPHD_ERR_ARG;
+    return create_filter(p, n_rigs, seeds, out);
+}
+
 void scphd_destroy(scphd_t* f) {
     if (!f) return;
     (void)hipSetDevice(f->dev);
@@ -571,13 +630,14 @@ void scphd_destroy(scphd_t* f) {
     if (f->side) (void)hipStreamSynchronize(f->side);
     DBuf<double>* dbs[] = {&f->ext, &f->pw, &f->pose, &f->loglik, &f->cum, &f->ext_tmp, &f->gw, &f->gw2,
                            &f->fit, &f->pre, &f->lik, &f->du, &f->dv, &f->eta, &f->sw, &f->smu, &f->mw, &f->mmu, &f->mP,
-                           &f->dneff};
+                           &f->dneff, &f->dbpcs};
     for (auto* b : dbs) b->release();
     DBuf<int32_t>* ibs[] = {&f->anc, &f->newcnt, &f->goff, &f->gparent, &f->gblock, &f->goff2, &f->gparent2,
                             &f->gblock2, &f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->ndfill, &f->nsurv, &f->survoff,
                             &f->nmerged, &f->mergeoff, &f->outcnt, &f->scantmp, &f->sg, &f->sm, &f->dflag, &f->mcnt, &f->mncand, &f->mcand,
-                            &f->ngroups, &f->glead, &f->gsta, &f->gend, &f->gmem};
+                            &f->ngroups, &f->glead, &f->gsta, &f->gend, &f->gmem, &f->dmoff, &f->drflag};
     for (auto* b : ibs) b->release();
+    f->dkeys.release();
     f->mscratch.release();
     f->gen[0].release();
     f->gen[1].release();
@@ -598,7 +658,7 @@ int scphd_predict(scphd_t* f, double dt) {
     HIPTRY(hipSetDevice(f->dev));
     int rc = materialize(f);
     if (rc) return rc;
-    HIPTRY(launch_parent_predict(f->stream, f->ext.p, f->Np, f->models.cam, dt, f->key, (uint64_t)f->n_predict));
+    HIPTRY(launch_parent_predict(f->stream, f->ext.p, f->Np, f->models.cam, dt, rig_keys(f), (uint64_t)f->n_predict));
     HIPTRY(launch_scale(f->stream, f->gw.p, f->Ng, f->prm.ps));
     f->pending = f->Ng > 0;
     f->pdt = dt;
@@ -610,6 +670,7 @@ int scphd_predict(scphd_t* f, double dt) {
 int scphd_update_device(scphd_t* f, const double* u_device, const double* v_device, int32_t n_meas,
                         int32_t fixed_camera) {
     if (check_handle(f) || n_meas < 0 || (n_meas > 0 && (!u_device || !v_device))) return SCPHD_ERR_ARG;
+    if (f->ensemble) return fail(f, SCPHD_ERR_ARG, "an ensemble: use scphd_update_ensemble");
     HIPTRY(hipSetDevice(f->dev));
     HIPTRY(f->du.ensure(std::max(n_meas, 1)));
     HIPTRY(f->dv.ensure(std::max(n_meas, 1)));
@@ -622,6 +683,7 @@ int scphd_update_device(scphd_t* f, const double* u_device, const double* v_devi
 
 int scphd_update(scphd_t* f, const double* u, const double* v, int32_t n_meas, int32_t fixed_camera) {
     if (check_handle(f) || n_meas < 0 || (n_meas > 0 && (!u || !v))) return SCPHD_ERR_ARG;
+    if (f->ensemble) return fail(f, SCPHD_ERR_ARG, "an ensemble: use scphd_update_ensemble");
     HIPTRY(hipSetDevice(f->dev));
     HIPTRY(f->du.ensure(std::max(n_meas, 1)));
     HIPTRY(f->dv.ensure(std::max(n_meas, 1)));
@@ -630,6 +692,37 @@ int scphd_update(scphd_t* f, const double* u, const double* v, int32_t n_meas, i
         HIPTRY(hipMemcpyAsync(f->dv.p, v, n_meas * sizeof(double), hipMemcpyHostToDevice, f->stream));
     }
     return update_common(f, n_meas, fixed_camera ? 1 : 0);
+}
+
+int scphd_update_ensemble(scphd_t* f, const double* u, const double* v, const int32_t* moff,
+                          int32_t fixed_camera) {
+    if (check_handle(f) || !moff) return SCPHD_ERR_ARG;
+    if (!f->ensemble) return fail(f, SCPHD_ERR_ARG, "not an ensemble: use scphd_update");
+    const int R = f->n_rigs;
+    if (moff[0] != 0) return fail(f, SCPHD_ERR_ARG, "moff[0] must be 0");
+    int Mmax = 0;
+    for (int r = 0; r < R; ++r) {
+        if (moff[r + 1] < moff[r]) return fail(f, SCPHD_ERR_ARG, "moff must not decrease");
+        Mmax = std::max(Mmax, moff[r + 1] - moff[r]);
+    }
+    const int32_t total = moff[R];
+    if (total > 0 && (!u || !v)) return SCPHD_ERR_ARG;
+    HIPTRY(hipSetDevice(f->dev));
+    f->hmoff.assign(moff, moff + R + 1);
+    f->hbpcs.resize(R);
+    const double kappa = f->models.meas.kappa();
+    for (int r = 0; r < R; ++r) f->hbpcs[r] = safe_log((moff[r + 1] - moff[r]) * f->prm.birth_w0 + kappa);
+    f->ens_meas = total;
+    HIPTRY(f->du.ensure(std::max(total, 1)));
+    HIPTRY(f->dv.ensure(std::max(total, 1)));
+    if (total > 0) {
+        HIPTRY(hipMemcpyAsync(f->du.p, u, total * sizeof(double), hipMemcpyHostToDevice, f->stream));
+        HIPTRY(hipMemcpyAsync(f->dv.p, v, total * sizeof(double), hipMemcpyHostToDevice, f->stream));
+    }
+    HIPTRY(hipMemcpyAsync(f->dmoff.p, f->hmoff.data(), (R + 1) * sizeof(int32_t), hipMemcpyHostToDevice,
+                          f->stream));
+    HIPTRY(hipMemcpyAsync(f->dbpcs.p, f->hbpcs.data(), R * sizeof(double), hipMemcpyHostToDevice, f->stream));
+    return update_common(f, Mmax, fixed_camera ? 1 : 0);
 }
 
 int scphd_resample(scphd_t* f, int32_t* resampled) {
@@ -647,12 +740,16 @@ int scphd_resample(scphd_t* f, int32_t* resampled) {
     ra.anc = f->anc.p;
     ra.newcnt = f->newcnt.p;
     ra.goff = f->goff.p;
-    ra.Np = Np;
+    ra.Np = f->npr;
+    ra.nrigs = f->n_rigs;
     ra.min_neff = f->prm.min_neff;
     ra.key = f->key;
+    ra.keys = f->ensemble ? f->dkeys.p : nullptr;
     ra.rstep = (uint64_t)f->n_resample;
     ra.flag = f->dflag.p;
+    ra.rflag = f->drflag.p;
     ra.neff = f->dneff.p;
+    HIPTRY(hipMemsetAsync(f->dflag.p, 0, sizeof(int32_t), s));
     HIPTRY(launch_resample(s, ra));
     HIPTRY(f->scantmp.ensure(scan_tmp_size(Np + 1) + 16));
     HIPTRY(launch_scan(s, f->newcnt.p, f->goff2.p, Np, f->scantmp.p));
@@ -874,6 +971,7 @@ int scphd_debug_normals(uint64_t seed, uint32_t stream, uint64_t step, uint32_t 
 
 int scphd_set_noise(scphd_t* f, int32_t mode, uint64_t seed, const double* injected, size_t n) {
     if (check_handle(f)) return SCPHD_ERR_ARG;
+    if (f->ensemble) return fail(f, SCPHD_ERR_ARG, "an ensemble's keys are fixed by scphd_create_ensemble");
     if (mode != 0 || injected || n)
         return fail(f, SCPHD_ERR_ARG, "only mode 0 (Philox streams keyed by seed) is provided");
     f->prm.seed = seed;
@@ -884,7 +982,7 @@ int scphd_set_noise(scphd_t* f, int32_t mode, uint64_t seed, const double* injec
 int scphd_get_estimate(scphd_t* f, double* mean12, double* cov144, double* cardinality) {
     if (check_handle(f)) return SCPHD_ERR_ARG;
     HIPTRY(hipSetDevice(f->dev));
-    HIPTRY(launch_estimate(f->stream, f->ext.p, f->pw.p, f->goff.p, f->gw.p, f->Np,
+    HIPTRY(launch_estimate(f->stream, f->ext.p, f->pw.p, f->goff.p, f->gw.p, f->Np, 1,
                            (double*)((char*)f->dmap + MAP_EST)));
     HIPTRY(hipStreamSynchronize(f->stream));
     const double* h = (const double*)((const char*)f->hmap + MAP_EST);
@@ -894,18 +992,45 @@ int scphd_get_estimate(scphd_t* f, double* mean12, double* cov144, double* cardi
     return SCPHD_OK;
 }
 
+int scphd_get_rig_estimates(scphd_t* f, double* mean12, double* cov144, double* cardinality) {
+    if (check_handle(f)) return SCPHD_ERR_ARG;
+    HIPTRY(hipSetDevice(f->dev));
+    HIPTRY(launch_estimate(f->stream, f->ext.p, f->pw.p, f->goff.p, f->gw.p, f->npr, f->n_rigs,
+                           (double*)((char*)f->dmap + MAP_EST)));
+    HIPTRY(hipStreamSynchronize(f->stream));
+    const double* h = (const double*)((const char*)f->hmap + MAP_EST);
+    for (int r = 0; r < f->n_rigs; ++r, h += EST_DOUBLES) {
+        if (mean12) std::memcpy(mean12 + 12 * r, h, 12 * sizeof(double));
+        if (cov144) std::memcpy(cov144 + 144 * r, h + 12, 144 * sizeof(double));
+        if (cardinality) cardinality[r] = h[156];
+    }
+    return SCPHD_OK;
+}
+
+int scphd_get_rig_resample(scphd_t* f, int32_t* resampled, double* neff) {
+    if (check_handle(f)) return SCPHD_ERR_ARG;
+    if (f->n_resample == 0) return fail(f, SCPHD_ERR_STATE, "no resampling step yet");
+    HIPTRY(hipSetDevice(f->dev));
+    if (resampled)
+        HIPTRY(hipMemcpyAsync(resampled, f->drflag.p, f->n_rigs * sizeof(int32_t), hipMemcpyDeviceToHost, f->stream));
+    if (neff) HIPTRY(hipMemcpyAsync(neff, f->dneff.p, f->n_rigs * sizeof(double), hipMemcpyDeviceToHost, f->stream));
+    HIPTRY(hipStreamSynchronize(f->stream));
+    return SCPHD_OK;
+}
+
 int scphd_get_memory(scphd_t* f, int64_t* bytes) {
     if (check_handle(f) || !bytes) return SCPHD_ERR_ARG;
     int64_t b = 0;
     DBuf<double>* dbs[] = {&f->ext, &f->pw, &f->pose, &f->loglik, &f->cum, &f->ext_tmp, &f->gw, &f->gw2,
                            &f->fit, &f->pre, &f->lik, &f->du, &f->dv, &f->eta, &f->sw, &f->smu, &f->mw, &f->mmu, &f->mP,
-                           &f->dneff};
+                           &f->dneff, &f->dbpcs};
     for (auto* d : dbs) b += d->n * (int64_t)sizeof(double);
     DBuf<int32_t>* ibs[] = {&f->anc, &f->newcnt, &f->goff, &f->gparent, &f->gblock, &f->goff2, &f->gparent2,
                             &f->gblock2, &f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->ndfill, &f->nsurv, &f->survoff,
                             &f->nmerged, &f->mergeoff, &f->outcnt, &f->scantmp, &f->sg, &f->sm, &f->dflag, &f->mcnt, &f->mncand, &f->mcand,
-                            &f->ngroups, &f->glead, &f->gsta, &f->gend, &f->gmem};
+                            &f->ngroups, &f->glead, &f->gsta, &f->gend, &f->gmem, &f->dmoff, &f->drflag};
     for (auto* d : ibs) b += d->n * (int64_t)sizeof(int32_t);
+    b += f->dkeys.n * (int64_t)sizeof(NoiseKey);
     b += f->mscratch.n;
     for (const auto& g : f->gen) b += g.cap_blocks * (int64_t)f->S * (int64_t)f->esz * 7;
     *bytes = b;

@@ -25,6 +25,25 @@
 
 namespace scphd {
 
+/* Noise keys: one rig (keys == nullptr: every parent uses `key`), or an
+ * ensemble of rigs with npr parents each, parent p belonging to rig
+ * p / npr, whose key it uses with its index inside the rig, so that every
+ * rig draws exactly the streams of a filter of its own. */
+struct RigKeys {
+    NoiseKey key;
+    const NoiseKey* keys;
+    int npr;
+    __device__ NoiseKey of(int p, int& pl) const {
+        if (!keys) {
+            pl = p;
+            return key;
+        }
+        const int r = p / npr;
+        pl = p - r * npr;
+        return keys[r];
+    }
+};
+
 /* per-Gaussian fit outputs, stride Ng (SoA planes) */
 enum FitPlane { FP_MU = 0, FP_P = 6, FP_PD = 27, FP_COUNT = 28 };
 /* per-Gaussian detection terms, stride Ng */
@@ -58,10 +77,10 @@ struct BirthModel {
 
 /* ---- prediction ---- */
 hipError_t launch_parent_predict(hipStream_t s, double* ext, int Np, OrientedLinearCVMotionModel3D cam,
-                                 double dt, NoiseKey key, uint64_t step);
+                                 double dt, RigKeys key, uint64_t step);
 hipError_t launch_scale(hipStream_t s, double* x, int64_t n, double a);
 hipError_t launch_prior(hipStream_t s, double* ext, double* pw, int Np, const double* a12, const double* b12,
-                        NoiseKey key);
+                        RigKeys key);
 hipError_t launch_pose(hipStream_t s, const double* ext, double* pose, int Np, int fixed);
 
 /* Apply a deferred sample prediction, writing every table entry to its own
@@ -69,7 +88,7 @@ hipError_t launch_pose(hipStream_t s, const double* ext, double* pose, int Np, i
 template <typename T>
 hipError_t launch_materialize(hipStream_t s, Planes<T> src, Planes<T> dst, const int32_t* gblock,
                               const int32_t* gparent, const int32_t* goff, int64_t Ng, int S,
-                              LinearCVMotionModel3D feat, double dt, NoiseKey key, uint64_t step);
+                              LinearCVMotionModel3D feat, double dt, RigKeys key, uint64_t step);
 
 /* ---- update ---- */
 struct FitArgs {
@@ -84,7 +103,7 @@ struct FitArgs {
     LinearCVMotionModel3D feat;
     int pending;
     double dt;
-    NoiseKey key;
+    RigKeys key;
     uint64_t pstep;
     double min_weight;
     double* fit;     /* FP_COUNT planes */
@@ -117,9 +136,12 @@ struct EtaArgs {
     int64_t Ng;        /* plane stride */
     const double* zu;
     const double* zv;
-    int M;
+    int M;             /* the measurement count; in an ensemble the largest, the stride of eta & co. */
     int Np;
     double bpc;        /* log(M w0 + kappa) */
+    const int32_t* moff = nullptr; /* ensemble: rig r's measurements at [moff[r], moff[r + 1]); null for one rig */
+    const double* bpcs = nullptr;  /* ensemble: bpc per rig */
+    int npr = 1;                   /* ensemble: parents per rig */
     double min_weight;
     BirthModel birth;
     double* eta;       /* Np*M */
@@ -184,8 +206,12 @@ hipError_t launch_readout(hipStream_t s, const int32_t* off, int n, int32_t limi
                           const int32_t* x1, int32_t* out);
 /* *out = the number of NaN entries of w[0, n) */
 hipError_t launch_nan_count(hipStream_t s, const double* w, int n, int32_t* out);
+/* *out = sum over rigs r of (goff[(r+1) npr] - goff[r npr]) (moff[r+1] - moff[r]) */
+hipError_t launch_pair_count(hipStream_t s, const int32_t* goff, const int32_t* moff, int npr, int nrigs,
+                             double* out);
 
-hipError_t launch_parent_weights(hipStream_t s, double* pw, const double* loglik, int Np);
+/* per rig of npr parents, nrigs rigs (one workgroup each) */
+hipError_t launch_parent_weights(hipStream_t s, double* pw, const double* loglik, int npr, int nrigs);
 
 /* out[6 n]: six normals of every element 0..n-1 of one noise stream */
 hipError_t launch_debug_normals(hipStream_t s, NoiseKey key, uint32_t stream, uint64_t step, uint32_t parent,
@@ -193,8 +219,10 @@ hipError_t launch_debug_normals(hipStream_t s, NoiseKey key, uint32_t stream, ui
 
 /* out[157]: weighted parent mean (12), covariance (144, row-major), summed
  * Gaussian weight of the first highest-weight parent */
+/* per rig of Np parents (nrigs rigs): EST_DOUBLES = mean[12], cov[144], cardinality */
+static constexpr int EST_DOUBLES = 157;
 hipError_t launch_estimate(hipStream_t s, const double* ext, const double* pw, const int32_t* goff,
-                           const double* gw, int Np, double* out);
+                           const double* gw, int Np, int nrigs, double* out);
 
 struct SampleArgs {
     const int32_t* mergeoff; /* Np+1 */
@@ -207,7 +235,7 @@ struct SampleArgs {
     int64_t Kt;
     int S;
     DisparityMeasurementModel meas;
-    NoiseKey key;
+    RigKeys key;
     uint64_t ustep;
     int64_t post_base;
     int off32; /* every destination plane is below 4 GiB: 32-bit byte offsets */
@@ -249,12 +277,15 @@ struct ResampleArgs {
     int32_t* anc;
     int32_t* newcnt;
     const int32_t* goff;
-    int Np;
+    int Np;                /* parents per rig */
+    int nrigs = 1;         /* one workgroup per rig */
     double min_neff;
-    NoiseKey key;
+    NoiseKey key;          /* one rig */
+    const NoiseKey* keys = nullptr; /* ensemble: per rig, or nullptr */
     uint64_t rstep;
-    int32_t* flag; /* out: 1 if resampled */
-    double* neff;  /* out */
+    int32_t* flag;  /* out: 1 if any rig resampled (zero before the launch) */
+    int32_t* rflag = nullptr; /* out: per rig, 1 if resampled (may be null) */
+    double* neff;   /* out: per rig */
 };
 hipError_t launch_resample(hipStream_t s, const ResampleArgs& a);
 hipError_t launch_table_resample(hipStream_t s, const int32_t* anc, const int32_t* goff_old,

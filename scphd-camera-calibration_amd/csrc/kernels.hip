@@ -124,27 +124,31 @@ struct Bounds12 {
 
 /* Uniform prior over [a, b] per extrinsic component, weights 1/N
  * (src/scphdcameracalibration.cu:835-980). */
-__global__ void k_prior(double* ext, double* pw, int Np, Bounds12 ab, NoiseKey key) {
+__global__ void k_prior(double* ext, double* pw, int Np, Bounds12 ab, RigKeys rk) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= Np) return;
+    int pl;
+    const NoiseKey key = rk.of(p, pl);
     for (int c = 0; c < 12; ++c) {
         uint32_t x[4];
-        noise_block(key, NS_PRIOR, 0, (uint32_t)p, 0, 0, (uint32_t)(c / 2), x);
+        noise_block(key, NS_PRIOR, 0, (uint32_t)pl, 0, 0, (uint32_t)(c / 2), x);
         const double r = (c % 2 == 0) ? uniform_co(x[0], x[1]) : uniform_co(x[2], x[3]);
         ext[(int64_t)p * 12 + c] = r * (ab.b[c] - ab.a[c]) + ab.a[c];
     }
-    pw[p] = 1.0 / Np;
+    pw[p] = 1.0 / rk.npr;
 }
 
 /* Parent extrinsics through OrientedLinearCVMotionModel3D::computeNoisyMotion
  * with N(0, std^2) accelerations (predict_camera functor,
  * src/scphdcameracalibration.cu:129-152, :1062-1100). */
-__global__ void k_parent_predict(double* ext, int Np, OrientedLinearCVMotionModel3D cam, double dt, NoiseKey key,
+__global__ void k_parent_predict(double* ext, int Np, OrientedLinearCVMotionModel3D cam, double dt, RigKeys rk,
                                  uint64_t step) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= Np) return;
+    int pl;
+    const NoiseKey key = rk.of(p, pl);
     double a[6];
-    normals<6>(key, NS_CAMERA, step, (uint32_t)p, 0, 0, a);
+    normals<6>(key, NS_CAMERA, step, (uint32_t)pl, 0, 0, a);
     double* e = ext + (int64_t)p * 12;
     EuclideanPoint c{e[0], e[1], e[2], e[3], e[4], e[5]};
     EuclideanPoint g{e[6], e[7], e[8], e[9], e[10], e[11]};
@@ -200,17 +204,19 @@ __device__ inline EuclideanPoint predict_sample(EuclideanPoint x, const LinearCV
 template <typename T>
 __global__ void k_materialize(Planes<T> src, Planes<T> dst, const int32_t* gblock, const int32_t* gparent,
                               const int32_t* goff, int64_t Ng, int S, LinearCVMotionModel3D feat, double dt,
-                              NoiseKey key, uint64_t step) {
+                              RigKeys rk, uint64_t step) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Ng * S) return;
     const int64_t g = i / S;
     const int s = (int)(i - g * S);
     const int p = gparent[g];
     const int k = (int)(g - goff[p]);
+    int pl;
+    const NoiseKey key = rk.of(p, pl);
     const int64_t si = (int64_t)gblock[g] * S + s;
     EuclideanPoint x{(double)src.p[0][si], (double)src.p[1][si], (double)src.p[2][si],
                      (double)src.p[3][si], (double)src.p[4][si], (double)src.p[5][si]};
-    x = predict_sample(x, feat, dt, key, step, (uint32_t)p, (uint32_t)k, (uint32_t)s);
+    x = predict_sample(x, feat, dt, key, step, (uint32_t)pl, (uint32_t)k, (uint32_t)s);
     const int64_t di = g * S + s;
     dst.p[0][di] = (T)x.x; dst.p[1][di] = (T)x.y; dst.p[2][di] = (T)x.z;
     dst.p[3][di] = (T)x.vx; dst.p[4][di] = (T)x.vy; dst.p[5][di] = (T)x.vz;
@@ -264,21 +270,22 @@ __device__ inline void load_sample(const Planes<T>& src, int64_t si, T r[7]) {
 }
 
 template <typename T>
-__device__ inline void eval_sample(const FitArgs& a, const T r[7], int p, int k, int s, const CameraPose& pose,
-                                   EuclideanPoint& x, DisparityPoint& d, bool& in, double& w) {
+__device__ inline void eval_sample(const FitArgs& a, const T r[7], NoiseKey kk, int pl, int k, int s,
+                                   const CameraPose& pose, EuclideanPoint& x, DisparityPoint& d, bool& in, double& w) {
     x = EuclideanPoint{(double)r[0], (double)r[1], (double)r[2], (double)r[3], (double)r[4], (double)r[5]};
     w = (double)r[6];
-    if (a.pending) x = predict_sample(x, a.feat, a.dt, a.key, a.pstep, (uint32_t)p, (uint32_t)k, (uint32_t)s);
+    if (a.pending) x = predict_sample(x, a.feat, a.dt, kk, a.pstep, (uint32_t)pl, (uint32_t)k, (uint32_t)s);
     d = disparity_fast(a.meas, pose, x);
     in = a.meas.inRange(d);
 }
 
 template <typename T>
-__device__ inline void fit_sample(const FitArgs& a, const Planes<T>& src, int64_t si, int p, int k, int s,
-                                  const CameraPose& pose, EuclideanPoint& x, DisparityPoint& d, bool& in, double& w) {
+__device__ inline void fit_sample(const FitArgs& a, const Planes<T>& src, int64_t si, NoiseKey kk, int pl, int k,
+                                  int s, const CameraPose& pose, EuclideanPoint& x, DisparityPoint& d, bool& in,
+                                  double& w) {
     T r[7];
     load_sample(src, si, r);
-    eval_sample(a, r, p, k, s, pose, x, d, in, w);
+    eval_sample(a, r, kk, pl, k, s, pose, x, d, in, w);
 }
 
 __device__ inline void write_fit(const FitArgs& a, int64_t g, int lane, const double* mu, const double* P, double pdg) {
@@ -346,6 +353,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
 #pragma unroll
     for (int i = 0; i < 3; ++i) pose.t[i] = s_pose[row * 12 + 9 + i];
     const int k = (int)(g - a.goff[p]);
+    int pl;
+    const NoiseKey kk = a.key.of(p, pl); /* the rig's key, p's index in its rig */
     const int64_t b0 = (int64_t)a.gblock[g] * a.S;
     const int S = a.S;
     const double wg = a.gw[g];
@@ -384,7 +393,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
         DisparityPoint d;
         bool in = false;
         double w = 0.0;
-        if (valid) eval_sample(a, cur, p, k, s, pose, x, d, in, w);
+        if (valid) eval_sample(a, cur, kk, pl, k, s, pose, x, d, in, w);
         const double q[6] = {d.u, d.v, d.d, d.vu, d.vv, d.vd};
         if (s0 == 0) {
             /* shift: the Gaussian's first sample, broadcast along the row */
@@ -475,6 +484,8 @@ __global__ __launch_bounds__(256) void k_nd_fill(FitArgs a, Planes<T> src, Plane
         const int64_t g = a.ndfill[i];
         const int p = a.gparent[g];
         const int k = (int)(g - a.goff[p]);
+        int pl;
+        const NoiseKey kk = a.key.of(p, pl);
         const int64_t b0 = (int64_t)a.gblock[g] * a.S;
         const int64_t d0 = (a.nd_base + g) * (int64_t)a.S;
         const CameraPose pose = load_pose(a.pose, p);
@@ -494,7 +505,7 @@ __global__ __launch_bounds__(256) void k_nd_fill(FitArgs a, Planes<T> src, Plane
                     DisparityPoint d;
                     bool in;
                     double w;
-                    eval_sample(a, r[j], p, k, s, pose, x, d, in, w);
+                    eval_sample(a, r[j], kk, pl, k, s, pose, x, d, in, w);
                     write_nd_off(a, dst, (Off)((d0 + s) * (int64_t)sizeof(T)), x, in);
                 }
             }
@@ -516,6 +527,8 @@ __global__ __launch_bounds__(256) void k_fit_fix(FitArgs a, Planes<T> src, Plane
         const int64_t g = a.redo[i];
         const int p = a.gparent[g];
         const int k = (int)(g - a.goff[p]);
+        int pl;
+        const NoiseKey kk = a.key.of(p, pl);
         const int64_t b0 = (int64_t)a.gblock[g] * S;
         const CameraPose pose = load_pose(a.pose, p);
         double sq[6] = {0, 0, 0, 0, 0, 0};
@@ -526,7 +539,7 @@ __global__ __launch_bounds__(256) void k_fit_fix(FitArgs a, Planes<T> src, Plane
                 DisparityPoint d;
                 bool in;
                 double w;
-                fit_sample(a, src, b0 + s, p, k, s, pose, x, d, in, w);
+                fit_sample(a, src, b0 + s, kk, pl, k, s, pose, x, d, in, w);
                 sq[0] += d.u; sq[1] += d.v; sq[2] += d.d; sq[3] += d.vu; sq[4] += d.vv; sq[5] += d.vd;
             }
         }
@@ -542,7 +555,7 @@ __global__ __launch_bounds__(256) void k_fit_fix(FitArgs a, Planes<T> src, Plane
                 DisparityPoint d;
                 bool in;
                 double w;
-                fit_sample(a, src, b0 + s, p, k, s, pose, x, d, in, w);
+                fit_sample(a, src, b0 + s, kk, pl, k, s, pose, x, d, in, w);
                 const double dv[6] = {d.u - mu[0], d.v - mu[1], d.d - mu[2], d.vu - mu[3], d.vv - mu[4], d.vd - mu[5]};
 #pragma unroll
                 for (int r = 0; r < 6; ++r)
@@ -654,7 +667,12 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
     const int G = a.goff[p + 1] - a.goff[p];
     double eta_sum = 0.0;
     int cnt = 0;
-    const int rounds = (a.M + NT - 1) / NT;
+    /* this parent's measurement set: its rig's in an ensemble */
+    const int rig = a.moff ? p / a.npr : 0;
+    const int M = a.moff ? a.moff[rig + 1] - a.moff[rig] : a.M;
+    const int64_t zb = a.moff ? a.moff[rig] : 0;
+    const double bpc = a.bpcs ? a.bpcs[rig] : a.bpc;
+    const int rounds = (M + NT - 1) / NT;
     /* Only a Gaussian with l > bpc can move the maximum off bpc, and
      * l <= C - log 2 pi - H (the value at dist = 0; the computed quadratic
      * form is never below -1e-9 of its scale), so the max pass visits only
@@ -668,7 +686,7 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
         if (j < G) {
             const double* rec = a.lik + (g0 + j) * LK_COUNT;
             const double ub = rec[LK_C] - 1.8378770664093453 - rec[LK_H];
-            h = rec[LK_OK] != 0.0 && ub + 1e-6 * (1.0 + fabs(ub)) > a.bpc;
+            h = rec[LK_OK] != 0.0 && ub + 1e-6 * (1.0 + fabs(ub)) > bpc;
         }
         int tot;
         const int pos = nheavy + block_exscan(h ? 1 : 0, sh, &tot);
@@ -679,12 +697,12 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
     const bool listed = nheavy <= ETA_HEAVY;
     for (int r = 0; r < rounds; ++r) {
         const int m = r * NT + t;
-        const bool valid = m < a.M;
-        const double zu = valid ? a.zu[m] : 0.0, zv = valid ? a.zv[m] : 0.0;
+        const bool valid = m < M;
+        const double zu = valid ? a.zu[zb + m] : 0.0, zv = valid ? a.zv[zb + m] : 0.0;
         /* eta = mx + log1p(sum of the other terms' exp(l - mx)), mx the
          * first maximum of (bpc, l_0, ...), terms summed g ascending then
          * bpc: the normaliser definition the oracle uses bit for bit */
-        double mx = a.bpc;
+        double mx = bpc;
         int am = -1;
         double T = 0.0;
         int mcnt = 0;
@@ -739,7 +757,7 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
                 }
             }
         }
-        if (am >= 0) T += exp(a.bpc - mx);
+        if (am >= 0) T += exp(bpc - mx);
         const double eta = mx + log1p(T);
         if (valid) {
             const int64_t pm = (int64_t)p * a.M + m;
@@ -796,11 +814,14 @@ __global__ __launch_bounds__(NT) void k_survivors(SurvArgs a) {
     const int64_t base = a.survoff[p];
     const int cap = a.survoff[p + 1] - a.survoff[p];
     int run = 0;
-    const int rounds = (e.M + NT - 1) / NT;
+    const int rig = e.moff ? p / e.npr : 0;
+    const int M = e.moff ? e.moff[rig + 1] - e.moff[rig] : e.M;
+    const int64_t zb = e.moff ? e.moff[rig] : 0;
+    const int rounds = (M + NT - 1) / NT;
     for (int r = 0; r < rounds; ++r) {
         const int m = r * NT + t;
-        const bool valid = m < e.M;
-        const double zu = valid ? e.zu[m] : 0.0, zv = valid ? e.zv[m] : 0.0;
+        const bool valid = m < M;
+        const double zu = valid ? e.zu[zb + m] : 0.0, zv = valid ? e.zv[zb + m] : 0.0;
         const double eta = valid ? e.eta[(int64_t)p * e.M + m] : 0.0;
         const int cnt = valid ? e.mcnt[(int64_t)p * e.M + m] : 0; /* detection terms kept, from k_eta */
         int tot;
@@ -849,7 +870,7 @@ __global__ __launch_bounds__(NT) void k_survivors(SurvArgs a) {
     }
     for (int r = 0; r < rounds; ++r) {
         const int m = r * NT + t;
-        const bool valid = m < e.M;
+        const bool valid = m < M;
         const double eta = valid ? e.eta[(int64_t)p * e.M + m] : 0.0;
         const double w = valid ? exp(e.birth.logw0 - eta) : 0.0;
         const bool ok = valid && w >= e.min_weight && e.birth.pd_ok;
@@ -860,8 +881,8 @@ __global__ __launch_bounds__(NT) void k_survivors(SurvArgs a) {
             a.sw[so] = w;
             a.sg[so] = -1 - m;
             a.sm[so] = m;
-            a.smu[0 * a.cap + so] = e.zu[m];
-            a.smu[1 * a.cap + so] = e.zv[m];
+            a.smu[0 * a.cap + so] = e.zu[zb + m];
+            a.smu[1 * a.cap + so] = e.zv[zb + m];
 #pragma unroll
             for (int c = 2; c < 6; ++c) a.smu[c * a.cap + so] = e.birth.mean[c];
         }
@@ -1553,6 +1574,8 @@ __global__ __launch_bounds__(NT) void k_merge_rounds(MergeArgs a) {
  * domain).  One workgroup. */
 __global__ __launch_bounds__(1024) void k_parent_weights(double* pw, const double* loglik, int Np) {
     __shared__ double red[16];
+    pw += (int64_t)blockIdx.x * Np; /* one workgroup per rig of Np parents */
+    loglik += (int64_t)blockIdx.x * Np;
     double mx = -INFINITY;
     for (int p = threadIdx.x; p < Np; p += blockDim.x) mx = fmax(mx, safe_log(pw[p]) + loglik[p]);
     mx = block_max(mx, red);
@@ -1593,14 +1616,21 @@ __device__ inline void block_sum_n(double (&v)[N], double* sh) {
     __syncthreads();
 }
 
-/* One 256-thread workgroup: weighted mean and covariance of the parent
- * extrinsics and the summed Gaussian weight of the first highest-weight
- * parent's map (exp_traj, cov_traj and cardinality of
- * load_disparity_results.m:27-34).  out: mean[12], cov[144] row-major,
- * cardinality. */
+/* One 256-thread workgroup per rig of Np parents: weighted mean and
+ * covariance of the parent extrinsics and the summed Gaussian weight of the
+ * first highest-weight parent's map (exp_traj, cov_traj and cardinality of
+ * load_disparity_results.m:27-34).  out per rig: mean[12], cov[144]
+ * row-major, cardinality (EST_DOUBLES). */
 __global__ __launch_bounds__(256) void k_estimate(const double* ext, const double* pw, const int32_t* goff,
                                                    const double* gw, int Np, double* out) {
     __shared__ double sh[4 * 78];
+    {
+        const int64_t P0 = (int64_t)blockIdx.x * Np;
+        ext += P0 * 12;
+        pw += P0;
+        goff += P0;
+        out += (int64_t)blockIdx.x * EST_DOUBLES;
+    }
     __shared__ double bws[4];
     __shared__ int bis[4];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -1686,6 +1716,8 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a, Planes<T> dst) {
     if (gm >= a.Kt) return; /* whole rows */
     const int p = upper_parent(a.mergeoff, a.Np, gm);
     const int k = (int)(gm - a.mergeoff[p]);
+    int pl;
+    const NoiseKey key = a.key.of(p, pl);
     const int64_t rec = (int64_t)a.survoff[p] + k;
     double mu[6], P[21], L[21];
 #pragma unroll
@@ -1698,7 +1730,7 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a, Planes<T> dst) {
     Off off = (Off)(((a.post_base + gm) * (int64_t)a.S + lane) * (int64_t)sizeof(T));
     for (int s = lane; s < a.S; s += 16) {
         double z[6];
-        normals<6>(a.key, NS_POSTERIOR, a.ustep, (uint32_t)p, (uint32_t)k, (uint32_t)s, z);
+        normals<6>(key, NS_POSTERIOR, a.ustep, (uint32_t)pl, (uint32_t)k, (uint32_t)s, z);
         double q[6];
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
@@ -1769,7 +1801,18 @@ __global__ __launch_bounds__(1024) void k_resample(ResampleArgs a) {
     __shared__ double chunk[RS_CH];
     __shared__ double s_em[16];
     __shared__ int s_am[16];
+    /* one workgroup per rig: its Np parents from P0, ancestors within it */
+    const int rig = blockIdx.x;
     const int N = a.Np;
+    const int64_t P0 = (int64_t)rig * N;
+    a.pw += P0;
+    a.ext += P0 * 12;
+    a.ext_tmp += P0 * 12;
+    a.cum += P0;
+    a.anc += P0;
+    a.newcnt += P0;
+    a.goff += P0;
+    const NoiseKey key = a.keys ? a.keys[rig] : a.key;
     const int t = threadIdx.x;
     double ss = 0.0;
     for (int p = t; p < N; p += blockDim.x) ss += a.pw[p] * a.pw[p];
@@ -1777,12 +1820,13 @@ __global__ __launch_bounds__(1024) void k_resample(ResampleArgs a) {
     const double neff = 1.0 / (N * ss);
     const bool go = !(neff >= a.min_neff);
     if (t == 0) {
-        *a.neff = neff;
-        *a.flag = go ? 1 : 0;
+        a.neff[rig] = neff;
+        if (a.rflag) a.rflag[rig] = go ? 1 : 0;
+        if (go) atomicOr(a.flag, 1); /* any rig resampled; zero before the launch */
     }
     if (!go) {
         for (int p = t; p < N; p += blockDim.x) {
-            a.anc[p] = p;
+            a.anc[p] = (int32_t)(P0 + p);
             a.newcnt[p] = a.goff[p + 1] - a.goff[p];
         }
         return;
@@ -1812,7 +1856,7 @@ __global__ __launch_bounds__(1024) void k_resample(ResampleArgs a) {
     }
     if (t == 0) {
         uint32_t x[4];
-        noise_block(a.key, NS_RESAMPLE, a.rstep, 0, 0, 0, 0, x);
+        noise_block(key, NS_RESAMPLE, a.rstep, 0, 0, 0, 0, x);
         s_u = uniform_co(x[0], x[1]) * (1.0 / N);
     }
     /* sequential cumulative sum */
@@ -1858,7 +1902,7 @@ __global__ __launch_bounds__(1024) void k_resample(ResampleArgs a) {
                 lo = mid + 1;
         }
         const int anc = lo < N ? lo : amax;
-        a.anc[j] = anc;
+        a.anc[j] = (int32_t)(P0 + anc);
         for (int q = 0; q < 12; ++q) a.ext_tmp[(int64_t)j * 12 + q] = a.ext[(int64_t)anc * 12 + q];
         a.newcnt[j] = a.goff[anc + 1] - a.goff[anc];
     }
@@ -1955,7 +1999,7 @@ static inline unsigned nblocks(int64_t n, int t) { return (unsigned)((n + t - 1)
 #define LAUNCH_CHECK() return hipGetLastError()
 
 hipError_t launch_prior(hipStream_t s, double* ext, double* pw, int Np, const double* a12, const double* b12,
-                        NoiseKey key) {
+                        RigKeys key) {
     Bounds12 ab;
     for (int i = 0; i < 12; ++i) {
         ab.a[i] = a12[i];
@@ -1966,7 +2010,7 @@ hipError_t launch_prior(hipStream_t s, double* ext, double* pw, int Np, const do
 }
 
 hipError_t launch_parent_predict(hipStream_t s, double* ext, int Np, OrientedLinearCVMotionModel3D cam, double dt,
-                                 NoiseKey key, uint64_t step) {
+                                 RigKeys key, uint64_t step) {
     k_parent_predict<<<nblocks(Np, 256), 256, 0, s>>>(ext, Np, cam, dt, key, step);
     LAUNCH_CHECK();
 }
@@ -1985,7 +2029,7 @@ hipError_t launch_pose(hipStream_t s, const double* ext, double* pose, int Np, i
 template <typename T>
 hipError_t launch_materialize(hipStream_t s, Planes<T> src, Planes<T> dst, const int32_t* gblock,
                               const int32_t* gparent, const int32_t* goff, int64_t Ng, int S,
-                              LinearCVMotionModel3D feat, double dt, NoiseKey key, uint64_t step) {
+                              LinearCVMotionModel3D feat, double dt, RigKeys key, uint64_t step) {
     if (Ng <= 0) return hipSuccess;
     k_materialize<T><<<nblocks(Ng * S, 256), 256, 0, s>>>(src, dst, gblock, gparent, goff, Ng, S, feat, dt, key,
                                                           step);
@@ -2046,8 +2090,8 @@ hipError_t launch_merge(hipStream_t s, const MergeArgs& a, int lds_bytes) {
     LAUNCH_CHECK();
 }
 
-hipError_t launch_parent_weights(hipStream_t s, double* pw, const double* loglik, int Np) {
-    k_parent_weights<<<1, 1024, 0, s>>>(pw, loglik, Np);
+hipError_t launch_parent_weights(hipStream_t s, double* pw, const double* loglik, int npr, int nrigs) {
+    k_parent_weights<<<nrigs, 1024, 0, s>>>(pw, loglik, npr);
     LAUNCH_CHECK();
 }
 
@@ -2059,9 +2103,9 @@ hipError_t launch_debug_normals(hipStream_t s, NoiseKey key, uint32_t stream, ui
 }
 
 hipError_t launch_estimate(hipStream_t s, const double* ext, const double* pw, const int32_t* goff,
-                           const double* gw, int Np, double* out) {
-    if (Np <= 0) return hipErrorInvalidValue;
-    k_estimate<<<1, 256, 0, s>>>(ext, pw, goff, gw, Np, out);
+                           const double* gw, int Np, int nrigs, double* out) {
+    if (Np <= 0 || nrigs <= 0) return hipErrorInvalidValue;
+    k_estimate<<<nrigs, 256, 0, s>>>(ext, pw, goff, gw, Np, out);
     LAUNCH_CHECK();
 }
 
@@ -2095,7 +2139,7 @@ hipError_t launch_table(hipStream_t s, const TableArgs& a) {
 }
 
 hipError_t launch_resample(hipStream_t s, const ResampleArgs& a) {
-    k_resample<<<1, 1024, 0, s>>>(a);
+    k_resample<<<a.nrigs, 1024, 0, s>>>(a);
     LAUNCH_CHECK();
 }
 
@@ -2153,6 +2197,25 @@ __global__ __launch_bounds__(1024) void k_nan_count(const double* w, int n, vola
     }
 }
 
+/* Detection pairs of an ensemble update: the sum over rigs of the rig's
+ * map size times its measurement count (exact in FP64 below 2^53).  One
+ * workgroup. */
+__global__ __launch_bounds__(256) void k_pair_count(const int32_t* goff, const int32_t* moff, int npr, int nrigs,
+                                                    volatile double* out) {
+    __shared__ double red[16];
+    double c = 0.0;
+    for (int r = threadIdx.x; r < nrigs; r += blockDim.x)
+        c += (double)(goff[(int64_t)(r + 1) * npr] - goff[(int64_t)r * npr]) * (double)(moff[r + 1] - moff[r]);
+    c = block_sum(c, red);
+    if (threadIdx.x == 0) *out = c;
+}
+
+hipError_t launch_pair_count(hipStream_t s, const int32_t* goff, const int32_t* moff, int npr, int nrigs,
+                             double* out) {
+    k_pair_count<<<1, 256, 0, s>>>(goff, moff, npr, nrigs, out);
+    LAUNCH_CHECK();
+}
+
 hipError_t launch_nan_count(hipStream_t s, const double* w, int n, int32_t* out) {
     k_nan_count<<<1, 1024, 0, s>>>(w, n, out);
     LAUNCH_CHECK();
@@ -2206,10 +2269,10 @@ hipError_t launch_scan(hipStream_t s, const int32_t* in, int32_t* out, int64_t n
 /* explicit instantiations */
 template hipError_t launch_materialize<float>(hipStream_t, Planes<float>, Planes<float>, const int32_t*,
                                               const int32_t*, const int32_t*, int64_t, int, LinearCVMotionModel3D,
-                                              double, NoiseKey, uint64_t);
+                                              double, RigKeys, uint64_t);
 template hipError_t launch_materialize<double>(hipStream_t, Planes<double>, Planes<double>, const int32_t*,
                                                const int32_t*, const int32_t*, int64_t, int, LinearCVMotionModel3D,
-                                               double, NoiseKey, uint64_t);
+                                               double, RigKeys, uint64_t);
 template hipError_t launch_fit<float>(hipStream_t, const FitArgs&, Planes<float>, Planes<float>);
 template hipError_t launch_fit<double>(hipStream_t, const FitArgs&, Planes<double>, Planes<double>);
 template hipError_t launch_fit_fix<float>(hipStream_t, const FitArgs&, Planes<float>, Planes<float>);

@@ -37,6 +37,7 @@ EXPORTED = [
     "scphd_batch_now", "scphd_rig_footprint", "scphd_batch_add_rig", "scphd_batch_remove_rig",
     "scphd_batch_submit", "scphd_batch_step", "scphd_batch_collect", "scphd_batch_get_stats",
     "scphd_batch_filter", "scphd_mat_read_measurements", "scphd_write_mat", "scphd_batch_set_output",
+    "scphd_create_ensemble", "scphd_update_ensemble", "scphd_get_rig_estimates", "scphd_get_rig_resample",
 ]
 
 
@@ -170,6 +171,10 @@ def lib():
                                                dp]),
         "scphd_write_mat": (_I32, [_P, ctypes.c_char_p]),
         "scphd_batch_set_output": (_I32, [_P, _I32, ctypes.c_char_p]),
+        "scphd_create_ensemble": (_I32, [pp, _I32, ctypes.POINTER(_U64), ctypes.POINTER(_P)]),
+        "scphd_update_ensemble": (_I32, [_P, dp, dp, ip, _I32]),
+        "scphd_get_rig_estimates": (_I32, [_P, dp, dp, dp]),
+        "scphd_get_rig_resample": (_I32, [_P, ip, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -384,6 +389,56 @@ class Filter:
         b = _I64(0)
         self._check(lib().scphd_get_memory(self.h, ctypes.byref(b)), "get_memory")
         return b.value
+
+
+class Ensemble(Filter):
+    """n_rigs rigs of one configuration stepped by the same kernel launches
+    (scphd_create_ensemble).  Rig r evolves as Filter(params with seed
+    seeds[r]) fed the same measurements; its parents are rows
+    [r * n_particles, (r + 1) * n_particles) of parents() and maps()."""
+
+    def __init__(self, params, seeds):
+        L = lib()
+        self.params = Params.from_dict(params) if isinstance(params, dict) else params
+        seeds = np.ascontiguousarray([int(x) & 0xFFFFFFFFFFFFFFFF for x in seeds], dtype=np.uint64)
+        h = _P()
+        rc = L.scphd_create_ensemble(ctypes.byref(self.params), len(seeds),
+                                     seeds.ctypes.data_as(ctypes.POINTER(_U64)), ctypes.byref(h))
+        if rc != SCPHD_OK:
+            raise ScphdError(rc, "scphd_create_ensemble failed")
+        self.h = h
+        self.n_rigs = len(seeds)
+        self.npr = self.params.n_particles
+        self.Np = self.n_rigs * self.npr
+        self.S = self.params.particles_per_feature
+
+    def update(self, us, vs, fixed):
+        """us[r], vs[r]: rig r's measurement set."""
+        if len(us) != self.n_rigs or len(vs) != self.n_rigs:
+            raise ValueError("one measurement set per rig")
+        moff = np.zeros(self.n_rigs + 1, dtype=np.int32)
+        moff[1:] = np.cumsum([len(x) for x in us])
+        u = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64) for x in us] + [np.zeros(1)]))
+        v = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64) for x in vs] + [np.zeros(1)]))
+        if any(len(a) != len(b) for a, b in zip(us, vs)):
+            raise ValueError("u and v differ in length")
+        self._check(lib().scphd_update_ensemble(self.h, _dp(u), _dp(v), _ip(moff), int(bool(fixed))),
+                    "update_ensemble")
+
+    def rig_estimates(self):
+        """Per rig: mean (n_rigs, 12), covariance (n_rigs, 12, 12), cardinality (n_rigs)."""
+        m = np.zeros((self.n_rigs, 12))
+        c = np.zeros((self.n_rigs, 12, 12))
+        card = np.zeros(self.n_rigs)
+        self._check(lib().scphd_get_rig_estimates(self.h, _dp(m), _dp(c), _dp(card)), "get_rig_estimates")
+        return m, c, card
+
+    def rig_resample(self):
+        """Per rig: resampled at the last resample() call, and n_eff."""
+        r = np.zeros(self.n_rigs, dtype=np.int32)
+        ne = np.zeros(self.n_rigs)
+        self._check(lib().scphd_get_rig_resample(self.h, _ip(r), _dp(ne)), "get_rig_resample")
+        return r.astype(bool), ne
 
 
 def rig_footprint(params, gaussians_per_parent):

@@ -161,6 +161,47 @@ def run_plain(rigs, progress=None):
                 pairs=pairs, latency_ms=dict(p50=_pct(lat, 50), p99=_pct(lat, 99), max=float(lat.max())))
 
 
+def run_ensemble(rigs, progress=None):
+    """The same rigs stepped together as one ensemble (scphd_create_ensemble):
+    the rigs share rigs[0]'s configuration apart from the seed and have equal
+    frame counts; an ensemble frame is one frame of every rig (predict,
+    update, NaN check, resample, every rig's estimate).  Latency is per
+    ensemble frame."""
+    p0 = dict(rigs[0]["p"])
+    for r in rigs:
+        q = dict(r["p"])
+        q["seed"] = p0["seed"]
+        if q != p0 or len(r["frames"]) != len(rigs[0]["frames"]) or r["s"]["dt"] != rigs[0]["s"]["dt"]:
+            raise ValueError("an ensemble needs one configuration and one frame sequence length")
+    e = scphd_amd.Ensemble(p0, [r["p"]["seed"] for r in rigs])
+    dt = rigs[0]["s"]["dt"]
+    lat, pairs, t_all = [], 0, 0.0
+    try:
+        for k in range(len(rigs[0]["frames"])):
+            us = [r["frames"][k][0] for r in rigs]
+            vs = [r["frames"][k][1] for r in rigs]
+            fixed = rigs[0]["frames"][k][2]
+            t0 = time.perf_counter()
+            e.predict(dt)
+            e.update(us, vs, fixed)
+            st = e.update_stats()
+            if e.check() != 0:
+                raise RuntimeError("nan weight")
+            e.resample()
+            e.rig_estimates()
+            t1 = time.perf_counter()
+            lat.append(t1 - t0)
+            pairs += st["n_pairs"]
+            t_all += t1 - t0
+            if progress and (k + 1) % 20 == 0:
+                progress(f"ensemble frames {k + 1}/{len(rigs[0]['frames'])}")
+    finally:
+        e.close()
+    lat = np.array(lat) * 1e3
+    return dict(rigs=len(rigs), frames=len(lat), wall_s=t_all, updates_per_s=pairs / t_all if t_all > 0 else 0.0,
+                pairs=pairs, latency_ms=dict(p50=_pct(lat, 50), p99=_pct(lat, 99), max=float(lat.max())))
+
+
 def offered_load(rigs, step_ms_per_gaussian_sample):
     """Rough device time the arrivals ask for per second of wall time, from a
     measured cost per (Gaussian x sample) of one step."""

@@ -1,0 +1,119 @@
+"""Ensembles (scphd_create_ensemble): many rigs of one configuration stepped
+by the same kernel launches.
+
+Each rig of an ensemble must evolve bit for bit as a plain filter created
+with that rig's seed and fed the same measurement sets: parents, weights,
+maps, samples, per-rig estimates and per-rig resampling decisions.  The
+rigs see different scenes, so their measurement counts differ every frame,
+and one rig gets an empty set on one frame (M = 0 beside M > 0).
+"""
+import numpy as np
+import pytest
+
+import scene
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+CFG = dict(n_particles=64, n_landmarks=25, lam=8.0, f=500.0, baseline=0.2, samples=64)
+SEEDS = [101, 202, 303, 404]
+FRAMES = 6
+EMPTY = (2, 3)  # rig 2 sees no measurement on frame 3
+
+
+def frames():
+    """Per rig: params and the measurement sets (u, v) of every frame; the
+    fixed-camera flag of rig 0's scene for every rig."""
+    rigs = []
+    for r, sd in enumerate(SEEDS):
+        p, s = workloads.rig(seed=sd, n_frames=FRAMES, scene_rig=r, **CFG)
+        sc = scene.generate(seed=sd, **s)
+        sets = []
+        for k in range(FRAMES):
+            u, v, fixed = scene.frame_measurements(sc, k)
+            if (r, k) == EMPTY:
+                u, v = np.zeros(0), np.zeros(0)
+            sets.append((np.asarray(u, dtype=np.float64), np.asarray(v, dtype=np.float64), fixed))
+        rigs.append((p, s, sets))
+    fixed = [rigs[0][2][k][2] for k in range(FRAMES)]
+    return rigs, fixed
+
+
+def run_alone(p, dt, sets, fixed):
+    import scphd_amd
+
+    f = scphd_amd.Filter(p)
+    est, rs = [], []
+    try:
+        for k, (u, v, _) in enumerate(sets):
+            f.predict(dt)
+            f.update(u, v, fixed[k])
+            assert f.check() == 0
+            rs.append(f.resample())
+            m, c, card = f.estimate()
+            est.append(np.concatenate([m, c.ravel(), [card]]))
+        return f.state(), np.array(est), rs
+    finally:
+        f.close()
+
+
+def test_ensemble_rigs_match_plain_filters_bit_for_bit():
+    import scphd_amd
+
+    rigs, fixed = frames()
+    dt = rigs[0][1]["dt"]
+    ref = [run_alone(p, dt, sets, fixed) for p, _, sets in rigs]
+
+    e = scphd_amd.Ensemble(rigs[0][0], SEEDS)
+    try:
+        est = [[] for _ in SEEDS]
+        rs = [[] for _ in SEEDS]
+        for k in range(FRAMES):
+            e.predict(dt)
+            e.update([r[2][k][0] for r in rigs], [r[2][k][1] for r in rigs], fixed[k])
+            assert e.check() == 0
+            e.resample()
+            flags, neff = e.rig_resample()
+            assert np.all(np.isfinite(neff))
+            m, c, card = e.rig_estimates()
+            for r in range(len(SEEDS)):
+                rs[r].append(bool(flags[r]))
+                est[r].append(np.concatenate([m[r], c[r].ravel(), [card[r]]]))
+        st = e.state()
+    finally:
+        e.close()
+
+    n = CFG["n_particles"]
+    S = CFG["samples"]
+    g0 = 0
+    for r, (rst, rest, rrs) in enumerate(ref):
+        sl = slice(r * n, (r + 1) * n)
+        np.testing.assert_array_equal(st["states"][sl], rst["states"], err_msg=f"rig {r} states")
+        np.testing.assert_array_equal(st["weights"][sl], rst["weights"], err_msg=f"rig {r} weights")
+        np.testing.assert_array_equal(st["gcount"][sl], rst["gcount"], err_msg=f"rig {r} gcount")
+        ng = int(rst["gcount"].sum())
+        np.testing.assert_array_equal(st["wg"][g0:g0 + ng], rst["wg"], err_msg=f"rig {r} map weights")
+        np.testing.assert_array_equal(st["samples"][:, g0 * S:(g0 + ng) * S], rst["samples"],
+                                      err_msg=f"rig {r} samples")
+        np.testing.assert_array_equal(np.array(est[r]), rest, err_msg=f"rig {r} estimates")
+        assert rs[r] == rrs, f"rig {r} resampling decisions"
+        g0 += ng
+    assert g0 == int(st["gcount"].sum())
+
+
+def test_ensemble_refuses_plain_updates_and_bad_offsets():
+    import scphd_amd
+
+    p, _ = workloads.rig(seed=1, **CFG)
+    e = scphd_amd.Ensemble(p, [1, 2])
+    try:
+        with pytest.raises(scphd_amd.ScphdError):
+            scphd_amd.Filter.update(e, np.zeros(3), np.zeros(3), True)
+        bad = np.array([0, 5, 3], dtype=np.int32)
+        u = np.zeros(5)
+        rc = scphd_amd.lib().scphd_update_ensemble(e.h, scphd_amd._dp(u), scphd_amd._dp(u), scphd_amd._ip(bad), 1)
+        assert scphd_amd.ERRORS.get(rc) == "SCPHD_ERR_ARG"
+        with pytest.raises(scphd_amd.ScphdError):
+            e.rig_resample()  # no resampling step yet
+    finally:
+        e.close()
