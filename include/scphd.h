@@ -232,20 +232,23 @@ int scphd_debug_normals(uint64_t seed, uint32_t stream, uint64_t step, uint32_t 
  * load_disparity_results.m:27-34 (exp_traj, cov_traj, cardinality). */
 int scphd_get_estimate(scphd_t* f, double* mean12, double* cov144, double* cardinality);
 
-/* ---- ensembles: many rigs of one configuration in one filter ----
+/* ---- ensembles: many rigs in one filter ----
  * No counterpart in the reference, which runs one rig per process
- * (src/main.cu:52-263).  An ensemble of n_rigs rigs, each of
- * p->n_particles parents, is stepped by the same kernel launches as one
- * filter of n_rigs * n_particles parents: rig r draws every noise stream
- * from the key of seeds[r] with its parents numbered from 0, normalises and
- * resamples its own parents, and reads its own measurement set, so each rig
- * evolves exactly as a plain filter created with seed seeds[r] and fed the
- * same measurements.  Parents of rig r are [r n_particles, (r + 1)
- * n_particles) in scphd_get_parents, scphd_get_maps and
- * scphd_set_state.  scphd_predict, scphd_resample (*resampled: any rig),
- * scphd_check and the getters take an ensemble handle; scphd_update,
- * scphd_update_device and scphd_set_noise refuse it. */
-int scphd_create_ensemble(const scphd_params* p, int32_t n_rigs, const uint64_t* seeds, scphd_t** out);
+ * (src/main.cu:52-263).  An ensemble of n_rigs rigs, rig r configured by
+ * rigs[r], is stepped by the same kernel launches as one filter of
+ * sum_r rigs[r].n_particles parents: rig r draws every noise stream from
+ * the key of rigs[r].seed with its parents numbered from 0, projects with
+ * its own camera model, normalises and resamples its own parents, and
+ * reads its own measurement set, so each rig evolves exactly as the plain
+ * filter scphd_create(&rigs[r]) fed the same measurements.  Rigs may differ
+ * in n_particles, the prior bounds, fx fy u0 v0, pd, lambda, the birth
+ * model and the seed; every other field must equal rigs[0]'s
+ * (SCPHD_ERR_ARG otherwise).  Rig r's parents follow those of rigs 0 .. r-1
+ * in scphd_get_parents, scphd_get_maps and scphd_set_state.
+ * scphd_predict, scphd_resample (*resampled: any rig), scphd_check and the
+ * getters take an ensemble handle; scphd_update, scphd_update_device and
+ * scphd_set_noise refuse it. */
+int scphd_create_ensemble(const scphd_params* rigs, int32_t n_rigs, scphd_t** out);
 
 /* Update every rig: rig r's measurements are u[moff[r] .. moff[r + 1]) and
  * v[...] (host arrays; moff has n_rigs + 1 entries, moff[0] = 0, never

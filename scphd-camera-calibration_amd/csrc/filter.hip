@@ -121,14 +121,20 @@ struct scphd_filter {
     BirthModel birth;
     NoiseKey key;
     int Np = 0, S = 0;
-    /* ensemble: n_rigs rigs of one configuration with npr parents each
-     * (Np = n_rigs npr), stepped by the same launches; rig r draws its
-     * noise from dkeys[r] and reads its measurements at [moff[r], moff[r+1]) */
+    /* ensemble: n_rigs rigs stepped by the same launches; rig r owns
+     * parents [poff[r], poff[r+1]) (prig: the rig of each parent), draws its
+     * noise from dkeys[r], projects with dmeas[r] and reads its measurements
+     * at [moff[r], moff[r+1]) */
     bool ensemble = false;
-    int n_rigs = 1, npr = 0;
+    int n_rigs = 1;
+    std::vector<scphd_params> rig_prm;
+    std::vector<DisparityMeasurementModel> rig_meas;
+    std::vector<int32_t> hpoff;
     DBuf<NoiseKey> dkeys;
-    DBuf<int32_t> dmoff, drflag;
+    DBuf<DisparityMeasurementModel> dmeas;
+    DBuf<int32_t> dpoff, dprig, dmoff, drflag;
     DBuf<double> dbpcs;
+    DBuf<BirthModel> dbirths; /* per rig; one for a plain filter */
     std::vector<int32_t> hmoff;
     std::vector<double> hbpcs;
     int64_t ens_meas = 0;
@@ -192,24 +198,20 @@ int hipfail(scphd_filter* f, hipError_t e, const char* what) {
 RigKeys rig_keys(const scphd_filter* f) {
     RigKeys r;
     r.key = f->key;
-    r.keys = f->ensemble ? f->dkeys.p : nullptr;
-    r.npr = f->npr;
+    if (f->ensemble) {
+        r.keys = f->dkeys.p;
+        r.prig = f->dprig.p;
+        r.poff = f->dpoff.p;
+    }
     return r;
 }
 
-#define HIPTRY(expr)                                              \
-    do {                                                          \
-        hipError_t e__ = (expr);                                  \
-        if (e__ != hipSuccess) return hipfail(f, e__, #expr);     \
-    } while (0)
+DisparityMeasurementModel meas_of(const scphd_params& p) {
+    return DisparityMeasurementModel(p.fx, p.fy, p.u0, p.v0, p.std_u, p.std_v, p.pd, p.lambda);
+}
 
-void set_models(scphd_filter* f) {
-    const scphd_params& p = f->prm;
-    f->models.meas = DisparityMeasurementModel(p.fx, p.fy, p.u0, p.v0, p.std_u, p.std_v, p.pd, p.lambda);
-    f->models.cam = OrientedLinearCVMotionModel3D(p.cam_accel_std[0], p.cam_accel_std[1], p.cam_accel_std[2],
-                                                  p.cam_accel_std[3], p.cam_accel_std[4], p.cam_accel_std[5]);
-    f->models.feat = LinearCVMotionModel3D(p.feat_accel_std[0], p.feat_accel_std[1], p.feat_accel_std[2]);
-    BirthModel& b = f->birth;
+BirthModel birth_of(const scphd_params& p) {
+    BirthModel b;
     b.mean[0] = b.mean[1] = 0.0;
     b.mean[2] = p.birth_d0;
     b.mean[3] = p.birth_vu0;
@@ -224,6 +226,22 @@ void set_models(scphd_filter* f) {
     b.P[pidx(5, 5)] = p.birth_var_vd0;
     b.logw0 = safe_log(p.birth_w0);
     b.pd_ok = is_pd6(b.P) ? 1 : 0;
+    return b;
+}
+
+#define HIPTRY(expr)                                              \
+    do {                                                          \
+        hipError_t e__ = (expr);                                  \
+        if (e__ != hipSuccess) return hipfail(f, e__, #expr);     \
+    } while (0)
+
+void set_models(scphd_filter* f) {
+    const scphd_params& p = f->prm;
+    f->models.meas = meas_of(p);
+    f->models.cam = OrientedLinearCVMotionModel3D(p.cam_accel_std[0], p.cam_accel_std[1], p.cam_accel_std[2],
+                                                  p.cam_accel_std[3], p.cam_accel_std[4], p.cam_accel_std[5]);
+    f->models.feat = LinearCVMotionModel3D(p.feat_accel_std[0], p.feat_accel_std[1], p.feat_accel_std[2]);
+    f->birth = birth_of(p);
     f->key = NoiseKey::from_seed(p.seed);
     /* largest survivor count whose merge workspace fits 60 KB of LDS */
     int cap = 64;
@@ -285,6 +303,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
         fa.Ng = Ng;
         fa.S = S;
         fa.meas = f->models.meas;
+        fa.rig_meas = f->ensemble ? f->dmeas.p : nullptr;
         fa.feat = f->models.feat;
         fa.pending = f->pending ? 1 : 0;
         fa.dt = f->pdt;
@@ -339,11 +358,12 @@ int update_t(scphd_filter* f, int M, int fixed) {
     if (f->ensemble) { /* M is the largest rig's count, the stride of eta, mcnt, mncand and mcand */
         ea.moff = f->dmoff.p;
         ea.bpcs = f->dbpcs.p;
-        ea.npr = f->npr;
-        HIPTRY(launch_pair_count(s, f->goff.p, f->dmoff.p, f->npr, f->n_rigs, (double*)((char*)f->dmap + MAP_PAIRS)));
+        ea.prig = f->dprig.p;
+        HIPTRY(launch_pair_count(s, f->goff.p, f->dmoff.p, f->dpoff.p, f->n_rigs,
+                                 (double*)((char*)f->dmap + MAP_PAIRS)));
     }
     ea.min_weight = P.min_weight;
-    ea.birth = f->birth;
+    ea.births = f->dbirths.p;
     ea.eta = f->eta.p;
     ea.loglik = f->loglik.p;
     ea.nsurv = f->nsurv.p;
@@ -395,7 +415,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
     ma.scap = scap;
     ma.pre = f->pre.p;
     ma.Ng = Ng;
-    ma.birth = f->birth;
+    ma.births = f->dbirths.p;
     ma.min_separation = P.min_separation;
     ma.Np = Np;
     ma.lds_cap = f->lds_cap;
@@ -420,7 +440,8 @@ int update_t(scphd_filter* f, int M, int fixed) {
         ma.scratch = f->mscratch.p;
     }
     HIPTRY(launch_merge(s, ma, merge_lds_bytes(std::min(std::max(maxU, 1), f->lds_cap))));
-    if (!fixed) HIPTRY(launch_parent_weights(s, f->pw.p, f->loglik.p, f->npr, f->n_rigs));
+    if (!fixed)
+        HIPTRY(launch_parent_weights(s, f->pw.p, f->loglik.p, Np, f->ensemble ? f->dpoff.p : nullptr, f->n_rigs));
     record(f, 3);
 
     /* output counts */
@@ -457,6 +478,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
     pa.Kt = Kt;
     pa.S = S;
     pa.meas = f->models.meas;
+    pa.rig_meas = f->ensemble ? f->dmeas.p : nullptr;
     pa.key = rig_keys(f);
     pa.ustep = (uint64_t)f->n_update;
     pa.post_base = Ng;
@@ -544,14 +566,33 @@ extern "C" {
 
 namespace {
 
-/* One filter of n_rigs rigs (n_rigs = 1 and no seeds: the plain filter). */
-int create_filter(const scphd_params* p, int32_t n_rigs, const uint64_t* seeds, scphd_t** out) {
-    if (!p || !out) return SCPHD_ERR_ARG;
+/* Parameters every rig of an ensemble shares: the sample count, storage,
+ * capacity, pruning, merging, resampling, survival, motion models and
+ * measurement noise.  Rigs may differ in parent count, prior, camera
+ * intrinsics, detection probability, clutter rate, birth model and seed. */
+bool same_shared(const scphd_params& a, const scphd_params& b) {
+    bool eq = a.particles_per_feature == b.particles_per_feature && a.max_gaussians == b.max_gaussians &&
+              a.storage == b.storage && a.min_weight == b.min_weight && a.min_separation == b.min_separation &&
+              a.min_neff == b.min_neff && a.ps == b.ps && a.std_u == b.std_u && a.std_v == b.std_v;
+    for (int i = 0; i < 6; ++i) eq = eq && a.cam_accel_std[i] == b.cam_accel_std[i];
+    for (int i = 0; i < 3; ++i) eq = eq && a.feat_accel_std[i] == b.feat_accel_std[i];
+    return eq;
+}
+
+/* One filter of n_rigs rigs; ensemble false: the plain filter of p[0]. */
+int create_filter(const scphd_params* p, int32_t n_rigs, bool ensemble, scphd_t** out) {
+    if (!p || !out || n_rigs <= 0 || (!ensemble && n_rigs != 1)) return SCPHD_ERR_ARG;
     *out = nullptr;
-    if (p->n_particles <= 0 || p->particles_per_feature <= 0 || p->particles_per_feature > 512)
-        return SCPHD_ERR_ARG;
-    if (n_rigs <= 0 || (int64_t)n_rigs * p->n_particles > (int64_t)1 << 30) return SCPHD_ERR_ARG;
-    if (p->storage != SCPHD_STORAGE_FP32 && p->storage != SCPHD_STORAGE_FP64) return SCPHD_ERR_ARG;
+    int64_t Np_total = 0;
+    for (int r = 0; r < n_rigs; ++r) {
+        const scphd_params& q = p[r];
+        if (q.n_particles <= 0 || q.particles_per_feature <= 0 || q.particles_per_feature > 512)
+            return SCPHD_ERR_ARG;
+        if (q.storage != SCPHD_STORAGE_FP32 && q.storage != SCPHD_STORAGE_FP64) return SCPHD_ERR_ARG;
+        if (!same_shared(q, p[0])) return SCPHD_ERR_ARG;
+        Np_total += q.n_particles;
+    }
+    if (Np_total > (int64_t)1 << 30) return SCPHD_ERR_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SCPHD_ERR_NODEVICE;
     /* the device with the most compute units (src/scphdcameracalibration.cu:1013-1024) */
@@ -566,10 +607,9 @@ int create_filter(const scphd_params* p, int32_t n_rigs, const uint64_t* seeds, 
     scphd_filter* f = new scphd_filter();
     f->prm = *p;
     f->dev = best;
-    f->ensemble = seeds != nullptr;
+    f->ensemble = ensemble;
     f->n_rigs = n_rigs;
-    f->npr = p->n_particles;
-    f->Np = n_rigs * p->n_particles;
+    f->Np = (int)Np_total;
     f->S = p->particles_per_feature;
     f->esz = p->storage == SCPHD_STORAGE_FP64 ? 8 : 4;
     set_models(f);
@@ -592,18 +632,43 @@ int create_filter(const scphd_params* p, int32_t n_rigs, const uint64_t* seeds, 
         f->drflag.ensure(n_rigs) || f->dflag.ensure(4) || f->gparent.ensure(1) || f->gblock.ensure(1) ||
         f->gw.ensure(1))
         return bail(SCPHD_ERR_HIP);
+    std::vector<BirthModel> births(n_rigs);
+    f->hpoff.assign(n_rigs + 1, 0);
+    for (int r = 0; r < n_rigs; ++r) {
+        births[r] = birth_of(p[r]);
+        f->hpoff[r + 1] = f->hpoff[r] + p[r].n_particles;
+    }
+    if (f->dbirths.ensure(n_rigs) ||
+        hipMemcpy(f->dbirths.p, births.data(), n_rigs * sizeof(BirthModel), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(SCPHD_ERR_HIP);
     if (f->ensemble) {
+        f->rig_prm.assign(p, p + n_rigs);
         std::vector<NoiseKey> keys(n_rigs);
-        for (int r = 0; r < n_rigs; ++r) keys[r] = NoiseKey::from_seed(seeds[r]);
-        if (f->dkeys.ensure(n_rigs) || f->dmoff.ensure(n_rigs + 1) || f->dbpcs.ensure(n_rigs) ||
-            hipMemcpy(f->dkeys.p, keys.data(), n_rigs * sizeof(NoiseKey), hipMemcpyHostToDevice) != hipSuccess)
+        std::vector<int32_t> prig(Np);
+        for (int r = 0; r < n_rigs; ++r) {
+            keys[r] = NoiseKey::from_seed(p[r].seed);
+            f->rig_meas.push_back(meas_of(p[r]));
+            for (int q = f->hpoff[r]; q < f->hpoff[r + 1]; ++q) prig[q] = r;
+        }
+        if (f->dkeys.ensure(n_rigs) || f->dmeas.ensure(n_rigs) || f->dpoff.ensure(n_rigs + 1) ||
+            f->dprig.ensure(Np) || f->dmoff.ensure(n_rigs + 1) || f->dbpcs.ensure(n_rigs) ||
+            hipMemcpy(f->dkeys.p, keys.data(), n_rigs * sizeof(NoiseKey), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(f->dmeas.p, f->rig_meas.data(), n_rigs * sizeof(DisparityMeasurementModel),
+                      hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(f->dpoff.p, f->hpoff.data(), (n_rigs + 1) * sizeof(int32_t), hipMemcpyHostToDevice) !=
+                hipSuccess ||
+            hipMemcpy(f->dprig.p, prig.data(), Np * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess)
             return bail(SCPHD_ERR_HIP);
     }
     if (hipHostMalloc((void**)&f->hmap, map_bytes(n_rigs), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
         hipHostGetDevicePointer((void**)&f->dmap, f->hmap, 0) != hipSuccess)
         return bail(SCPHD_ERR_HIP);
-    if (launch_prior(f->stream, f->ext.p, f->pw.p, Np, p->prior_a, p->prior_b, rig_keys(f)) != hipSuccess)
-        return bail(SCPHD_ERR_HIP);
+    for (int r = 0; r < n_rigs; ++r) { /* each rig's prior, as a filter of its own draws it */
+        const int64_t P0 = f->hpoff[r];
+        if (launch_prior(f->stream, f->ext.p + 12 * P0, f->pw.p + P0, p[r].n_particles, p[r].prior_a, p[r].prior_b,
+                         NoiseKey::from_seed(p[r].seed)) != hipSuccess)
+            return bail(SCPHD_ERR_HIP);
+    }
     if (hipMemsetAsync(f->goff.p, 0, (size_t)(Np + 1) * sizeof(int32_t), f->stream) != hipSuccess)
         return bail(SCPHD_ERR_HIP);
     if (launch_iota(f->stream, f->anc.p, Np) != hipSuccess) return bail(SCPHD_ERR_HIP);
@@ -616,11 +681,10 @@ int create_filter(const scphd_params* p, int32_t n_rigs, const uint64_t* seeds, 
 
 extern "C" {
 
-int scphd_create(const scphd_params* p, scphd_t** out) { return create_filter(p, 1, nullptr, out); }
+int scphd_create(const scphd_params* p, scphd_t** out) { return create_filter(p, 1, false, out); }
 
-int scphd_create_ensemble(const scphd_params* p, int32_t n_rigs, const uint64_t* seeds, scphd_t** out) {
-    if (!seeds) return SCPHD_ERR_ARG;
-    return create_filter(p, n_rigs, seeds, out);
+int scphd_create_ensemble(const scphd_params* rigs, int32_t n_rigs, scphd_t** out) {
+    return create_filter(rigs, n_rigs, true, out);
 }
 
 void scphd_destroy(scphd_t* f) {
@@ -635,9 +699,11 @@ void scphd_destroy(scphd_t* f) {
     DBuf<int32_t>* ibs[] = {&f->anc, &f->newcnt, &f->goff, &f->gparent, &f->gblock, &f->goff2, &f->gparent2,
                             &f->gblock2, &f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->ndfill, &f->nsurv, &f->survoff,
                             &f->nmerged, &f->mergeoff, &f->outcnt, &f->scantmp, &f->sg, &f->sm, &f->dflag, &f->mcnt, &f->mncand, &f->mcand,
-                            &f->ngroups, &f->glead, &f->gsta, &f->gend, &f->gmem, &f->dmoff, &f->drflag};
+                            &f->ngroups, &f->glead, &f->gsta, &f->gend, &f->gmem, &f->dmoff, &f->drflag, &f->dpoff, &f->dprig};
     for (auto* b : ibs) b->release();
     f->dkeys.release();
+    f->dmeas.release();
+    f->dbirths.release();
     f->mscratch.release();
     f->gen[0].release();
     f->gen[1].release();
@@ -710,8 +776,8 @@ int scphd_update_ensemble(scphd_t* f, const double* u, const double* v, const in
     HIPTRY(hipSetDevice(f->dev));
     f->hmoff.assign(moff, moff + R + 1);
     f->hbpcs.resize(R);
-    const double kappa = f->models.meas.kappa();
-    for (int r = 0; r < R; ++r) f->hbpcs[r] = safe_log((moff[r + 1] - moff[r]) * f->prm.birth_w0 + kappa);
+    for (int r = 0; r < R; ++r)
+        f->hbpcs[r] = safe_log((moff[r + 1] - moff[r]) * f->rig_prm[r].birth_w0 + f->rig_meas[r].kappa());
     f->ens_meas = total;
     HIPTRY(f->du.ensure(std::max(total, 1)));
     HIPTRY(f->dv.ensure(std::max(total, 1)));
@@ -740,7 +806,8 @@ int scphd_resample(scphd_t* f, int32_t* resampled) {
     ra.anc = f->anc.p;
     ra.newcnt = f->newcnt.p;
     ra.goff = f->goff.p;
-    ra.Np = f->npr;
+    ra.Np = Np;
+    ra.poff = f->ensemble ? f->dpoff.p : nullptr;
     ra.nrigs = f->n_rigs;
     ra.min_neff = f->prm.min_neff;
     ra.key = f->key;
@@ -982,7 +1049,7 @@ int scphd_set_noise(scphd_t* f, int32_t mode, uint64_t seed, const double* injec
 int scphd_get_estimate(scphd_t* f, double* mean12, double* cov144, double* cardinality) {
     if (check_handle(f)) return SCPHD_ERR_ARG;
     HIPTRY(hipSetDevice(f->dev));
-    HIPTRY(launch_estimate(f->stream, f->ext.p, f->pw.p, f->goff.p, f->gw.p, f->Np, 1,
+    HIPTRY(launch_estimate(f->stream, f->ext.p, f->pw.p, f->goff.p, f->gw.p, f->Np, nullptr, 1,
                            (double*)((char*)f->dmap + MAP_EST)));
     HIPTRY(hipStreamSynchronize(f->stream));
     const double* h = (const double*)((const char*)f->hmap + MAP_EST);
@@ -995,7 +1062,8 @@ int scphd_get_estimate(scphd_t* f, double* mean12, double* cov144, double* cardi
 int scphd_get_rig_estimates(scphd_t* f, double* mean12, double* cov144, double* cardinality) {
     if (check_handle(f)) return SCPHD_ERR_ARG;
     HIPTRY(hipSetDevice(f->dev));
-    HIPTRY(launch_estimate(f->stream, f->ext.p, f->pw.p, f->goff.p, f->gw.p, f->npr, f->n_rigs,
+    HIPTRY(launch_estimate(f->stream, f->ext.p, f->pw.p, f->goff.p, f->gw.p, f->Np,
+                           f->ensemble ? f->dpoff.p : nullptr, f->n_rigs,
                            (double*)((char*)f->dmap + MAP_EST)));
     HIPTRY(hipStreamSynchronize(f->stream));
     const double* h = (const double*)((const char*)f->hmap + MAP_EST);
@@ -1028,9 +1096,10 @@ int scphd_get_memory(scphd_t* f, int64_t* bytes) {
     DBuf<int32_t>* ibs[] = {&f->anc, &f->newcnt, &f->goff, &f->gparent, &f->gblock, &f->goff2, &f->gparent2,
                             &f->gblock2, &f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->ndfill, &f->nsurv, &f->survoff,
                             &f->nmerged, &f->mergeoff, &f->outcnt, &f->scantmp, &f->sg, &f->sm, &f->dflag, &f->mcnt, &f->mncand, &f->mcand,
-                            &f->ngroups, &f->glead, &f->gsta, &f->gend, &f->gmem, &f->dmoff, &f->drflag};
+                            &f->ngroups, &f->glead, &f->gsta, &f->gend, &f->gmem, &f->dmoff, &f->drflag, &f->dpoff, &f->dprig};
     for (auto* d : ibs) b += d->n * (int64_t)sizeof(int32_t);
-    b += f->dkeys.n * (int64_t)sizeof(NoiseKey);
+    b += f->dkeys.n * (int64_t)sizeof(NoiseKey) + f->dmeas.n * (int64_t)sizeof(DisparityMeasurementModel) +
+         f->dbirths.n * (int64_t)sizeof(BirthModel);
     b += f->mscratch.n;
     for (const auto& g : f->gen) b += g.cap_blocks * (int64_t)f->S * (int64_t)f->esz * 7;
     *bytes = b;

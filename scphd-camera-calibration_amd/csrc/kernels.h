@@ -26,20 +26,22 @@
 namespace scphd {
 
 /* Noise keys: one rig (keys == nullptr: every parent uses `key`), or an
- * ensemble of rigs with npr parents each, parent p belonging to rig
- * p / npr, whose key it uses with its index inside the rig, so that every
- * rig draws exactly the streams of a filter of its own. */
+ * ensemble of rigs, rig r owning parents [poff[r], poff[r + 1]) and
+ * prig[p] the rig of parent p; a parent draws with its rig's key and its
+ * index inside the rig, so that every rig draws exactly the streams of a
+ * filter of its own. */
 struct RigKeys {
     NoiseKey key;
-    const NoiseKey* keys;
-    int npr;
+    const NoiseKey* keys = nullptr;
+    const int32_t* prig = nullptr;
+    const int32_t* poff = nullptr;
     __device__ NoiseKey of(int p, int& pl) const {
         if (!keys) {
             pl = p;
             return key;
         }
-        const int r = p / npr;
-        pl = p - r * npr;
+        const int r = prig[p];
+        pl = p - poff[r];
         return keys[r];
     }
 };
@@ -79,8 +81,9 @@ struct BirthModel {
 hipError_t launch_parent_predict(hipStream_t s, double* ext, int Np, OrientedLinearCVMotionModel3D cam,
                                  double dt, RigKeys key, uint64_t step);
 hipError_t launch_scale(hipStream_t s, double* x, int64_t n, double a);
+/* one rig's Np parents (weights 1 / Np) */
 hipError_t launch_prior(hipStream_t s, double* ext, double* pw, int Np, const double* a12, const double* b12,
-                        RigKeys key);
+                        NoiseKey key);
 hipError_t launch_pose(hipStream_t s, const double* ext, double* pose, int Np, int fixed);
 
 /* Apply a deferred sample prediction, writing every table entry to its own
@@ -100,6 +103,7 @@ struct FitArgs {
     int64_t Ng;
     int S;
     DisparityMeasurementModel meas;
+    const DisparityMeasurementModel* rig_meas = nullptr; /* ensemble: per rig (rigs may differ), else null */
     LinearCVMotionModel3D feat;
     int pending;
     double dt;
@@ -141,9 +145,9 @@ struct EtaArgs {
     double bpc;        /* log(M w0 + kappa) */
     const int32_t* moff = nullptr; /* ensemble: rig r's measurements at [moff[r], moff[r + 1]); null for one rig */
     const double* bpcs = nullptr;  /* ensemble: bpc per rig */
-    int npr = 1;                   /* ensemble: parents per rig */
+    const int32_t* prig = nullptr; /* ensemble: the rig of each parent */
     double min_weight;
-    BirthModel birth;
+    const BirthModel* births; /* per rig (one for a plain filter), device memory */
     double* eta;       /* Np*M */
     double* loglik;    /* Np   */
     int32_t* nsurv;    /* Np   */
@@ -160,7 +164,7 @@ struct SurvArgs {
     const double* fit;
     const int32_t* survoff; /* Np+1 */
     double* sw;
-    int32_t* sg;            /* table index, or -1 - m for a birth term */
+    int32_t* sg;            /* table index, or -1 - rig for a birth term */
     int32_t* sm;
     double* smu;            /* 6 planes, stride cap */
     int64_t cap;
@@ -177,7 +181,7 @@ struct MergeArgs {
     int64_t scap;
     const double* pre; /* PP_PN planes of the table entries */
     int64_t Ng;
-    BirthModel birth;
+    const BirthModel* births; /* per rig, device memory: birth terms' covariance */
     double min_separation;
     int Np;
     int32_t lds_cap;   /* parents with more survivors use global scratch */
@@ -206,23 +210,25 @@ hipError_t launch_readout(hipStream_t s, const int32_t* off, int n, int32_t limi
                           const int32_t* x1, int32_t* out);
 /* *out = the number of NaN entries of w[0, n) */
 hipError_t launch_nan_count(hipStream_t s, const double* w, int n, int32_t* out);
-/* *out = sum over rigs r of (goff[(r+1) npr] - goff[r npr]) (moff[r+1] - moff[r]) */
-hipError_t launch_pair_count(hipStream_t s, const int32_t* goff, const int32_t* moff, int npr, int nrigs,
-                             double* out);
+/* *out = sum over rigs r of (goff[poff[r+1]] - goff[poff[r]]) (moff[r+1] - moff[r]) */
+hipError_t launch_pair_count(hipStream_t s, const int32_t* goff, const int32_t* moff, const int32_t* poff,
+                             int nrigs, double* out);
 
-/* per rig of npr parents, nrigs rigs (one workgroup each) */
-hipError_t launch_parent_weights(hipStream_t s, double* pw, const double* loglik, int npr, int nrigs);
+/* Per-rig launches: one workgroup per rig, rig r's parents [poff[r],
+ * poff[r + 1]); poff == nullptr: one rig of Np parents. */
+hipError_t launch_parent_weights(hipStream_t s, double* pw, const double* loglik, int Np, const int32_t* poff,
+                                 int nrigs);
 
 /* out[6 n]: six normals of every element 0..n-1 of one noise stream */
 hipError_t launch_debug_normals(hipStream_t s, NoiseKey key, uint32_t stream, uint64_t step, uint32_t parent,
                                 uint32_t gauss, int n, double* out);
 
-/* out[157]: weighted parent mean (12), covariance (144, row-major), summed
- * Gaussian weight of the first highest-weight parent */
-/* per rig of Np parents (nrigs rigs): EST_DOUBLES = mean[12], cov[144], cardinality */
+/* per rig, EST_DOUBLES at out + EST_DOUBLES r: weighted parent mean (12),
+ * covariance (144, row-major), summed Gaussian weight of the first
+ * highest-weight parent */
 static constexpr int EST_DOUBLES = 157;
 hipError_t launch_estimate(hipStream_t s, const double* ext, const double* pw, const int32_t* goff,
-                           const double* gw, int Np, int nrigs, double* out);
+                           const double* gw, int Np, const int32_t* poff, int nrigs, double* out);
 
 struct SampleArgs {
     const int32_t* mergeoff; /* Np+1 */
@@ -235,6 +241,7 @@ struct SampleArgs {
     int64_t Kt;
     int S;
     DisparityMeasurementModel meas;
+    const DisparityMeasurementModel* rig_meas = nullptr; /* ensemble: per rig, else null */
     RigKeys key;
     uint64_t ustep;
     int64_t post_base;
@@ -277,7 +284,8 @@ struct ResampleArgs {
     int32_t* anc;
     int32_t* newcnt;
     const int32_t* goff;
-    int Np;                /* parents per rig */
+    int Np;                /* parents of the one rig when poff is null */
+    const int32_t* poff = nullptr; /* ensemble: rig r's parents [poff[r], poff[r + 1]) */
     int nrigs = 1;         /* one workgroup per rig */
     double min_neff;
     NoiseKey key;          /* one rig */

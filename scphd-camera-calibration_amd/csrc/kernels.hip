@@ -124,18 +124,16 @@ struct Bounds12 {
 
 /* Uniform prior over [a, b] per extrinsic component, weights 1/N
  * (src/scphdcameracalibration.cu:835-980). */
-__global__ void k_prior(double* ext, double* pw, int Np, Bounds12 ab, RigKeys rk) {
+__global__ void k_prior(double* ext, double* pw, int Np, Bounds12 ab, NoiseKey key) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= Np) return;
-    int pl;
-    const NoiseKey key = rk.of(p, pl);
     for (int c = 0; c < 12; ++c) {
         uint32_t x[4];
-        noise_block(key, NS_PRIOR, 0, (uint32_t)pl, 0, 0, (uint32_t)(c / 2), x);
+        noise_block(key, NS_PRIOR, 0, (uint32_t)p, 0, 0, (uint32_t)(c / 2), x);
         const double r = (c % 2 == 0) ? uniform_co(x[0], x[1]) : uniform_co(x[2], x[3]);
         ext[(int64_t)p * 12 + c] = r * (ab.b[c] - ab.a[c]) + ab.a[c];
     }
-    pw[p] = 1.0 / rk.npr;
+    pw[p] = 1.0 / Np;
 }
 
 /* Parent extrinsics through OrientedLinearCVMotionModel3D::computeNoisyMotion
@@ -253,14 +251,15 @@ __device__ inline void load_sample_off(const Planes<T>& src, Off off, T r[7]) {
 }
 
 template <typename T, typename Off>
-__device__ inline void write_nd_off(const FitArgs& a, Planes<T>& dst, Off off, const EuclideanPoint& x, bool in) {
+__device__ inline void write_nd_off(const DisparityMeasurementModel& meas, Planes<T>& dst, Off off,
+                                    const EuclideanPoint& x, bool in) {
     at(dst.p[0], off) = (T)x.x;
     at(dst.p[1], off) = (T)x.y;
     at(dst.p[2], off) = (T)x.z;
     at(dst.p[3], off) = (T)x.vx;
     at(dst.p[4], off) = (T)x.vy;
     at(dst.p[5], off) = (T)x.vz;
-    at(dst.p[6], off) = (T)(1.0 - (in ? a.meas.pd() : 0.0));
+    at(dst.p[6], off) = (T)(1.0 - (in ? meas.pd() : 0.0));
 }
 
 template <typename T>
@@ -270,22 +269,23 @@ __device__ inline void load_sample(const Planes<T>& src, int64_t si, T r[7]) {
 }
 
 template <typename T>
-__device__ inline void eval_sample(const FitArgs& a, const T r[7], NoiseKey kk, int pl, int k, int s,
-                                   const CameraPose& pose, EuclideanPoint& x, DisparityPoint& d, bool& in, double& w) {
+__device__ inline void eval_sample(const FitArgs& a, const DisparityMeasurementModel& meas, const T r[7], NoiseKey kk,
+                                   int pl, int k, int s, const CameraPose& pose, EuclideanPoint& x, DisparityPoint& d,
+                                   bool& in, double& w) {
     x = EuclideanPoint{(double)r[0], (double)r[1], (double)r[2], (double)r[3], (double)r[4], (double)r[5]};
     w = (double)r[6];
     if (a.pending) x = predict_sample(x, a.feat, a.dt, kk, a.pstep, (uint32_t)pl, (uint32_t)k, (uint32_t)s);
-    d = disparity_fast(a.meas, pose, x);
-    in = a.meas.inRange(d);
+    d = disparity_fast(meas, pose, x);
+    in = meas.inRange(d);
 }
 
 template <typename T>
-__device__ inline void fit_sample(const FitArgs& a, const Planes<T>& src, int64_t si, NoiseKey kk, int pl, int k,
-                                  int s, const CameraPose& pose, EuclideanPoint& x, DisparityPoint& d, bool& in,
-                                  double& w) {
+__device__ inline void fit_sample(const FitArgs& a, const DisparityMeasurementModel& meas, const Planes<T>& src,
+                                  int64_t si, NoiseKey kk, int pl, int k, int s, const CameraPose& pose,
+                                  EuclideanPoint& x, DisparityPoint& d, bool& in, double& w) {
     T r[7];
     load_sample(src, si, r);
-    eval_sample(a, r, kk, pl, k, s, pose, x, d, in, w);
+    eval_sample(a, meas, r, kk, pl, k, s, pose, x, d, in, w);
 }
 
 __device__ inline void write_fit(const FitArgs& a, int64_t g, int lane, const double* mu, const double* P, double pdg) {
@@ -329,12 +329,31 @@ __device__ inline void write_fit(const FitArgs& a, int64_t g, int lane, const do
  * normalises to weight 1 per sample, is listed in redo[] and refitted by
  * k_fit_fix when pd_g > 0; with pd_g = 0 its fit cannot reach any output and
  * is the first sample with zero covariance. */
+/* The measurement model of parent p's rig, `m`: the kernel argument itself
+ * for one rig (RG false, the model stays in scalar registers), else a copy
+ * of the rig's entry of per_rig. */
+template <bool RG>
+struct RowMeas;
+template <>
+struct RowMeas<false> {
+    const DisparityMeasurementModel& m;
+    __device__ RowMeas(const DisparityMeasurementModel& one, const DisparityMeasurementModel*, const RigKeys&, int)
+        : m(one) {}
+};
+template <>
+struct RowMeas<true> {
+    DisparityMeasurementModel m;
+    __device__ RowMeas(const DisparityMeasurementModel&, const DisparityMeasurementModel* per_rig, const RigKeys& rk,
+                       int p)
+        : m(per_rig[rk.prig[p]]) {}
+};
+
 /* waves per SIMD the fit kernel is compiled for (register budget 512 / n) */
 #ifndef SCPHD_FIT_WAVES
 #define SCPHD_FIT_WAVES 2
 #endif
 
-template <typename T, typename Off>
+template <typename T, typename Off, bool RG>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_WAVES, SCPHD_FIT_WAVES))) void k_fit(
     FitArgs a, Planes<T> src, Planes<T> dst) {
     /* the rows' camera poses are staged through LDS once and then held in
@@ -355,10 +374,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
     const int k = (int)(g - a.goff[p]);
     int pl;
     const NoiseKey kk = a.key.of(p, pl); /* the rig's key, p's index in its rig */
+    const RowMeas<RG> rm(a.meas, a.rig_meas, a.key, p);
+    const DisparityMeasurementModel& meas = rm.m;
     const int64_t b0 = (int64_t)a.gblock[g] * a.S;
     const int S = a.S;
     const double wg = a.gw[g];
-    const double Pd = a.meas.pd();
+    const double Pd = meas.pd();
     const bool nd_sure = wg * (1.0 - Pd) >= a.min_weight; /* kept for certain: cloud written in the sweep */
     const int64_t d0 = (a.nd_base + g) * (int64_t)S;
 
@@ -393,7 +414,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
         DisparityPoint d;
         bool in = false;
         double w = 0.0;
-        if (valid) eval_sample(a, cur, kk, pl, k, s, pose, x, d, in, w);
+        if (valid) eval_sample(a, meas, cur, kk, pl, k, s, pose, x, d, in, w);
         const double q[6] = {d.u, d.v, d.d, d.vu, d.vv, d.vd};
         if (s0 == 0) {
             /* shift: the Gaussian's first sample, broadcast along the row */
@@ -414,7 +435,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
         for (int r = 0; r < 6; ++r)
 #pragma unroll
             for (int c = 0; c <= r; ++c) m2[pidx(r, c)] += wp * dv[r] * dv[c];
-        if (nd_sure && valid) write_nd_off(a, dst, dof, x, in);
+        if (nd_sure && valid) write_nd_off(meas, dst, dof, x, in);
         dof += (Off)(16 * sizeof(T));
     };
     const std::integral_constant<bool, true> full;
@@ -475,7 +496,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
  * weights 1 - pd_i, computed by the same code as in the sweep.  One 16-lane
  * row per listed term; workgroups stride over the list (its length is read
  * on the device). */
-template <typename T, typename Off>
+template <typename T, typename Off, bool RG>
 __global__ __launch_bounds__(256) void k_nd_fill(FitArgs a, Planes<T> src, Planes<T> dst) {
     const int lane = threadIdx.x & 15;
     const int n = *a.ndfill_n;
@@ -486,6 +507,8 @@ __global__ __launch_bounds__(256) void k_nd_fill(FitArgs a, Planes<T> src, Plane
         const int k = (int)(g - a.goff[p]);
         int pl;
         const NoiseKey kk = a.key.of(p, pl);
+        const RowMeas<RG> rm(a.meas, a.rig_meas, a.key, p);
+        const DisparityMeasurementModel& meas = rm.m;
         const int64_t b0 = (int64_t)a.gblock[g] * a.S;
         const int64_t d0 = (a.nd_base + g) * (int64_t)a.S;
         const CameraPose pose = load_pose(a.pose, p);
@@ -505,8 +528,8 @@ __global__ __launch_bounds__(256) void k_nd_fill(FitArgs a, Planes<T> src, Plane
                     DisparityPoint d;
                     bool in;
                     double w;
-                    eval_sample(a, r[j], kk, pl, k, s, pose, x, d, in, w);
-                    write_nd_off(a, dst, (Off)((d0 + s) * (int64_t)sizeof(T)), x, in);
+                    eval_sample(a, meas, r[j], kk, pl, k, s, pose, x, d, in, w);
+                    write_nd_off(meas, dst, (Off)((d0 + s) * (int64_t)sizeof(T)), x, in);
                 }
             }
         }
@@ -518,7 +541,7 @@ __global__ __launch_bounds__(256) void k_nd_fill(FitArgs a, Planes<T> src, Plane
  * of the samples, as the reference's normalisation of an all-zero cloud
  * gives).  Workgroups stride over the list (its length is read on the
  * device). */
-template <typename T>
+template <typename T, bool RG>
 __global__ __launch_bounds__(256) void k_fit_fix(FitArgs a, Planes<T> src, Planes<T> dst) {
     const int lane = threadIdx.x & 15;
     const int nredo = *a.redo_n;
@@ -529,6 +552,8 @@ __global__ __launch_bounds__(256) void k_fit_fix(FitArgs a, Planes<T> src, Plane
         const int k = (int)(g - a.goff[p]);
         int pl;
         const NoiseKey kk = a.key.of(p, pl);
+        const RowMeas<RG> rm(a.meas, a.rig_meas, a.key, p);
+        const DisparityMeasurementModel& meas = rm.m;
         const int64_t b0 = (int64_t)a.gblock[g] * S;
         const CameraPose pose = load_pose(a.pose, p);
         double sq[6] = {0, 0, 0, 0, 0, 0};
@@ -539,7 +564,7 @@ __global__ __launch_bounds__(256) void k_fit_fix(FitArgs a, Planes<T> src, Plane
                 DisparityPoint d;
                 bool in;
                 double w;
-                fit_sample(a, src, b0 + s, kk, pl, k, s, pose, x, d, in, w);
+                fit_sample(a, meas, src, b0 + s, kk, pl, k, s, pose, x, d, in, w);
                 sq[0] += d.u; sq[1] += d.v; sq[2] += d.d; sq[3] += d.vu; sq[4] += d.vv; sq[5] += d.vd;
             }
         }
@@ -555,7 +580,7 @@ __global__ __launch_bounds__(256) void k_fit_fix(FitArgs a, Planes<T> src, Plane
                 DisparityPoint d;
                 bool in;
                 double w;
-                fit_sample(a, src, b0 + s, kk, pl, k, s, pose, x, d, in, w);
+                fit_sample(a, meas, src, b0 + s, kk, pl, k, s, pose, x, d, in, w);
                 const double dv[6] = {d.u - mu[0], d.v - mu[1], d.d - mu[2], d.vu - mu[3], d.vv - mu[4], d.vd - mu[5]};
 #pragma unroll
                 for (int r = 0; r < 6; ++r)
@@ -668,10 +693,12 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
     double eta_sum = 0.0;
     int cnt = 0;
     /* this parent's measurement set: its rig's in an ensemble */
-    const int rig = a.moff ? p / a.npr : 0;
+    const int rig = a.moff ? a.prig[p] : 0;
     const int M = a.moff ? a.moff[rig + 1] - a.moff[rig] : a.M;
     const int64_t zb = a.moff ? a.moff[rig] : 0;
     const double bpc = a.bpcs ? a.bpcs[rig] : a.bpc;
+    const double logw0 = a.births[rig].logw0;
+    const bool birth_ok = a.births[rig].pd_ok != 0;
     const int rounds = (M + NT - 1) / NT;
     /* Only a Gaussian with l > bpc can move the maximum off bpc, and
      * l <= C - log 2 pi - H (the value at dist = 0; the computed quadratic
@@ -785,8 +812,8 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
             a.mncand[pm] = nc;
             a.mcnt[pm] = mcnt;
             cnt += mcnt;
-            const double w = exp(a.birth.logw0 - eta);
-            cnt += (w >= a.min_weight && a.birth.pd_ok) ? 1 : 0;
+            const double w = exp(logw0 - eta);
+            cnt += (w >= a.min_weight && birth_ok) ? 1 : 0;
         }
     }
     double swg = 0.0;
@@ -814,7 +841,8 @@ __global__ __launch_bounds__(NT) void k_survivors(SurvArgs a) {
     const int64_t base = a.survoff[p];
     const int cap = a.survoff[p + 1] - a.survoff[p];
     int run = 0;
-    const int rig = e.moff ? p / e.npr : 0;
+    const int rig = e.moff ? e.prig[p] : 0;
+    const BirthModel& birth = e.births[rig];
     const int M = e.moff ? e.moff[rig + 1] - e.moff[rig] : e.M;
     const int64_t zb = e.moff ? e.moff[rig] : 0;
     const int rounds = (M + NT - 1) / NT;
@@ -872,19 +900,19 @@ __global__ __launch_bounds__(NT) void k_survivors(SurvArgs a) {
         const int m = r * NT + t;
         const bool valid = m < M;
         const double eta = valid ? e.eta[(int64_t)p * e.M + m] : 0.0;
-        const double w = valid ? exp(e.birth.logw0 - eta) : 0.0;
-        const bool ok = valid && w >= e.min_weight && e.birth.pd_ok;
+        const double w = valid ? exp(birth.logw0 - eta) : 0.0;
+        const bool ok = valid && w >= e.min_weight && birth.pd_ok;
         int tot;
         const int o = run + block_exscan(ok ? 1 : 0, sh, &tot);
         if (ok && o < cap) {
             const int64_t so = base + o;
             a.sw[so] = w;
-            a.sg[so] = -1 - m;
+            a.sg[so] = -1 - rig;
             a.sm[so] = m;
             a.smu[0 * a.cap + so] = e.zu[zb + m];
             a.smu[1 * a.cap + so] = e.zv[zb + m];
 #pragma unroll
-            for (int c = 2; c < 6; ++c) a.smu[c * a.cap + so] = e.birth.mean[c];
+            for (int c = 2; c < 6; ++c) a.smu[c * a.cap + so] = birth.mean[c];
         }
         run += tot;
     }
@@ -940,14 +968,11 @@ int64_t merge_scratch_bytes(int64_t Stot, int Np) {
 }
 
 /* Packed covariance entry e of a survivor's term: the updated covariance of
- * table entry g, or the birth covariance (g < 0).  The birth entry passes
- * through an empty asm so that it is a value, not a load: a choice between
- * two loads is otherwise folded into one load from a chosen address, global
- * or kernel-argument memory, which compiles to a flat load. */
+ * table entry g, or (g = -1 - r < 0) the birth covariance of rig r.  Both
+ * live in global memory, so the choice is one global load from a chosen
+ * address. */
 __device__ inline double term_P(const MergeArgs& a, int g, int e) {
-    double b = a.birth.P[e];
-    asm volatile("" : "+v"(b));
-    return g >= 0 ? a.pre[(int64_t)(PP_PN + e) * a.Ng + g] : b;
+    return g >= 0 ? a.pre[(int64_t)(PP_PN + e) * a.Ng + g] : a.births[-1 - g].P[e];
 }
 
 __device__ inline void survivor_P(const MergeArgs& a, int64_t si, double* P) {
@@ -1572,10 +1597,14 @@ __global__ __launch_bounds__(NT) void k_merge_rounds(MergeArgs a) {
 /* w_p <- w_p exp(loglik_p), normalised, in the log domain with a max shift
  * (src/scphdcameracalibration.cu:1851-1875 multiplies in the linear
  * domain).  One workgroup. */
-__global__ __launch_bounds__(1024) void k_parent_weights(double* pw, const double* loglik, int Np) {
+__global__ __launch_bounds__(1024) void k_parent_weights(double* pw, const double* loglik, int Np,
+                                                         const int32_t* poff) {
     __shared__ double red[16];
-    pw += (int64_t)blockIdx.x * Np; /* one workgroup per rig of Np parents */
-    loglik += (int64_t)blockIdx.x * Np;
+    if (poff) { /* one workgroup per rig */
+        pw += poff[blockIdx.x];
+        loglik += poff[blockIdx.x];
+        Np = poff[blockIdx.x + 1] - poff[blockIdx.x];
+    }
     double mx = -INFINITY;
     for (int p = threadIdx.x; p < Np; p += blockDim.x) mx = fmax(mx, safe_log(pw[p]) + loglik[p]);
     mx = block_max(mx, red);
@@ -1622,10 +1651,11 @@ __device__ inline void block_sum_n(double (&v)[N], double* sh) {
  * load_disparity_results.m:27-34).  out per rig: mean[12], cov[144]
  * row-major, cardinality (EST_DOUBLES). */
 __global__ __launch_bounds__(256) void k_estimate(const double* ext, const double* pw, const int32_t* goff,
-                                                   const double* gw, int Np, double* out) {
+                                                   const double* gw, int Np, const int32_t* poff, double* out) {
     __shared__ double sh[4 * 78];
-    {
-        const int64_t P0 = (int64_t)blockIdx.x * Np;
+    if (poff) { /* one workgroup per rig */
+        const int64_t P0 = poff[blockIdx.x];
+        Np = poff[blockIdx.x + 1] - poff[blockIdx.x];
         ext += P0 * 12;
         pw += P0;
         goff += P0;
@@ -1709,7 +1739,7 @@ __global__ __launch_bounds__(256) void k_estimate(const double* ext, const doubl
  * src/kernels.cu:49-80), sample weight 1/S (:2009-2010).  Destination
  * addresses are byte offsets from the plane bases (32-bit when every plane
  * is below 4 GiB). */
-template <typename T, typename Off>
+template <typename T, typename Off, bool RG>
 __global__ __launch_bounds__(256) void k_sample(SampleArgs a, Planes<T> dst) {
     const int lane = threadIdx.x & 15;
     const int64_t gm = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
@@ -1718,6 +1748,8 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a, Planes<T> dst) {
     const int k = (int)(gm - a.mergeoff[p]);
     int pl;
     const NoiseKey key = a.key.of(p, pl);
+    const RowMeas<RG> rm(a.meas, a.rig_meas, a.key, p);
+    const DisparityMeasurementModel& meas = rm.m;
     const int64_t rec = (int64_t)a.survoff[p] + k;
     double mu[6], P[21], L[21];
 #pragma unroll
@@ -1740,7 +1772,7 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a, Planes<T> dst) {
             q[r] = acc + mu[r];
         }
         const DisparityPoint d{q[0], q[1], q[2], q[3], q[4], q[5]};
-        const EuclideanPoint x = world_from_disparity_fast(a.meas, pose, d);
+        const EuclideanPoint x = world_from_disparity_fast(meas, pose, d);
         at(dst.p[0], off) = (T)x.x;
         at(dst.p[1], off) = (T)x.y;
         at(dst.p[2], off) = (T)x.z;
@@ -1803,8 +1835,8 @@ __global__ __launch_bounds__(1024) void k_resample(ResampleArgs a) {
     __shared__ int s_am[16];
     /* one workgroup per rig: its Np parents from P0, ancestors within it */
     const int rig = blockIdx.x;
-    const int N = a.Np;
-    const int64_t P0 = (int64_t)rig * N;
+    const int N = a.poff ? a.poff[rig + 1] - a.poff[rig] : a.Np;
+    const int64_t P0 = a.poff ? a.poff[rig] : 0;
     a.pw += P0;
     a.ext += P0 * 12;
     a.ext_tmp += P0 * 12;
@@ -1999,7 +2031,7 @@ static inline unsigned nblocks(int64_t n, int t) { return (unsigned)((n + t - 1)
 #define LAUNCH_CHECK() return hipGetLastError()
 
 hipError_t launch_prior(hipStream_t s, double* ext, double* pw, int Np, const double* a12, const double* b12,
-                        RigKeys key) {
+                        NoiseKey key) {
     Bounds12 ab;
     for (int i = 0; i < 12; ++i) {
         ab.a[i] = a12[i];
@@ -2039,17 +2071,27 @@ hipError_t launch_materialize(hipStream_t s, Planes<T> src, Planes<T> dst, const
 template <typename T>
 hipError_t launch_fit(hipStream_t s, const FitArgs& a, Planes<T> src, Planes<T> dst) {
     if (a.Ng <= 0) return hipSuccess;
-    if (a.off32)
-        k_fit<T, uint32_t><<<nblocks(a.Ng, 16), 256, 0, s>>>(a, src, dst);
+    const unsigned nb = nblocks(a.Ng, 16);
+    if (a.rig_meas) {
+        if (a.off32)
+            k_fit<T, uint32_t, true><<<nb, 256, 0, s>>>(a, src, dst);
+        else
+            k_fit<T, uint64_t, true><<<nb, 256, 0, s>>>(a, src, dst);
+    } else if (a.off32)
+        k_fit<T, uint32_t, false><<<nb, 256, 0, s>>>(a, src, dst);
     else
-        k_fit<T, uint64_t><<<nblocks(a.Ng, 16), 256, 0, s>>>(a, src, dst);
+        k_fit<T, uint64_t, false><<<nb, 256, 0, s>>>(a, src, dst);
     LAUNCH_CHECK();
 }
 
 template <typename T>
 hipError_t launch_fit_fix(hipStream_t s, const FitArgs& a, Planes<T> src, Planes<T> dst) {
     if (a.Ng <= 0) return hipSuccess;
-    k_fit_fix<T><<<std::min<int64_t>(nblocks(a.Ng, 16), 256), 256, 0, s>>>(a, src, dst);
+    const unsigned nb = (unsigned)std::min<int64_t>(nblocks(a.Ng, 16), 256);
+    if (a.rig_meas)
+        k_fit_fix<T, true><<<nb, 256, 0, s>>>(a, src, dst);
+    else
+        k_fit_fix<T, false><<<nb, 256, 0, s>>>(a, src, dst);
     LAUNCH_CHECK();
 }
 
@@ -2057,10 +2099,15 @@ template <typename T>
 hipError_t launch_nd_fill(hipStream_t s, const FitArgs& a, Planes<T> src, Planes<T> dst) {
     if (a.Ng <= 0) return hipSuccess;
     const int64_t blocks = std::min<int64_t>(nblocks(a.Ng, 16), 2048); /* the list is at most Ng long */
-    if (a.off32)
-        k_nd_fill<T, uint32_t><<<blocks, 256, 0, s>>>(a, src, dst);
+    if (a.rig_meas) {
+        if (a.off32)
+            k_nd_fill<T, uint32_t, true><<<blocks, 256, 0, s>>>(a, src, dst);
+        else
+            k_nd_fill<T, uint64_t, true><<<blocks, 256, 0, s>>>(a, src, dst);
+    } else if (a.off32)
+        k_nd_fill<T, uint32_t, false><<<blocks, 256, 0, s>>>(a, src, dst);
     else
-        k_nd_fill<T, uint64_t><<<blocks, 256, 0, s>>>(a, src, dst);
+        k_nd_fill<T, uint64_t, false><<<blocks, 256, 0, s>>>(a, src, dst);
     LAUNCH_CHECK();
 }
 
@@ -2090,8 +2137,9 @@ hipError_t launch_merge(hipStream_t s, const MergeArgs& a, int lds_bytes) {
     LAUNCH_CHECK();
 }
 
-hipError_t launch_parent_weights(hipStream_t s, double* pw, const double* loglik, int npr, int nrigs) {
-    k_parent_weights<<<nrigs, 1024, 0, s>>>(pw, loglik, npr);
+hipError_t launch_parent_weights(hipStream_t s, double* pw, const double* loglik, int Np, const int32_t* poff,
+                                 int nrigs) {
+    k_parent_weights<<<poff ? nrigs : 1, 1024, 0, s>>>(pw, loglik, Np, poff);
     LAUNCH_CHECK();
 }
 
@@ -2103,9 +2151,9 @@ hipError_t launch_debug_normals(hipStream_t s, NoiseKey key, uint32_t stream, ui
 }
 
 hipError_t launch_estimate(hipStream_t s, const double* ext, const double* pw, const int32_t* goff,
-                           const double* gw, int Np, int nrigs, double* out) {
-    if (Np <= 0 || nrigs <= 0) return hipErrorInvalidValue;
-    k_estimate<<<nrigs, 256, 0, s>>>(ext, pw, goff, gw, Np, out);
+                           const double* gw, int Np, const int32_t* poff, int nrigs, double* out) {
+    if (poff ? nrigs <= 0 : Np <= 0) return hipErrorInvalidValue;
+    k_estimate<<<poff ? nrigs : 1, 256, 0, s>>>(ext, pw, goff, gw, Np, poff, out);
     LAUNCH_CHECK();
 }
 
@@ -2113,10 +2161,15 @@ template <typename T>
 hipError_t launch_sample(hipStream_t s, const SampleArgs& a, Planes<T> dst) {
     if (a.Kt <= 0) return hipSuccess;
     const unsigned nb = nblocks(a.Kt, 16);
-    if (a.off32)
-        k_sample<T, uint32_t><<<nb, 256, 0, s>>>(a, dst);
+    if (a.rig_meas) {
+        if (a.off32)
+            k_sample<T, uint32_t, true><<<nb, 256, 0, s>>>(a, dst);
+        else
+            k_sample<T, uint64_t, true><<<nb, 256, 0, s>>>(a, dst);
+    } else if (a.off32)
+        k_sample<T, uint32_t, false><<<nb, 256, 0, s>>>(a, dst);
     else
-        k_sample<T, uint64_t><<<nb, 256, 0, s>>>(a, dst);
+        k_sample<T, uint64_t, false><<<nb, 256, 0, s>>>(a, dst);
     LAUNCH_CHECK();
 }
 
@@ -2139,7 +2192,7 @@ hipError_t launch_table(hipStream_t s, const TableArgs& a) {
 }
 
 hipError_t launch_resample(hipStream_t s, const ResampleArgs& a) {
-    k_resample<<<a.nrigs, 1024, 0, s>>>(a);
+    k_resample<<<a.poff ? a.nrigs : 1, 1024, 0, s>>>(a);
     LAUNCH_CHECK();
 }
 
@@ -2200,19 +2253,19 @@ __global__ __launch_bounds__(1024) void k_nan_count(const double* w, int n, vola
 /* Detection pairs of an ensemble update: the sum over rigs of the rig's
  * map size times its measurement count (exact in FP64 below 2^53).  One
  * workgroup. */
-__global__ __launch_bounds__(256) void k_pair_count(const int32_t* goff, const int32_t* moff, int npr, int nrigs,
-                                                    volatile double* out) {
+__global__ __launch_bounds__(256) void k_pair_count(const int32_t* goff, const int32_t* moff, const int32_t* poff,
+                                                    int nrigs, volatile double* out) {
     __shared__ double red[16];
     double c = 0.0;
     for (int r = threadIdx.x; r < nrigs; r += blockDim.x)
-        c += (double)(goff[(int64_t)(r + 1) * npr] - goff[(int64_t)r * npr]) * (double)(moff[r + 1] - moff[r]);
+        c += (double)(goff[poff[r + 1]] - goff[poff[r]]) * (double)(moff[r + 1] - moff[r]);
     c = block_sum(c, red);
     if (threadIdx.x == 0) *out = c;
 }
 
-hipError_t launch_pair_count(hipStream_t s, const int32_t* goff, const int32_t* moff, int npr, int nrigs,
-                             double* out) {
-    k_pair_count<<<1, 256, 0, s>>>(goff, moff, npr, nrigs, out);
+hipError_t launch_pair_count(hipStream_t s, const int32_t* goff, const int32_t* moff, const int32_t* poff,
+                             int nrigs, double* out) {
+    k_pair_count<<<1, 256, 0, s>>>(goff, moff, poff, nrigs, out);
     LAUNCH_CHECK();
 }
 

@@ -2,14 +2,23 @@
 """Batched ragged workload: many independent rigs stepped together on one GPU.
 
     python run_c2.py [--rigs 64] [--frames 100] [--workers 8] [--plain-rigs N]
+                     [--mode batch|ensemble|both] [--ensemble-gb 200]
 
 Rigs as workloads.c2_rigs: parents uniform in {256, ..., 4096}, landmarks
 uniform in [20, 300], clutter lambda uniform in [5, 80], baseline uniform in
 [0.1, 0.5] m, f uniform in [400, 800] px, S = 100, seeds from SplitMix64 of
-a master seed.  Every rig's frames are queued at once; rigs are admitted as
-the HBM budget allows (scphd_batch_*), and a rig leaves when its frames are
-done.  Prints one JSON line: component updates/s of the batch, of the plain
-path on the first --plain-rigs rigs run one at a time, and the ratio.
+a master seed.
+
+* batch: every rig's frames are queued at once; rigs are admitted as the HBM
+  budget allows (scphd_batch_*), and a rig leaves when its frames are done.
+* ensemble: the rigs are packed, in order, into groups whose summed
+  footprint (scphd_rig_footprint at the expected map size, x 1.5) fits
+  --ensemble-gb; each group is one ensemble (scphd_create_ensemble) whose
+  frames step every rig of the group with one launch per kernel; the groups
+  run one after the other.
+
+Prints one JSON line: component updates/s of each mode, of the plain path on
+the first --plain-rigs rigs run one at a time, and the ratios.
 """
 import argparse
 import json
@@ -33,6 +42,8 @@ def main(argv=None):
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--plain-rigs", type=int, default=8)
     ap.add_argument("--master", type=int, default=1)
+    ap.add_argument("--mode", default="both", choices=["batch", "ensemble", "both"])
+    ap.add_argument("--ensemble-gb", type=float, default=200.0)
     args = ap.parse_args(argv)
 
     rigs = []
@@ -41,16 +52,41 @@ def main(argv=None):
         rigs.append(dict(t=0.0, p=p, s=s, frames=[scene.frame_measurements(sc, k) for k in range(s["n_frames"])]))
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     t0 = time.perf_counter()
-    bat = serve.run_served(rigs, n_workers=args.workers, paced=False, progress=log)
-    plain = serve.run_plain(rigs[:args.plain_rigs], progress=log)
     out = dict(workload=f"c2: {args.rigs} ragged rigs x {args.frames} frames, S = 100, FP32 storage",
-               parents=[r["p"]["n_particles"] for r in rigs],
-               batched=dict(updates_per_s=bat["updates_per_s"], wall_s=bat["wall_s"], frames=bat["frames"],
-                            peak_active_rigs=bat["peak_active_rigs"], peak_reserved_gb=bat["peak_reserved_gb"],
-                            peak_used_gb=bat["peak_used_gb"], refused_admissions=bat["refused_admissions"]),
-               plain=dict(rigs=plain["rigs"], updates_per_s=plain["updates_per_s"], latency_ms=plain["latency_ms"]),
-               ratio=bat["updates_per_s"] / plain["updates_per_s"] if plain["updates_per_s"] else None,
-               total_s=time.perf_counter() - t0)
+               parents=[r["p"]["n_particles"] for r in rigs])
+    if args.mode in ("batch", "both"):
+        bat = serve.run_served(rigs, n_workers=args.workers, paced=False, progress=log)
+        out["batched"] = dict(updates_per_s=bat["updates_per_s"], wall_s=bat["wall_s"], frames=bat["frames"],
+                              peak_active_rigs=bat["peak_active_rigs"], peak_reserved_gb=bat["peak_reserved_gb"],
+                              peak_used_gb=bat["peak_used_gb"], refused_admissions=bat["refused_admissions"])
+        print(json.dumps(out), flush=True)
+    if args.mode in ("ensemble", "both"):
+        groups, cur, used = [], [], 0.0
+        for r in rigs:
+            need = 1.5 * scphd_amd.rig_footprint(r["p"], serve.expected_gaussians_per_parent(r["s"])) / 1e9
+            if cur and used + need > args.ensemble_gb:
+                groups.append(cur)
+                cur, used = [], 0.0
+            cur.append(r)
+            used += need
+        if cur:
+            groups.append(cur)
+        pairs, wall, lat = 0, 0.0, []
+        for i, g in enumerate(groups):
+            e = serve.run_ensemble(g, progress=log)
+            pairs += e["pairs"]
+            wall += e["wall_s"]
+            lat.append(e["latency_ms"])
+            log(f"ensemble group {i + 1}/{len(groups)}: {len(g)} rigs, {e['updates_per_s']:.3g} updates/s")
+        out["ensemble"] = dict(updates_per_s=pairs / wall if wall > 0 else 0.0, wall_s=wall,
+                               groups=[len(g) for g in groups], group_latency_ms=lat)
+        print(json.dumps(out), flush=True)
+    plain = serve.run_plain(rigs[:args.plain_rigs], progress=log)
+    out["plain"] = dict(rigs=plain["rigs"], updates_per_s=plain["updates_per_s"], latency_ms=plain["latency_ms"])
+    for k in ("batched", "ensemble"):
+        if k in out and plain["updates_per_s"]:
+            out[f"ratio_{k}"] = out[k]["updates_per_s"] / plain["updates_per_s"]
+    out["total_s"] = time.perf_counter() - t0
     print(json.dumps(out))
     return 0
 

@@ -171,7 +171,7 @@ def lib():
                                                dp]),
         "scphd_write_mat": (_I32, [_P, ctypes.c_char_p]),
         "scphd_batch_set_output": (_I32, [_P, _I32, ctypes.c_char_p]),
-        "scphd_create_ensemble": (_I32, [pp, _I32, ctypes.POINTER(_U64), ctypes.POINTER(_P)]),
+        "scphd_create_ensemble": (_I32, [pp, _I32, ctypes.POINTER(_P)]),
         "scphd_update_ensemble": (_I32, [_P, dp, dp, ip, _I32]),
         "scphd_get_rig_estimates": (_I32, [_P, dp, dp, dp]),
         "scphd_get_rig_resample": (_I32, [_P, ip, dp]),
@@ -392,24 +392,24 @@ class Filter:
 
 
 class Ensemble(Filter):
-    """n_rigs rigs of one configuration stepped by the same kernel launches
-    (scphd_create_ensemble).  Rig r evolves as Filter(params with seed
-    seeds[r]) fed the same measurements; its parents are rows
-    [r * n_particles, (r + 1) * n_particles) of parents() and maps()."""
+    """Rigs stepped by the same kernel launches (scphd_create_ensemble).
+    Rig r evolves as Filter(rigs[r]) fed the same measurements; its parents
+    follow those of rigs 0 .. r-1 in parents() and maps()."""
 
-    def __init__(self, params, seeds):
+    def __init__(self, rigs):
         L = lib()
-        self.params = Params.from_dict(params) if isinstance(params, dict) else params
-        seeds = np.ascontiguousarray([int(x) & 0xFFFFFFFFFFFFFFFF for x in seeds], dtype=np.uint64)
+        self.rig_params = [Params.from_dict(p) if isinstance(p, dict) else p for p in rigs]
+        arr = (Params * len(rigs))(*self.rig_params)
         h = _P()
-        rc = L.scphd_create_ensemble(ctypes.byref(self.params), len(seeds),
-                                     seeds.ctypes.data_as(ctypes.POINTER(_U64)), ctypes.byref(h))
+        rc = L.scphd_create_ensemble(arr, len(rigs), ctypes.byref(h))
         if rc != SCPHD_OK:
             raise ScphdError(rc, "scphd_create_ensemble failed")
         self.h = h
-        self.n_rigs = len(seeds)
-        self.npr = self.params.n_particles
-        self.Np = self.n_rigs * self.npr
+        self.params = self.rig_params[0]
+        self.n_rigs = len(rigs)
+        self.rig_np = [p.n_particles for p in self.rig_params]
+        self.poff = np.concatenate([[0], np.cumsum(self.rig_np)]).astype(int)
+        self.Np = int(self.poff[-1])
         self.S = self.params.particles_per_feature
 
     def update(self, us, vs, fixed):

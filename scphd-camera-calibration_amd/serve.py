@@ -163,17 +163,14 @@ def run_plain(rigs, progress=None):
 
 def run_ensemble(rigs, progress=None):
     """The same rigs stepped together as one ensemble (scphd_create_ensemble):
-    the rigs share rigs[0]'s configuration apart from the seed and have equal
-    frame counts; an ensemble frame is one frame of every rig (predict,
-    update, NaN check, resample, every rig's estimate).  Latency is per
-    ensemble frame."""
-    p0 = dict(rigs[0]["p"])
+    the rigs share the sample count, pruning, merging and motion settings
+    (the library checks) and the frame sequence length and dt; an ensemble
+    frame is one frame of every rig (predict, update, NaN check, resample,
+    every rig's estimate).  Latency is per ensemble frame."""
     for r in rigs:
-        q = dict(r["p"])
-        q["seed"] = p0["seed"]
-        if q != p0 or len(r["frames"]) != len(rigs[0]["frames"]) or r["s"]["dt"] != rigs[0]["s"]["dt"]:
-            raise ValueError("an ensemble needs one configuration and one frame sequence length")
-    e = scphd_amd.Ensemble(p0, [r["p"]["seed"] for r in rigs])
+        if len(r["frames"]) != len(rigs[0]["frames"]) or r["s"]["dt"] != rigs[0]["s"]["dt"]:
+            raise ValueError("an ensemble needs one frame sequence length and one dt")
+    e = scphd_amd.Ensemble([r["p"] for r in rigs])
     dt = rigs[0]["s"]["dt"]
     lat, pairs, t_all = [], 0, 0.0
     try:

@@ -1,11 +1,13 @@
-"""Ensembles (scphd_create_ensemble): many rigs of one configuration stepped
-by the same kernel launches.
+"""Ensembles (scphd_create_ensemble): many rigs stepped by the same kernel
+launches.
 
 Each rig of an ensemble must evolve bit for bit as a plain filter created
-with that rig's seed and fed the same measurement sets: parents, weights,
-maps, samples, per-rig estimates and per-rig resampling decisions.  The
-rigs see different scenes, so their measurement counts differ every frame,
-and one rig gets an empty set on one frame (M = 0 beside M > 0).
+with that rig's parameters and fed the same measurement sets: parents,
+weights, maps, samples, per-rig estimates and per-rig resampling decisions.
+The rigs are ragged: their parent counts, focal lengths, baselines (prior),
+clutter rates and birth models differ, they see different scenes, so their
+measurement counts differ every frame, and one rig gets an empty set on one
+frame (M = 0 beside M > 0).
 """
 import numpy as np
 import pytest
@@ -15,8 +17,11 @@ import workloads
 
 pytestmark = pytest.mark.gpu
 
-CFG = dict(n_particles=64, n_landmarks=25, lam=8.0, f=500.0, baseline=0.2, samples=64)
-SEEDS = [101, 202, 303, 404]
+SAMPLES = 64
+RIGS = [dict(seed=101, n_particles=64, n_landmarks=25, lam=8.0, f=500.0, baseline=0.2),
+        dict(seed=202, n_particles=96, n_landmarks=40, lam=15.0, f=650.0, baseline=0.3),
+        dict(seed=303, n_particles=32, n_landmarks=15, lam=4.0, f=420.0, baseline=0.15),
+        dict(seed=404, n_particles=48, n_landmarks=30, lam=10.0, f=560.0, baseline=0.25)]
 FRAMES = 6
 EMPTY = (2, 3)  # rig 2 sees no measurement on frame 3
 
@@ -25,9 +30,9 @@ def frames():
     """Per rig: params and the measurement sets (u, v) of every frame; the
     fixed-camera flag of rig 0's scene for every rig."""
     rigs = []
-    for r, sd in enumerate(SEEDS):
-        p, s = workloads.rig(seed=sd, n_frames=FRAMES, scene_rig=r, **CFG)
-        sc = scene.generate(seed=sd, **s)
+    for r, cfg in enumerate(RIGS):
+        p, s = workloads.rig(n_frames=FRAMES, scene_rig=r, samples=SAMPLES, **cfg)
+        sc = scene.generate(seed=cfg["seed"], **s)
         sets = []
         for k in range(FRAMES):
             u, v, fixed = scene.frame_measurements(sc, k)
@@ -64,10 +69,10 @@ def test_ensemble_rigs_match_plain_filters_bit_for_bit():
     dt = rigs[0][1]["dt"]
     ref = [run_alone(p, dt, sets, fixed) for p, _, sets in rigs]
 
-    e = scphd_amd.Ensemble(rigs[0][0], SEEDS)
+    e = scphd_amd.Ensemble([r[0] for r in rigs])
     try:
-        est = [[] for _ in SEEDS]
-        rs = [[] for _ in SEEDS]
+        est = [[] for _ in RIGS]
+        rs = [[] for _ in RIGS]
         for k in range(FRAMES):
             e.predict(dt)
             e.update([r[2][k][0] for r in rigs], [r[2][k][1] for r in rigs], fixed[k])
@@ -76,18 +81,19 @@ def test_ensemble_rigs_match_plain_filters_bit_for_bit():
             flags, neff = e.rig_resample()
             assert np.all(np.isfinite(neff))
             m, c, card = e.rig_estimates()
-            for r in range(len(SEEDS)):
+            for r in range(len(RIGS)):
                 rs[r].append(bool(flags[r]))
                 est[r].append(np.concatenate([m[r], c[r].ravel(), [card[r]]]))
         st = e.state()
     finally:
         e.close()
 
-    n = CFG["n_particles"]
-    S = CFG["samples"]
-    g0 = 0
+    S = SAMPLES
+    g0, p0 = 0, 0
     for r, (rst, rest, rrs) in enumerate(ref):
-        sl = slice(r * n, (r + 1) * n)
+        n = RIGS[r]["n_particles"]
+        sl = slice(p0, p0 + n)
+        p0 += n
         np.testing.assert_array_equal(st["states"][sl], rst["states"], err_msg=f"rig {r} states")
         np.testing.assert_array_equal(st["weights"][sl], rst["weights"], err_msg=f"rig {r} weights")
         np.testing.assert_array_equal(st["gcount"][sl], rst["gcount"], err_msg=f"rig {r} gcount")
@@ -104,8 +110,12 @@ def test_ensemble_rigs_match_plain_filters_bit_for_bit():
 def test_ensemble_refuses_plain_updates_and_bad_offsets():
     import scphd_amd
 
-    p, _ = workloads.rig(seed=1, **CFG)
-    e = scphd_amd.Ensemble(p, [1, 2])
+    p, _ = workloads.rig(samples=SAMPLES, **RIGS[0])
+    q, _ = workloads.rig(samples=SAMPLES, **RIGS[1])
+    bad_rig = dict(q, min_weight=1e-3)  # a shared parameter that differs
+    with pytest.raises(scphd_amd.ScphdError):
+        scphd_amd.Ensemble([p, bad_rig])
+    e = scphd_amd.Ensemble([p, q])
     try:
         with pytest.raises(scphd_amd.ScphdError):
             scphd_amd.Filter.update(e, np.zeros(3), np.zeros(3), True)
