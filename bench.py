@@ -152,6 +152,29 @@ def served_leg(args, seed):
     }
 
 
+def ensemble_leg(args, seed):
+    """Many small ragged rigs (workloads.small_ragged_rigs) stepped as one
+    ensemble, one launch per kernel per frame, against the same rigs run one
+    at a time through the plain path."""
+    import serve  # noqa: E402
+
+    arr = [(0.0, p, s) for p, s in workloads.small_ragged_rigs(master=seed, n_rigs=args.ensemble_rigs,
+                                                               n_frames=args.ensemble_frames)]
+    rigs = serve.prepare(arr)
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    ens = serve.run_ensemble(rigs, progress=log)
+    pl = serve.run_plain(rigs, progress=log)
+    return {
+        "workload": f"{args.ensemble_rigs} ragged rigs x {args.ensemble_frames} frames: parents in {{256, 512}}, "
+                    f"landmarks in [20, 100], lambda in [5, 40], f in [400, 800] px, baseline in [0.1, 0.5] m, "
+                    f"S = 100, FP32 storage",
+        "ensemble": ens,
+        "plain_one_rig_at_a_time": pl,
+        "throughput_ratio_ensemble_over_plain":
+            ens["updates_per_s"] / pl["updates_per_s"] if pl["updates_per_s"] else None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,6 +195,9 @@ def main():
     ap.add_argument("--served-lambda", type=float, nargs=2, default=[5.0, 60.0])
     ap.add_argument("--served-duration", type=float, nargs=2, default=[2.0, 6.0])
     ap.add_argument("--served-workers", type=int, default=8)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-rigs", type=int, default=64)
+    ap.add_argument("--ensemble-frames", type=int, default=40)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X and the filter step "
@@ -293,6 +319,8 @@ def main():
     }
     if not args.no_served:
         out["served"] = served_leg(args, p["seed"])
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_leg(args, p["seed"])
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cpu_v, cpu_s, cpu_gp = cpu_baseline(p, s, args.cpu_parents, args.cpu_warm, args.cpu_timed, threads)

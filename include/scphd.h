@@ -266,6 +266,13 @@ int scphd_get_rig_estimates(scphd_t* f, double* mean12, double* cov144, double* 
  * n_rigs entries each; a plain filter is one rig). */
 int scphd_get_rig_resample(scphd_t* f, int32_t* resampled, double* neff);
 
+/* Grow the device buffers now to hold n_gaussians map entries over all
+ * parents (and as many survivors) and max_meas measurements per rig, so that
+ * steps up to those sizes allocate nothing; without it the buffers grow
+ * geometrically as the maps grow, and the steps that grow them carry the
+ * allocation time.  No counterpart in the reference. */
+int scphd_reserve(scphd_t* f, int64_t n_gaussians, int32_t max_meas);
+
 /* Device memory currently held by the handle, in bytes (buffers grow
  * geometrically and are kept between steps).  No counterpart in the
  * reference. */

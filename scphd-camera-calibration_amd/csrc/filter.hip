@@ -1086,6 +1086,50 @@ int scphd_get_rig_resample(scphd_t* f, int32_t* resampled, double* neff) {
     return SCPHD_OK;
 }
 
+int scphd_reserve(scphd_t* f, int64_t n_gaussians, int32_t max_meas) {
+    if (check_handle(f) || n_gaussians < 0 || max_meas < 0) return SCPHD_ERR_ARG;
+    HIPTRY(hipSetDevice(f->dev));
+    const int64_t G = std::max<int64_t>(n_gaussians, f->Ng);
+    const int64_t Np = f->Np, M = std::max(max_meas, 1);
+    /* the current generation keeps every block (after resampling the table
+     * may reference any of them); the other one holds nothing between calls */
+    Generation& cur = f->gen[f->cur];
+    HIPTRY(cur.ensure(G, cur.cap_blocks, f->S, f->esz, f->stream));
+    HIPTRY(f->gen[1 - f->cur].ensure(G, 0, f->S, f->esz, f->stream));
+    HIPTRY(f->fit.ensure((int64_t)FP_COUNT * G));
+    HIPTRY(f->pre.ensure((int64_t)PP_COUNT * G));
+    HIPTRY(f->lik.ensure((int64_t)LK_COUNT * (G + 3)));
+    HIPTRY(f->keep.ensure(G));
+    HIPTRY(f->keepscan.ensure(G + 1));
+    HIPTRY(f->redo.ensure(G + 1));
+    HIPTRY(f->ndfill.ensure(G + 1));
+    HIPTRY(f->ndlist.ensure(G));
+    HIPTRY(f->scantmp.ensure(scan_tmp_size(std::max<int64_t>(G, Np) + 1) + 16));
+    /* the successor table (the current one holds live entries) */
+    HIPTRY(f->gparent2.ensure(G + 1));
+    HIPTRY(f->gblock2.ensure(G + 1));
+    HIPTRY(f->gw2.ensure(G + 1));
+    /* survivors: about one per Gaussian in steady state */
+    HIPTRY(f->sw.ensure(G + 1));
+    HIPTRY(f->sg.ensure(G + 1));
+    HIPTRY(f->sm.ensure(G + 1));
+    const int64_t scap = f->sw.n;
+    HIPTRY(f->smu.ensure(6 * scap));
+    HIPTRY(f->mw.ensure(scap));
+    HIPTRY(f->mmu.ensure(6 * scap));
+    HIPTRY(f->mP.ensure(21 * scap));
+    for (DBuf<int32_t>* b : {&f->glead, &f->gsta, &f->gend, &f->gmem}) HIPTRY(b->ensure(G + 1));
+    /* per (parent, measurement) and the measurement sets */
+    HIPTRY(f->eta.ensure(Np * M));
+    HIPTRY(f->mcnt.ensure(Np * M));
+    HIPTRY(f->mncand.ensure(Np * M));
+    HIPTRY(f->mcand.ensure(Np * M * ETA_NCAND));
+    HIPTRY(f->du.ensure((int64_t)f->n_rigs * M));
+    HIPTRY(f->dv.ensure((int64_t)f->n_rigs * M));
+    HIPTRY(hipStreamSynchronize(f->stream));
+    return SCPHD_OK;
+}
+
 int scphd_get_memory(scphd_t* f, int64_t* bytes) {
     if (check_handle(f) || !bytes) return SCPHD_ERR_ARG;
     int64_t b = 0;

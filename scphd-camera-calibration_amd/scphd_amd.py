@@ -38,6 +38,7 @@ EXPORTED = [
     "scphd_batch_submit", "scphd_batch_step", "scphd_batch_collect", "scphd_batch_get_stats",
     "scphd_batch_filter", "scphd_mat_read_measurements", "scphd_write_mat", "scphd_batch_set_output",
     "scphd_create_ensemble", "scphd_update_ensemble", "scphd_get_rig_estimates", "scphd_get_rig_resample",
+    "scphd_reserve",
 ]
 
 
@@ -175,6 +176,7 @@ def lib():
         "scphd_update_ensemble": (_I32, [_P, dp, dp, ip, _I32]),
         "scphd_get_rig_estimates": (_I32, [_P, dp, dp, dp]),
         "scphd_get_rig_resample": (_I32, [_P, ip, dp]),
+        "scphd_reserve": (_I32, [_P, _I64, _I32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -384,6 +386,10 @@ class Filter:
     def write_mat(self, path):
         """writeMat: parents and the highest-weight parent's map (scphd_write_mat)."""
         self._check(lib().scphd_write_mat(self.h, path.encode()), "write_mat")
+
+    def reserve(self, n_gaussians, max_meas):
+        """Pre-size the device buffers (scphd_reserve)."""
+        self._check(lib().scphd_reserve(self.h, int(n_gaussians), int(max_meas)), "reserve")
 
     def memory(self):
         b = _I64(0)

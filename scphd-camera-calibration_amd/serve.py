@@ -14,8 +14,11 @@ Per-frame latency = completion time - due time.  Throughput = component
 updates (N_g * M per frame) / wall time.
 
 run_plain() runs the same rigs through the plain path: one rig at a time, its
-frames back to back on one Filter, for the throughput and per-frame latency
-of one rig per launch.
+frames back to back on one Filter (buffers reserved at the expected map size
+first), for the throughput and per-frame latency of one rig per launch.
+
+run_ensemble() steps the same rigs together as one ensemble: one launch per
+kernel per frame for all of them.
 
 Neither loads anything but the library; scene generation happens before the
 clock starts.
@@ -35,6 +38,13 @@ def expected_gaussians_per_parent(s):
     plus 2-3 non-detection clouds, and clutter births survive about one step:
     3.5 * 0.95 * 0.9 * L + lambda."""
     return int(3.5 * 0.95 * 0.9 * s["n_landmarks"] + s["lam"]) + 16
+
+
+def expected_measurements(s):
+    """A measurement count a frame rarely exceeds: the detected visible
+    landmarks (0.95 * 0.9 * L) plus clutter lambda and four standard
+    deviations of it."""
+    return int(0.95 * 0.9 * s["n_landmarks"] + s["lam"] + 4.0 * math.sqrt(s["lam"])) + 8
 
 
 def prepare(arrivals):
@@ -139,6 +149,10 @@ def run_plain(rigs, progress=None):
     lat, pairs, t_all = [], 0, 0.0
     for i, r in enumerate(rigs):
         f = scphd_amd.Filter(r["p"])
+        try:  # steady-state sizes up front, as for an ensemble
+            f.reserve(r["p"]["n_particles"] * expected_gaussians_per_parent(r["s"]), expected_measurements(r["s"]))
+        except scphd_amd.ScphdError:
+            pass
         dt = r["s"]["dt"]
         for u, v, fixed in r["frames"]:
             t0 = time.perf_counter()
@@ -171,6 +185,11 @@ def run_ensemble(rigs, progress=None):
         if len(r["frames"]) != len(rigs[0]["frames"]) or r["s"]["dt"] != rigs[0]["s"]["dt"]:
             raise ValueError("an ensemble needs one frame sequence length and one dt")
     e = scphd_amd.Ensemble([r["p"] for r in rigs])
+    try:  # the maps' steady-state size, allocated before the first frame
+        e.reserve(sum(r["p"]["n_particles"] * expected_gaussians_per_parent(r["s"]) for r in rigs),
+                  max(expected_measurements(r["s"]) for r in rigs))
+    except scphd_amd.ScphdError:
+        pass  # not enough memory to reserve up front: the buffers grow as the maps do
     dt = rigs[0]["s"]["dt"]
     lat, pairs, t_all = [], 0, 0.0
     try:

@@ -120,6 +120,21 @@ def c2_rigs(master=1, n_rigs=64, samples=100, n_frames=100):
     return out
 
 
+def small_ragged_rigs(master=1, n_rigs=64, samples=100, n_frames=40):
+    """The C2 draw at small sizes, for the ensemble leg: parents uniform in
+    {256, 512}, landmarks uniform in [20, 100], clutter lambda uniform in
+    [5, 40], baseline uniform in [0.1, 0.5] m, f uniform in [400, 800] px;
+    seeds and scene draws from SplitMix64(master, r)."""
+    out = []
+    for r in range(n_rigs):
+        sd = _splitmix64(master, r)
+        u = [((_splitmix64(sd, j) >> 11) * 2.0 ** -53) for j in range(5)]
+        out.append(rig(seed=sd, n_particles=[256, 512][min(int(u[0] * 2), 1)], n_landmarks=20 + int(u[1] * 81),
+                       lam=5.0 + 35.0 * u[2], baseline=0.1 + 0.4 * u[3], f=400.0 + 400.0 * u[4], samples=samples,
+                       n_frames=n_frames, scene_rig=r))
+    return out
+
+
 def c4_arrivals(master=1, rate=20.0, horizon=30.0, samples=100, fps=30.0, dur=(2.0, 10.0), parents=(256, 8192),
                 landmarks=(20, 400), lam=(5.0, 100.0)):
     """Served workload: rigs arriving as a Poisson process of `rate` rigs/s

@@ -127,3 +127,29 @@ def test_ensemble_refuses_plain_updates_and_bad_offsets():
             e.rig_resample()  # no resampling step yet
     finally:
         e.close()
+
+
+def test_reserve_changes_no_result():
+    """scphd_reserve pre-sizes the buffers; the steps that follow give the
+    same bits as steps that grow the buffers as they go."""
+    import scphd_amd
+
+    rigs, fixed = frames()
+    p, s, sets = rigs[1]
+    out = []
+    for reserve in (False, True):
+        f = scphd_amd.Filter(p)
+        try:
+            if reserve:
+                before = f.memory()
+                f.reserve(p["n_particles"] * 200, 64)
+                assert f.memory() > before
+            for k, (u, v, _) in enumerate(sets):
+                f.predict(s["dt"])
+                f.update(u, v, fixed[k])
+                f.resample()
+            out.append(f.state())
+        finally:
+            f.close()
+    for key in ("states", "weights", "gcount", "wg", "samples"):
+        np.testing.assert_array_equal(out[0][key], out[1][key], err_msg=key)
