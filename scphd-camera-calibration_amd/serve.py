@@ -4,7 +4,8 @@ stream frames at a fixed rate, and share one GPU through the batch API
 
 run_served() replays an arrival list in real time:
   * an arriving rig waits in a FIFO until its reservation fits the HBM budget
-    (scphd_batch_add_rig); the wait is its admission delay;
+    (scphd_batch_add_rig); the wait is its admission delay; on admission its
+    buffers are allocated at the expected map size (scphd_reserve);
   * once admitted, its frame k is due at t_admit + k / fps and is submitted
     when due (frames that come due while earlier ones are still queued or on
     the GPU wait in the rig's queue: that wait is part of their latency);
@@ -92,6 +93,11 @@ def run_served(rigs, hbm_budget_bytes=0, n_workers=8, reserve_factor=1.5, progre
                     blocked = True  # offered again once a rig leaves
                     break
                 waiting.pop(0)
+                try:  # the expected map size allocated at admission, inside the reservation
+                    b.filter(rid).reserve(r["p"]["n_particles"] * expected_gaussians_per_parent(r["s"]),
+                                          expected_measurements(r["s"]))
+                except scphd_amd.ScphdError:
+                    pass
                 adm.append(now - r["t"] if paced else 0.0)
                 active[rid] = dict(r=r, t_admit=now, sent=0, done=0, due={})
             for rid, a in active.items():
