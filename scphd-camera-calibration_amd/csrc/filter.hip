@@ -177,6 +177,7 @@ struct scphd_filter {
 
     scphd_update_stats stats{};
     int64_t last_M = 0;
+    int64_t kt_hint = 0; /* merged Gaussians of the last update */
     bool timing = false;
     hipEvent_t ev[10] = {};  /* 0..5 phase boundaries, 6..7 fit kernel, 8..9 sample kernel */
     double phase_ms[8] = {};
@@ -286,7 +287,10 @@ int update_t(scphd_filter* f, int M, int fixed) {
     HIPTRY(f->keepscan.ensure(Ng + 1));
     HIPTRY(f->scantmp.ensure(scan_tmp_size(std::max<int64_t>(Ng, Np) + 1) + 16));
     if (Ng > 0) {
-        HIPTRY(f->gen[nxt].ensure(Ng, 0, S, f->esz, s));
+        /* room for this step's posterior clouds too (the last step's count
+         * plus a quarter), so that sizing them after the merge rarely has to
+         * grow the generation and copy the non-detection clouds already in it */
+        HIPTRY(f->gen[nxt].ensure(Ng + f->kt_hint + f->kt_hint / 4, 0, S, f->esz, s));
         HIPTRY(f->fit.ensure((int64_t)FP_COUNT * Ng));
         HIPTRY(f->pre.ensure((int64_t)PP_COUNT * Ng));
         HIPTRY(f->lik.ensure((int64_t)LK_COUNT * (Ng + 3))); /* 3 records of padding: read four at a time */
@@ -530,6 +534,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
     f->stats.n_gaussians_out = Nout;
     f->stats.max_survivors_per_parent = maxU;
     f->last_M = M;
+    f->kt_hint = Kt;
     if (f->timing) {
         HIPTRY(hipEventSynchronize(f->ev[5]));
         for (int i = 0; i < 5; ++i) {
