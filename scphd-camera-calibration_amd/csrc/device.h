@@ -19,14 +19,22 @@
 namespace scphd {
 
 /* ----------------------------------------------------------------- noise */
-/* x ^ y ^ k for a key word k that is the same in every lane (it is read
- * from a scalar register): one three-input bitwise instruction, truth
- * table 0x96, on the device */
+/* x ^ y ^ k.  A key word k that is the same in every lane (LANE false) is
+ * read from a scalar register: one three-input bitwise instruction, truth
+ * table 0x96, on the device.  A key word that may differ between lanes
+ * (LANE true: the rigs of an ensemble) stays a plain expression; forcing it
+ * into a scalar register would make the compiler loop over its distinct
+ * values. */
+template <bool LANE>
 SCPHD_HD inline uint32_t xor3_key(uint32_t x, uint32_t y, uint32_t k) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    uint32_t r;
-    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(x), "v"(y), "s"(k));
-    return r;
+    if constexpr (!LANE) {
+        uint32_t r;
+        asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(x), "v"(y), "s"(k));
+        return r;
+    } else {
+        return x ^ y ^ k;
+    }
 #else
     return x ^ y ^ k;
 #endif
@@ -35,14 +43,15 @@ SCPHD_HD inline uint32_t xor3_key(uint32_t x, uint32_t y, uint32_t k) {
 /* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
  * easy as 1, 2, 3", SC'11): 10 rounds of two 32x32->64 multiplies, key
  * bumped by the Weyl constants between rounds. */
+template <bool LANE = false>
 SCPHD_HD inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
                                    uint32_t k1, uint32_t out[4]) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         const uint64_t a = (uint64_t)0xD2511F53u * c0;
         const uint64_t b = (uint64_t)0xCD9E8D57u * c2;
-        const uint32_t n0 = xor3_key((uint32_t)(b >> 32), c1, k0);
-        const uint32_t n2 = xor3_key((uint32_t)(a >> 32), c3, k1);
+        const uint32_t n0 = xor3_key<LANE>((uint32_t)(b >> 32), c1, k0);
+        const uint32_t n2 = xor3_key<LANE>((uint32_t)(a >> 32), c3, k1);
         c1 = (uint32_t)b;
         c3 = (uint32_t)a;
         c0 = n0;
@@ -76,9 +85,19 @@ struct NoiseKey {
     }
 };
 
+/* a key that may differ between the lanes of a wavefront (ensembles: each
+ * rig has its own); the same streams as the NoiseKey of the same words */
+struct LaneKey {
+    uint32_t k0, k1;
+};
+
 SCPHD_HD inline void noise_block(NoiseKey key, uint32_t stream, uint64_t step, uint32_t parent,
                                  uint32_t gauss, uint32_t elem, uint32_t block, uint32_t out[4]) {
-    philox4x32_10((elem << 2) | block, gauss, parent, (uint32_t)((step << 4) | stream), key.k0, key.k1, out);
+    philox4x32_10<false>((elem << 2) | block, gauss, parent, (uint32_t)((step << 4) | stream), key.k0, key.k1, out);
+}
+SCPHD_HD inline void noise_block(LaneKey key, uint32_t stream, uint64_t step, uint32_t parent,
+                                 uint32_t gauss, uint32_t elem, uint32_t block, uint32_t out[4]) {
+    philox4x32_10<true>((elem << 2) | block, gauss, parent, (uint32_t)((step << 4) | stream), key.k0, key.k1, out);
 }
 
 /* 53-bit uniforms from two 32-bit words. */
@@ -159,8 +178,8 @@ __device__ inline void box_muller(uint32_t x0, uint32_t x1, double& z0, double& 
 
 /* N (<= 8) standard normals, four per Philox block: words (x0, x1) give
  * (z0, z1), words (x2, x3) give (z2, z3) (box_muller). */
-template <int N>
-__device__ inline void normals(NoiseKey key, uint32_t stream, uint64_t step, uint32_t parent, uint32_t gauss,
+template <int N, typename Key>
+__device__ inline void normals(Key key, uint32_t stream, uint64_t step, uint32_t parent, uint32_t gauss,
                                uint32_t elem, double z[N]) {
 #pragma unroll
     for (int b = 0; 4 * b < N; ++b) {

@@ -35,14 +35,15 @@ struct RigKeys {
     const NoiseKey* keys = nullptr;
     const int32_t* prig = nullptr;
     const int32_t* poff = nullptr;
-    __device__ NoiseKey of(int p, int& pl) const {
+    __device__ LaneKey of(int p, int& pl) const {
         if (!keys) {
             pl = p;
-            return key;
+            return LaneKey{key.k0, key.k1};
         }
         const int r = prig[p];
         pl = p - poff[r];
-        return keys[r];
+        const NoiseKey k = keys[r];
+        return LaneKey{k.k0, k.k1};
     }
 };
 
@@ -117,6 +118,7 @@ struct FitArgs {
     int32_t* ndfill;   /* kept terms whose clouds k_nd_fill writes (decided after the sweep) */
     int32_t* ndfill_n; /* their number; zero before k_fit */
     int64_t nd_base; /* destination block of the non-detection cloud of entry g: nd_base + g */
+    const uint8_t* ghint = nullptr; /* per entry: its cloud was wholly out of view at its last update */
     int off32;       /* every source and destination plane is below 4 GiB: 32-bit byte offsets */
 };
 
@@ -272,6 +274,7 @@ struct TableArgs {
     int32_t* gparent;
     int32_t* gblock;
     double* gwo;
+    uint8_t* ghint; /* non-detection entries whose source cloud had pd_g = 0 */
 };
 hipError_t launch_table(hipStream_t s, const TableArgs& a);
 
@@ -298,7 +301,8 @@ struct ResampleArgs {
 hipError_t launch_resample(hipStream_t s, const ResampleArgs& a);
 hipError_t launch_table_resample(hipStream_t s, const int32_t* anc, const int32_t* goff_old,
                                  const int32_t* goff_new, const int32_t* gblock_old, const double* gw_old,
-                                 int Np, int64_t Nnew, int32_t* gparent, int32_t* gblock, double* gw);
+                                 int Np, int64_t Nnew, int32_t* gparent, int32_t* gblock, double* gw,
+                                 const uint8_t* ghint_old, uint8_t* ghint);
 
 /* ---- read-back ---- */
 template <typename T>

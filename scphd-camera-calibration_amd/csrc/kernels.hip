@@ -143,10 +143,14 @@ __global__ void k_parent_predict(double* ext, int Np, OrientedLinearCVMotionMode
                                  uint64_t step) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= Np) return;
-    int pl;
-    const NoiseKey key = rk.of(p, pl);
     double a[6];
-    normals<6>(key, NS_CAMERA, step, (uint32_t)pl, 0, 0, a);
+    if (!rk.keys) { /* one rig: the key stays in scalar registers */
+        normals<6>(rk.key, NS_CAMERA, step, (uint32_t)p, 0, 0, a);
+    } else {
+        int pl;
+        const LaneKey key = rk.of(p, pl);
+        normals<6>(key, NS_CAMERA, step, (uint32_t)pl, 0, 0, a);
+    }
     double* e = ext + (int64_t)p * 12;
     EuclideanPoint c{e[0], e[1], e[2], e[3], e[4], e[5]};
     EuclideanPoint g{e[6], e[7], e[8], e[9], e[10], e[11]};
@@ -192,8 +196,9 @@ __device__ inline CameraPose load_pose(const double* pose, int p) {
 
 /* Landmark sample prediction (predict_features functor,
  * src/scphdcameracalibration.cu:154-178) for the sample (p, k, s). */
+template <typename Key>
 __device__ inline EuclideanPoint predict_sample(EuclideanPoint x, const LinearCVMotionModel3D& feat, double dt,
-                                                NoiseKey key, uint64_t step, uint32_t p, uint32_t k, uint32_t s) {
+                                                Key key, uint64_t step, uint32_t p, uint32_t k, uint32_t s) {
     double a[3];
     normals<3>(key, NS_FEATURE, step, p, k, s, a);
     return feat.computeNoisyMotion(x, dt, a[0] * feat.std_ax(), a[1] * feat.std_ay(), a[2] * feat.std_az());
@@ -209,12 +214,16 @@ __global__ void k_materialize(Planes<T> src, Planes<T> dst, const int32_t* gbloc
     const int s = (int)(i - g * S);
     const int p = gparent[g];
     const int k = (int)(g - goff[p]);
-    int pl;
-    const NoiseKey key = rk.of(p, pl);
     const int64_t si = (int64_t)gblock[g] * S + s;
     EuclideanPoint x{(double)src.p[0][si], (double)src.p[1][si], (double)src.p[2][si],
                      (double)src.p[3][si], (double)src.p[4][si], (double)src.p[5][si]};
-    x = predict_sample(x, feat, dt, key, step, (uint32_t)pl, (uint32_t)k, (uint32_t)s);
+    if (!rk.keys) { /* one rig: the key stays in scalar registers */
+        x = predict_sample(x, feat, dt, rk.key, step, (uint32_t)p, (uint32_t)k, (uint32_t)s);
+    } else {
+        int pl;
+        const LaneKey key = rk.of(p, pl);
+        x = predict_sample(x, feat, dt, key, step, (uint32_t)pl, (uint32_t)k, (uint32_t)s);
+    }
     const int64_t di = g * S + s;
     dst.p[0][di] = (T)x.x; dst.p[1][di] = (T)x.y; dst.p[2][di] = (T)x.z;
     dst.p[3][di] = (T)x.vx; dst.p[4][di] = (T)x.vy; dst.p[5][di] = (T)x.vz;
@@ -268,8 +277,8 @@ __device__ inline void load_sample(const Planes<T>& src, int64_t si, T r[7]) {
     for (int c = 0; c < 7; ++c) r[c] = src.p[c][si];
 }
 
-template <typename T>
-__device__ inline void eval_sample(const FitArgs& a, const DisparityMeasurementModel& meas, const T r[7], NoiseKey kk,
+template <typename T, typename Key>
+__device__ inline void eval_sample(const FitArgs& a, const DisparityMeasurementModel& meas, const T r[7], Key kk,
                                    int pl, int k, int s, const CameraPose& pose, EuclideanPoint& x, DisparityPoint& d,
                                    bool& in, double& w) {
     x = EuclideanPoint{(double)r[0], (double)r[1], (double)r[2], (double)r[3], (double)r[4], (double)r[5]};
@@ -279,9 +288,9 @@ __device__ inline void eval_sample(const FitArgs& a, const DisparityMeasurementM
     in = meas.inRange(d);
 }
 
-template <typename T>
+template <typename T, typename Key>
 __device__ inline void fit_sample(const FitArgs& a, const DisparityMeasurementModel& meas, const Planes<T>& src,
-                                  int64_t si, NoiseKey kk, int pl, int k, int s, const CameraPose& pose,
+                                  int64_t si, Key kk, int pl, int k, int s, const CameraPose& pose,
                                   EuclideanPoint& x, DisparityPoint& d, bool& in, double& w) {
     T r[7];
     load_sample(src, si, r);
@@ -329,6 +338,18 @@ __device__ inline void write_fit(const FitArgs& a, int64_t g, int lane, const do
  * normalises to weight 1 per sample, is listed in redo[] and refitted by
  * k_fit_fix when pd_g > 0; with pd_g = 0 its fit cannot reach any output and
  * is the first sample with zero covariance. */
+/* The noise key of parent p and p's index in its rig: the kernel argument
+ * (in scalar registers) for one rig, the rig's key per lane in an ensemble. */
+template <bool RG>
+__device__ inline std::conditional_t<RG, LaneKey, NoiseKey> row_key(const RigKeys& rk, int p, int& pl) {
+    if constexpr (RG) {
+        return rk.of(p, pl);
+    } else {
+        pl = p;
+        return rk.key;
+    }
+}
+
 /* The measurement model of parent p's rig, `m`: the kernel argument itself
  * for one rig (RG false, the model stays in scalar registers), else a copy
  * of the rig's entry of per_rig. */
@@ -373,14 +394,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
     for (int i = 0; i < 3; ++i) pose.t[i] = s_pose[row * 12 + 9 + i];
     const int k = (int)(g - a.goff[p]);
     int pl;
-    const NoiseKey kk = a.key.of(p, pl); /* the rig's key, p's index in its rig */
+    const auto kk = row_key<RG>(a.key, p, pl); /* the rig's key, p's index in its rig */
     const RowMeas<RG> rm(a.meas, a.rig_meas, a.key, p);
     const DisparityMeasurementModel& meas = rm.m;
     const int64_t b0 = (int64_t)a.gblock[g] * a.S;
     const int S = a.S;
     const double wg = a.gw[g];
     const double Pd = meas.pd();
-    const bool nd_sure = wg * (1.0 - Pd) >= a.min_weight; /* kept for certain: cloud written in the sweep */
+    const bool nd_sure = wg * (1.0 - Pd) >= a.min_weight; /* kept for certain */
+    /* the cloud is written in the sweep when the term is kept for certain, or
+     * when its cloud was wholly out of view at its last update (pd_g = 0 then;
+     * the term is kept if it still is): a guess that costs only the write of
+     * a cloud nothing references if it proves wrong */
+    const bool nd_write = nd_sure || (a.ghint && a.ghint[g] && wg >= a.min_weight);
     const int64_t d0 = (a.nd_base + g) * (int64_t)S;
 
     double spd = 0.0, sw = 0.0, sw2 = 0.0, m1[6], m2[21], K[6];
@@ -435,7 +461,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
         for (int r = 0; r < 6; ++r)
 #pragma unroll
             for (int c = 0; c <= r; ++c) m2[pidx(r, c)] += wp * dv[r] * dv[c];
-        if (nd_sure && valid) write_nd_off(meas, dst, dof, x, in);
+        if (nd_write && valid) write_nd_off(meas, dst, dof, x, in);
         dof += (Off)(16 * sizeof(T));
     };
     const std::integral_constant<bool, true> full;
@@ -487,7 +513,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
     if (lane == 0) {
         a.keep[g] = keep ? 1 : 0;
         if (sw == 0 && spd != 0.0) a.redo[atomicAdd(a.redo_n, 1)] = (int32_t)g;
-        if (keep && !nd_sure) a.ndfill[atomicAdd(a.ndfill_n, 1)] = (int32_t)g;
+        if (keep && !nd_write) a.ndfill[atomicAdd(a.ndfill_n, 1)] = (int32_t)g;
     }
 }
 
@@ -506,7 +532,7 @@ __global__ __launch_bounds__(256) void k_nd_fill(FitArgs a, Planes<T> src, Plane
         const int p = a.gparent[g];
         const int k = (int)(g - a.goff[p]);
         int pl;
-        const NoiseKey kk = a.key.of(p, pl);
+        const auto kk = row_key<RG>(a.key, p, pl);
         const RowMeas<RG> rm(a.meas, a.rig_meas, a.key, p);
         const DisparityMeasurementModel& meas = rm.m;
         const int64_t b0 = (int64_t)a.gblock[g] * a.S;
@@ -551,7 +577,7 @@ __global__ __launch_bounds__(256) void k_fit_fix(FitArgs a, Planes<T> src, Plane
         const int p = a.gparent[g];
         const int k = (int)(g - a.goff[p]);
         int pl;
-        const NoiseKey kk = a.key.of(p, pl);
+        const auto kk = row_key<RG>(a.key, p, pl);
         const RowMeas<RG> rm(a.meas, a.rig_meas, a.key, p);
         const DisparityMeasurementModel& meas = rm.m;
         const int64_t b0 = (int64_t)a.gblock[g] * S;
@@ -1747,7 +1773,7 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a, Planes<T> dst) {
     const int p = upper_parent(a.mergeoff, a.Np, gm);
     const int k = (int)(gm - a.mergeoff[p]);
     int pl;
-    const NoiseKey key = a.key.of(p, pl);
+    const auto key = row_key<RG>(a.key, p, pl);
     const RowMeas<RG> rm(a.meas, a.rig_meas, a.key, p);
     const DisparityMeasurementModel& meas = rm.m;
     const int64_t rec = (int64_t)a.survoff[p] + k;
@@ -1810,10 +1836,13 @@ __global__ void k_table(TableArgs a) {
     if (j < K) {
         a.gblock[o] = (int32_t)(a.post_base + a.mergeoff[p] + j);
         a.gwo[o] = a.mw[(int64_t)a.survoff[p] + j];
+        a.ghint[o] = 0;
     } else {
         const int g = a.ndlist[a.keepscan[a.goff[p]] + (j - K)];
+        const double pdg = a.fit[(int64_t)FP_PD * a.Ng_in + g];
         a.gblock[o] = (int32_t)(a.nd_base + g);
-        a.gwo[o] = a.gw[g] * (1 - a.fit[(int64_t)FP_PD * a.Ng_in + g]);
+        a.gwo[o] = a.gw[g] * (1 - pdg);
+        a.ghint[o] = pdg == 0.0 ? 1 : 0;
     }
     a.gparent[o] = p;
 }
@@ -1952,7 +1981,8 @@ __global__ void k_iota(int32_t* x, int64_t n) {
 
 __global__ void k_table_resample(const int32_t* anc, const int32_t* goff_old, const int32_t* goff_new,
                                  const int32_t* gblock_old, const double* gw_old, int Np, int64_t Nnew,
-                                 int32_t* gparent, int32_t* gblock, double* gw) {
+                                 int32_t* gparent, int32_t* gblock, double* gw, const uint8_t* ghint_old,
+                                 uint8_t* ghint) {
     const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= Nnew) return;
     const int p = upper_parent(goff_new, Np, o);
@@ -1960,6 +1990,7 @@ __global__ void k_table_resample(const int32_t* anc, const int32_t* goff_old, co
     gparent[o] = p;
     gblock[o] = gblock_old[src];
     gw[o] = gw_old[src];
+    if (ghint) ghint[o] = ghint_old[src];
 }
 
 /* ------------------------------------------------------------ read-back */
@@ -2288,10 +2319,11 @@ hipError_t launch_iota(hipStream_t s, int32_t* x, int64_t n) {
 
 hipError_t launch_table_resample(hipStream_t s, const int32_t* anc, const int32_t* goff_old,
                                  const int32_t* goff_new, const int32_t* gblock_old, const double* gw_old, int Np,
-                                 int64_t Nnew, int32_t* gparent, int32_t* gblock, double* gw) {
+                                 int64_t Nnew, int32_t* gparent, int32_t* gblock, double* gw,
+                                 const uint8_t* ghint_old, uint8_t* ghint) {
     if (Nnew <= 0) return hipSuccess;
     k_table_resample<<<nblocks(Nnew, 256), 256, 0, s>>>(anc, goff_old, goff_new, gblock_old, gw_old, Np, Nnew,
-                                                         gparent, gblock, gw);
+                                                         gparent, gblock, gw, ghint_old, ghint);
     LAUNCH_CHECK();
 }
 
