@@ -149,9 +149,9 @@ struct scphd_filter {
     DBuf<double> gw;
     DBuf<int32_t> goff2, gparent2, gblock2;
     DBuf<double> gw2;
-    /* per entry: its cloud was wholly out of view at its last update (valid
-     * once an update or a resampling built the table; set_state clears it) */
-    DBuf<uint8_t> ghint, ghint2;
+    /* per entry: pd_g of its source term at the last update, 1 for posterior
+     * terms (valid once an update built the table; set_state clears it) */
+    DBuf<float> gpd, gpd2;
     bool hint_ok = false;
     /* sample generations */
     Generation gen[2];
@@ -327,7 +327,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
         fa.ndfill_n = f->ndfill.p + Ng;
         HIPTRY(hipMemsetAsync(fa.ndfill_n, 0, sizeof(int32_t), s));
         fa.nd_base = 0;
-        fa.ghint = f->hint_ok ? f->ghint.p : nullptr;
+        fa.gpd = f->hint_ok ? f->gpd.p : nullptr;
         fa.off32 = std::max(f->gen[0].cap_blocks, f->gen[1].cap_blocks) * (int64_t)S * (int64_t)f->esz <
                    (int64_t)0xFFFFFFFFll - 64 * (int64_t)f->esz;
         record(f, 6);
@@ -502,7 +502,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
     HIPTRY(f->gparent2.ensure(Nout + 1));
     HIPTRY(f->gblock2.ensure(Nout + 1));
     HIPTRY(f->gw2.ensure(Nout + 1));
-    HIPTRY(f->ghint2.ensure(Nout + 1));
+    HIPTRY(f->gpd2.ensure(Nout + 1));
     TableArgs ta;
     ta.outoff = f->goff2.p;
     ta.mergeoff = f->mergeoff.p;
@@ -521,14 +521,14 @@ int update_t(scphd_filter* f, int M, int fixed) {
     ta.gparent = f->gparent2.p;
     ta.gblock = f->gblock2.p;
     ta.gwo = f->gw2.p;
-    ta.ghint = f->ghint2.p;
+    ta.gpd = f->gpd2.p;
     HIPTRY(launch_table(s, ta));
     record(f, 5);
     std::swap(f->goff, f->goff2);
     std::swap(f->gparent, f->gparent2);
     std::swap(f->gblock, f->gblock2);
     std::swap(f->gw, f->gw2);
-    std::swap(f->ghint, f->ghint2);
+    std::swap(f->gpd, f->gpd2);
     f->hint_ok = true;
     f->cur = nxt;
     f->Ng = Nout;
@@ -718,8 +718,8 @@ void scphd_destroy(scphd_t* f) {
     f->dkeys.release();
     f->dmeas.release();
     f->dbirths.release();
-    f->ghint.release();
-    f->ghint2.release();
+    f->gpd.release();
+    f->gpd2.release();
     f->mscratch.release();
     f->gen[0].release();
     f->gen[1].release();
@@ -846,15 +846,15 @@ int scphd_resample(scphd_t* f, int32_t* resampled) {
     HIPTRY(f->gparent2.ensure(Nnew + 1));
     HIPTRY(f->gblock2.ensure(Nnew + 1));
     HIPTRY(f->gw2.ensure(Nnew + 1));
-    if (f->hint_ok) HIPTRY(f->ghint2.ensure(Nnew + 1));
+    if (f->hint_ok) HIPTRY(f->gpd2.ensure(Nnew + 1));
     HIPTRY(launch_table_resample(s, f->anc.p, f->goff.p, f->goff2.p, f->gblock.p, f->gw.p, Np, Nnew,
-                                 f->gparent2.p, f->gblock2.p, f->gw2.p, f->hint_ok ? f->ghint.p : nullptr,
-                                 f->hint_ok ? f->ghint2.p : nullptr));
+                                 f->gparent2.p, f->gblock2.p, f->gw2.p, f->hint_ok ? f->gpd.p : nullptr,
+                                 f->hint_ok ? f->gpd2.p : nullptr));
     std::swap(f->goff, f->goff2);
     std::swap(f->gparent, f->gparent2);
     std::swap(f->gblock, f->gblock2);
     std::swap(f->gw, f->gw2);
-    if (f->hint_ok) std::swap(f->ghint, f->ghint2);
+    if (f->hint_ok) std::swap(f->gpd, f->gpd2);
     f->Ng = Nnew;
     return SCPHD_OK;
 }
@@ -1129,7 +1129,7 @@ int scphd_reserve(scphd_t* f, int64_t n_gaussians, int32_t max_meas) {
     HIPTRY(f->gparent2.ensure(G + 1));
     HIPTRY(f->gblock2.ensure(G + 1));
     HIPTRY(f->gw2.ensure(G + 1));
-    HIPTRY(f->ghint2.ensure(G + 1));
+    HIPTRY(f->gpd2.ensure(G + 1));
     /* survivors: about one per Gaussian in steady state */
     HIPTRY(f->sw.ensure(G + 1));
     HIPTRY(f->sg.ensure(G + 1));
@@ -1163,7 +1163,7 @@ int scphd_get_memory(scphd_t* f, int64_t* bytes) {
                             &f->nmerged, &f->mergeoff, &f->outcnt, &f->scantmp, &f->sg, &f->sm, &f->dflag, &f->mcnt, &f->mncand, &f->mcand,
                             &f->ngroups, &f->glead, &f->gsta, &f->gend, &f->gmem, &f->dmoff, &f->drflag, &f->dpoff, &f->dprig};
     for (auto* d : ibs) b += d->n * (int64_t)sizeof(int32_t);
-    b += f->ghint.n + f->ghint2.n;
+    b += (f->gpd.n + f->gpd2.n) * (int64_t)sizeof(float);
     b += f->dkeys.n * (int64_t)sizeof(NoiseKey) + f->dmeas.n * (int64_t)sizeof(DisparityMeasurementModel) +
          f->dbirths.n * (int64_t)sizeof(BirthModel);
     b += f->mscratch.n;

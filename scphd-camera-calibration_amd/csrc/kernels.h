@@ -118,7 +118,7 @@ struct FitArgs {
     int32_t* ndfill;   /* kept terms whose clouds k_nd_fill writes (decided after the sweep) */
     int32_t* ndfill_n; /* their number; zero before k_fit */
     int64_t nd_base; /* destination block of the non-detection cloud of entry g: nd_base + g */
-    const uint8_t* ghint = nullptr; /* per entry: its cloud was wholly out of view at its last update */
+    const float* gpd = nullptr; /* per entry: pd_g of its source term at the last update (1: posterior term) */
     int off32;       /* every source and destination plane is below 4 GiB: 32-bit byte offsets */
 };
 
@@ -274,7 +274,7 @@ struct TableArgs {
     int32_t* gparent;
     int32_t* gblock;
     double* gwo;
-    uint8_t* ghint; /* non-detection entries whose source cloud had pd_g = 0 */
+    float* gpd;     /* per entry: the source term's pd_g (non-detection entries), 1 (posterior entries) */
 };
 hipError_t launch_table(hipStream_t s, const TableArgs& a);
 
@@ -302,7 +302,7 @@ hipError_t launch_resample(hipStream_t s, const ResampleArgs& a);
 hipError_t launch_table_resample(hipStream_t s, const int32_t* anc, const int32_t* goff_old,
                                  const int32_t* goff_new, const int32_t* gblock_old, const double* gw_old,
                                  int Np, int64_t Nnew, int32_t* gparent, int32_t* gblock, double* gw,
-                                 const uint8_t* ghint_old, uint8_t* ghint);
+                                 const float* gpd_old, float* gpd);
 
 /* ---- read-back ---- */
 template <typename T>

@@ -403,10 +403,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
     const double Pd = meas.pd();
     const bool nd_sure = wg * (1.0 - Pd) >= a.min_weight; /* kept for certain */
     /* the cloud is written in the sweep when the term is kept for certain, or
-     * when its cloud was wholly out of view at its last update (pd_g = 0 then;
-     * the term is kept if it still is): a guess that costs only the write of
-     * a cloud nothing references if it proves wrong */
-    const bool nd_write = nd_sure || (a.ghint && a.ghint[g] && wg >= a.min_weight);
+     * when it would be kept at the pd_g of its source term at the last update
+     * (a non-detection term's cloud moves little in one step): a guess that
+     * costs only the write of a cloud nothing references if it proves wrong */
+    const bool nd_write = nd_sure || (a.gpd && wg * (1.0 - (double)a.gpd[g]) >= a.min_weight);
     const int64_t d0 = (a.nd_base + g) * (int64_t)S;
 
     double spd = 0.0, sw = 0.0, sw2 = 0.0, m1[6], m2[21], K[6];
@@ -1836,13 +1836,13 @@ __global__ void k_table(TableArgs a) {
     if (j < K) {
         a.gblock[o] = (int32_t)(a.post_base + a.mergeoff[p] + j);
         a.gwo[o] = a.mw[(int64_t)a.survoff[p] + j];
-        a.ghint[o] = 0;
+        a.gpd[o] = 1.0f;
     } else {
         const int g = a.ndlist[a.keepscan[a.goff[p]] + (j - K)];
         const double pdg = a.fit[(int64_t)FP_PD * a.Ng_in + g];
         a.gblock[o] = (int32_t)(a.nd_base + g);
         a.gwo[o] = a.gw[g] * (1 - pdg);
-        a.ghint[o] = pdg == 0.0 ? 1 : 0;
+        a.gpd[o] = (float)pdg;
     }
     a.gparent[o] = p;
 }
@@ -1981,8 +1981,8 @@ __global__ void k_iota(int32_t* x, int64_t n) {
 
 __global__ void k_table_resample(const int32_t* anc, const int32_t* goff_old, const int32_t* goff_new,
                                  const int32_t* gblock_old, const double* gw_old, int Np, int64_t Nnew,
-                                 int32_t* gparent, int32_t* gblock, double* gw, const uint8_t* ghint_old,
-                                 uint8_t* ghint) {
+                                 int32_t* gparent, int32_t* gblock, double* gw, const float* gpd_old,
+                                 float* gpd) {
     const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= Nnew) return;
     const int p = upper_parent(goff_new, Np, o);
@@ -1990,7 +1990,7 @@ __global__ void k_table_resample(const int32_t* anc, const int32_t* goff_old, co
     gparent[o] = p;
     gblock[o] = gblock_old[src];
     gw[o] = gw_old[src];
-    if (ghint) ghint[o] = ghint_old[src];
+    if (gpd) gpd[o] = gpd_old[src];
 }
 
 /* ------------------------------------------------------------ read-back */
@@ -2320,10 +2320,10 @@ hipError_t launch_iota(hipStream_t s, int32_t* x, int64_t n) {
 hipError_t launch_table_resample(hipStream_t s, const int32_t* anc, const int32_t* goff_old,
                                  const int32_t* goff_new, const int32_t* gblock_old, const double* gw_old, int Np,
                                  int64_t Nnew, int32_t* gparent, int32_t* gblock, double* gw,
-                                 const uint8_t* ghint_old, uint8_t* ghint) {
+                                 const float* gpd_old, float* gpd) {
     if (Nnew <= 0) return hipSuccess;
     k_table_resample<<<nblocks(Nnew, 256), 256, 0, s>>>(anc, goff_old, goff_new, gblock_old, gw_old, Np, Nnew,
-                                                         gparent, gblock, gw, ghint_old, ghint);
+                                                         gparent, gblock, gw, gpd_old, gpd);
     LAUNCH_CHECK();
 }
 
