@@ -2,7 +2,8 @@
 """Batched ragged workload: many independent rigs stepped together on one GPU.
 
     python run_c2.py [--rigs 64] [--frames 100] [--workers 8] [--plain-rigs N]
-                     [--mode batch|ensemble|both] [--ensemble-gb 120] [--workload c2|small]
+                     [--mode batch|ensemble|hybrid|both] [--ensemble-gb 120] [--workload c2|small]
+                     [--alone-parents 2048]
 
 Rigs as workloads.c2_rigs: parents uniform in {256, ..., 4096}, landmarks
 uniform in [20, 300], clutter lambda uniform in [5, 80], baseline uniform in
@@ -20,6 +21,8 @@ a master seed (--workload small: workloads.small_ragged_rigs, parents in
   of landmarks that left the view decay only by the survival probability),
   to about twice the expected size, so the budget leaves that headroom below
   the 288 GB of the device.
+* hybrid: rigs with at least --alone-parents parents run alone (one rig fills
+  the GPU), the others in ensembles as above.
 
 Prints one JSON line: component updates/s of each mode, of the plain path on
 the first --plain-rigs rigs run one at a time, and the ratios.
@@ -46,7 +49,8 @@ def main(argv=None):
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--plain-rigs", type=int, default=8)
     ap.add_argument("--master", type=int, default=1)
-    ap.add_argument("--mode", default="both", choices=["batch", "ensemble", "both"])
+    ap.add_argument("--mode", default="both", choices=["batch", "ensemble", "hybrid", "both"])
+    ap.add_argument("--alone-parents", type=int, default=2048)
     ap.add_argument("--ensemble-gb", type=float, default=120.0)
     ap.add_argument("--workload", default="c2", choices=["c2", "small"])
     args = ap.parse_args(argv)
@@ -66,9 +70,13 @@ def main(argv=None):
                               peak_active_rigs=bat["peak_active_rigs"], peak_reserved_gb=bat["peak_reserved_gb"],
                               peak_used_gb=bat["peak_used_gb"], refused_admissions=bat["refused_admissions"])
         print(json.dumps(out), flush=True)
-    if args.mode in ("ensemble", "both"):
+    if args.mode in ("ensemble", "hybrid", "both"):
+        hybrid = args.mode == "hybrid"
+        alone = [r for r in rigs if hybrid and r["p"]["n_particles"] >= args.alone_parents]
         groups, cur, used = [], [], 0.0
         for r in rigs:
+            if hybrid and r["p"]["n_particles"] >= args.alone_parents:
+                continue
             need = 1.5 * scphd_amd.rig_footprint(r["p"], serve.expected_gaussians_per_parent(r["s"])) / 1e9
             if cur and used + need > args.ensemble_gb:
                 groups.append(cur)
@@ -84,13 +92,19 @@ def main(argv=None):
             wall += e["wall_s"]
             lat.append(e["latency_ms"])
             log(f"ensemble group {i + 1}/{len(groups)}: {len(g)} rigs, {e['updates_per_s']:.3g} updates/s")
-        out["ensemble"] = dict(updates_per_s=pairs / wall if wall > 0 else 0.0, wall_s=wall,
-                               groups=[len(g) for g in groups], group_latency_ms=lat)
+        if alone:
+            a = serve.run_plain(alone, progress=log)
+            pairs += a["pairs"]
+            wall += a["wall_s"]
+            log(f"{len(alone)} rigs alone: {a['updates_per_s']:.3g} updates/s")
+        out["hybrid" if hybrid else "ensemble"] = dict(updates_per_s=pairs / wall if wall > 0 else 0.0, wall_s=wall,
+                                                       groups=[len(g) for g in groups], alone=len(alone),
+                                                       group_latency_ms=lat)
         print(json.dumps(out), flush=True)
     if args.plain_rigs > 0:
         plain = serve.run_plain(rigs[:args.plain_rigs], progress=log)
         out["plain"] = dict(rigs=plain["rigs"], updates_per_s=plain["updates_per_s"], latency_ms=plain["latency_ms"])
-        for k in ("batched", "ensemble"):
+        for k in ("batched", "ensemble", "hybrid"):
             if k in out and plain["updates_per_s"]:
                 out[f"ratio_{k}"] = out[k]["updates_per_s"] / plain["updates_per_s"]
     out["total_s"] = time.perf_counter() - t0
