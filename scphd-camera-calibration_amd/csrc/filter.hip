@@ -15,6 +15,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -25,54 +27,189 @@ using namespace scphd;
 
 namespace {
 
+/* ---------------------------------------------------- device memory cache */
+/* Released device blocks are kept, per device and size, and handed out
+ * again, so that steady-state steps and rig turnover call neither hipMalloc
+ * nor hipFree: hipFree waits for the whole device (stalling every other
+ * rig's stream), and hipMalloc of large blocks becomes orders of magnitude
+ * slower when most of the device memory is in use.  A block enters the cache
+ * only when no queued work can still touch it: a handle's blocks replaced
+ * during a call are retired to the handle and handed over once its streams
+ * have drained.  When hipMalloc fails, the device's cached blocks are freed
+ * and the allocation is tried once more.  The cache holds at most an eighth
+ * of the device's memory (a block that would exceed it is freed instead), so
+ * that fresh allocations keep the device's free memory to work with.  The
+ * cache lives as long as the process (its blocks are not freed at exit, when
+ * the runtime may already be gone). */
+struct Block {
+    void* p;
+    size_t bytes;
+};
+
+struct BlockCache {
+    std::mutex mu;
+    std::multimap<std::pair<int, size_t>, void*> blocks; /* (device, bytes) -> block */
+    std::map<int, size_t> held;                          /* cached bytes per device */
+};
+
+BlockCache& block_cache() {
+    static BlockCache* c = new BlockCache();
+    return *c;
+}
+
+/* sizes: powers of two from 4 KiB below 1 MiB, whole MiB above */
+size_t block_size(size_t bytes) {
+    if (bytes >= (1u << 20)) return (bytes + (1u << 20) - 1) & ~(size_t)((1u << 20) - 1);
+    size_t b = 4096;
+    while (b < bytes) b <<= 1;
+    return b;
+}
+
+void cache_trim(int dev) {
+    std::vector<void*> drop;
+    {
+        BlockCache& c = block_cache();
+        std::lock_guard<std::mutex> lk(c.mu);
+        for (auto it = c.blocks.begin(); it != c.blocks.end();) {
+            if (it->first.first == dev) {
+                drop.push_back(it->second);
+                c.held[dev] -= it->first.second;
+                it = c.blocks.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
+    for (void* p : drop) (void)hipFree(p);
+}
+
+/* a block of at least *bytes on the current device (a cached one a little
+ * larger, else a new one); *bytes returns its size */
+hipError_t cache_alloc(void** p, size_t* bytes) {
+    *p = nullptr;
+    const size_t want = block_size(*bytes);
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    {
+        BlockCache& c = block_cache();
+        std::lock_guard<std::mutex> lk(c.mu);
+        /* a cached block up to twice the size for small requests, a quarter
+         * more for large ones (64 MiB and up) */
+        const size_t most = want >= ((size_t)64 << 20) ? want + want / 4 : 2 * want;
+        auto it = c.blocks.lower_bound({dev, want});
+        if (it != c.blocks.end() && it->first.first == dev && it->first.second <= most) {
+            *p = it->second;
+            *bytes = it->first.second;
+            c.held[dev] -= it->first.second;
+            c.blocks.erase(it);
+            return hipSuccess;
+        }
+    }
+    e = hipMalloc(p, want);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        cache_trim(dev);
+        e = hipMalloc(p, want);
+    }
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return e;
+    }
+    *bytes = want;
+    return hipSuccess;
+}
+
+/* hand a block no queued work touches to the cache of the current device,
+ * or free it when the cache is full */
+void cache_put(const Block& b) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    static const size_t limit = [] {
+        size_t fr = 0, tot = 0;
+        return hipMemGetInfo(&fr, &tot) == hipSuccess ? tot / 8 : (size_t)8 << 30;
+    }();
+    {
+        BlockCache& c = block_cache();
+        std::lock_guard<std::mutex> lk(c.mu);
+        if (c.held[dev] + b.bytes <= limit) {
+            c.blocks.insert({{dev, b.bytes}, b.p});
+            c.held[dev] += b.bytes;
+            return;
+        }
+    }
+    (void)hipFree(b.p);
+}
+
+/* a released block: to the owner's retired list while its streams may still
+ * use it, else straight to the cache */
+void retire(std::vector<Block>* retired, void* p, size_t bytes) {
+    if (!p) return;
+    if (retired)
+        retired->push_back(Block{p, bytes});
+    else
+        cache_put(Block{p, bytes});
+}
+
 template <typename T>
 struct DBuf {
     T* p = nullptr;
     int64_t n = 0;
+    size_t bytes = 0;
+    std::vector<Block>* retired = nullptr; /* the owning handle's list */
     hipError_t ensure(int64_t need) {
         if (need <= n) return hipSuccess;
         release();
         const int64_t cap = std::max<int64_t>(need + need / 4, 64);
-        hipError_t e = hipMalloc((void**)&p, (size_t)cap * sizeof(T));
-        if (e != hipSuccess) {
-            p = nullptr;
-            return e;
-        }
-        n = cap;
+        size_t b = (size_t)cap * sizeof(T);
+        void* q = nullptr;
+        hipError_t e = cache_alloc(&q, &b);
+        if (e != hipSuccess) return e;
+        p = (T*)q;
+        bytes = b;
+        n = (int64_t)(b / sizeof(T));
         return hipSuccess;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        retire(retired, p, bytes);
         p = nullptr;
         n = 0;
+        bytes = 0;
     }
 };
 
 /* One generation of sample blocks: 7 planes of float or double. */
 struct Generation {
     void* pl[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    size_t pbytes[7] = {};
     int64_t cap_blocks = 0;
+    std::vector<Block>* retired = nullptr; /* the owning handle's list */
     void release() {
-        for (auto& x : pl) {
-            if (x) (void)hipFree(x);
-            x = nullptr;
+        for (int c = 0; c < 7; ++c) {
+            retire(retired, pl[c], pbytes[c]);
+            pl[c] = nullptr;
+            pbytes[c] = 0;
         }
         cap_blocks = 0;
     }
     /* grow to at least need blocks keeping the first keep blocks.  Planes
-     * are replaced one at a time (allocate, copy the kept blocks, free the
-     * old plane), so the transient is one plane, not a whole generation; if
-     * an allocation fails, every plane still holds the kept blocks and at
-     * least the old capacity, which stays the recorded one. */
+     * are replaced one at a time (allocate, copy the kept blocks, wait, hand
+     * the old plane back), so the transient is one plane, not a whole
+     * generation; if an allocation fails, every plane still holds the kept
+     * blocks and at least the old capacity, which stays the recorded one.
+     * Keeping blocks happens only while no other stream works on the
+     * generation, so after the wait an old plane goes straight to the cache. */
     hipError_t ensure(int64_t need, int64_t keep, int S, size_t esz, hipStream_t s) {
         if (need <= cap_blocks) return hipSuccess;
         const int64_t cap = std::max<int64_t>(need + need / 4, 16);
         if (keep <= 0) {
-            release(); /* nothing to keep: free first, then allocate */
+            release(); /* nothing to keep: release first, then allocate */
         }
+        int64_t got = INT64_MAX;
         for (int c = 0; c < 7; ++c) {
             void* np = nullptr;
-            hipError_t e = hipMalloc(&np, (size_t)cap * S * esz);
+            size_t b = (size_t)cap * S * esz;
+            hipError_t e = cache_alloc(&np, &b);
             if (e != hipSuccess) {
                 if (keep <= 0) release();
                 return e;
@@ -81,14 +218,18 @@ struct Generation {
                 e = hipMemcpyAsync(np, pl[c], (size_t)keep * S * esz, hipMemcpyDeviceToDevice, s);
                 if (e == hipSuccess) e = hipStreamSynchronize(s);
                 if (e != hipSuccess) {
-                    (void)hipFree(np);
+                    retire(retired, np, b);
                     return e;
                 }
+                cache_put(Block{pl[c], pbytes[c]});
+            } else {
+                retire(retired, pl[c], pbytes[c]);
             }
-            if (pl[c]) (void)hipFree(pl[c]);
             pl[c] = np;
+            pbytes[c] = b;
+            got = std::min<int64_t>(got, (int64_t)(b / ((size_t)S * esz)));
         }
-        cap_blocks = cap;
+        cap_blocks = got;
         return hipSuccess;
     }
     template <typename T>
@@ -179,6 +320,10 @@ struct scphd_filter {
     double pdt = 0.0;
     uint64_t pstep = 0;
 
+    /* blocks replaced during a call, handed to the cache once the streams
+     * have drained */
+    std::vector<Block> retired;
+
     scphd_update_stats stats{};
     int64_t last_M = 0;
     int64_t kt_hint = 0; /* merged Gaussians of the last update */
@@ -209,6 +354,28 @@ RigKeys rig_keys(const scphd_filter* f) {
         r.poff = f->dpoff.p;
     }
     return r;
+}
+
+/* every device buffer of a handle */
+template <typename Fn>
+void for_each_buf(scphd_filter* f, Fn&& fn) {
+    fn(f->dkeys); fn(f->dmeas); fn(f->dpoff); fn(f->dprig); fn(f->dmoff); fn(f->drflag); fn(f->dbpcs);
+    fn(f->dbirths); fn(f->ext); fn(f->pw); fn(f->pose); fn(f->loglik); fn(f->cum); fn(f->ext_tmp); fn(f->anc);
+    fn(f->newcnt); fn(f->goff); fn(f->gparent); fn(f->gblock); fn(f->gw); fn(f->goff2); fn(f->gparent2);
+    fn(f->gblock2); fn(f->gw2); fn(f->gpd); fn(f->gpd2); fn(f->fit); fn(f->pre); fn(f->lik); fn(f->keep);
+    fn(f->keepscan); fn(f->ndlist); fn(f->redo); fn(f->ndfill); fn(f->du); fn(f->dv); fn(f->eta); fn(f->nsurv);
+    fn(f->survoff); fn(f->nmerged); fn(f->mergeoff); fn(f->outcnt); fn(f->scantmp); fn(f->sg); fn(f->sm);
+    fn(f->dflag); fn(f->mcnt); fn(f->mncand); fn(f->mcand); fn(f->ngroups); fn(f->glead); fn(f->gsta);
+    fn(f->gend); fn(f->gmem); fn(f->sw); fn(f->smu); fn(f->mw); fn(f->mmu); fn(f->mP); fn(f->dneff);
+    fn(f->mscratch);
+}
+
+/* hand the retired blocks to the cache: call only when no queued work on
+ * either stream can still use them (after a wait on the main stream that
+ * follows the join of the second stream) */
+void flush_retired(scphd_filter* f) {
+    for (const Block& b : f->retired) cache_put(b);
+    f->retired.clear();
 }
 
 DisparityMeasurementModel meas_of(const scphd_params& p) {
@@ -467,6 +634,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
     HIPTRY(hipStreamWaitEvent(s, f->ev_nd, 0));
     HIPTRY(launch_readout(s, f->goff2.p, Np, P.max_gaussians, f->mergeoff.p + Np, f->dflag.p, f->dmap));
     HIPTRY(hipStreamSynchronize(s));
+    flush_retired(f); /* both streams have drained */
     const int64_t Nout = f->hmap[0];
     const int64_t Kt = f->hmap[3];
     if (f->hmap[4] != 0) return fail(f, SCPHD_ERR_STATE, "survivor count mismatch between passes");
@@ -619,6 +787,8 @@ int create_filter(const scphd_params* p, int32_t n_rigs, bool ensemble, scphd_t*
         }
     }
     scphd_filter* f = new scphd_filter();
+    for_each_buf(f, [f](auto& b) { b.retired = &f->retired; });
+    f->gen[0].retired = f->gen[1].retired = &f->retired;
     f->prm = *p;
     f->dev = best;
     f->ensemble = ensemble;
@@ -706,23 +876,10 @@ void scphd_destroy(scphd_t* f) {
     (void)hipSetDevice(f->dev);
     if (f->stream) (void)hipStreamSynchronize(f->stream);
     if (f->side) (void)hipStreamSynchronize(f->side);
-    DBuf<double>* dbs[] = {&f->ext, &f->pw, &f->pose, &f->loglik, &f->cum, &f->ext_tmp, &f->gw, &f->gw2,
-                           &f->fit, &f->pre, &f->lik, &f->du, &f->dv, &f->eta, &f->sw, &f->smu, &f->mw, &f->mmu, &f->mP,
-                           &f->dneff, &f->dbpcs};
-    for (auto* b : dbs) b->release();
-    DBuf<int32_t>* ibs[] = {&f->anc, &f->newcnt, &f->goff, &f->gparent, &f->gblock, &f->goff2, &f->gparent2,
-                            &f->gblock2, &f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->ndfill, &f->nsurv, &f->survoff,
-                            &f->nmerged, &f->mergeoff, &f->outcnt, &f->scantmp, &f->sg, &f->sm, &f->dflag, &f->mcnt, &f->mncand, &f->mcand,
-                            &f->ngroups, &f->glead, &f->gsta, &f->gend, &f->gmem, &f->dmoff, &f->drflag, &f->dpoff, &f->dprig};
-    for (auto* b : ibs) b->release();
-    f->dkeys.release();
-    f->dmeas.release();
-    f->dbirths.release();
-    f->gpd.release();
-    f->gpd2.release();
-    f->mscratch.release();
+    for_each_buf(f, [](auto& b) { b.release(); });
     f->gen[0].release();
     f->gen[1].release();
+    flush_retired(f); /* the streams have drained above */
     if (f->hmap) (void)hipHostFree(f->hmap);
     for (auto& e : f->ev)
         if (e) (void)hipEventDestroy(e);
@@ -838,6 +995,7 @@ int scphd_resample(scphd_t* f, int32_t* resampled) {
     HIPTRY(launch_scan(s, f->newcnt.p, f->goff2.p, Np, f->scantmp.p));
     HIPTRY(launch_readout(s, f->goff2.p, Np, 0, f->dflag.p, nullptr, f->dmap));
     HIPTRY(hipStreamSynchronize(s));
+    flush_retired(f);
     f->n_resample++;
     const int go = f->hmap[3];
     if (resampled) *resampled = go;
@@ -916,8 +1074,10 @@ int scphd_get_maps(scphd_t* f, int32_t* gcount, double* wg, double* samples) {
         HIPTRY(hipMemcpyAsync(wg, f->gw.p, (size_t)f->Ng * sizeof(double), hipMemcpyDeviceToHost, s));
     if (samples && f->Ng > 0) {
         const int64_t n = f->Ng * f->S;
-        double* d = nullptr;
-        HIPTRY(hipMalloc((void**)&d, (size_t)n * 7 * sizeof(double)));
+        void* dv = nullptr;
+        size_t db = (size_t)n * 7 * sizeof(double);
+        HIPTRY(cache_alloc(&dv, &db));
+        double* d = (double*)dv;
         hipError_t e = f->esz == 8 ? launch_extract<double>(s, f->gen[f->cur].planes<double>(), f->gblock.p, f->Ng,
                                                             f->S, d)
                                    : launch_extract<float>(s, f->gen[f->cur].planes<float>(), f->gblock.p, f->Ng,
@@ -925,10 +1085,11 @@ int scphd_get_maps(scphd_t* f, int32_t* gcount, double* wg, double* samples) {
         if (e == hipSuccess)
             e = hipMemcpyAsync(samples, d, (size_t)n * 7 * sizeof(double), hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
-        (void)hipFree(d);
+        retire(&f->retired, dv, db);
         if (e != hipSuccess) return hipfail(f, e, "get_maps");
     }
     HIPTRY(hipStreamSynchronize(s));
+    flush_retired(f);
     return SCPHD_OK;
 }
 
@@ -962,18 +1123,21 @@ int scphd_set_state(scphd_t* f, const double* states, const double* weights, con
         HIPTRY(hipMemcpyAsync(f->gw.p, wg, (size_t)Ng * sizeof(double), hipMemcpyHostToDevice, s));
         HIPTRY(launch_iota(s, f->gblock.p, Ng));
         HIPTRY(f->gen[f->cur].ensure(Ng, 0, S, f->esz, s));
-        double* d = nullptr;
         const int64_t n = Ng * S;
-        HIPTRY(hipMalloc((void**)&d, (size_t)n * 7 * sizeof(double)));
+        void* dv = nullptr;
+        size_t db = (size_t)n * 7 * sizeof(double);
+        HIPTRY(cache_alloc(&dv, &db));
+        double* d = (double*)dv;
         hipError_t e = hipMemcpyAsync(d, samples, (size_t)n * 7 * sizeof(double), hipMemcpyHostToDevice, s);
         if (e == hipSuccess)
             e = f->esz == 8 ? launch_inject<double>(s, d, Ng, S, f->gen[f->cur].planes<double>())
                             : launch_inject<float>(s, d, Ng, S, f->gen[f->cur].planes<float>());
         if (e == hipSuccess) e = hipStreamSynchronize(s);
-        (void)hipFree(d);
+        retire(&f->retired, dv, db);
         if (e != hipSuccess) return hipfail(f, e, "set_state");
     }
     HIPTRY(hipStreamSynchronize(s));
+    flush_retired(f);
     f->Ng = Ng;
     f->n_predict = n_predict;
     f->n_update = n_update;
@@ -1148,26 +1312,16 @@ int scphd_reserve(scphd_t* f, int64_t n_gaussians, int32_t max_meas) {
     HIPTRY(f->du.ensure((int64_t)f->n_rigs * M));
     HIPTRY(f->dv.ensure((int64_t)f->n_rigs * M));
     HIPTRY(hipStreamSynchronize(f->stream));
+    flush_retired(f);
     return SCPHD_OK;
 }
 
 int scphd_get_memory(scphd_t* f, int64_t* bytes) {
     if (check_handle(f) || !bytes) return SCPHD_ERR_ARG;
     int64_t b = 0;
-    DBuf<double>* dbs[] = {&f->ext, &f->pw, &f->pose, &f->loglik, &f->cum, &f->ext_tmp, &f->gw, &f->gw2,
-                           &f->fit, &f->pre, &f->lik, &f->du, &f->dv, &f->eta, &f->sw, &f->smu, &f->mw, &f->mmu, &f->mP,
-                           &f->dneff, &f->dbpcs};
-    for (auto* d : dbs) b += d->n * (int64_t)sizeof(double);
-    DBuf<int32_t>* ibs[] = {&f->anc, &f->newcnt, &f->goff, &f->gparent, &f->gblock, &f->goff2, &f->gparent2,
-                            &f->gblock2, &f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->ndfill, &f->nsurv, &f->survoff,
-                            &f->nmerged, &f->mergeoff, &f->outcnt, &f->scantmp, &f->sg, &f->sm, &f->dflag, &f->mcnt, &f->mncand, &f->mcand,
-                            &f->ngroups, &f->glead, &f->gsta, &f->gend, &f->gmem, &f->dmoff, &f->drflag, &f->dpoff, &f->dprig};
-    for (auto* d : ibs) b += d->n * (int64_t)sizeof(int32_t);
-    b += (f->gpd.n + f->gpd2.n) * (int64_t)sizeof(float);
-    b += f->dkeys.n * (int64_t)sizeof(NoiseKey) + f->dmeas.n * (int64_t)sizeof(DisparityMeasurementModel) +
-         f->dbirths.n * (int64_t)sizeof(BirthModel);
-    b += f->mscratch.n;
-    for (const auto& g : f->gen) b += g.cap_blocks * (int64_t)f->S * (int64_t)f->esz * 7;
+    for_each_buf(f, [&b](auto& d) { b += (int64_t)d.bytes; });
+    for (const auto& g : f->gen)
+        for (size_t x : g.pbytes) b += (int64_t)x;
     *bytes = b;
     return SCPHD_OK;
 }

@@ -430,6 +430,21 @@ class Ensemble(Filter):
             raise ValueError("u and v differ in length")
         self._check(lib().scphd_update_ensemble(self.h, _dp(u), _dp(v), _ip(moff), int(bool(fixed))),
                     "update_ensemble")
+        self._mstride = max(len(x) for x in us) if us else 0
+
+    def update_debug(self):
+        """As Filter.update_debug; eta rows have the largest rig's measurement
+        count (a rig with fewer uses the first moff[r+1]-moff[r] columns)."""
+        M = getattr(self, "_mstride", 0)
+        eta = np.zeros(max(self.Np * M, 1))
+        ll = np.zeros(self.Np)
+        nm = np.zeros(self.Np, dtype=np.int32)
+        nn = np.zeros(self.Np, dtype=np.int32)
+        ns = np.zeros(self.Np, dtype=np.int32)
+        self._check(lib().scphd_get_update_debug(self.h, _dp(eta), _dp(ll), _ip(nm), _ip(nn), _ip(ns)),
+                    "get_update_debug")
+        return dict(eta=eta[:self.Np * M].reshape(self.Np, M), loglik=ll, nmerged=nm, nnondetect=nn,
+                    nsurvivors=ns)
 
     def rig_estimates(self):
         """Per rig: mean (n_rigs, 12), covariance (n_rigs, 12, 12), cardinality (n_rigs)."""

@@ -53,11 +53,12 @@ def run_alone(p, dt, sets, fixed):
         for k, (u, v, _) in enumerate(sets):
             f.predict(dt)
             f.update(u, v, fixed[k])
+            dbg = f.update_debug()
             assert f.check() == 0
             rs.append(f.resample())
             m, c, card = f.estimate()
             est.append(np.concatenate([m, c.ravel(), [card]]))
-        return f.state(), np.array(est), rs
+        return f.state(), np.array(est), rs, dbg
     finally:
         f.close()
 
@@ -76,6 +77,7 @@ def test_ensemble_rigs_match_plain_filters_bit_for_bit():
         for k in range(FRAMES):
             e.predict(dt)
             e.update([r[2][k][0] for r in rigs], [r[2][k][1] for r in rigs], fixed[k])
+            dbg = e.update_debug()
             assert e.check() == 0
             e.resample()
             flags, neff = e.rig_resample()
@@ -90,10 +92,13 @@ def test_ensemble_rigs_match_plain_filters_bit_for_bit():
 
     S = SAMPLES
     g0, p0 = 0, 0
-    for r, (rst, rest, rrs) in enumerate(ref):
+    for r, (rst, rest, rrs, rdbg) in enumerate(ref):
         n = RIGS[r]["n_particles"]
         sl = slice(p0, p0 + n)
         p0 += n
+        m = rdbg["eta"].shape[1]  # the last frame's normalisers and parent log-likelihoods
+        np.testing.assert_array_equal(dbg["eta"][sl, :m], rdbg["eta"], err_msg=f"rig {r} eta")
+        np.testing.assert_array_equal(dbg["loglik"][sl], rdbg["loglik"], err_msg=f"rig {r} loglik")
         np.testing.assert_array_equal(st["states"][sl], rst["states"], err_msg=f"rig {r} states")
         np.testing.assert_array_equal(st["weights"][sl], rst["weights"], err_msg=f"rig {r} weights")
         np.testing.assert_array_equal(st["gcount"][sl], rst["gcount"], err_msg=f"rig {r} gcount")
