@@ -36,9 +36,12 @@ namespace {
  * only when no queued work can still touch it: a handle's blocks replaced
  * during a call are retired to the handle and handed over once its streams
  * have drained.  When hipMalloc fails, the device's cached blocks are freed
- * and the allocation is tried once more.  The cache holds at most an eighth
- * of the device's memory (a block that would exceed it is freed instead), so
- * that fresh allocations keep the device's free memory to work with.  The
+ * and the allocation is tried once more; no allocation may leave less than
+ * 1/16 of the device's memory free.  The cache holds at most 1/32 of
+ * the device's memory, and nothing while less than 1/16 of it is free (a
+ * block is then freed instead, and a new allocation first empties the
+ * cache): the runtime allocates memory of its own on demand (scratch for
+ * kernels that spill, queues), and aborts when it cannot.  The
  * cache lives as long as the process (its blocks are not freed at exit, when
  * the runtime may already be gone). */
 struct Block {
@@ -83,6 +86,14 @@ void cache_trim(int dev) {
     for (void* p : drop) (void)hipFree(p);
 }
 
+/* the device's total memory and whether less than 1/16 of it is free */
+bool memory_low(size_t* total) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return true;
+    if (total) *total = tot;
+    return fr < tot / 16;
+}
+
 /* a block of at least *bytes on the current device (a cached one a little
  * larger, else a new one); *bytes returns its size */
 hipError_t cache_alloc(void** p, size_t* bytes) {
@@ -106,6 +117,17 @@ hipError_t cache_alloc(void** p, size_t* bytes) {
             return hipSuccess;
         }
     }
+    /* never into the last 1/16 of the device's memory: the runtime's own
+     * allocations (scratch, queues) need it, and the runtime aborts the
+     * process when they fail, where a failed allocation here is an error
+     * the caller sees */
+    size_t fr = 0, tot = 0;
+    if (memory_low(nullptr)) cache_trim(dev);
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess && (fr < want || fr - want < tot / 16)) {
+        cache_trim(dev);
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess && (fr < want || fr - want < tot / 16))
+            return hipErrorOutOfMemory;
+    }
     e = hipMalloc(p, want);
     if (e == hipErrorOutOfMemory) {
         (void)hipGetLastError();
@@ -125,14 +147,12 @@ hipError_t cache_alloc(void** p, size_t* bytes) {
 void cache_put(const Block& b) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return;
-    static const size_t limit = [] {
-        size_t fr = 0, tot = 0;
-        return hipMemGetInfo(&fr, &tot) == hipSuccess ? tot / 8 : (size_t)8 << 30;
-    }();
-    {
+    size_t total = 0;
+    const bool low = memory_low(&total);
+    if (!low) {
         BlockCache& c = block_cache();
         std::lock_guard<std::mutex> lk(c.mu);
-        if (c.held[dev] + b.bytes <= limit) {
+        if (c.held[dev] + b.bytes <= total / 32) {
             c.blocks.insert({{dev, b.bytes}, b.p});
             c.held[dev] += b.bytes;
             return;

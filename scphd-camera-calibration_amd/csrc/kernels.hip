@@ -626,7 +626,8 @@ __global__ __launch_bounds__(256) void k_fit_fix(FitArgs a, Planes<T> src, Plane
 /* Measurement-independent detection terms of each fitted Gaussian
  * (computeDetectionsKernel, src/kernels.cu:492-598): innovation covariance,
  * gain, Joseph covariance and its PD test, log(pd) + log(w), log det. */
-__global__ void k_precompute(const double* fit, const double* gw, int64_t Ng, double var_u, double var_v,
+__global__ __launch_bounds__(128) void k_precompute(const double* fit, const double* gw, int64_t Ng, double var_u,
+                                                    double var_v,
                              double* pre, double* lik) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= Ng) return;

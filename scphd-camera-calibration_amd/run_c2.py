@@ -3,7 +3,7 @@
 
     python run_c2.py [--rigs 64] [--frames 100] [--workers 8] [--plain-rigs N]
                      [--mode batch|ensemble|hybrid|both] [--ensemble-gb 120] [--workload c2|small]
-                     [--alone-parents 2048]
+                     [--alone-parents 2048] [--growth 2]
 
 Rigs as workloads.c2_rigs: parents uniform in {256, ..., 4096}, landmarks
 uniform in [20, 300], clutter lambda uniform in [5, 80], baseline uniform in
@@ -14,13 +14,15 @@ a master seed (--workload small: workloads.small_ragged_rigs, parents in
 * batch: every rig's frames are queued at once; rigs are admitted as the HBM
   budget allows (scphd_batch_*), and a rig leaves when its frames are done.
 * ensemble: the rigs are packed, in order, into groups whose summed
-  footprint (scphd_rig_footprint at the expected map size, x 1.5) fits
-  --ensemble-gb; each group is one ensemble (scphd_create_ensemble) whose
+  footprint (scphd_rig_footprint at --growth times the expected map size)
+  fits --ensemble-gb; each group is one ensemble (scphd_create_ensemble),
+  its buffers reserved for that size before the first frame, whose
   frames step every rig of the group with one launch per kernel; the groups
   run one after the other.  The C2 maps keep growing over 100 frames (terms
   of landmarks that left the view decay only by the survival probability),
-  to about twice the expected size, so the budget leaves that headroom below
-  the 288 GB of the device.
+  to about twice the expected size: hence --growth 2, so that the maps grow
+  into reserved memory rather than allocating large blocks near the
+  device's capacity, which is slow.
 * hybrid: rigs with at least --alone-parents parents run alone (one rig fills
   the GPU), the others in ensembles as above.
 
@@ -51,6 +53,8 @@ def main(argv=None):
     ap.add_argument("--master", type=int, default=1)
     ap.add_argument("--mode", default="both", choices=["batch", "ensemble", "hybrid", "both"])
     ap.add_argument("--alone-parents", type=int, default=2048)
+    ap.add_argument("--growth", type=float, default=2.0,
+                    help="ensembles reserve and are packed for this multiple of the expected map size")
     ap.add_argument("--ensemble-gb", type=float, default=120.0)
     ap.add_argument("--workload", default="c2", choices=["c2", "small"])
     args = ap.parse_args(argv)
@@ -77,7 +81,7 @@ def main(argv=None):
         for r in rigs:
             if hybrid and r["p"]["n_particles"] >= args.alone_parents:
                 continue
-            need = 1.5 * scphd_amd.rig_footprint(r["p"], serve.expected_gaussians_per_parent(r["s"])) / 1e9
+            need = scphd_amd.rig_footprint(r["p"], int(args.growth * serve.expected_gaussians_per_parent(r["s"]))) / 1e9
             if cur and used + need > args.ensemble_gb:
                 groups.append(cur)
                 cur, used = [], 0.0
@@ -87,7 +91,7 @@ def main(argv=None):
             groups.append(cur)
         pairs, wall, lat = 0, 0.0, []
         for i, g in enumerate(groups):
-            e = serve.run_ensemble(g, progress=log)
+            e = serve.run_ensemble(g, progress=log, growth=args.growth)
             pairs += e["pairs"]
             wall += e["wall_s"]
             lat.append(e["latency_ms"])

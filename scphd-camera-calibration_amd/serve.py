@@ -4,8 +4,7 @@ stream frames at a fixed rate, and share one GPU through the batch API
 
 run_served() replays an arrival list in real time:
   * an arriving rig waits in a FIFO until its reservation fits the HBM budget
-    (scphd_batch_add_rig); the wait is its admission delay; on admission its
-    buffers are allocated at the expected map size (scphd_reserve);
+    (scphd_batch_add_rig); the wait is its admission delay;
   * once admitted, its frame k is due at t_admit + k / fps and is submitted
     when due (frames that come due while earlier ones are still queued or on
     the GPU wait in the rig's queue: that wait is part of their latency);
@@ -93,11 +92,6 @@ def run_served(rigs, hbm_budget_bytes=0, n_workers=8, reserve_factor=1.5, progre
                     blocked = True  # offered again once a rig leaves
                     break
                 waiting.pop(0)
-                try:  # the expected map size allocated at admission, inside the reservation
-                    b.filter(rid).reserve(r["p"]["n_particles"] * expected_gaussians_per_parent(r["s"]),
-                                          expected_measurements(r["s"]))
-                except scphd_amd.ScphdError:
-                    pass
                 adm.append(now - r["t"] if paced else 0.0)
                 active[rid] = dict(r=r, t_admit=now, sent=0, done=0, due={})
             for rid, a in active.items():
@@ -181,18 +175,20 @@ def run_plain(rigs, progress=None):
                 pairs=pairs, latency_ms=dict(p50=_pct(lat, 50), p99=_pct(lat, 99), max=float(lat.max())))
 
 
-def run_ensemble(rigs, progress=None):
+def run_ensemble(rigs, progress=None, growth=1.0):
     """The same rigs stepped together as one ensemble (scphd_create_ensemble):
     the rigs share the sample count, pruning, merging and motion settings
     (the library checks) and the frame sequence length and dt; an ensemble
     frame is one frame of every rig (predict, update, NaN check, resample,
-    every rig's estimate).  Latency is per ensemble frame."""
+    every rig's estimate).  Latency is per ensemble frame.  The buffers are
+    reserved for `growth` times the expected map size before the first
+    frame."""
     for r in rigs:
         if len(r["frames"]) != len(rigs[0]["frames"]) or r["s"]["dt"] != rigs[0]["s"]["dt"]:
             raise ValueError("an ensemble needs one frame sequence length and one dt")
     e = scphd_amd.Ensemble([r["p"] for r in rigs])
     try:  # the maps' steady-state size, allocated before the first frame
-        e.reserve(sum(r["p"]["n_particles"] * expected_gaussians_per_parent(r["s"]) for r in rigs),
+        e.reserve(int(growth * sum(r["p"]["n_particles"] * expected_gaussians_per_parent(r["s"]) for r in rigs)),
                   max(expected_measurements(r["s"]) for r in rigs))
     except scphd_amd.ScphdError:
         pass  # not enough memory to reserve up front: the buffers grow as the maps do
