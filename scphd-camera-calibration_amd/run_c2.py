@@ -15,7 +15,8 @@ a master seed (--workload small: workloads.small_ragged_rigs, parents in
   budget allows (scphd_batch_*), and a rig leaves when its frames are done.
 * ensemble: the rigs are packed, in order, into groups whose summed
   footprint (scphd_rig_footprint at --growth times the expected map size)
-  fits --ensemble-gb; each group is one ensemble (scphd_create_ensemble),
+  fits --ensemble-gb and whose sample planes stay below 4 GiB (the kernels'
+  32-bit offset form); each group is one ensemble (scphd_create_ensemble),
   its buffers reserved for that size before the first frame, whose
   frames step every rig of the group with one launch per kernel; the groups
   run one after the other.  The C2 maps keep growing over 100 frames (terms
@@ -77,16 +78,20 @@ def main(argv=None):
     if args.mode in ("ensemble", "hybrid", "both"):
         hybrid = args.mode == "hybrid"
         alone = [r for r in rigs if hybrid and r["p"]["n_particles"] >= args.alone_parents]
-        groups, cur, used = [], [], 0.0
+        groups, cur, used, blocks = [], [], 0.0, 0.0
+        plane_blocks = (4 << 30) / (100 * 4) / 1.25  # sample blocks a plane holds with 32-bit offsets
         for r in rigs:
             if hybrid and r["p"]["n_particles"] >= args.alone_parents:
                 continue
-            need = scphd_amd.rig_footprint(r["p"], int(args.growth * serve.expected_gaussians_per_parent(r["s"]))) / 1e9
-            if cur and used + need > args.ensemble_gb:
+            gp = int(args.growth * serve.expected_gaussians_per_parent(r["s"]))
+            need = scphd_amd.rig_footprint(r["p"], gp) / 1e9
+            nb = r["p"]["n_particles"] * gp
+            if cur and (used + need > args.ensemble_gb or blocks + nb > plane_blocks):
                 groups.append(cur)
-                cur, used = [], 0.0
+                cur, used, blocks = [], 0.0, 0.0
             cur.append(r)
             used += need
+            blocks += nb
         if cur:
             groups.append(cur)
         pairs, wall, lat = 0, 0.0, []
