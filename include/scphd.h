@@ -177,9 +177,11 @@ typedef struct scphd_update_stats {
 int scphd_get_update_stats(scphd_t* f, scphd_update_stats* s);
 
 /* Per-parent intermediates of the last update, for verification: eta
- * [N_p * M] log normalisers (parent-major), loglik[N_p] parent
- * log-likelihood, nmerged/nnondetect/nsurvivors[N_p] counts.  Any pointer may
- * be NULL.  No counterpart in the reference. */
+ * [N_p * M] log normalisers (parent-major; in an ensemble M is the largest
+ * rig's measurement count and a rig with fewer leaves the rest of its rows
+ * unwritten), loglik[N_p] parent log-likelihood,
+ * nmerged/nnondetect/nsurvivors[N_p] counts.  Any pointer may be NULL.  No
+ * counterpart in the reference. */
 int scphd_get_update_debug(scphd_t* f, double* eta, double* loglik, int32_t* nmerged,
                            int32_t* nnondetect, int32_t* nsurvivors);
 
