@@ -709,6 +709,32 @@ __device__ inline double det_lw(const double* r, double zu, double zv) {
  * likelihood records read by scalar loads. */
 static constexpr int ETA_HEAVY = 2048; /* LDS list of the Gaussians that can exceed bpc */
 
+/* e^x for -746 < x <= 0, the normaliser's terms (within 1 ulp of a
+ * correctly rounded exp): x = k ln2 + r with |r| <= ln2/2 by Cody and
+ * Waite's two-part reduction (ln2 split so that k ln2_hi is exact), e^r by
+ * its Taylor polynomial to degree 13 (truncation below 5e-18 relative) in
+ * Horner form, scaled by 2^k (gradual underflow below 2^-1022).  Cheaper
+ * than the general exp, which also handles overflow and special values. */
+__device__ inline double exp_nonpos(double x) {
+    const double k = rint(x * 1.4426950408889634);
+    const double r = fma(k, -1.90821492927058770002e-10, fma(k, -6.93147180369123816490e-01, x));
+    double p = 1.6059043836821613e-10; /* 1/13! */
+    p = fma(p, r, 2.08767569878681e-09);
+    p = fma(p, r, 2.505210838544172e-08);
+    p = fma(p, r, 2.755731922398589e-07);
+    p = fma(p, r, 2.7557319223985893e-06);
+    p = fma(p, r, 2.48015873015873e-05);
+    p = fma(p, r, 0.0001984126984126984);
+    p = fma(p, r, 0.001388888888888889);
+    p = fma(p, r, 0.008333333333333333);
+    p = fma(p, r, 0.041666666666666664);
+    p = fma(p, r, 0.16666666666666666);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(p, (int)k);
+}
+
 __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
     __shared__ double red[16];
     __shared__ int sh[8];
@@ -797,7 +823,7 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
                 for (int q = 0; q < 4; ++q) {
                     const double x = det_lw(R.v[q], zu, zv) - mx;
                     if (j0 + q < G && j0 + q != am && x > thr) {
-                        T += exp(x);
+                        T += exp_nonpos(x);
                         thr = fmax(-746.0, (double)(__builtin_amdgcn_frexp_exp(T) - 55) * 0.6931471805599453);
                     }
                     if (j0 + q < G && x > cgate && R.v[q][LK_OK] != 0.0) {
