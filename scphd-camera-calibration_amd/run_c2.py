@@ -77,23 +77,8 @@ def main(argv=None):
         print(json.dumps(out), flush=True)
     if args.mode in ("ensemble", "hybrid", "both"):
         hybrid = args.mode == "hybrid"
-        alone = [r for r in rigs if hybrid and r["p"]["n_particles"] >= args.alone_parents]
-        groups, cur, used, blocks = [], [], 0.0, 0.0
-        plane_blocks = (4 << 30) / (100 * 4) / 1.25  # sample blocks a plane holds with 32-bit offsets
-        for r in rigs:
-            if hybrid and r["p"]["n_particles"] >= args.alone_parents:
-                continue
-            gp = int(args.growth * serve.expected_gaussians_per_parent(r["s"]))
-            need = scphd_amd.rig_footprint(r["p"], gp) / 1e9
-            nb = r["p"]["n_particles"] * gp
-            if cur and (used + need > args.ensemble_gb or blocks + nb > plane_blocks):
-                groups.append(cur)
-                cur, used, blocks = [], 0.0, 0.0
-            cur.append(r)
-            used += need
-            blocks += nb
-        if cur:
-            groups.append(cur)
+        groups, alone = serve.pack_ensembles(rigs, args.ensemble_gb, args.growth,
+                                             args.alone_parents if hybrid else None)
         pairs, wall, lat = 0, 0.0, []
         for i, g in enumerate(groups):
             e = serve.run_ensemble(g, progress=log, growth=args.growth)

@@ -220,6 +220,35 @@ def run_ensemble(rigs, progress=None, growth=1.0):
                 pairs=pairs, latency_ms=dict(p50=_pct(lat, 50), p99=_pct(lat, 99), max=float(lat.max())))
 
 
+def pack_ensembles(rigs, budget_gb, growth=2.0, alone_parents=None, plane_bytes=4 << 30):
+    """Split prepared rigs into ensembles, in order: a group closes when the
+    next rig would take its summed footprint (scphd_rig_footprint at `growth`
+    times the expected map size) past budget_gb, or its sample planes (at
+    that size, with the buffers' growth slack of 1.25) past plane_bytes, the
+    limit of the kernels' 32-bit offsets.  Rigs with at least alone_parents
+    parents (when given) are returned apart, to run alone.  Returns (groups,
+    alone)."""
+    alone = [r for r in rigs if alone_parents is not None and r["p"]["n_particles"] >= alone_parents]
+    groups, cur, used, blocks = [], [], 0.0, 0.0
+    for r in rigs:
+        if alone_parents is not None and r["p"]["n_particles"] >= alone_parents:
+            continue
+        gp = int(growth * expected_gaussians_per_parent(r["s"]))
+        need = scphd_amd.rig_footprint(r["p"], gp) / 1e9
+        esz = 8 if r["p"].get("storage", 0) == 1 else 4
+        max_blocks = plane_bytes / (r["p"]["particles_per_feature"] * esz) / 1.25
+        nb = r["p"]["n_particles"] * gp
+        if cur and (used + need > budget_gb or blocks + nb > max_blocks):
+            groups.append(cur)
+            cur, used, blocks = [], 0.0, 0.0
+        cur.append(r)
+        used += need
+        blocks += nb
+    if cur:
+        groups.append(cur)
+    return groups, alone
+
+
 def offered_load(rigs, step_ms_per_gaussian_sample):
     """Rough device time the arrivals ask for per second of wall time, from a
     measured cost per (Gaussian x sample) of one step."""

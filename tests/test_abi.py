@@ -276,3 +276,31 @@ def test_seed_is_parsed_exactly_to_64_bits(tmp_path):
     cfg = tmp_path / "hex.cfg"
     cfg.write_text(workloads.to_cfg(p).replace(f"seed = {p['seed']}", "seed = 0xFFFFFFFFFFFFFFC5L"))
     assert scphd_amd.load_params(str(cfg))["seed"] == 2 ** 64 - 59
+
+
+def test_ensemble_packing_respects_budget_and_plane_limit():
+    """serve.pack_ensembles (host logic only): every rig in exactly one group,
+    in order; each group's footprint within the budget and its sample planes
+    within the 32-bit offset limit (a single rig may exceed either alone);
+    rigs from the alone threshold up are set apart."""
+    import serve
+    import scphd_amd
+
+    rigs = [dict(t=0.0, p=p, s=s, frames=[]) for p, s in workloads.c2_rigs(master=1, n_rigs=64, n_frames=1)]
+    budget, growth = 120.0, 2.0
+    for alone_parents in (None, 2048):
+        groups, alone = serve.pack_ensembles(rigs, budget, growth, alone_parents)
+        flat = [r for g in groups for r in g]
+        assert [id(r) for r in flat] == [id(r) for r in rigs if r not in alone]
+        if alone_parents is None:
+            assert alone == []
+        else:
+            assert alone and all(r["p"]["n_particles"] >= alone_parents for r in alone)
+            assert all(r["p"]["n_particles"] < alone_parents for r in flat)
+        for g in groups:
+            gps = [int(growth * serve.expected_gaussians_per_parent(r["s"])) for r in g]
+            foot = sum(scphd_amd.rig_footprint(r["p"], gp) for r, gp in zip(g, gps)) / 1e9
+            blocks = sum(r["p"]["n_particles"] * gp for r, gp in zip(g, gps))
+            if len(g) > 1:
+                assert foot <= budget
+                assert blocks * 100 * 4 * 1.25 <= 4 << 30
