@@ -1,11 +1,14 @@
-"""The plain filter step at its full benchmark size -- 4096 parents, S = 200,
-200 landmarks, clutter lambda 50, FP32 storage -- checked through a property
-that holds at every size.
+"""The plain filter step at full benchmark sizes, checked through a property
+that holds at every size:
+  * 4096 parents, S = 200, 200 landmarks, clutter lambda 50 (the bench
+    workload), with FP32 and with FP64 sample storage;
+  * 16384 parents, S = 100, 300 landmarks, clutter lambda 100 (the precision
+    study), FP32 storage.
 
 Each parent's map update depends only on that parent's own state and on
 noise counters that carry its parent index (and the Gaussian index within
 the run's layout, which is the same for the first parents of any run). The
-first K parents of the 4096-parent GPU run therefore equal a K-parent oracle
+first K parents of the full-size GPU run therefore equal a K-parent oracle
 run from the same seed, as long as no resampling mixes the parents. Frame 0
 has a fixed camera and births only, so the weights stay uniform and nothing
 is resampled. Frame 1 moves the camera and brings detection, non-detection,
@@ -15,10 +18,11 @@ merge and birth terms. For parents 0 .. K-1 the test compares:
   * within tolerance: predicted parent states (1e-13), normalisers eta
     (1e-11 relative), parent log-likelihoods (1e-11 relative, 1e-9
     absolute), and the parent weights as ratios to parent 0 (1e-8
-    relative). Ratios, because the GPU normalises over 4096 parents and the
+    relative). Ratios, because the GPU normalises over all parents and the
     oracle over K.
-Every launch, scan and compaction runs at its full-size shape (4096
-workgroups per per-parent kernel, ~10^6 Gaussians after frame 1).
+Every launch, scan and compaction runs at its full-size shape (one
+workgroup per parent in the per-parent kernels, 1-10 million Gaussians
+after frame 1).
 """
 import numpy as np
 import pytest
@@ -43,11 +47,14 @@ def _close(a, b, rtol, atol, what):
                            f"{int(np.argmax(err))} ({a.flat[np.argmax(err)]!r} vs {b.flat[np.argmax(err)]!r})")
 
 
-def test_first_parents_of_the_full_size_step_equal_a_small_oracle_run():
+@pytest.mark.parametrize("name,storage,parents,samples", [("c1", 0, 4096, 200), ("c1", 1, 4096, 200),
+                                                            ("c3", 0, 16384, 100)],
+                         ids=["c1-fp32", "c1-fp64", "c3-fp32"])
+def test_first_parents_of_the_full_size_step_equal_a_small_oracle_run(name, storage, parents, samples):
     import scphd_amd
 
-    p, s = workloads.workload("c1")
-    assert p["n_particles"] == 4096 and p["particles_per_feature"] == 200 and p["storage"] == 0
+    p, s = workloads.workload(name, storage=storage)
+    assert p["n_particles"] == parents and p["particles_per_feature"] == samples
     sc = scene.generate(seed=p["seed"], **dict(s, n_frames=2))
     gpu = scphd_amd.Filter(p)
     orc = oracle.Oracle(dict(p, n_particles=K), threads=16)
@@ -63,7 +70,7 @@ def test_first_parents_of_the_full_size_step_equal_a_small_oracle_run():
             gpu.update(u, v, fixed)
             orc.update(u, v, fixed)
             dg, do = gpu.update_debug(), orc.update_debug()
-            assert dg["eta"].shape == (4096, len(u))
+            assert dg["eta"].shape == (parents, len(u))
             for key in ("nsurvivors", "nmerged", "nnondetect"):
                 np.testing.assert_array_equal(dg[key][:K], do[key], err_msg=f"frame {k} {key}")
             _close(dg["eta"][:K], do["eta"], 1e-11, 1e-11, f"frame {k} eta")
@@ -76,8 +83,8 @@ def test_first_parents_of_the_full_size_step_equal_a_small_oracle_run():
                 rg, ro = gpu.resample(), orc.resample()
                 assert rg == ro == 0, "frame 0 resampled: parents mixed, the slice comparison would not hold"
         c = gpu.counts()
-        print(f"full-size step: {c['n_gaussians']} Gaussians over 4096 parents after frame 1, "
+        print(f"full-size step: {c['n_gaussians']} Gaussians over {parents} parents after frame 1, "
               f"{len(u)} measurements")
-        assert c["n_gaussians"] > 4096 * 100
+        assert c["n_gaussians"] > parents * 100
     finally:
         gpu.close()
