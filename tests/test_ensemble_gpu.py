@@ -158,3 +158,68 @@ def test_reserve_changes_no_result():
             f.close()
     for key in ("states", "weights", "gcount", "wg", "samples"):
         np.testing.assert_array_equal(out[0][key], out[1][key], err_msg=key)
+
+
+BIG = [dict(seed=505, n_particles=4096, n_landmarks=200, lam=50.0, f=500.0, baseline=0.2),
+       dict(seed=606, n_particles=1024, n_landmarks=120, lam=30.0, f=600.0, baseline=0.25)]
+
+
+def test_ensemble_of_full_size_rigs_matches_plain_filters_frame_by_frame():
+    """An ensemble of a 4096-parent rig with S = 200 (the bench workload's
+    size) and a 1024-parent rig, stepped beside one plain filter per rig:
+    every frame, parents, weights, normalisers, log-likelihoods, per-parent
+    survivor / merged / non-detection counts, per-rig estimates and
+    resampling decisions are bit-identical.  The maps themselves (tens of GB
+    at this size) are covered through those counts and the normalisers,
+    which depend on every sample."""
+    import scphd_amd
+
+    frames_n = 4
+    rigs = []
+    for r, cfg in enumerate(BIG):
+        p, s = workloads.rig(n_frames=frames_n, scene_rig=r, samples=200, **cfg)
+        sc = scene.generate(seed=cfg["seed"], **s)
+        rigs.append((p, s, [scene.frame_measurements(sc, k) for k in range(frames_n)]))
+    dt = rigs[0][1]["dt"]
+    plain = [scphd_amd.Filter(p) for p, _, _ in rigs]
+    e = scphd_amd.Ensemble([p for p, _, _ in rigs])
+    try:
+        for k in range(frames_n):
+            fixed = rigs[0][2][k][2]
+            us = [np.asarray(r[2][k][0], dtype=np.float64) for r in rigs]
+            vs = [np.asarray(r[2][k][1], dtype=np.float64) for r in rigs]
+            e.predict(dt)
+            e.update(us, vs, fixed)
+            de = e.update_debug()
+            rse = e.resample()
+            flags, _ = e.rig_resample()
+            me, ce, carde = e.rig_estimates()
+            se, we = e.parents()
+            p0 = 0
+            for r, f in enumerate(plain):
+                f.predict(dt)
+                f.update(us[r], vs[r], fixed)
+                dp = f.update_debug()
+                rs = f.resample()
+                m, c, card = f.estimate()
+                s_, w_ = f.parents()
+                n = BIG[r]["n_particles"]
+                sl = slice(p0, p0 + n)
+                p0 += n
+                M = len(us[r])
+                what = f"frame {k} rig {r}"
+                np.testing.assert_array_equal(de["eta"][sl, :M], dp["eta"], err_msg=f"{what} eta")
+                for key in ("loglik", "nsurvivors", "nmerged", "nnondetect"):
+                    np.testing.assert_array_equal(de[key][sl], dp[key], err_msg=f"{what} {key}")
+                np.testing.assert_array_equal(se[sl], s_, err_msg=f"{what} states")
+                np.testing.assert_array_equal(we[sl], w_, err_msg=f"{what} weights")
+                assert bool(flags[r]) == rs, f"{what} resampling decision"
+                np.testing.assert_array_equal(me[r], m, err_msg=f"{what} mean")
+                np.testing.assert_array_equal(ce[r], c, err_msg=f"{what} covariance")
+                assert carde[r] == card, f"{what} cardinality"
+            assert rse == any(bool(x) for x in flags)
+        assert e.counts()["n_gaussians"] > 4096 * 100
+    finally:
+        e.close()
+        for f in plain:
+            f.close()
