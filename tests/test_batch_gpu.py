@@ -3,7 +3,8 @@
 * A rig's results are the same alone and in a mix: each rig is run alone
   through a plain Filter, then together with other rigs through one Batch
   with its frames interleaved with theirs; parents, weights, maps and the
-  per-frame estimates must be bit-identical.
+  per-frame estimates must be bit-identical; also at full size (a
+  4096-parent rig with S = 200 beside a 2048-parent rig).
 * Admission by HBM reservation, removal, and re-admission.
 * The device-side estimate (scphd_get_estimate) against numpy on the
   parents read back.
@@ -208,3 +209,54 @@ def test_check_reports_a_nan_parent_weight():
     f.set_state(st)
     assert f.check() == 0
     f.close()
+
+
+def test_full_size_rigs_are_identical_alone_and_in_a_batch():
+    """A 4096-parent rig with S = 200 (the bench workload's size) and a
+    2048-parent rig with S = 100, served together by one Batch with their
+    frames interleaved: per-frame estimates, and the parents and weights at
+    the end, are bit-identical to each rig run alone.  The maps (tens of GB
+    here) are left out; the estimates depend on every map through the
+    parent weights."""
+    import scphd_amd
+
+    big = [dict(seed=21, n_particles=4096, n_landmarks=200, lam=50.0, samples=200),
+           dict(seed=22, n_particles=2048, n_landmarks=150, lam=40.0, f=600.0, baseline=0.25, samples=100)]
+    frames = 4
+    made = []
+    for i, cfg in enumerate(big):
+        p, s = workloads.rig(n_frames=frames, scene_rig=i, **cfg)
+        sc = scene.generate(seed=p["seed"], **s)
+        made.append((p, s, [scene.frame_measurements(sc, k) for k in range(frames)]))
+    ref = []
+    for p, s, sets in made:
+        f = scphd_amd.Filter(p)
+        est = []
+        for u, v, fixed in sets:
+            f.predict(s["dt"])
+            f.update(u, v, fixed)
+            assert f.check() == 0
+            f.resample()
+            est.append(f.estimate()[0])
+        ref.append((f.parents(), np.array(est)))
+        f.close()
+    b = scphd_amd.Batch(n_workers=2)
+    try:
+        rigs = [b.add_rig(p) for p, _, _ in made]
+        assert None not in rigs
+        for k in range(frames):
+            for r, (_, s, sets) in zip(rigs, made):
+                u, v, fixed = sets[k]
+                b.submit(r, k, s["dt"], u, v, fixed)
+        assert b.step() == frames * len(big)
+        res = b.collect()
+        assert len(res) == frames * len(big) and all(x["status"] == 0 for x in res)
+        for i, r in enumerate(rigs):
+            mine = sorted((x for x in res if x["rig"] == r), key=lambda x: x["frame_id"])
+            np.testing.assert_array_equal(np.array([x["estimate"] for x in mine]), ref[i][1],
+                                          err_msg=f"rig {i} estimates")
+            states, weights = b.filter(r).parents()
+            np.testing.assert_array_equal(states, ref[i][0][0], err_msg=f"rig {i} states")
+            np.testing.assert_array_equal(weights, ref[i][0][1], err_msg=f"rig {i} weights")
+    finally:
+        b.close()
