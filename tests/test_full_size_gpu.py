@@ -23,6 +23,10 @@ merge and birth terms. For parents 0 .. K-1 the test compares:
 Every launch, scan and compaction runs at its full-size shape (one
 workgroup per parent in the per-parent kernels, 1-10 million Gaussians
 after frame 1).
+
+Resampling mixes all parents, so it is checked on its own: from the same
+weights and counters at 4096, 16384 and 65536 parents, the decision and
+every ancestor index equal the oracle's.
 """
 import numpy as np
 import pytest
@@ -86,5 +90,47 @@ def test_first_parents_of_the_full_size_step_equal_a_small_oracle_run(name, stor
         print(f"full-size step: {c['n_gaussians']} Gaussians over {parents} parents after frame 1, "
               f"{len(u)} measurements")
         assert c["n_gaussians"] > parents * 100
+    finally:
+        gpu.close()
+
+
+@pytest.mark.parametrize("parents", [4096, 16384, 65536])
+@pytest.mark.parametrize("weights", ["spread", "near-uniform", "zeros-and-one-heavy"])
+def test_resampling_at_full_size_equals_the_oracle(parents, weights):
+    """Systematic resampling over all parents at once (one FP64 prefix sum,
+    one uniform from the resampling stream): the decision and every ancestor
+    index equal the oracle's, from the same weights and counters.  The maps
+    are empty, because resampling reads only the parent weights.  Weight
+    sets: log-normal spread (resamples), near-uniform (n_eff above the
+    threshold: no resampling), and a set with zero weights and one parent
+    holding half the mass."""
+    import scphd_amd
+
+    p, _ = workloads.workload("c1", n_particles=parents)
+    rng = np.random.default_rng(parents)
+    if weights == "spread":
+        w = rng.lognormal(0.0, 2.0, parents)
+    elif weights == "near-uniform":
+        w = 1.0 + 0.01 * rng.random(parents)
+    else:
+        w = rng.random(parents)
+        w[rng.random(parents) < 0.3] = 0.0
+        w[parents // 3] = w.sum()
+    w /= w.sum()
+    st = dict(states=rng.normal(0.0, 0.1, (parents, 12)), weights=w, gcount=np.zeros(parents, dtype=np.int32),
+              wg=np.zeros(0), samples=np.zeros((7, 0)), n_predict=3, n_update=3, n_resample=2)
+    gpu = scphd_amd.Filter(p)
+    orc = oracle.Oracle(p, threads=16)
+    try:
+        gpu.set_state(st)
+        orc.set_state(dict(st, wg=np.zeros(1), samples=np.zeros((7, 1))))
+        rg, ro = gpu.resample(), orc.resample()
+        assert rg == ro == (weights != "near-uniform")
+        if ro:
+            np.testing.assert_array_equal(gpu.ancestors(), orc.ancestors()[0])
+        sg, wg = gpu.parents()
+        so, wo = orc.parents()
+        np.testing.assert_array_equal(sg, so)
+        np.testing.assert_array_equal(wg, wo)
     finally:
         gpu.close()
