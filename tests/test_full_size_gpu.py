@@ -12,7 +12,9 @@ first K parents of the full-size GPU run therefore equal a K-parent oracle
 run from the same seed, as long as no resampling mixes the parents. Frame 0
 has a fixed camera and births only, so the weights stay uniform and nothing
 is resampled. Frame 1 moves the camera and brings detection, non-detection,
-merge and birth terms. For parents 0 .. K-1 the test compares:
+merge and birth terms. Frame 2 follows without resampling, with an empty
+measurement set (non-detection terms only). For parents 0 .. K-1 the test
+compares after every frame:
   * bit-exact: survivor, merged and kept non-detection counts, and the
     resampling decision of frame 0;
   * within tolerance: predicted parent states (1e-13), normalisers eta
@@ -66,8 +68,10 @@ def test_first_parents_of_the_full_size_step_equal_a_small_oracle_run(name, stor
         sg, wg = gpu.parents()
         so, wo = orc.parents()
         _close(sg[:K], so, 1e-14, 1e-15, "prior")
-        for k in range(2):
-            u, v, fixed = scene.frame_measurements(sc, k)
+        for k in range(3):
+            u, v, fixed = scene.frame_measurements(sc, min(k, 1))
+            if k == 2:  # an empty measurement set: non-detection terms only
+                u, v = np.zeros(0), np.zeros(0)
             gpu.predict(s["dt"])
             orc.predict(s["dt"])
             _close(gpu.parents()[0][:K], orc.parents()[0], 1e-13, 1e-13, f"frame {k} predicted parents")
@@ -87,8 +91,8 @@ def test_first_parents_of_the_full_size_step_equal_a_small_oracle_run(name, stor
                 rg, ro = gpu.resample(), orc.resample()
                 assert rg == ro == 0, "frame 0 resampled: parents mixed, the slice comparison would not hold"
         c = gpu.counts()
-        print(f"full-size step: {c['n_gaussians']} Gaussians over {parents} parents after frame 1, "
-              f"{len(u)} measurements")
+        print(f"full-size step: {c['n_gaussians']} Gaussians over {parents} parents after frame 2 "
+              f"(empty measurement set)")
         assert c["n_gaussians"] > parents * 100
     finally:
         gpu.close()
