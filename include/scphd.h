@@ -208,6 +208,23 @@ int scphd_get_ancestors(scphd_t* f, int32_t* ancestors);
 int scphd_set_timing(scphd_t* f, int32_t enabled);
 int scphd_get_phase_ms(scphd_t* f, double* ms8);
 
+/* Verification limits: lower the capacities at which the update falls back
+ * to its general code paths, so that small inputs exercise them.
+ *   merge_lds_cap       parents with more survivors merge in global scratch
+ *                       (k_merge_groups<true>); default: what fits 60 KB LDS
+ *   eta_heavy_cap       0..2048: more Gaussians above the birth/clutter term
+ *                       -> the normaliser max pass sweeps the whole map
+ *   eta_ncand_cap       0..4: more candidate terms per measurement -> the
+ *                       normaliser count and the survivor pass sweep the map
+ *   merge_force_rounds  nonzero: every parent merges in the round loop
+ *                       (k_merge_rounds), the path of candidate or
+ *                       neighbour-list overflow
+ * -1 (0 for the last) restores the normal limit.  Every setting gives the
+ * same results; no counterpart in the reference, whose merge has no
+ * capacity limits (src/kernels.cu:662-874). */
+int scphd_debug_set_limits(scphd_t* f, int32_t merge_lds_cap, int32_t eta_heavy_cap, int32_t eta_ncand_cap,
+                           int32_t merge_force_rounds);
+
 /* Noise source of the handle.  mode 0: the counter-based Philox4x32-10
  * streams, keyed by seed from the next call on (the same key as
  * scphd_params.seed).  Injected noise buffers (the reference's parity
@@ -236,7 +253,7 @@ int scphd_get_estimate(scphd_t* f, double* mean12, double* cov144, double* cardi
 
 /* ---- ensembles: many rigs in one filter ----
  * No counterpart in the reference, which runs one rig per process
- * (src/main.cu:52-263).  An ensemble of n_rigs rigs, rig r configured by
+ * (src/main.cpp:53-99).  An ensemble of n_rigs rigs, rig r configured by
  * rigs[r], is stepped by the same kernel launches as one filter of
  * sum_r rigs[r].n_particles parents: rig r draws every noise stream from
  * the key of rigs[r].seed with its parents numbered from 0, projects with

@@ -351,6 +351,8 @@ struct scphd_filter {
     hipEvent_t ev[10] = {};  /* 0..5 phase boundaries, 6..7 fit kernel, 8..9 sample kernel */
     double phase_ms[8] = {};
     int lds_cap = 0;
+    /* verification limits (scphd_debug_set_limits); -1 = the normal limit */
+    int dbg_lds_cap = -1, dbg_heavy = -1, dbg_ncand = -1, dbg_rounds = 0;
 };
 
 namespace {
@@ -570,6 +572,8 @@ int update_t(scphd_filter* f, int M, int fixed) {
     ea.mncand = f->mncand.p;
     ea.mcand = f->mcand.p;
     ea.lw_gate = P.min_weight > 0 ? std::log(P.min_weight) - 1e-3 : -INFINITY;
+    ea.heavy_cap = f->dbg_heavy >= 0 ? std::min(f->dbg_heavy, ETA_HEAVY) : ETA_HEAVY;
+    ea.ncand_cap = f->dbg_ncand >= 0 ? std::min(f->dbg_ncand, ETA_NCAND) : ETA_NCAND;
     HIPTRY(launch_eta(s, ea));
     HIPTRY(launch_scan(s, f->nsurv.p, f->survoff.p, Np, f->scantmp.p));
     HIPTRY(launch_readout(s, f->survoff.p, Np, 0, nullptr, nullptr, f->dmap));
@@ -614,7 +618,9 @@ int update_t(scphd_filter* f, int M, int fixed) {
     ma.births = f->dbirths.p;
     ma.min_separation = P.min_separation;
     ma.Np = Np;
-    ma.lds_cap = f->lds_cap;
+    const int lds_cap = f->dbg_lds_cap >= 0 ? std::min(f->dbg_lds_cap, f->lds_cap) : f->lds_cap;
+    ma.lds_cap = lds_cap;
+    ma.force_rounds = f->dbg_rounds;
     ma.scratch = nullptr;
     ma.mw = f->mw.p;
     ma.mmu = f->mmu.p;
@@ -631,11 +637,11 @@ int update_t(scphd_filter* f, int M, int fixed) {
     ma.gend = f->gend.p;
     ma.gmem = f->gmem.p;
     ma.Stot = Stot;
-    if (maxU > f->lds_cap) {
+    if (maxU > lds_cap) {
         HIPTRY(f->mscratch.ensure(merge_scratch_bytes(Stot, Np)));
         ma.scratch = f->mscratch.p;
     }
-    HIPTRY(launch_merge(s, ma, merge_lds_bytes(std::min(std::max(maxU, 1), f->lds_cap))));
+    HIPTRY(launch_merge(s, ma, merge_lds_bytes(std::max(1, std::min(maxU, lds_cap)))));
     if (!fixed)
         HIPTRY(launch_parent_weights(s, f->pw.p, f->loglik.p, Np, f->ensemble ? f->dpoff.p : nullptr, f->n_rigs));
     record(f, 3);
@@ -1222,6 +1228,19 @@ int scphd_set_timing(scphd_t* f, int32_t enabled) {
 int scphd_get_phase_ms(scphd_t* f, double* ms8) {
     if (check_handle(f) || !ms8) return SCPHD_ERR_ARG;
     for (int i = 0; i < 8; ++i) ms8[i] = f->phase_ms[i];
+    return SCPHD_OK;
+}
+
+int scphd_debug_set_limits(scphd_t* f, int32_t merge_lds_cap, int32_t eta_heavy_cap, int32_t eta_ncand_cap,
+                           int32_t merge_force_rounds) {
+    if (check_handle(f)) return SCPHD_ERR_ARG;
+    if (merge_lds_cap < -1 || eta_heavy_cap < -1 || eta_ncand_cap < -1 || eta_heavy_cap > ETA_HEAVY ||
+        eta_ncand_cap > ETA_NCAND)
+        return fail(f, SCPHD_ERR_ARG, "debug limits out of range");
+    f->dbg_lds_cap = merge_lds_cap;
+    f->dbg_heavy = eta_heavy_cap;
+    f->dbg_ncand = eta_ncand_cap;
+    f->dbg_rounds = merge_force_rounds != 0;
     return SCPHD_OK;
 }
 

@@ -157,8 +157,11 @@ struct EtaArgs {
     int32_t* mncand;   /* Np*M candidates per (parent, measurement): terms that may be kept */
     int32_t* mcand;    /* Np*M*ETA_NCAND their map indices (ascending), when at most ETA_NCAND */
     double lw_gate;    /* log(min_weight) - 1e-3: below it exp() < min_weight for certain */
+    int heavy_cap;     /* at most ETA_HEAVY: more Gaussians above bpc than this -> whole-map max pass */
+    int ncand_cap;     /* at most ETA_NCAND: more candidates than this -> whole-map sweeps */
 };
 static constexpr int ETA_NCAND = 4;
+static constexpr int ETA_HEAVY = 2048; /* LDS list of the Gaussians that can exceed bpc */
 hipError_t launch_eta(hipStream_t s, const EtaArgs& a);
 
 struct SurvArgs {
@@ -187,6 +190,7 @@ struct MergeArgs {
     double min_separation;
     int Np;
     int32_t lds_cap;   /* parents with more survivors use global scratch */
+    int32_t force_rounds; /* nonzero: every parent takes the round loop (verification of that path) */
     char* scratch;     /* global scratch for big parents: merge_scratch_bytes(Stot, Np) */
     double* mw;        /* merged outputs at survoff[p] + k */
     double* mmu;       /* 6 planes, stride scap */

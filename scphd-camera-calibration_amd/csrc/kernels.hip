@@ -707,7 +707,6 @@ __device__ inline double det_lw(const double* r, double zu, double zv) {
  * min_weight (updateKernel's prune flag, src/kernels.cu:608-659).
  * One workgroup per parent, one thread per measurement, the Gaussians'
  * likelihood records read by scalar loads. */
-static constexpr int ETA_HEAVY = 2048; /* LDS list of the Gaussians that can exceed bpc */
 
 /* e^x for -746 < x <= 0, the normaliser's terms (within 1 ulp of a
  * correctly rounded exp): x = k ln2 + r with |r| <= ln2/2 by Cody and
@@ -757,8 +756,8 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
      * l <= C - log 2 pi - H (the value at dist = 0; the computed quadratic
      * form is never below -1e-9 of its scale), so the max pass visits only
      * those whose bound exceeds bpc (with a 1e-6 relative margin), in
-     * ascending order: the same first maximum.  More than ETA_HEAVY of them:
-     * the whole map. */
+     * ascending order: the same first maximum.  More than heavy_cap (at most
+     * ETA_HEAVY) of them: the whole map. */
     int nheavy = 0;
     for (int j0 = 0; j0 < G; j0 += NT) {
         const int j = j0 + t;
@@ -774,7 +773,7 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
         nheavy += tot;
     }
     __syncthreads();
-    const bool listed = nheavy <= ETA_HEAVY;
+    const bool listed = nheavy <= a.heavy_cap;
     for (int r = 0; r < rounds; ++r) {
         const int m = r * NT + t;
         const bool valid = m < M;
@@ -824,7 +823,10 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
                     const double x = det_lw(R.v[q], zu, zv) - mx;
                     if (j0 + q < G && j0 + q != am && x > thr) {
                         T += exp_nonpos(x);
-                        thr = fmax(-746.0, (double)(__builtin_amdgcn_frexp_exp(T) - 55) * 0.6931471805599453);
+                        /* the bound holds only once T > 0: a term that
+                         * underflowed to +0 leaves the threshold at -746 */
+                        thr = T > 0.0 ? fmax(-746.0, (double)(__builtin_amdgcn_frexp_exp(T) - 55) * 0.6931471805599453)
+                                      : -746.0;
                     }
                     if (j0 + q < G && x > cgate && R.v[q][LK_OK] != 0.0) {
                         const int j = j0 + q;
@@ -843,7 +845,7 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
             const int64_t pm = (int64_t)p * a.M + m;
             a.eta[pm] = eta;
             eta_sum += eta;
-            if (nc <= ETA_NCAND) { /* the candidates alone */
+            if (nc <= a.ncand_cap) { /* the candidates alone */
                 for (int k = 0; k < nc; ++k) {
                     const int j = k == 0 ? c0 : k == 1 ? c1 : k == 2 ? c2 : c3;
                     const double* rec = a.lik + (g0 + j) * LK_COUNT;
@@ -933,7 +935,7 @@ __global__ __launch_bounds__(NT) void k_survivors(SurvArgs a) {
                 ++o;
             };
             const int nc = e.mncand[pm];
-            if (nc <= ETA_NCAND) { /* the candidates k_eta listed, ascending */
+            if (nc <= e.ncand_cap) { /* the candidates k_eta listed, ascending */
                 for (int k = 0; k < nc; ++k) {
                     const int j = e.mcand[pm * ETA_NCAND + k];
                     term(j, e.lik + (g0 + j) * LK_COUNT);
@@ -1197,7 +1199,7 @@ __global__ __launch_bounds__(NT) void k_merge_groups(MergeArgs a) {
         }
         return;
     }
-    if (U >= 65536) { /* candidate pairs pack two 16-bit indices: the round loop takes it */
+    if (U >= 65536 || a.force_rounds) { /* candidate pairs pack two 16-bit indices: the round loop takes it */
         if (t == 0) {
             a.ngroups[p] = -1;
             a.nmerged[p] = 0;

@@ -9,7 +9,10 @@ Philox noise streams, and compares:
     parent log-likelihoods, Gaussian weights, sample clouds.
 Tolerances: FP64 storage: 1e-9 relative on weights / normalisers, 1e-7 on
 samples (sample positions go through 1/d); FP32 storage: samples within 4
-float ulps (2.5e-7 relative) plus 1e-6 absolute.
+float ulps (2.5e-7 relative) plus 1e-6 absolute.  The refit of every cloud
+(disparity-space mean, covariance, pd_g) is compared too: means within
+1e-12 relative, covariances within 1e-9 of sqrt(P_aa P_bb) (the GPU sums
+the moments in one pass, the oracle in two), pd_g within 1e-13.
 """
 import numpy as np
 import pytest
@@ -42,12 +45,39 @@ def _close(a, b, rtol, atol, what):
     return float((err / np.maximum(np.maximum(np.abs(a), np.abs(b)), 1e-300)).max())
 
 
-def run_lockstep(p, s, frames, warm=0):
+def compare_fit(fg, fo, what):
+    """Per-cloud refit planes (mean 6, packed covariance 21, pd_g)."""
+    assert fg.shape == fo.shape, f"{what}: fit shape {fg.shape} vs {fo.shape}"
+    if fo.shape[1] == 0:
+        return
+    _close(fg[:6], fo[:6], 1e-12, 1e-12, f"{what} fit means")
+    diag = {}
+    e = 6
+    for a in range(6):
+        for b in range(a + 1):
+            if a == b:
+                diag[a] = np.abs(fo[e])
+            e += 1
+    e = 6
+    for a in range(6):
+        for b in range(a + 1):
+            scale = np.sqrt(diag[a] * diag[b])
+            err = np.abs(fg[e] - fo[e])
+            bad = err > 1e-9 * scale + 1e-12
+            assert not bad.any(), (f"{what} fit covariance ({a},{b}): {bad.sum()} of {err.size} differ, worst "
+                                   f"{err.max():.3e}")
+            e += 1
+    _close(fg[27], fo[27], 1e-13, 1e-15, f"{what} fit pd_g")
+
+
+def run_lockstep(p, s, frames, warm=0, limits=None):
     import scphd_amd
 
     sc = scene.generate(seed=p["seed"], **s)
     orc = oracle.Oracle(p, threads=8)
     gpu = scphd_amd.Filter(p)
+    if limits:
+        gpu.set_debug_limits(**limits)
     # same prior draw
     so, wo = orc.parents()
     sg, wg = gpu.parents()
@@ -72,6 +102,8 @@ def run_lockstep(p, s, frames, warm=0):
         do, dg = orc.update_debug(), gpu.update_debug()
         for key in ("nsurvivors", "nmerged", "nnondetect"):
             np.testing.assert_array_equal(dg[key], do[key], err_msg=f"frame {k} {key}")
+        ng = gpu.update_stats()["n_gaussians_in"]
+        compare_fit(gpu.fit_debug(), orc.fit_debug(ng), f"frame {k}")
         _close(dg["eta"], do["eta"], 1e-11, 1e-11, f"frame {k} eta")
         _close(dg["loglik"], do["loglik"], 1e-11, 1e-9, f"frame {k} loglik")
         gco, wgo, smo = orc.maps()
@@ -84,7 +116,7 @@ def run_lockstep(p, s, frames, warm=0):
         assert ro == rg, f"frame {k} resample decision"
         if ro:
             np.testing.assert_array_equal(gpu.ancestors(), orc.ancestors()[0], err_msg=f"frame {k} ancestors")
-        report.append((k, int(gco.sum()), e))
+        report.append((k, int(gco.sum()), e, int(do["nsurvivors"].max())))
     gpu.close()
     return report
 
@@ -177,3 +209,49 @@ def test_device_noise_equals_the_oracle_bit_for_bit():
         dev = scphd_amd.device_normals(seed, stream, step, parent, gauss, n)
         ref = np.stack([oracle.normals(seed, stream, step, parent, gauss, e, 6) for e in range(n)])
         np.testing.assert_array_equal(dev, ref)
+
+
+# ---- general code paths, forced at small sizes through the verification
+# ---- limits (scphd_debug_set_limits); each must give the oracle's results
+
+def _grown(n_particles=24, storage=0):
+    """c0 geometry, 24 parents, from a state grown by 6 oracle frames: maps
+    of ~150 Gaussians and ~100-300 survivors per parent."""
+    return small(storage, n_particles=n_particles)
+
+
+def test_lockstep_merge_in_global_scratch():
+    """Survivor lists longer than 16 merge in global scratch
+    (k_merge_groups<true>), the path of parents beyond the LDS workspace."""
+    p, s = _grown()
+    rep = run_lockstep(p, s, frames=3, warm=6, limits=dict(merge_lds_cap=16))
+    assert max(r[3] for r in rep) > 16
+
+
+def test_lockstep_merge_round_loop_lds():
+    """Every parent merges in the round loop (k_merge_rounds<false>), the
+    path of candidate or neighbour-list overflow."""
+    p, s = _grown(storage=1)
+    run_lockstep(p, s, frames=3, warm=6, limits=dict(merge_force_rounds=True))
+
+
+def test_lockstep_merge_round_loop_global_scratch():
+    """Round loop with its workspace in global scratch (k_merge_rounds<true>)."""
+    p, s = _grown()
+    run_lockstep(p, s, frames=3, warm=6, limits=dict(merge_lds_cap=16, merge_force_rounds=True))
+
+
+def test_lockstep_normaliser_whole_map_max_pass():
+    """No list of the Gaussians above the birth/clutter term: the normaliser
+    max pass sweeps the whole map (the path of more than 2048 such
+    Gaussians)."""
+    p, s = _grown()
+    run_lockstep(p, s, frames=3, warm=6, limits=dict(eta_heavy_cap=0))
+
+
+def test_lockstep_whole_map_candidate_sweeps():
+    """No candidate lists: the normaliser's kept-term count and the survivor
+    pass sweep the whole map for every measurement (the path of more than
+    four candidate terms per measurement)."""
+    p, s = _grown(storage=1)
+    run_lockstep(p, s, frames=3, warm=6, limits=dict(eta_ncand_cap=0))
