@@ -15,11 +15,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
-#include <map>
-#include <mutex>
 #include <string>
 #include <vector>
 
+#include "devmem.h"
 #include "kernels.h"
 #include "scphd.h"
 
@@ -27,139 +26,11 @@ using namespace scphd;
 
 namespace {
 
-/* ---------------------------------------------------- device memory cache */
-/* Released device blocks are kept, per device and size, and handed out
- * again, so that steady-state steps and rig turnover call neither hipMalloc
- * nor hipFree: hipFree waits for the whole device (stalling every other
- * rig's stream), and hipMalloc of large blocks becomes orders of magnitude
- * slower when most of the device memory is in use.  A block enters the cache
- * only when no queued work can still touch it: a handle's blocks replaced
- * during a call are retired to the handle and handed over once its streams
- * have drained.  When hipMalloc fails, the device's cached blocks are freed
- * and the allocation is tried once more; no allocation may leave less than
- * 1/16 of the device's memory free.  The cache holds at most 1/32 of
- * the device's memory, and nothing while less than 1/16 of it is free (a
- * block is then freed instead, and a new allocation first empties the
- * cache): the runtime allocates memory of its own on demand (scratch for
- * kernels that spill, queues), and aborts when it cannot.  The
- * cache lives as long as the process (its blocks are not freed at exit, when
- * the runtime may already be gone). */
-struct Block {
-    void* p;
-    size_t bytes;
-};
-
-struct BlockCache {
-    std::mutex mu;
-    std::multimap<std::pair<int, size_t>, void*> blocks; /* (device, bytes) -> block */
-    std::map<int, size_t> held;                          /* cached bytes per device */
-};
-
-BlockCache& block_cache() {
-    static BlockCache* c = new BlockCache();
-    return *c;
-}
-
-/* sizes: powers of two from 4 KiB below 1 MiB, whole MiB above */
-size_t block_size(size_t bytes) {
-    if (bytes >= (1u << 20)) return (bytes + (1u << 20) - 1) & ~(size_t)((1u << 20) - 1);
-    size_t b = 4096;
-    while (b < bytes) b <<= 1;
-    return b;
-}
-
-void cache_trim(int dev) {
-    std::vector<void*> drop;
-    {
-        BlockCache& c = block_cache();
-        std::lock_guard<std::mutex> lk(c.mu);
-        for (auto it = c.blocks.begin(); it != c.blocks.end();) {
-            if (it->first.first == dev) {
-                drop.push_back(it->second);
-                c.held[dev] -= it->first.second;
-                it = c.blocks.erase(it);
-            } else {
-                ++it;
-            }
-        }
-    }
-    for (void* p : drop) (void)hipFree(p);
-}
-
-/* the device's total memory and whether less than 1/16 of it is free */
-bool memory_low(size_t* total) {
-    size_t fr = 0, tot = 0;
-    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return true;
-    if (total) *total = tot;
-    return fr < tot / 16;
-}
-
-/* a block of at least *bytes on the current device (a cached one a little
- * larger, else a new one); *bytes returns its size */
-hipError_t cache_alloc(void** p, size_t* bytes) {
-    *p = nullptr;
-    const size_t want = block_size(*bytes);
-    int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e != hipSuccess) return e;
-    {
-        BlockCache& c = block_cache();
-        std::lock_guard<std::mutex> lk(c.mu);
-        /* a cached block up to twice the size for small requests, a quarter
-         * more for large ones (64 MiB and up) */
-        const size_t most = want >= ((size_t)64 << 20) ? want + want / 4 : 2 * want;
-        auto it = c.blocks.lower_bound({dev, want});
-        if (it != c.blocks.end() && it->first.first == dev && it->first.second <= most) {
-            *p = it->second;
-            *bytes = it->first.second;
-            c.held[dev] -= it->first.second;
-            c.blocks.erase(it);
-            return hipSuccess;
-        }
-    }
-    /* never into the last 1/16 of the device's memory: the runtime's own
-     * allocations (scratch, queues) need it, and the runtime aborts the
-     * process when they fail, where a failed allocation here is an error
-     * the caller sees */
-    size_t fr = 0, tot = 0;
-    if (memory_low(nullptr)) cache_trim(dev);
-    if (hipMemGetInfo(&fr, &tot) == hipSuccess && (fr < want || fr - want < tot / 16)) {
-        cache_trim(dev);
-        if (hipMemGetInfo(&fr, &tot) == hipSuccess && (fr < want || fr - want < tot / 16))
-            return hipErrorOutOfMemory;
-    }
-    e = hipMalloc(p, want);
-    if (e == hipErrorOutOfMemory) {
-        (void)hipGetLastError();
-        cache_trim(dev);
-        e = hipMalloc(p, want);
-    }
-    if (e != hipSuccess) {
-        *p = nullptr;
-        return e;
-    }
-    *bytes = want;
-    return hipSuccess;
-}
-
-/* hand a block no queued work touches to the cache of the current device,
- * or free it when the cache is full */
-void cache_put(const Block& b) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return;
-    size_t total = 0;
-    const bool low = memory_low(&total);
-    if (!low) {
-        BlockCache& c = block_cache();
-        std::lock_guard<std::mutex> lk(c.mu);
-        if (c.held[dev] + b.bytes <= total / 32) {
-            c.blocks.insert({{dev, b.bytes}, b.p});
-            c.held[dev] += b.bytes;
-            return;
-        }
-    }
-    (void)hipFree(b.p);
-}
+/* Device buffers come from the process-wide block cache (devmem.cpp),
+ * which enforces the allocation floor.  A block a handle replaces during a
+ * call is retired to the handle and reaches the cache only once the
+ * handle's streams have drained, so another handle's stream never receives
+ * a block that queued work still uses. */
 
 /* a released block: to the owner's retired list while its streams may still
  * use it, else straight to the cache */
