@@ -323,8 +323,9 @@ __device__ inline void write_fit(const FitArgs& a, int64_t g, int lane, const do
  *
  * Samples stream through once: each lane accumulates, with weights
  * w' = w pd, the sums of w', w'^2, w' (q - K) and w' (q - K)(q - K)^T about
- * the Gaussian's first sample K, from which the normalised mean and
- * centred covariance follow; 30 row sums by DPP.
+ * a shift K, from which the normalised mean and centred covariance follow;
+ * 30 row sums by DPP.  K is the first sample with w' > 0 (the first sample
+ * when there is none).
  *
  * The non-detection cloud (kept when w_g (1 - pd_g) >= min_weight,
  * computeNonDetections src/scphdcameracalibration.cu:1290-1420; sample
@@ -410,6 +411,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
     const int64_t d0 = (a.nd_base + g) * (int64_t)S;
 
     double spd = 0.0, sw = 0.0, sw2 = 0.0, m1[6], m2[21], K[6];
+    bool kw = false; /* K is a weighted sample (uniform along the row) */
 #pragma unroll
     for (int c = 0; c < 6; ++c) m1[c] = 0.0;
 #pragma unroll
@@ -442,13 +444,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
         double w = 0.0;
         if (valid) eval_sample(a, meas, cur, kk, pl, k, s, pose, x, d, in, w);
         const double q[6] = {d.u, d.v, d.d, d.vu, d.vv, d.vd};
+        const double pd = in ? Pd : 0.0;
+        const double wp = valid ? w * pd : 0.0;
         if (s0 == 0) {
-            /* shift: the Gaussian's first sample, broadcast along the row */
+            /* shift: the Gaussian's first sample, broadcast along the row
+             * (the fit of a cloud with no weighted sample) */
 #pragma unroll
             for (int c = 0; c < 6; ++c) K[c] = __shfl(q[c], 0, 16);
         }
-        const double pd = in ? Pd : 0.0;
-        const double wp = valid ? w * pd : 0.0;
+        if (__ballot(!kw)) {
+            /* rows still without a weighted sample take the first one
+             * that appears as the shift: every sum is exactly zero until
+             * then, and a shift inside the weighted cloud keeps the
+             * one-pass moments exact to rounding (a single weighted
+             * sample gives zero covariance exactly) */
+            const uint32_t rb = (uint32_t)(__ballot(wp > 0.0) >> (threadIdx.x & 48)) & 0xFFFFu;
+            if (!kw && rb != 0) {
+                const int l = __builtin_ctz(rb);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) K[c] = __shfl(q[c], l, 16);
+                kw = true;
+            }
+        }
         spd += pd;
         sw += wp;
         sw2 += wp * wp;
