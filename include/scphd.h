@@ -219,11 +219,15 @@ int scphd_get_phase_ms(scphd_t* f, double* ms8);
  *   merge_force_rounds  nonzero: every parent merges in the round loop
  *                       (k_merge_rounds), the path of candidate or
  *                       neighbour-list overflow
- * -1 (0 for the last) restores the normal limit.  Every setting gives the
+ *   merge_all_pairs     nonzero: merge candidates come from every pair of
+ *                       survivors, the path of survivors whose (u, v)
+ *                       covariance is ill-conditioned, instead of windows
+ *                       in u
+ * -1 (0 for the last two) restores the normal limit.  Every setting gives the
  * same results; no counterpart in the reference, whose merge has no
  * capacity limits (src/kernels.cu:662-874). */
 int scphd_debug_set_limits(scphd_t* f, int32_t merge_lds_cap, int32_t eta_heavy_cap, int32_t eta_ncand_cap,
-                           int32_t merge_force_rounds);
+                           int32_t merge_force_rounds, int32_t merge_all_pairs);
 
 /* Noise source of the handle.  mode 0: the counter-based Philox4x32-10
  * streams, keyed by seed from the next call on (the same key as

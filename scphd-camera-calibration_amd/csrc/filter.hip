@@ -223,7 +223,7 @@ struct scphd_filter {
     double phase_ms[8] = {};
     int lds_cap = 0;
     /* verification limits (scphd_debug_set_limits); -1 = the normal limit */
-    int dbg_lds_cap = -1, dbg_heavy = -1, dbg_ncand = -1, dbg_rounds = 0;
+    int dbg_lds_cap = -1, dbg_heavy = -1, dbg_ncand = -1, dbg_rounds = 0, dbg_all_pairs = 0;
 };
 
 namespace {
@@ -492,6 +492,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
     const int lds_cap = f->dbg_lds_cap >= 0 ? std::min(f->dbg_lds_cap, f->lds_cap) : f->lds_cap;
     ma.lds_cap = lds_cap;
     ma.force_rounds = f->dbg_rounds;
+    ma.all_pairs = f->dbg_all_pairs;
     ma.scratch = nullptr;
     ma.mw = f->mw.p;
     ma.mmu = f->mmu.p;
@@ -1103,7 +1104,7 @@ int scphd_get_phase_ms(scphd_t* f, double* ms8) {
 }
 
 int scphd_debug_set_limits(scphd_t* f, int32_t merge_lds_cap, int32_t eta_heavy_cap, int32_t eta_ncand_cap,
-                           int32_t merge_force_rounds) {
+                           int32_t merge_force_rounds, int32_t merge_all_pairs) {
     if (check_handle(f)) return SCPHD_ERR_ARG;
     if (merge_lds_cap < -1 || eta_heavy_cap < -1 || eta_ncand_cap < -1 || eta_heavy_cap > ETA_HEAVY ||
         eta_ncand_cap > ETA_NCAND)
@@ -1112,6 +1113,7 @@ int scphd_debug_set_limits(scphd_t* f, int32_t merge_lds_cap, int32_t eta_heavy_
     f->dbg_heavy = eta_heavy_cap;
     f->dbg_ncand = eta_ncand_cap;
     f->dbg_rounds = merge_force_rounds != 0;
+    f->dbg_all_pairs = merge_all_pairs != 0;
     return SCPHD_OK;
 }
 

@@ -191,6 +191,7 @@ struct MergeArgs {
     int Np;
     int32_t lds_cap;   /* parents with more survivors use global scratch */
     int32_t force_rounds; /* nonzero: every parent takes the round loop (verification of that path) */
+    int32_t all_pairs;    /* nonzero: candidates from all pairs, as for ill-conditioned survivors (verification) */
     char* scratch;     /* global scratch for big parents: merge_scratch_bytes(Stot, Np) */
     double* mw;        /* merged outputs at survoff[p] + k */
     double* mmu;       /* 6 planes, stride scap */

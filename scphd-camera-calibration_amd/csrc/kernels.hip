@@ -998,9 +998,11 @@ __global__ __launch_bounds__(NT) void k_survivors(SurvArgs a) {
  *   flag  uint8[U]    merged;               selfm uint8[U]  member of its own group
  *   nbo   int[U+1]    neighbour offsets;    cnt   int[U]    neighbour counts / fill cursors
  *   mem   int[U]      group members;
- *   nbr   int[ecap]   neighbour lists, ecap = 4 U + 64
- *   cand  uint32[ccap] candidate pairs i << 16 | j (i < j), ccap = 8 U + 64
- *   res   uint8[ccap] their membership                                         */
+ *   nbr   int[ecap]   neighbour lists, ecap = 8 U + 64
+ *   cand  uint32[ccap] candidate pairs i << 16 | j (i < j), ccap = 16 U + 64
+ *   res   uint8[ccap] their membership
+ * While the candidates are enumerated, ord holds the (u, index) order,
+ * selfm the ill-conditioned flags and mem the list of those survivors. */
 __host__ __device__ inline int next_pow2(int x) {
     int v = 1;
     while (v < x) v <<= 1;
@@ -1021,17 +1023,17 @@ struct MergeLayout {
         cnt = nbo + ((int64_t)U + 1) * 4;
         mem = cnt + (int64_t)U * 4;
         nbr = mem + (int64_t)U * 4;
-        ecap = 4 * (int64_t)U + 64;
+        ecap = 8 * (int64_t)U + 64;
         cand = nbr + ecap * 4;
-        ccap = 8 * (int64_t)U + 64;
+        ccap = 16 * (int64_t)U + 64;
         res = cand + ccap * 4;
         bytes = res + ccap;
     }
 };
 
 /* global scratch for parents that do not fit in LDS: MergeLayout(U).bytes
- * <= 134 U + 600 since Up < 2 U */
-static constexpr int64_t MERGE_WS_PER_SLOT = 176, MERGE_WS_PER_PARENT = 1024;
+ * <= 190 U + 600 since Up < 2 U */
+static constexpr int64_t MERGE_WS_PER_SLOT = 240, MERGE_WS_PER_PARENT = 1024;
 
 int merge_lds_bytes(int cap) { return (int)MergeLayout(cap).bytes; }
 
@@ -1128,28 +1130,12 @@ __device__ inline bool merge_pair(const MergeArgs& a, int64_t sbase, const Merge
     return merge_bound(w, i, j, thr) && merge_full(a, sbase, i, j, thr);
 }
 
-/* Load one parent's survivors into the workspace and sort them by
- * (weight desc, index asc), the order in which the reference's argmax
- * loop picks leaders (ties: lowest pack index). */
-__device__ inline void merge_load_sort(const MergeArgs& a, int64_t sbase, int U, MergeWs& w) {
+/* Bitonic sort of ord[0, Up) by key_w: ascending (key, index) when ASC,
+ * else descending key with ascending index on ties.  Padding slots carry
+ * keys that sort last.  Ends with a barrier. */
+template <bool ASC>
+__device__ inline void merge_bitonic(MergeWs& w, int Up) {
     const int t = threadIdx.x;
-    const int Up = next_pow2(U);
-    for (int i = t; i < Up; i += NT) {
-        if (i < U) {
-            w.key_w[i] = a.sw[sbase + i];
-            w.ord[i] = i;
-            w.bu[i] = a.smu[0 * a.scap + sbase + i];
-            w.bv[i] = a.smu[1 * a.scap + sbase + i];
-            const int g = a.sg[sbase + i];
-            w.bs[3 * i] = term_P(a, g, pidx(0, 0));
-            w.bs[3 * i + 1] = term_P(a, g, pidx(1, 0));
-            w.bs[3 * i + 2] = term_P(a, g, pidx(1, 1));
-            w.flag[i] = 0;
-        } else {
-            w.key_w[i] = -INFINITY;
-            w.ord[i] = 0x7fffffff;
-        }
-    }
     __syncthreads();
     for (int k = 2; k <= Up; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
@@ -1158,7 +1144,8 @@ __device__ inline void merge_load_sort(const MergeArgs& a, int64_t sbase, int U,
                 if (l > i) {
                     const double wi = w.key_w[i], wl = w.key_w[l];
                     const int ii = w.ord[i], il = w.ord[l];
-                    const bool l_before_i = (wl > wi) || (wl == wi && il < ii);
+                    const bool l_before_i = ASC ? ((wl < wi) || (wl == wi && il < ii))
+                                                : ((wl > wi) || (wl == wi && il < ii));
                     const bool asc = (i & k) == 0;
                     if (asc == l_before_i) {
                         w.key_w[i] = wl;
@@ -1171,6 +1158,51 @@ __device__ inline void merge_load_sort(const MergeArgs& a, int64_t sbase, int U,
             __syncthreads();
         }
     }
+}
+
+/* Load one parent's survivors' (u, v) marginals into the workspace, clear
+ * the merged flags, and sort them by (u, index) into ord; a survivor whose
+ * u is not finite sorts last. */
+__device__ inline void merge_load_sort_u(const MergeArgs& a, int64_t sbase, int U, MergeWs& w) {
+    const int t = threadIdx.x;
+    const int Up = next_pow2(U);
+    for (int i = t; i < Up; i += NT) {
+        if (i < U) {
+            w.bu[i] = a.smu[0 * a.scap + sbase + i];
+            w.bv[i] = a.smu[1 * a.scap + sbase + i];
+            const int g = a.sg[sbase + i];
+            w.bs[3 * i] = term_P(a, g, pidx(0, 0));
+            w.bs[3 * i + 1] = term_P(a, g, pidx(1, 0));
+            w.bs[3 * i + 2] = term_P(a, g, pidx(1, 1));
+            w.flag[i] = 0;
+            w.key_w[i] = isfinite(w.bu[i]) ? w.bu[i] : INFINITY;
+            w.ord[i] = i;
+        } else {
+            w.key_w[i] = INFINITY;
+            w.ord[i] = 0x7fffffff;
+        }
+    }
+    merge_bitonic<true>(w, Up);
+}
+
+/* Sort one parent's survivors by (weight desc, index asc) into ord, the
+ * order in which the reference's argmax loop picks leaders (ties: lowest
+ * pack index). */
+__device__ inline void merge_sort_w(const MergeArgs& a, int64_t sbase, int U, MergeWs& w) {
+    const int t = threadIdx.x;
+    const int Up = next_pow2(U);
+    __syncthreads();
+    for (int i = t; i < Up; i += NT) {
+        w.key_w[i] = i < U ? a.sw[sbase + i] : -INFINITY;
+        w.ord[i] = i < U ? i : 0x7fffffff;
+    }
+    merge_bitonic<false>(w, Up);
+}
+
+/* Load and sort by weight (the round loop). */
+__device__ inline void merge_load_sort(const MergeArgs& a, int64_t sbase, int U, MergeWs& w) {
+    merge_load_sort_u(a, sbase, U, w);
+    merge_sort_w(a, sbase, U, w);
 }
 
 __device__ inline bool merge_self(const MergeArgs& a, int64_t si, double thr) {
@@ -1188,8 +1220,11 @@ __device__ inline bool merge_self(const MergeArgs& a, int64_t si, double thr) {
  *
  * Evaluated in phases instead of the reference's barrier-bound rounds.
  * k_merge_groups (one workgroup per parent):
- *   1. sort by (weight desc, index asc);
- *   2. candidate pairs i < j that pass merge_bound, compacted in LDS;
+ *   1. sort by (u, index);
+ *   2. candidate pairs i < j that pass merge_bound, found by scanning a
+ *      window in u around each survivor (and every pair of an
+ *      ill-conditioned survivor), compacted in the workspace; then sort by
+ *      (weight desc, index asc);
  *   3. the full 6x6 test, one lane per candidate pair (and per self pair),
  *      so that lanes stay busy however sparse the candidates are;
  *   4. neighbour lists from the members, each sorted ascending;
@@ -1229,49 +1264,69 @@ __global__ __launch_bounds__(NT) void k_merge_groups(MergeArgs a) {
     MergeWs w(ws, U);
     const double thr = a.min_separation;
     if (t == 0) s_nc = 0;
-    merge_load_sort(a, sbase, U, w); /* ends with a barrier */
+    merge_load_sort_u(a, sbase, U, w); /* ends with a barrier */
 
-    /* 2. candidates: the U (U - 1) / 2 pairs i < j in row-major order, split
-     * into NT contiguous stretches (row i starts at i U - i (i + 1) / 2);
-     * survivor i's bound data stays in registers along its row */
+    /* 2. candidates: the pairs that pass merge_bound, enumerated without
+     * visiting all U (U - 1) / 2 pairs.
+     *   A survivor is ill-conditioned when its (u, v) block is not PD with
+     *   det > 2e-6 puu pvv (or holds a non-finite value).  For two that are
+     *   not, the block of their sum has det > 1e-6 suu svv (the correlation
+     *   of a sum of PSD 2x2 blocks is at most the larger of theirs:
+     *   |auv + buv| <= r (sqrt(auu avv) + sqrt(buu bvv)) <= r sqrt(suu svv)
+     *   by Cauchy-Schwarz), so merge_bound decides by q <= 1.001 thr det,
+     *   and q suu - du^2 det = (suv du - suu dv)^2 >= 0 makes that imply
+     *   du^2 <= 1.001 thr suu.  Such pairs lie within
+     *   du^2 <= 1.002 thr (puu_i + max puu) of each other in u: each
+     *   survivor scans its successors in (u, index) order up to that
+     *   window.  (Every survivor of this filter has puu <= std_u^2: a
+     *   Kalman update with H = [I2 0] never raises the measured variance
+     *   above R, and births carry R; the windows span a few pixels.)
+     *   Pairs with an ill-conditioned survivor are tested against all
+     *   others.  The candidate set is the one the full U^2 sweep finds. */
+    __shared__ int s_nill;
+    __shared__ double s_red[8];
     for (int i = t; i < U; i += NT) w.cnt[i] = 0;
+    if (t == 0) s_nill = 0;
+    __syncthreads();
+    double smax = 0.0;
+    for (int i = t; i < U; i += NT) {
+        const double suu = w.bs[3 * i], suv = w.bs[3 * i + 1], svv = w.bs[3 * i + 2];
+        const bool good = !a.all_pairs && isfinite(w.bu[i]) && isfinite(w.bv[i]) && isfinite(suu) && isfinite(suv) &&
+                          isfinite(svv) && suu > 0.0 && svv > 0.0 && suu * svv - suv * suv > 2e-6 * suu * svv;
+        w.selfm[i] = good ? 0 : 1;
+        if (good) {
+            smax = fmax(smax, suu);
+        } else {
+            w.mem[atomicAdd(&s_nill, 1)] = i;
+        }
+    }
+    smax = block_max(smax, s_red); /* ends with a barrier after the flags */
     {
-        const int64_t npairs = (int64_t)U * (U - 1) / 2;
-        const int64_t per = (npairs + NT - 1) / NT;
-        const int64_t q0 = (int64_t)t * per, q1 = min(npairs, q0 + per);
-        if (q0 < q1) {
-            auto row_start = [U](int64_t r) { return r * U - r * (r + 1) / 2; };
-            const double b = 2.0 * U - 1.0;
-            int64_t i = (int64_t)((b - sqrt(fmax(b * b - 8.0 * (double)q0, 0.0))) * 0.5);
-            i = max<int64_t>(0, min<int64_t>(i, U - 2));
-            while (i > 0 && row_start(i) > q0) --i;
-            while (i + 1 < U - 1 && row_start(i + 1) <= q0) ++i;
-            int j = (int)(i + 1 + (q0 - row_start(i)));
-            double ui = w.bu[i], vi = w.bv[i], ai = w.bs[3 * i], bi = w.bs[3 * i + 1], ci = w.bs[3 * i + 2];
-            for (int64_t q = q0; q < q1; ++q) {
-                const double du = w.bu[j] - ui, dv = w.bv[j] - vi;
-                const double suu = ai + w.bs[3 * j], suv = bi + w.bs[3 * j + 1], svv = ci + w.bs[3 * j + 2];
-                const double det = suu * svv - suv * suv;
-                /* merge_bound: reject only a well-conditioned (u, v) block whose
-                 * marginal distance exceeds the threshold by 0.1 % */
-                const bool cand = !(det > 1e-6 * suu * svv) ||
-                                  !(svv * du * du - 2.0 * suv * du * dv + suu * dv * dv > 1.001 * thr * det);
-                if (cand) {
-                    const int c = atomicAdd(&s_nc, 1);
-                    if (c < w.ccap) w.cand[c] = ((uint32_t)i << 16) | (uint32_t)j;
-                }
-                if (++j == U) {
-                    ++i;
-                    j = (int)i + 1;
-                    if (i < U - 1) {
-                        ui = w.bu[i];
-                        vi = w.bv[i];
-                        ai = w.bs[3 * i];
-                        bi = w.bs[3 * i + 1];
-                        ci = w.bs[3 * i + 2];
-                    }
-                }
+        auto add = [&](int i, int j) {
+            const int lo = min(i, j), hi = max(i, j);
+            if (merge_bound(w, lo, hi, thr)) {
+                const int c = atomicAdd(&s_nc, 1);
+                if (c < w.ccap) w.cand[c] = ((uint32_t)lo << 16) | (uint32_t)hi;
             }
+        };
+        for (int r = t; r < U; r += NT) {
+            const int i = w.ord[r];
+            if (w.selfm[i]) continue;
+            const double ui = w.bu[i];
+            const double lim2 = 1.002 * thr * (w.bs[3 * i] + smax);
+            for (int r2 = r + 1; r2 < U; ++r2) {
+                const int j = w.ord[r2];
+                const double du = w.bu[j] - ui;
+                if (!(du * du <= lim2)) break;
+                if (!w.selfm[j]) add(i, j);
+            }
+        }
+        const int nill = s_nill;
+        const int64_t nx = (int64_t)nill * U;
+        for (int64_t e = t; e < nx; e += NT) {
+            const int i = w.mem[e / U], j = (int)(e % U);
+            if (j == i || (w.selfm[j] && j < i)) continue; /* two ill survivors: once */
+            add(i, j);
         }
     }
     __syncthreads();
@@ -1283,6 +1338,7 @@ __global__ __launch_bounds__(NT) void k_merge_groups(MergeArgs a) {
         }
         return;
     }
+    merge_sort_w(a, sbase, U, w); /* ord: the leaders' order; ends with a barrier */
     /* 3. full tests: candidates, then every survivor against itself */
     for (int c = t; c < nc + U; c += NT) {
         if (c < nc) {

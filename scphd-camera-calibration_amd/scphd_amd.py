@@ -151,7 +151,7 @@ def lib():
         "scphd_set_timing": (_I32, [_P, _I32]),
         "scphd_get_phase_ms": (_I32, [_P, dp]),
         "scphd_get_fit_debug": (_I32, [_P, dp]),
-        "scphd_debug_set_limits": (_I32, [_P, _I32, _I32, _I32, _I32]),
+        "scphd_debug_set_limits": (_I32, [_P, _I32, _I32, _I32, _I32, _I32]),
         "scphd_get_estimate": (_I32, [_P, dp, dp, dp]),
         "scphd_set_noise": (_I32, [_P, _I32, _U64, dp, ctypes.c_size_t]),
         "scphd_debug_normals": (_I32, [_U64, ctypes.c_uint32, _U64, ctypes.c_uint32, ctypes.c_uint32, _I32, dp]),
@@ -358,11 +358,13 @@ class Filter:
         self._check(lib().scphd_get_fit_debug(self.h, _dp(out)), "get_fit_debug")
         return out[:28 * n].reshape(28, n)
 
-    def set_debug_limits(self, merge_lds_cap=-1, eta_heavy_cap=-1, eta_ncand_cap=-1, merge_force_rounds=False):
+    def set_debug_limits(self, merge_lds_cap=-1, eta_heavy_cap=-1, eta_ncand_cap=-1, merge_force_rounds=False,
+                         merge_all_pairs=False):
         """Lower the capacities at which the update takes its general paths
         (scphd_debug_set_limits); -1 keeps the normal limit."""
         self._check(lib().scphd_debug_set_limits(self.h, int(merge_lds_cap), int(eta_heavy_cap), int(eta_ncand_cap),
-                                                 int(bool(merge_force_rounds))), "debug_set_limits")
+                                                 int(bool(merge_force_rounds)), int(bool(merge_all_pairs))),
+                    "debug_set_limits")
 
     def ancestors(self):
         a = np.zeros(self.Np, dtype=np.int32)

@@ -255,3 +255,12 @@ def test_lockstep_whole_map_candidate_sweeps():
     four candidate terms per measurement)."""
     p, s = _grown(storage=1)
     run_lockstep(p, s, frames=3, warm=6, limits=dict(eta_ncand_cap=0))
+
+
+def test_lockstep_merge_candidates_from_all_pairs():
+    """Every survivor treated as ill-conditioned: merge candidates come from
+    all pairs (in LDS and in global scratch) instead of windows in u, the
+    path of survivors whose (u, v) covariance is near singular."""
+    p, s = _grown(storage=1)
+    run_lockstep(p, s, frames=2, warm=6, limits=dict(merge_all_pairs=True))
+    run_lockstep(p, s, frames=2, warm=6, limits=dict(merge_all_pairs=True, merge_lds_cap=16))
