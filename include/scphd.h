@@ -214,7 +214,7 @@ int scphd_get_phase_ms(scphd_t* f, double* ms8);
  *                       (k_merge_groups<true>); default: what fits 60 KB LDS
  *   eta_heavy_cap       0..2048: more Gaussians above the birth/clutter term
  *                       -> the normaliser max pass sweeps the whole map
- *   eta_ncand_cap       0..4: more candidate terms per measurement -> the
+ *   eta_ncand_cap       0..8: more candidate terms per measurement -> the
  *                       normaliser count and the survivor pass sweep the map
  *   merge_force_rounds  nonzero: every parent merges in the round loop
  *                       (k_merge_rounds), the path of candidate or

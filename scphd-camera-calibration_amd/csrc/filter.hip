@@ -188,6 +188,8 @@ struct scphd_filter {
     /* sample generations */
     Generation gen[2];
     int cur = 0;
+    /* every block the table references is below this index of gen[cur] */
+    int64_t blocks_used = 0;
     /* per-Gaussian intermediates */
     DBuf<double> fit, pre, lik;
     DBuf<int32_t> keep, keepscan, ndlist, redo, ndfill;
@@ -327,6 +329,7 @@ int materialize_t(scphd_filter* f) {
                                  f->gparent.p, f->goff.p, f->Ng, f->S, f->models.feat, f->pdt, rig_keys(f), f->pstep));
     HIPTRY(launch_iota(f->stream, f->gblock.p, f->Ng));
     f->cur = nxt;
+    f->blocks_used = f->Ng;
     return SCPHD_OK;
 }
 
@@ -388,7 +391,10 @@ int update_t(scphd_filter* f, int M, int fixed) {
         HIPTRY(hipMemsetAsync(fa.ndfill_n, 0, sizeof(int32_t), s));
         fa.nd_base = 0;
         fa.gpd = f->hint_ok ? f->gpd.p : nullptr;
-        fa.off32 = std::max(f->gen[0].cap_blocks, f->gen[1].cap_blocks) * (int64_t)S * (int64_t)f->esz <
+        /* 32-bit byte offsets when every block read (below blocks_used of
+         * the current generation) and written (non-detection clouds below
+         * Ng of the next) lies below 4 GiB */
+        fa.off32 = std::max(f->blocks_used, Ng) * (int64_t)S * (int64_t)f->esz <
                    (int64_t)0xFFFFFFFFll - 64 * (int64_t)f->esz;
         record(f, 6);
         HIPTRY(launch_fit<T>(s, fa, f->gen[f->cur].planes<T>(), f->gen[nxt].planes<T>()));
@@ -556,10 +562,15 @@ int update_t(scphd_filter* f, int M, int fixed) {
     pa.rig_meas = f->ensemble ? f->dmeas.p : nullptr;
     pa.key = rig_keys(f);
     pa.ustep = (uint64_t)f->n_update;
-    pa.post_base = Ng;
-    pa.off32 = f->gen[nxt].cap_blocks * (int64_t)S * (int64_t)f->esz < (int64_t)0xFFFFFFFFll - 64 * (int64_t)f->esz;
+    /* the posterior clouds go to blocks [Ng, Ng + Kt): the planes are
+     * passed from block Ng on, so that offsets start at 0 (32-bit while
+     * Kt S esz is below 4 GiB) */
+    pa.post_base = 0;
+    pa.off32 = Kt * (int64_t)S * (int64_t)f->esz < (int64_t)0xFFFFFFFFll - 64 * (int64_t)f->esz;
+    Planes<T> post = f->gen[nxt].planes<T>();
+    for (int c = 0; c < 7; ++c) post.p[c] += Ng * (int64_t)S;
     record(f, 8);
-    HIPTRY(launch_sample<T>(s, pa, f->gen[nxt].planes<T>()));
+    HIPTRY(launch_sample<T>(s, pa, post));
     record(f, 9);
     record(f, 4);
 
@@ -598,6 +609,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
     f->hint_ok = true;
     f->cur = nxt;
     f->Ng = Nout;
+    f->blocks_used = Ng + Kt; /* non-detection clouds below Ng, posteriors from Ng */
     f->n_update++;
 
     f->stats.n_meas = f->ensemble ? f->ens_meas : M;
@@ -1037,6 +1049,7 @@ int scphd_set_state(scphd_t* f, const double* states, const double* weights, con
     HIPTRY(hipStreamSynchronize(s));
     flush_retired(f);
     f->Ng = Ng;
+    f->blocks_used = Ng;
     f->n_predict = n_predict;
     f->n_update = n_update;
     f->n_resample = n_resample;

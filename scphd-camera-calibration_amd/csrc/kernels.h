@@ -51,8 +51,8 @@ struct RigKeys {
 enum FitPlane { FP_MU = 0, FP_P = 6, FP_PD = 27, FP_COUNT = 28 };
 /* per-Gaussian detection terms, stride Ng */
 enum PrePlane {
-    PP_K = 0,   /* gain, 12 */
-    PP_PN = 12, /* updated covariance, 21 packed */
+    PP_K = 0,   /* gain, 12 planes */
+    PP_PN = 12, /* updated covariance: from plane 12 on, 21 packed doubles per Gaussian, contiguous */
     PP_COUNT = 33
 };
 /* per-Gaussian likelihood record, LK_COUNT consecutive doubles (64 B):
@@ -160,7 +160,7 @@ struct EtaArgs {
     int heavy_cap;     /* at most ETA_HEAVY: more Gaussians above bpc than this -> whole-map max pass */
     int ncand_cap;     /* at most ETA_NCAND: more candidates than this -> whole-map sweeps */
 };
-static constexpr int ETA_NCAND = 4;
+static constexpr int ETA_NCAND = 8;
 static constexpr int ETA_HEAVY = 2048; /* LDS list of the Gaussians that can exceed bpc */
 hipError_t launch_eta(hipStream_t s, const EtaArgs& a);
 
@@ -184,7 +184,7 @@ struct MergeArgs {
     const int32_t* sm;
     const double* smu;
     int64_t scap;
-    const double* pre; /* PP_PN planes of the table entries */
+    const double* pre; /* updated covariances of the table entries (PP_PN records) */
     int64_t Ng;
     const BirthModel* births; /* per rig, device memory: birth terms' covariance */
     double min_separation;

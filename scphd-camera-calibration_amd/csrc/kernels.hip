@@ -667,7 +667,7 @@ __global__ __launch_bounds__(128) void k_precompute(const double* fit, const dou
 #pragma unroll
     for (int i = 0; i < 12; ++i) pre[(PP_K + i) * Ng + g] = r.K[i];
 #pragma unroll
-    for (int e = 0; e < 21; ++e) pre[(PP_PN + e) * Ng + g] = r.Pn[e];
+    for (int e = 0; e < 21; ++e) pre[PP_PN * Ng + g * 21 + e] = r.Pn[e];
 }
 
 /* ------------------------------------------- normalisers and survivors */
@@ -804,7 +804,9 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
         int mcnt = 0;
         /* candidates: terms with l - mx above the gate (less 1e-6), the only
          * ones that can pass it against eta >= mx; the first ETA_NCAND kept */
-        int nc = 0, c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+        int nc = 0, cc[ETA_NCAND];
+#pragma unroll
+        for (int q = 0; q < ETA_NCAND; ++q) cc[q] = 0;
         const double cgate = a.lw_gate - 1e-6;
         if (valid) {
             if (listed) {
@@ -847,10 +849,8 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
                     }
                     if (j0 + q < G && x > cgate && R.v[q][LK_OK] != 0.0) {
                         const int j = j0 + q;
-                        c0 = nc == 0 ? j : c0;
-                        c1 = nc == 1 ? j : c1;
-                        c2 = nc == 2 ? j : c2;
-                        c3 = nc == 3 ? j : c3;
+#pragma unroll
+                        for (int e = 0; e < ETA_NCAND; ++e) cc[e] = nc == e ? j : cc[e];
                         ++nc;
                     }
                 }
@@ -863,8 +863,10 @@ __global__ __launch_bounds__(NT) void k_eta(EtaArgs a) {
             a.eta[pm] = eta;
             eta_sum += eta;
             if (nc <= a.ncand_cap) { /* the candidates alone */
-                for (int k = 0; k < nc; ++k) {
-                    const int j = k == 0 ? c0 : k == 1 ? c1 : k == 2 ? c2 : c3;
+#pragma unroll
+                for (int k = 0; k < ETA_NCAND; ++k) {
+                    if (k >= nc) break;
+                    const int j = cc[k];
                     const double* rec = a.lik + (g0 + j) * LK_COUNT;
                     const double x = det_lw(rec, zu, zv) - eta;
                     if (x > a.lw_gate && rec[LK_OK] != 0.0) mcnt += exp(x) >= a.min_weight ? 1 : 0;
@@ -1046,7 +1048,7 @@ int64_t merge_scratch_bytes(int64_t Stot, int Np) {
  * live in global memory, so the choice is one global load from a chosen
  * address. */
 __device__ inline double term_P(const MergeArgs& a, int g, int e) {
-    return g >= 0 ? a.pre[(int64_t)(PP_PN + e) * a.Ng + g] : a.births[-1 - g].P[e];
+    return g >= 0 ? a.pre[(int64_t)PP_PN * a.Ng + (int64_t)g * 21 + e] : a.births[-1 - g].P[e];
 }
 
 __device__ inline void survivor_P(const MergeArgs& a, int64_t si, double* P) {

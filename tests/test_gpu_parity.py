@@ -252,7 +252,7 @@ def test_lockstep_normaliser_whole_map_max_pass():
 def test_lockstep_whole_map_candidate_sweeps():
     """No candidate lists: the normaliser's kept-term count and the survivor
     pass sweep the whole map for every measurement (the path of more than
-    four candidate terms per measurement)."""
+    eight candidate terms per measurement)."""
     p, s = _grown(storage=1)
     run_lockstep(p, s, frames=3, warm=6, limits=dict(eta_ncand_cap=0))
 
