@@ -515,9 +515,14 @@ int update_t(scphd_filter* f, int M, int fixed) {
     ma.gend = f->gend.p;
     ma.gmem = f->gmem.p;
     ma.Stot = Stot;
+    ma.big_lds_up = 0;
     if (maxU > lds_cap) {
         HIPTRY(f->mscratch.ensure(merge_scratch_bytes(Stot, Np)));
         ma.scratch = f->mscratch.p;
+        /* the big parents' sort keys in LDS: 12 B per slot, at most 48 KB */
+        int up = 1;
+        while (up < maxU) up <<= 1;
+        ma.big_lds_up = up <= 4096 ? up : 0;
     }
     HIPTRY(launch_merge(s, ma, merge_lds_bytes(std::max(1, std::min(maxU, lds_cap)))));
     if (!fixed)

@@ -1136,7 +1136,7 @@ __device__ inline bool merge_pair(const MergeArgs& a, int64_t sbase, const Merge
  * else descending key with ascending index on ties.  Padding slots carry
  * keys that sort last.  Ends with a barrier. */
 template <bool ASC>
-__device__ inline void merge_bitonic(MergeWs& w, int Up) {
+__device__ inline void merge_bitonic(double* key, int* ord, int Up) {
     const int t = threadIdx.x;
     __syncthreads();
     for (int k = 2; k <= Up; k <<= 1) {
@@ -1144,16 +1144,16 @@ __device__ inline void merge_bitonic(MergeWs& w, int Up) {
             for (int i = t; i < Up; i += NT) {
                 const int l = i ^ j;
                 if (l > i) {
-                    const double wi = w.key_w[i], wl = w.key_w[l];
-                    const int ii = w.ord[i], il = w.ord[l];
+                    const double wi = key[i], wl = key[l];
+                    const int ii = ord[i], il = ord[l];
                     const bool l_before_i = ASC ? ((wl < wi) || (wl == wi && il < ii))
                                                 : ((wl > wi) || (wl == wi && il < ii));
                     const bool asc = (i & k) == 0;
                     if (asc == l_before_i) {
-                        w.key_w[i] = wl;
-                        w.key_w[l] = wi;
-                        w.ord[i] = il;
-                        w.ord[l] = ii;
+                        key[i] = wl;
+                        key[l] = wi;
+                        ord[i] = il;
+                        ord[l] = ii;
                     }
                 }
             }
@@ -1163,9 +1163,11 @@ __device__ inline void merge_bitonic(MergeWs& w, int Up) {
 }
 
 /* Load one parent's survivors' (u, v) marginals into the workspace, clear
- * the merged flags, and sort them by (u, index) into ord; a survivor whose
- * u is not finite sorts last. */
-__device__ inline void merge_load_sort_u(const MergeArgs& a, int64_t sbase, int U, MergeWs& w) {
+ * the merged flags, and sort them by (u, index): key[r] is the u of rank r
+ * (+inf for a u that is not finite, which sorts last), ord[r] its survivor.
+ * key and ord hold next_pow2(U) entries. */
+__device__ inline void merge_load_sort_u(const MergeArgs& a, int64_t sbase, int U, MergeWs& w, double* key,
+                                         int* ord) {
     const int t = threadIdx.x;
     const int Up = next_pow2(U);
     for (int i = t; i < Up; i += NT) {
@@ -1177,34 +1179,34 @@ __device__ inline void merge_load_sort_u(const MergeArgs& a, int64_t sbase, int 
             w.bs[3 * i + 1] = term_P(a, g, pidx(1, 0));
             w.bs[3 * i + 2] = term_P(a, g, pidx(1, 1));
             w.flag[i] = 0;
-            w.key_w[i] = isfinite(w.bu[i]) ? w.bu[i] : INFINITY;
-            w.ord[i] = i;
+            key[i] = isfinite(w.bu[i]) ? w.bu[i] : INFINITY;
+            ord[i] = i;
         } else {
-            w.key_w[i] = INFINITY;
-            w.ord[i] = 0x7fffffff;
+            key[i] = INFINITY;
+            ord[i] = 0x7fffffff;
         }
     }
-    merge_bitonic<true>(w, Up);
+    merge_bitonic<true>(key, ord, Up);
 }
 
 /* Sort one parent's survivors by (weight desc, index asc) into ord, the
  * order in which the reference's argmax loop picks leaders (ties: lowest
  * pack index). */
-__device__ inline void merge_sort_w(const MergeArgs& a, int64_t sbase, int U, MergeWs& w) {
+__device__ inline void merge_sort_w(const MergeArgs& a, int64_t sbase, int U, double* key, int* ord) {
     const int t = threadIdx.x;
     const int Up = next_pow2(U);
     __syncthreads();
     for (int i = t; i < Up; i += NT) {
-        w.key_w[i] = i < U ? a.sw[sbase + i] : -INFINITY;
-        w.ord[i] = i < U ? i : 0x7fffffff;
+        key[i] = i < U ? a.sw[sbase + i] : -INFINITY;
+        ord[i] = i < U ? i : 0x7fffffff;
     }
-    merge_bitonic<false>(w, Up);
+    merge_bitonic<false>(key, ord, Up);
 }
 
 /* Load and sort by weight (the round loop). */
 __device__ inline void merge_load_sort(const MergeArgs& a, int64_t sbase, int U, MergeWs& w) {
-    merge_load_sort_u(a, sbase, U, w);
-    merge_sort_w(a, sbase, U, w);
+    merge_load_sort_u(a, sbase, U, w, w.key_w, w.ord);
+    merge_sort_w(a, sbase, U, w.key_w, w.ord);
 }
 
 __device__ inline bool merge_self(const MergeArgs& a, int64_t si, double thr) {
@@ -1212,6 +1214,75 @@ __device__ inline bool merge_self(const MergeArgs& a, int64_t si, double thr) {
     survivor_mu(a, si, m);
     survivor_P(a, si, P);
     return mahal6(m, P, m, P) < thr;
+}
+
+/* Step 2 of k_merge_groups: the merge candidates, from the survivors
+ * sorted by (u, index) (key[r] = u of rank r, ord[r] its survivor; in LDS
+ * when they fit, else in the workspace). */
+__device__ inline void merge_candidates(const MergeArgs& a, int U, MergeWs& w, double thr, const double* key,
+                                        const int* ord, int* s_nc, int* s_nill, double* s_red) {
+    const int t = threadIdx.x;
+    /* 2. candidates: the pairs that pass merge_bound, enumerated without
+     * visiting all U (U - 1) / 2 pairs.
+     *   A survivor is ill-conditioned when its (u, v) block is not PD with
+     *   det > 2e-6 puu pvv (or holds a non-finite value).  For two that are
+     *   not, the block of their sum has det > 1e-6 suu svv (the correlation
+     *   of a sum of PSD 2x2 blocks is at most the larger of theirs:
+     *   |auv + buv| <= r (sqrt(auu avv) + sqrt(buu bvv)) <= r sqrt(suu svv)
+     *   by Cauchy-Schwarz), so merge_bound decides by q <= 1.001 thr det,
+     *   and q suu - du^2 det = (suv du - suu dv)^2 >= 0 makes that imply
+     *   du^2 <= 1.001 thr suu.  Such pairs lie within
+     *   du^2 <= 1.002 thr (puu_i + max puu) of each other in u: each
+     *   survivor scans its successors in (u, index) order up to that
+     *   window.  (Every survivor of this filter has puu <= std_u^2: a
+     *   Kalman update with H = [I2 0] never raises the measured variance
+     *   above R, and births carry R; the windows span a few pixels.)
+     *   Pairs with an ill-conditioned survivor are tested against all
+     *   others.  The candidate set is the one the full U^2 sweep finds. */
+    for (int i = t; i < U; i += NT) w.cnt[i] = 0;
+    if (t == 0) *s_nill = 0;
+    __syncthreads();
+    double smax = 0.0;
+    for (int i = t; i < U; i += NT) {
+        const double suu = w.bs[3 * i], suv = w.bs[3 * i + 1], svv = w.bs[3 * i + 2];
+        const bool good = !a.all_pairs && isfinite(w.bu[i]) && isfinite(w.bv[i]) && isfinite(suu) && isfinite(suv) &&
+                          isfinite(svv) && suu > 0.0 && svv > 0.0 && suu * svv - suv * suv > 2e-6 * suu * svv;
+        w.selfm[i] = good ? 0 : 1;
+        if (good) {
+            smax = fmax(smax, suu);
+        } else {
+            w.mem[atomicAdd(s_nill, 1)] = i;
+        }
+    }
+    smax = block_max(smax, s_red); /* ends with a barrier after the flags */
+    {
+        auto add = [&](int i, int j) {
+            const int lo = min(i, j), hi = max(i, j);
+            if (merge_bound(w, lo, hi, thr)) {
+                const int c = atomicAdd(s_nc, 1);
+                if (c < w.ccap) w.cand[c] = ((uint32_t)lo << 16) | (uint32_t)hi;
+            }
+        };
+        for (int r = t; r < U; r += NT) {
+            const int i = ord[r];
+            if (w.selfm[i]) continue;
+            const double ui = key[r];
+            const double lim2 = 1.002 * thr * (w.bs[3 * i] + smax);
+            for (int r2 = r + 1; r2 < U; ++r2) {
+                const int j = ord[r2];
+                const double du = key[r2] - ui;
+                if (!(du * du <= lim2)) break;
+                if (!w.selfm[j]) add(i, j);
+            }
+        }
+        const int nill = *s_nill;
+        const int64_t nx = (int64_t)nill * U;
+        for (int64_t e = t; e < nx; e += NT) {
+            const int i = w.mem[e / U], j = (int)(e % U);
+            if (j == i || (w.selfm[j] && j < i)) continue; /* two ill survivors: once */
+            add(i, j);
+        }
+    }
 }
 
 /* Greedy Gaussian-mixture merge (phdUpdateMergeKernel, src/kernels.cu:662-874):
@@ -1266,71 +1337,30 @@ __global__ __launch_bounds__(NT) void k_merge_groups(MergeArgs a) {
     MergeWs w(ws, U);
     const double thr = a.min_separation;
     if (t == 0) s_nc = 0;
-    merge_load_sort_u(a, sbase, U, w); /* ends with a barrier */
-
-    /* 2. candidates: the pairs that pass merge_bound, enumerated without
-     * visiting all U (U - 1) / 2 pairs.
-     *   A survivor is ill-conditioned when its (u, v) block is not PD with
-     *   det > 2e-6 puu pvv (or holds a non-finite value).  For two that are
-     *   not, the block of their sum has det > 1e-6 suu svv (the correlation
-     *   of a sum of PSD 2x2 blocks is at most the larger of theirs:
-     *   |auv + buv| <= r (sqrt(auu avv) + sqrt(buu bvv)) <= r sqrt(suu svv)
-     *   by Cauchy-Schwarz), so merge_bound decides by q <= 1.001 thr det,
-     *   and q suu - du^2 det = (suv du - suu dv)^2 >= 0 makes that imply
-     *   du^2 <= 1.001 thr suu.  Such pairs lie within
-     *   du^2 <= 1.002 thr (puu_i + max puu) of each other in u: each
-     *   survivor scans its successors in (u, index) order up to that
-     *   window.  (Every survivor of this filter has puu <= std_u^2: a
-     *   Kalman update with H = [I2 0] never raises the measured variance
-     *   above R, and births carry R; the windows span a few pixels.)
-     *   Pairs with an ill-conditioned survivor are tested against all
-     *   others.  The candidate set is the one the full U^2 sweep finds. */
+    /* 1-2. sort by (u, index), candidates, sort by weight: the sort keys
+     * in LDS for a big parent whose keys fit the launch's LDS, else in the
+     * workspace */
     __shared__ int s_nill;
     __shared__ double s_red[8];
-    for (int i = t; i < U; i += NT) w.cnt[i] = 0;
-    if (t == 0) s_nill = 0;
-    __syncthreads();
-    double smax = 0.0;
-    for (int i = t; i < U; i += NT) {
-        const double suu = w.bs[3 * i], suv = w.bs[3 * i + 1], svv = w.bs[3 * i + 2];
-        const bool good = !a.all_pairs && isfinite(w.bu[i]) && isfinite(w.bv[i]) && isfinite(suu) && isfinite(suv) &&
-                          isfinite(svv) && suu > 0.0 && svv > 0.0 && suu * svv - suv * suv > 2e-6 * suu * svv;
-        w.selfm[i] = good ? 0 : 1;
-        if (good) {
-            smax = fmax(smax, suu);
-        } else {
-            w.mem[atomicAdd(&s_nill, 1)] = i;
+    const int Up = next_pow2(U);
+    const bool keys_lds = BIG && Up <= a.big_lds_up;
+    if (keys_lds) {
+        double* key = (double*)lds;
+        int* ord = (int*)(lds + (size_t)8 * Up);
+        merge_load_sort_u(a, sbase, U, w, key, ord); /* ends with a barrier */
+        merge_candidates(a, U, w, thr, key, ord, &s_nc, &s_nill, s_red);
+        __syncthreads();
+        if (s_nc <= w.ccap) {
+            merge_sort_w(a, sbase, U, key, ord);
+            for (int i = t; i < U; i += NT) w.ord[i] = ord[i];
         }
+    } else {
+        merge_load_sort_u(a, sbase, U, w, w.key_w, w.ord);
+        merge_candidates(a, U, w, thr, w.key_w, w.ord, &s_nc, &s_nill, s_red);
+        __syncthreads();
+        if (s_nc <= w.ccap) merge_sort_w(a, sbase, U, w.key_w, w.ord);
     }
-    smax = block_max(smax, s_red); /* ends with a barrier after the flags */
-    {
-        auto add = [&](int i, int j) {
-            const int lo = min(i, j), hi = max(i, j);
-            if (merge_bound(w, lo, hi, thr)) {
-                const int c = atomicAdd(&s_nc, 1);
-                if (c < w.ccap) w.cand[c] = ((uint32_t)lo << 16) | (uint32_t)hi;
-            }
-        };
-        for (int r = t; r < U; r += NT) {
-            const int i = w.ord[r];
-            if (w.selfm[i]) continue;
-            const double ui = w.bu[i];
-            const double lim2 = 1.002 * thr * (w.bs[3 * i] + smax);
-            for (int r2 = r + 1; r2 < U; ++r2) {
-                const int j = w.ord[r2];
-                const double du = w.bu[j] - ui;
-                if (!(du * du <= lim2)) break;
-                if (!w.selfm[j]) add(i, j);
-            }
-        }
-        const int nill = s_nill;
-        const int64_t nx = (int64_t)nill * U;
-        for (int64_t e = t; e < nx; e += NT) {
-            const int i = w.mem[e / U], j = (int)(e % U);
-            if (j == i || (w.selfm[j] && j < i)) continue; /* two ill survivors: once */
-            add(i, j);
-        }
-    }
+
     __syncthreads();
     const int nc = s_nc;
     if (nc > w.ccap) {
@@ -1340,7 +1370,6 @@ __global__ __launch_bounds__(NT) void k_merge_groups(MergeArgs a) {
         }
         return;
     }
-    merge_sort_w(a, sbase, U, w); /* ord: the leaders' order; ends with a barrier */
     /* 3. full tests: candidates, then every survivor against itself */
     for (int c = t; c < nc + U; c += NT) {
         if (c < nc) {
@@ -1429,76 +1458,112 @@ __global__ __launch_bounds__(NT) void k_merge_groups(MergeArgs a) {
         if (t == 0) s_tot = nn;
     }
     __syncthreads();
-    if (t < 64) {
-        const int lane = t, nn = s_tot;
-        int tp = 0;
-        for (int k = 0; k < nn; ++k) {
-            const int r = nl[k], L = w.ord[r];
-            if (w.flag[L]) continue; /* taken by an earlier leader */
-            const bool sm = w.selfm[L] != 0;
-            const int e0 = w.nbo[L], e1 = w.nbo[L + 1];
-            const int st = tp;
-            int gsz;
-            if (e1 - e0 <= 64) {
-                /* one pass: members and their flags written at once (a
-                 * neighbour appears once in a list, and never the leader) */
-                const int ei = e0 + lane;
-                const int j = ei < e1 ? w.nbr[ei] : 0x7fffffff;
-                const bool tk = ei < e1 && !w.flag[j];
-                const uint64_t bm = __ballot(tk);
-                const int nlt = __popcll(__ballot(tk && j < L));
-                gsz = __popcll(bm) + (sm ? 1 : 0);
-                if (tk) {
-                    const int q = __popcll(bm & ((1ull << lane) - 1ull));
-                    tmp[st + q + ((sm && q >= nlt) ? 1 : 0)] = j;
+    /* The walk per connected component.  A leader takes only unmerged
+     * neighbours, so the groups formed in one component of the neighbour
+     * graph depend only on that component's ranks and flags: each
+     * component is walked on its own, in rank order, by one lane.  (The
+     * reference's loop ends at the first leader that is not its own member;
+     * a component walked past that rank forms groups the rank-ordered
+     * placement below discards, exactly as if the walk had stopped.)
+     *   labels: the smallest survivor index of the component, by min
+     *           propagation along the neighbour lists;
+     *   buckets: the connected ranks grouped by label (counts, scan,
+     *           scatter), each bucket sorted by rank by its lane;
+     *   members of a group go to tmp inside its bucket's range, ascending
+     *           by survivor index with the leader in its place. */
+    {
+        const int nn = s_tot;
+        int* lab = (int*)w.bv;           /* label per survivor (bv is free after the candidates) */
+        int* boff = (int*)w.bs;          /* bucket offsets [U + 1] (bs is free after the candidates) */
+        int* bcur = boff + (U + 1);      /* scatter cursors [U] */
+        int* bkt = bcur + U;             /* connected ranks by bucket [nn] */
+        int* seg = (int*)w.key_w;        /* non-empty buckets (key_w is free after the sorts) */
+        for (int i = t; i < U; i += NT) {
+            lab[i] = i;
+            boff[i] = 0;
+        }
+        if (t == 0) {
+            boff[U] = 0;
+            s_nc = 0;
+        }
+        __syncthreads();
+        for (;;) {
+            int changed = 0;
+            for (int k = t; k < nn; k += NT) {
+                const int i = w.ord[nl[k]];
+                int m = lab[i];
+                for (int e = w.nbo[i]; e < w.nbo[i + 1]; ++e) m = min(m, lab[w.nbr[e]]);
+                if (m < lab[i]) {
+                    lab[i] = m;
+                    changed = 1;
+                }
+            }
+            if (!__syncthreads_or(changed)) break;
+        }
+        for (int k = t; k < nn; k += NT) atomicAdd(&boff[lab[w.ord[nl[k]]]], 1);
+        __syncthreads();
+        {
+            int mine = 0;
+            for (int i = r0; i < r1; ++i) mine += boff[i];
+            int tot;
+            int base = block_exscan(mine, sh, &tot);
+            for (int i = r0; i < r1; ++i) {
+                const int c = boff[i];
+                boff[i] = base;
+                bcur[i] = base;
+                if (c > 0) seg[atomicAdd(&s_nc, 1)] = i;
+                base += c;
+            }
+            if (t == 0) boff[U] = tot;
+        }
+        __syncthreads();
+        for (int k = t; k < nn; k += NT) {
+            const int r = nl[k];
+            bkt[atomicAdd(&bcur[lab[w.ord[r]]], 1)] = r;
+        }
+        __syncthreads();
+        const int nseg = s_nc;
+        for (int q = t; q < nseg; q += NT) {
+            const int b0 = seg[q];
+            const int k0 = boff[b0], k1 = bcur[b0]; /* the cursor ends at the bucket's end */
+            for (int k = k0 + 1; k < k1; ++k) { /* ranks ascending: buckets are small */
+                const int x = bkt[k];
+                int f = k - 1;
+                while (f >= k0 && bkt[f] > x) {
+                    bkt[f + 1] = bkt[f];
+                    --f;
+                }
+                bkt[f + 1] = x;
+            }
+            int tp = k0;
+            for (int k = k0; k < k1; ++k) {
+                const int r = bkt[k], L = w.ord[r];
+                if (w.flag[L]) continue; /* taken by an earlier leader */
+                const bool sm = w.selfm[L] != 0;
+                const int st = tp;
+                bool placed = !sm;
+                for (int e = w.nbo[L]; e < w.nbo[L + 1]; ++e) {
+                    const int j = w.nbr[e];
+                    if (w.flag[j]) continue;
+                    if (!placed && j > L) {
+                        tmp[tp++] = L;
+                        placed = true;
+                    }
+                    tmp[tp++] = j;
                     w.flag[j] = 1;
                 }
-                if (lane == 0 && sm) {
-                    tmp[st + nlt] = L;
-                    w.flag[L] = 1;
-                }
-            } else {
-                /* taken neighbours below L: the leader's slot when it is a member */
-                int nlt = 0, ntk = 0;
-                for (int e = e0; e < e1; e += 64) {
-                    const int ei = e + lane;
-                    const int j = ei < e1 ? w.nbr[ei] : 0x7fffffff;
-                    const bool tk = ei < e1 && !w.flag[j];
-                    nlt += __popcll(__ballot(tk && j < L));
-                    ntk += __popcll(__ballot(tk));
-                }
-                gsz = ntk + (sm ? 1 : 0);
-                if (gsz > 0) {
-                    int seen = 0;
-                    for (int e = e0; e < e1; e += 64) {
-                        const int ei = e + lane;
-                        const int j = ei < e1 ? w.nbr[ei] : 0x7fffffff;
-                        const bool tk = ei < e1 && !w.flag[j];
-                        const uint64_t bm = __ballot(tk);
-                        if (tk) {
-                            const int q = seen + __popcll(bm & ((1ull << lane) - 1ull));
-                            tmp[st + q + ((sm && q >= nlt) ? 1 : 0)] = j;
-                        }
-                        seen += __popcll(bm);
-                    }
-                    if (lane == 0 && sm) tmp[st + nlt] = L;
-                    __builtin_amdgcn_wave_barrier();
-                    for (int e = st + lane; e < st + gsz; e += 64) w.flag[tmp[e]] = 1;
-                }
-            }
-            /* empty group: zero weight, the reference stops; a leader outside
-             * its group is taken again, finds nothing and stops */
-            if (lane == 0) {
+                if (!placed) tmp[tp++] = L;
+                if (sm) w.flag[L] = 1;
+                const int gsz = tp - st;
+                /* empty group: zero weight, the reference stops; a leader
+                 * outside its group is taken again, finds nothing and stops */
                 rinfo[2 * r] = gsz | (sm ? 0 : STOP);
                 rinfo[2 * r + 1] = st;
+                if (!sm || gsz == 0) break;
             }
-            tp += gsz;
-            /* LDS accesses of one wavefront complete in order: the next
-             * iteration sees these flags without waiting */
-            __builtin_amdgcn_wave_barrier();
-            if (!sm || gsz == 0) break;
         }
     }
+    __syncthreads();
     if (t == 0) s_nc = U; /* first stopping rank */
     __syncthreads();
     for (int r = r0; r < r1; ++r)
@@ -2265,7 +2330,7 @@ hipError_t launch_survivors(hipStream_t s, const SurvArgs& a) {
 
 hipError_t launch_merge(hipStream_t s, const MergeArgs& a, int lds_bytes) {
     k_merge_groups<false><<<a.Np, NT, lds_bytes, s>>>(a);
-    if (a.scratch) k_merge_groups<true><<<a.Np, NT, 0, s>>>(a);
+    if (a.scratch) k_merge_groups<true><<<a.Np, NT, 12 * a.big_lds_up, s>>>(a);
     if (a.Stot > 0) k_merge_moments<<<nblocks(a.Stot, 128), 128, 0, s>>>(a);
     k_merge_rounds<false><<<a.Np, NT, lds_bytes, s>>>(a);
     if (a.scratch) k_merge_rounds<true><<<a.Np, NT, 0, s>>>(a);
