@@ -33,11 +33,13 @@ import scphd_amd
 
 
 def expected_gaussians_per_parent(s):
-    """Steady-state map size estimate: each visible landmark (about 90 % of
-    the box seen by a 640 x 480, f = 500 camera) carries one detection term
-    plus 2-3 non-detection clouds, and clutter births survive about one step:
-    3.5 * 0.95 * 0.9 * L + lambda."""
-    return int(3.5 * 0.95 * 0.9 * s["n_landmarks"] + s["lam"]) + 16
+    """Steady-state map size estimate: each landmark carries a detection
+    term, a chain of 3-4 non-detection clouds (weights (1 - Pd)^k down to
+    min_weight) and the terms of the landmarks that have left the view
+    (weights ps^k, alive for many frames), and clutter births survive a frame
+    or two: about 4.75 L + 1.5 lambda (measured: 150 Gaussians per parent for
+    30 landmarks and lambda 10, 1000 for 200 landmarks and lambda 50)."""
+    return int(4.75 * s["n_landmarks"] + 1.5 * s["lam"]) + 16
 
 
 def expected_measurements(s):
@@ -62,11 +64,14 @@ def _pct(x, q):
 
 
 def run_served(rigs, hbm_budget_bytes=0, n_workers=8, reserve_factor=1.5, progress=None, max_wall=None,
-               paced=True):
+               paced=True, keep_estimates=False):
     """Replay prepared rigs (see prepare) in real time; returns a summary
     dict.  paced=False submits all of a rig's frames when it is admitted and
     admits every rig at once (as the budget allows): the saturated
-    throughput of the served form."""
+    throughput of the served form.  keep_estimates: the summary also holds
+    "estimates", per rig (in the order of rigs) the per-frame calibration
+    estimates (12 values each), in frame order."""
+    est = [dict() for _ in rigs]
     b = scphd_amd.Batch(hbm_budget_bytes=hbm_budget_bytes, n_workers=n_workers)
     try:
         waiting = []          # indices of arrived, not yet admitted rigs (FIFO)
@@ -91,9 +96,9 @@ def run_served(rigs, hbm_budget_bytes=0, n_workers=8, reserve_factor=1.5, progre
                 if rid is None:
                     blocked = True  # offered again once a rig leaves
                     break
-                waiting.pop(0)
+                idx = waiting.pop(0)
                 adm.append(now - r["t"] if paced else 0.0)
-                active[rid] = dict(r=r, t_admit=now, sent=0, done=0, due={})
+                active[rid] = dict(r=r, idx=idx, t_admit=now, sent=0, done=0, due={})
             for rid, a in active.items():
                 fps_dt = a["r"]["s"]["dt"]
                 fr = a["r"]["frames"]
@@ -109,6 +114,8 @@ def run_served(rigs, hbm_budget_bytes=0, n_workers=8, reserve_factor=1.5, progre
                     raise RuntimeError(f"rig {x['rig']} frame {x['frame_id']} failed: status {x['status']}")
                 lat.append(x["t_end"] - a["due"].pop(x["frame_id"]))
                 svc.append(x["t_end"] - x["t_start"])
+                if keep_estimates:
+                    est[a["idx"]][x["frame_id"]] = np.array(x["estimate"])
                 pairs += x["n_pairs"]
                 n_frames += 1
                 a["done"] += 1
@@ -134,7 +141,7 @@ def run_served(rigs, hbm_budget_bytes=0, n_workers=8, reserve_factor=1.5, progre
     lat = np.array(lat) * 1e3
     svc = np.array(svc) * 1e3
     adm = np.array(adm)
-    return dict(rigs=len(rigs), rigs_done=done_rigs, frames=n_frames, wall_s=wall,
+    out = dict(rigs=len(rigs), rigs_done=done_rigs, frames=n_frames, wall_s=wall,
                 updates_per_s=pairs / wall if wall > 0 else 0.0, pairs=pairs,
                 latency_ms=dict(p50=_pct(lat, 50), p99=_pct(lat, 99), max=float(lat.max()) if len(lat) else None),
                 service_ms=dict(p50=_pct(svc, 50), p99=_pct(svc, 99)),
@@ -142,6 +149,9 @@ def run_served(rigs, hbm_budget_bytes=0, n_workers=8, reserve_factor=1.5, progre
                 peak_active_rigs=peak_active, peak_reserved_gb=peak_res / 1e9, peak_used_gb=peak_used / 1e9,
                 refused_admissions=st["refused"], budget_gb=st["budget_bytes"] / 1e9, n_workers=n_workers,
                 paced=paced)
+    if keep_estimates:
+        out["estimates"] = [np.array([e[k] for k in sorted(e)]) for e in est]
+    return out
 
 
 def run_plain(rigs, progress=None):
