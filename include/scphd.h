@@ -120,6 +120,20 @@ int scphd_update_device(scphd_t* f, const double* u_device, const double* v_devi
  * src/scphdcameracalibration.cu:2651-2817 and computeNeff :2641-2649. */
 int scphd_resample(scphd_t* f, int32_t* resampled);
 
+/* scphd_resample that allocates no device memory.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * results as scphd_resample; its scratch (cumulative weights, gathered
+ * extrinsics, ancestors, new map sizes, scan partials, the decision flag)
+ * lives in a static 8 MiB device array that every call on one GPU shares,
+ * so calls that overlap on one GPU race.  Plain filters only
+ * (SCPHD_ERR_ARG for an ensemble).  SCPHD_ERR_CAPACITY, with the filter
+ * unchanged, when the parents need more than the scratch holds (about
+ * 77,000), when a deferred predict needs a larger sample generation, or
+ * when the resampled map table exceeds the handle's spare table --
+ * scphd_reserve sizes both. */
+int scphd_resample_noalloc(scphd_t* f, int32_t* resampled);
+
 /* NaN guard on the parent weights: SCPHD_ERR_NAN instead of exit(2).
  * Replaces checkStuff(), src/scphdcameracalibration.cu:812-821. */
 int scphd_check(scphd_t* f);

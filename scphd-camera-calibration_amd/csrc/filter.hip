@@ -15,6 +15,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -68,6 +70,30 @@ struct DBuf {
         bytes = 0;
     }
 };
+
+/* Grow b to at least need elements keeping its first keep (at most its
+ * current size); the old block is retired to the owner. */
+template <typename T>
+hipError_t grow_keep(DBuf<T>& b, int64_t need, int64_t keep, hipStream_t s) {
+    if (need <= b.n) return hipSuccess;
+    DBuf<T> nb;
+    nb.retired = b.retired;
+    hipError_t e = nb.ensure(need);
+    if (e != hipSuccess) return e;
+    keep = std::min<int64_t>(keep, b.n);
+    if (keep > 0) {
+        e = hipMemcpyAsync(nb.p, b.p, (size_t)keep * sizeof(T), hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) {
+            nb.release();
+            return e;
+        }
+    }
+    b.release();
+    b.p = nb.p;
+    b.n = nb.n;
+    b.bytes = nb.bytes;
+    return hipSuccess;
+}
 
 /* One generation of sample blocks: 7 planes of float or double. */
 struct Generation {
@@ -932,6 +958,125 @@ int scphd_resample(scphd_t* f, int32_t* resampled) {
     return SCPHD_OK;
 }
 
+/* ---- resampling without device allocation ----
+ * The scratch of scphd_resample_noalloc: a static device array, one per GPU
+ * the library runs on, held until the process ends.  Host code never passes
+ * the array's name to a kernel (on the host it names a stand-in, not a
+ * device address): the device address is looked up once per device with
+ * hipGetSymbolAddress and kept here. */
+static constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)8 << 20;
+static __device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+
+struct ScratchAddrs {
+    std::mutex mu;
+    std::map<int, char*> by_dev;
+};
+
+static ScratchAddrs& scratch_addrs() {
+    static ScratchAddrs* a = new ScratchAddrs();
+    return *a;
+}
+
+/* device address of `scratch` on dev, which must be the current device */
+static hipError_t scratch_address(int dev, char** out) {
+    ScratchAddrs& sa = scratch_addrs();
+    std::lock_guard<std::mutex> lk(sa.mu);
+    auto it = sa.by_dev.find(dev);
+    if (it == sa.by_dev.end()) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        it = sa.by_dev.emplace(dev, (char*)p).first;
+    }
+    *out = it->second;
+    return hipSuccess;
+}
+
+/* byte offsets into the scratch for n parents: cumulative weights,
+ * gathered extrinsics, ancestors, new map sizes, scan partials, flags */
+struct NoAllocLayout {
+    size_t cum, ext_tmp, anc, newcnt, scantmp, flag, bytes;
+    explicit NoAllocLayout(int n) {
+        auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+        cum = 0;
+        ext_tmp = up(cum + (size_t)n * sizeof(double));
+        anc = up(ext_tmp + (size_t)n * 12 * sizeof(double));
+        newcnt = up(anc + (size_t)n * sizeof(int32_t));
+        scantmp = up(newcnt + ((size_t)n + 1) * sizeof(int32_t));
+        flag = up(scantmp + ((size_t)scan_tmp_size((int64_t)n + 1) + 16) * sizeof(int32_t));
+        bytes = up(flag + 4 * sizeof(int32_t));
+    }
+};
+
+int scphd_resample_noalloc(scphd_t* f, int32_t* resampled) {
+    if (check_handle(f)) return SCPHD_ERR_ARG;
+    if (f->ensemble) return fail(f, SCPHD_ERR_ARG, "scphd_resample_noalloc takes a plain filter, not an ensemble");
+    const int Np = f->Np;
+    const NoAllocLayout ly(Np);
+    if (ly.bytes > NOALLOC_SCRATCH_BYTES)
+        return fail(f, SCPHD_ERR_CAPACITY, "scphd_resample_noalloc: " + std::to_string(Np) +
+                                               " parents need more than the static scratch of 8 MiB (" +
+                                               std::to_string(ly.bytes) + " bytes)");
+    HIPTRY(hipSetDevice(f->dev));
+    char* base = nullptr;
+    HIPTRY(scratch_address(f->dev, &base));
+    /* a deferred predict is applied into the other generation, which must
+     * already hold the map */
+    if (f->pending && f->Ng > 0 && f->gen[1 - f->cur].cap_blocks < f->Ng)
+        return fail(f, SCPHD_ERR_CAPACITY,
+                    "scphd_resample_noalloc: the deferred predict needs a larger sample generation (scphd_reserve)");
+    int rc = materialize(f);
+    if (rc) return rc;
+    hipStream_t s = f->stream;
+    ResampleArgs ra;
+    ra.pw = f->pw.p;
+    ra.ext = f->ext.p;
+    ra.ext_tmp = (double*)(base + ly.ext_tmp);
+    ra.cum = (double*)(base + ly.cum);
+    ra.anc = (int32_t*)(base + ly.anc);
+    ra.newcnt = (int32_t*)(base + ly.newcnt);
+    ra.goff = f->goff.p;
+    ra.Np = Np;
+    ra.min_neff = f->prm.min_neff;
+    ra.key = f->key;
+    ra.rstep = (uint64_t)f->n_resample;
+    ra.flag = (int32_t*)(base + ly.flag);
+    ra.rflag = f->drflag.p;
+    ra.neff = f->dneff.p;
+    ra.defer = 1;
+    /* the scratch keeps the last call's contents: clear the flag */
+    HIPTRY(launch_fill_i32(s, ra.flag, 0, 1));
+    HIPTRY(launch_resample(s, ra));
+    HIPTRY(launch_scan(s, ra.newcnt, f->goff2.p, Np, (int32_t*)(base + ly.scantmp)));
+    HIPTRY(launch_readout(s, f->goff2.p, Np, 0, ra.flag, nullptr, f->dmap));
+    HIPTRY(hipStreamSynchronize(s));
+    const int go = f->hmap[3];
+    const int64_t Nnew = f->hmap[0];
+    if (go && (Nnew + 1 > f->gparent2.n || Nnew + 1 > f->gblock2.n || Nnew + 1 > f->gw2.n ||
+               (f->hint_ok && Nnew + 1 > f->gpd2.n)))
+        return fail(f, SCPHD_ERR_CAPACITY, "scphd_resample_noalloc: the resampled map table needs " +
+                                               std::to_string(Nnew + 1) + " entries, more than the handle holds " +
+                                               "(scphd_reserve); the filter is unchanged");
+    HIPTRY(launch_resample_commit(s, f->ext.p, ra.ext_tmp, f->pw.p, f->anc.p, ra.anc, Np, go));
+    f->n_resample++;
+    if (resampled) *resampled = go;
+    if (go) {
+        HIPTRY(launch_table_resample(s, ra.anc, f->goff.p, f->goff2.p, f->gblock.p, f->gw.p, Np, Nnew,
+                                     f->gparent2.p, f->gblock2.p, f->gw2.p, f->hint_ok ? f->gpd.p : nullptr,
+                                     f->hint_ok ? f->gpd2.p : nullptr));
+        std::swap(f->goff, f->goff2);
+        std::swap(f->gparent, f->gparent2);
+        std::swap(f->gblock, f->gblock2);
+        std::swap(f->gw, f->gw2);
+        if (f->hint_ok) std::swap(f->gpd, f->gpd2);
+        f->Ng = Nnew;
+    }
+    /* the caller may reuse the scratch (another handle on this GPU) once
+     * this returns */
+    HIPTRY(hipStreamSynchronize(s));
+    return SCPHD_OK;
+}
+
 int scphd_check(scphd_t* f) {
     if (check_handle(f)) return SCPHD_ERR_ARG;
     HIPTRY(hipSetDevice(f->dev));
@@ -1219,11 +1364,16 @@ int scphd_reserve(scphd_t* f, int64_t n_gaussians, int32_t max_meas) {
     HIPTRY(f->ndfill.ensure(G + 1));
     HIPTRY(f->ndlist.ensure(G));
     HIPTRY(f->scantmp.ensure(scan_tmp_size(std::max<int64_t>(G, Np) + 1) + 16));
-    /* the successor table (the current one holds live entries) */
+    /* the successor table, and the current one keeping its entries, so that
+     * the table swaps of later updates and resamplings find both sized */
     HIPTRY(f->gparent2.ensure(G + 1));
     HIPTRY(f->gblock2.ensure(G + 1));
     HIPTRY(f->gw2.ensure(G + 1));
     HIPTRY(f->gpd2.ensure(G + 1));
+    HIPTRY(grow_keep(f->gparent, G + 1, f->Ng, f->stream));
+    HIPTRY(grow_keep(f->gblock, G + 1, f->Ng, f->stream));
+    HIPTRY(grow_keep(f->gw, G + 1, f->Ng, f->stream));
+    HIPTRY(grow_keep(f->gpd, G + 1, f->Ng, f->stream));
     /* survivors: about one per Gaussian in steady state */
     HIPTRY(f->sw.ensure(G + 1));
     HIPTRY(f->sg.ensure(G + 1));

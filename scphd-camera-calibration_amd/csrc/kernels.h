@@ -303,8 +303,14 @@ struct ResampleArgs {
     int32_t* flag;  /* out: 1 if any rig resampled (zero before the launch) */
     int32_t* rflag = nullptr; /* out: per rig, 1 if resampled (may be null) */
     double* neff;   /* out: per rig */
+    int defer = 0;  /* nonzero: leave ext and pw unchanged (k_resample_commit applies the gather) */
 };
 hipError_t launch_resample(hipStream_t s, const ResampleArgs& a);
+/* Apply a deferred resampling of one rig's N parents: anc_out <- anc, and
+ * when go, ext <- ext_tmp and pw <- 1/N. */
+hipError_t launch_resample_commit(hipStream_t s, double* ext, const double* ext_tmp, double* pw, int32_t* anc_out,
+                                  const int32_t* anc, int N, int go);
+hipError_t launch_fill_i32(hipStream_t s, int32_t* x, int32_t v, int64_t n);
 hipError_t launch_table_resample(hipStream_t s, const int32_t* anc, const int32_t* goff_old,
                                  const int32_t* goff_new, const int32_t* gblock_old, const double* gw_old,
                                  int Np, int64_t Nnew, int32_t* gparent, int32_t* gblock, double* gw,

@@ -2136,11 +2136,28 @@ __global__ __launch_bounds__(1024) void k_resample(ResampleArgs a) {
         for (int q = 0; q < 12; ++q) a.ext_tmp[(int64_t)j * 12 + q] = a.ext[(int64_t)anc * 12 + q];
         a.newcnt[j] = a.goff[anc + 1] - a.goff[anc];
     }
+    if (a.defer) return; /* k_resample_commit applies the gather */
     __syncthreads();
     for (int j = t; j < N; j += blockDim.x) {
         for (int q = 0; q < 12; ++q) a.ext[(int64_t)j * 12 + q] = a.ext_tmp[(int64_t)j * 12 + q];
         a.pw[j] = 1.0 / N;
     }
+}
+
+__global__ void k_resample_commit(double* ext, const double* ext_tmp, double* pw, int32_t* anc_out,
+                                  const int32_t* anc, int N, int go) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    anc_out[j] = anc[j];
+    if (go) {
+        for (int q = 0; q < 12; ++q) ext[(int64_t)j * 12 + q] = ext_tmp[(int64_t)j * 12 + q];
+        pw[j] = 1.0 / N;
+    }
+}
+
+__global__ void k_fill_i32(int32_t* x, int32_t v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
 }
 
 __global__ void k_iota(int32_t* x, int64_t n) {
@@ -2477,6 +2494,19 @@ hipError_t launch_nan_count(hipStream_t s, const double* w, int n, int32_t* out)
 hipError_t launch_readout(hipStream_t s, const int32_t* off, int n, int32_t limit, const int32_t* x0,
                           const int32_t* x1, int32_t* out) {
     k_readout<<<1, 1024, 0, s>>>(off, n, limit, x0, x1, out);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_resample_commit(hipStream_t s, double* ext, const double* ext_tmp, double* pw, int32_t* anc_out,
+                                  const int32_t* anc, int N, int go) {
+    if (N <= 0) return hipSuccess;
+    k_resample_commit<<<nblocks(N, 256), 256, 0, s>>>(ext, ext_tmp, pw, anc_out, anc, N, go);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_fill_i32(hipStream_t s, int32_t* x, int32_t v, int64_t n) {
+    if (n <= 0) return hipSuccess;
+    k_fill_i32<<<nblocks(n, 256), 256, 0, s>>>(x, v, n);
     LAUNCH_CHECK();
 }
 

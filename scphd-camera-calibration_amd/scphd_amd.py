@@ -38,7 +38,7 @@ EXPORTED = [
     "scphd_batch_submit", "scphd_batch_step", "scphd_batch_collect", "scphd_batch_get_stats",
     "scphd_batch_filter", "scphd_mat_read_measurements", "scphd_write_mat", "scphd_batch_set_output",
     "scphd_create_ensemble", "scphd_update_ensemble", "scphd_get_rig_estimates", "scphd_get_rig_resample",
-    "scphd_reserve", "scphd_debug_set_limits",
+    "scphd_reserve", "scphd_debug_set_limits", "scphd_resample_noalloc",
 ]
 
 
@@ -139,6 +139,7 @@ def lib():
         "scphd_update": (_I32, [_P, dp, dp, _I32, _I32]),
         "scphd_update_device": (_I32, [_P, _P, _P, _I32, _I32]),
         "scphd_resample": (_I32, [_P, ip]),
+        "scphd_resample_noalloc": (_I32, [_P, ip]),
         "scphd_check": (_I32, [_P]),
         "scphd_sync": (_I32, [_P]),
         "scphd_get_counts": (_I32, [_P, ctypes.POINTER(Counts)]),
@@ -285,6 +286,12 @@ class Filter:
     def resample(self):
         r = _I32(0)
         self._check(lib().scphd_resample(self.h, ctypes.byref(r)), "resample")
+        return bool(r.value)
+
+    def resample_noalloc(self):
+        """scphd_resample with no device allocation (static scratch)."""
+        r = _I32(0)
+        self._check(lib().scphd_resample_noalloc(self.h, ctypes.byref(r)), "resample_noalloc")
         return bool(r.value)
 
     def check(self):
