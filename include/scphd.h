@@ -253,11 +253,15 @@ int scphd_debug_set_limits(scphd_t* f, int32_t merge_lds_cap, int32_t eta_heavy_
  * (src/scphdcameracalibration.cu:1047-1118). */
 int scphd_set_noise(scphd_t* f, int32_t mode, uint64_t seed, const double* injected, size_t n);
 
-/* Six standard normals per element elem = 0..n_elem-1 of one noise stream
- * (counter: elem << 2 | block, gauss, parent, step << 4 | stream; key =
- * seed), drawn on the device exactly as the filter kernels draw them, into
- * out[6 * n_elem].  For verifying the device noise against a host
- * restatement; no counterpart in the reference. */
+/* Standard normals of one noise stream (counter: elem << 2 | block, gauss,
+ * parent, step << 4 | stream; key = seed), drawn on the device exactly as
+ * the filter kernels draw them, six per element i = 0..n_elem-1 into
+ * out[6 * n_elem]: for stream 2 (landmark prediction) the three normals of
+ * sample i, then three zeros; for stream 3 (posterior sampling) the six
+ * normals of sample i (samples share Philox blocks in groups, see the
+ * kernels); for the other streams the six normals of element i.  For
+ * verifying the device noise against a host restatement; no counterpart in
+ * the reference. */
 int scphd_debug_normals(uint64_t seed, uint32_t stream, uint64_t step, uint32_t parent, uint32_t gauss,
                         int32_t n_elem, double* out);
 

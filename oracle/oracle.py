@@ -92,6 +92,7 @@ def lib():
         "orc_get_fit_debug": (None, [_P, dp]),
         "orc_philox": (None, [up, up, up]),
         "orc_normals": (None, [_U64, _U32, _U64, _U32, _U32, _U32, _I32, dp]),
+        "orc_stream_normals": (None, [_U64, _U32, _U64, _U32, _U32, _U32, dp]),
         "orc_box_muller": (None, [_U32, _U32, dp]),
         "orc_wrap_angle": (_D, [_D]),
         "orc_safe_log": (_D, [_D]),
@@ -247,6 +248,14 @@ def normals(seed, stream, step, parent, gauss, elem, n):
     z = np.zeros(n)
     lib().orc_normals(seed, stream, step, parent, gauss, elem, n, _dp(z))
     return z
+
+
+def stream_normals(seed, stream, step, parent, gauss, s):
+    """The normals of sample s of stream 2 (three: landmark prediction) or 3
+    (six: posterior sampling), as the filter draws them."""
+    z = np.zeros(6)
+    lib().orc_stream_normals(seed, stream, step, parent, gauss, s, _dp(z))
+    return z[:3] if stream == 2 else z
 
 
 def rotation(e):

@@ -11,7 +11,9 @@
  *   D1  noise: Philox4x32-10 counter streams + Box-Muller (box_muller,
  *       single-precision polynomial log / sin / cos) instead of thrust's
  *       time-seeded default_random_engine (reference :92-127, :1062-1118,
- *       :640-690, :2663-2666, prior :912-979).
+ *       :640-690, :2663-2666, prior :912-979).  The landmark-prediction
+ *       normals of four samples share three Philox blocks, the posterior
+ *       normals of two samples share three (feature_elem, posterior_elem).
  *   D2  exact per-parent segmentation of normalisers, cardinalities and map
  *       offsets (reference reduce_by_key misaligns when a parent map is empty
  *       or M == 1, :1750-1756, :1824-1837, :2566-2572).
@@ -395,6 +397,45 @@ static void normals(uint64_t seed, uint32_t stream, uint64_t step, uint32_t pare
     }
 }
 
+/* The normals of a group of samples that share Philox blocks: blocks
+ * b = 0 .. nb - 1 of element elem give n[4 b + 2 h], n[4 b + 2 h + 1] =
+ * box_muller(words 2 h, 2 h + 1 of block b). */
+static void group_normals(uint64_t seed, uint32_t stream, uint64_t step, uint32_t parent, uint32_t gauss,
+                          uint32_t elem, int nb, double* n) {
+    for (int b = 0; b < nb; ++b) {
+        uint32_t x[4];
+        draw(seed, stream, step, parent, gauss, elem, (uint32_t)b, x);
+        box_muller(x[0], x[1], n[4 * b], n[4 * b + 1]);
+        box_muller(x[2], x[3], n[4 * b + 2], n[4 * b + 3]);
+    }
+}
+
+/* D1, landmark prediction: three normals per sample, four samples to three
+ * blocks.  Sample s is member i = (s >> 4) & 3 of the group with element
+ * (s >> 6) << 4 | (s & 15) -- the samples s, s + 16, s + 32, s + 48 of one
+ * 64-sample stretch -- and takes the group's normals n[3 i .. 3 i + 2]. */
+static inline uint32_t feature_elem(uint32_t s) { return ((s >> 6) << 4) | (s & 15u); }
+
+/* D1, posterior sampling: six normals per sample, two samples to three
+ * blocks.  Sample s is member i = (s >> 4) & 1 of the group with element
+ * (s >> 5) << 4 | (s & 15) and takes n[6 i .. 6 i + 5]. */
+static inline uint32_t posterior_elem(uint32_t s) { return ((s >> 5) << 4) | (s & 15u); }
+
+/* the normals of one sample of either stream (the unit tests' view) */
+static void stream_normals(uint64_t seed, uint32_t stream, uint64_t step, uint32_t parent, uint32_t gauss,
+                           uint32_t s, double* z) {
+    double n[12];
+    if (stream == ST_FEATURE) {
+        const int i = (int)((s >> 4) & 3u);
+        group_normals(seed, stream, step, parent, gauss, feature_elem(s), 3, n);
+        for (int c = 0; c < 3; ++c) z[c] = n[3 * i + c];
+    } else {
+        const int i = (int)((s >> 4) & 1u);
+        group_normals(seed, stream, step, parent, gauss, posterior_elem(s), 3, n);
+        for (int c = 0; c < 6; ++c) z[c] = n[6 * i + c];
+    }
+}
+
 /* exported for the unit tests: one Box-Muller pair from raw words */
 extern "C" void orc_box_muller(uint32_t x0, uint32_t x1, double* z) { box_muller(x0, x1, z[0], z[1]); }
 
@@ -485,15 +526,21 @@ static void predict(State& st, double dt) {
     for (int p = 0; p < st.Np; ++p) {
         for (int64_t g = goff[p]; g < goff[p + 1]; ++g) {
             const uint32_t k = (uint32_t)(g - goff[p]);
-            for (int s = 0; s < S; ++s) {
-                const size_t i = (size_t)g * S + s;
-                double a[3];
-                normals(P.seed, ST_FEATURE, step, (uint32_t)p, k, (uint32_t)s, 3, a);
-                for (int c = 0; c < 3; ++c) {
-                    const double ac = a[c] * P.feat_accel_std[c];
-                    const double x = st.smp[c][i], v = st.smp[3 + c][i];
-                    st.smp[c][i] = x + v * dt + 0.5 * ac * dt * dt;
-                    st.smp[3 + c][i] = v + ac * dt;
+            /* groups of samples s0, s0 + 16, s0 + 32, s0 + 48 (D1): the
+             * blocks their members need, then their motion */
+            for (int s0 = 0; s0 < S; s0 += ((s0 & 15) == 15 ? 49 : 1)) {
+                int nm = 0;
+                while (nm < 4 && s0 + 16 * nm < S) ++nm;
+                double n[12];
+                group_normals(P.seed, ST_FEATURE, step, (uint32_t)p, k, feature_elem((uint32_t)s0), (3 * nm + 3) / 4, n);
+                for (int m = 0; m < nm; ++m) {
+                    const size_t i = (size_t)g * S + s0 + 16 * m;
+                    for (int c = 0; c < 3; ++c) {
+                        const double ac = n[3 * m + c] * P.feat_accel_std[c];
+                        const double x = st.smp[c][i], v = st.smp[3 + c][i];
+                        st.smp[c][i] = x + v * dt + 0.5 * ac * dt * dt;
+                        st.smp[3 + c][i] = v + ac * dt;
+                    }
                 }
             }
             st.wg[g] *= P.ps;
@@ -785,9 +832,15 @@ static int update(State& st, const double* zu, const double* zv, int M, int fixe
             new_wg[p][k] = mg[k].w;
             double L[36];
             cholesky6(mg[k].P, L);
-            for (int s = 0; s < S; ++s) {
-                double z[6];
-                normals(P.seed, ST_POSTERIOR, (uint64_t)st.n_update, (uint32_t)p, (uint32_t)k, (uint32_t)s, 6, z);
+            /* D1: samples s0 and s0 + 16 of a 32-sample stretch share three blocks */
+            for (int s0 = 0; s0 < S; s0 += ((s0 & 15) == 15 ? 17 : 1)) {
+              const int nm = s0 + 16 < S ? 2 : 1;
+              double gn[12];
+              group_normals(P.seed, ST_POSTERIOR, (uint64_t)st.n_update, (uint32_t)p, (uint32_t)k,
+                            posterior_elem((uint32_t)s0), nm == 2 ? 3 : 2, gn);
+              for (int m = 0; m < nm; ++m) {
+                const int s = s0 + 16 * m;
+                const double* z = gn + 6 * m;
                 double q[6];
                 for (int r = 0; r < 6; ++r) {
                     double a = 0.0;
@@ -800,6 +853,7 @@ static int update(State& st, const double* zu, const double* zv, int M, int fixe
                 const size_t o = (size_t)k * S + s;
                 for (int c = 0; c < 6; ++c) new_smp[c][p][o] = pwd[c];
                 new_smp[6][p][o] = 1.0 / S;
+              }
             }
         }
         for (size_t j = 0; j < keep.size(); ++j) {
@@ -1051,6 +1105,13 @@ void orc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) { philo
 void orc_normals(uint64_t seed, uint32_t stream, uint64_t step, uint32_t parent, uint32_t gauss,
                  uint32_t elem, int n, double* z) {
     normals(seed, stream, step, parent, gauss, elem, n, z);
+}
+
+/* the landmark-prediction (stream 2: 3 values) or posterior (stream 3: 6
+ * values) normals of sample s */
+void orc_stream_normals(uint64_t seed, uint32_t stream, uint64_t step, uint32_t parent, uint32_t gauss, uint32_t s,
+                        double* z) {
+    stream_normals(seed, stream, step, parent, gauss, s, z);
 }
 
 double orc_wrap_angle(double a) { return wrap_angle(a); }

@@ -121,8 +121,8 @@ SCPHD_HD inline double uniform_oc(uint32_t lo, uint32_t hi) { /* (0, 1] */
  *       and cos(pi/4 t) = C(t^2) are Taylor polynomials to t^9 and t^8
  *       (|error| < 4e-8);
  *   r = sqrt(-2 log a); z0 = r cos 2 pi c, z1 = r sin 2 pi c (single
- *   precision, widened to double). */
-__device__ inline void box_muller(uint32_t x0, uint32_t x1, double& z0, double& z1) {
+ *   precision; box_muller widens them to double). */
+__device__ inline void box_muller_f(uint32_t x0, uint32_t x1, float& z0, float& z1) {
 #pragma clang fp contract(off)
     const uint32_t n = x0 + 1u; /* 0 stands for a = 1 (log a = 0) */
     const uint32_t nn = n != 0u ? n : 1u;
@@ -172,8 +172,15 @@ __device__ inline void box_muller(uint32_t x0, uint32_t x1, double& z0, double& 
     const float sn0 = swap ? pc : sv, cs0 = swap ? sv : pc;
     const float sn = __uint_as_float(__float_as_uint(sn0) ^ ((q & 2u) << 30));
     const float cs = __uint_as_float(__float_as_uint(cs0) ^ (((q + 1u) & 2u) << 30));
-    z0 = (double)(r * cs);
-    z1 = (double)(r * sn);
+    z0 = r * cs;
+    z1 = r * sn;
+}
+
+__device__ inline void box_muller(uint32_t x0, uint32_t x1, double& z0, double& z1) {
+    float a, b;
+    box_muller_f(x0, x1, a, b);
+    z0 = (double)a;
+    z1 = (double)b;
 }
 
 /* N (<= 8) standard normals, four per Philox block: words (x0, x1) give
@@ -196,6 +203,129 @@ __device__ inline void normals(Key key, uint32_t stream, uint64_t step, uint32_t
             }
         }
     }
+}
+
+/* Landmark-prediction normals: three per sample, four samples to three
+ * Philox blocks.  Sample s is member i = (s >> 4) & 3 of the group whose
+ * element is feature_elem(s) = (s >> 6) << 4 | (s & 15): the four samples
+ * that one lane of a 16-lane row visits in four consecutive steps of 16.
+ * The group's blocks b = 0, 1, 2 give twelve normals, n[4 b + 2 h] and
+ * n[4 b + 2 h + 1] from box_muller of block b's words 2 h and 2 h + 1, and
+ * member i takes n[3 i], n[3 i + 1], n[3 i + 2]. */
+SCPHD_HD constexpr uint32_t feature_elem(uint32_t s) { return ((s >> 6) << 4) | (s & 15u); }
+
+/* any one sample (one or two blocks) */
+template <typename Key>
+__device__ inline void feature_normals(Key key, uint64_t step, uint32_t p, uint32_t k, uint32_t s, double a[3]) {
+    const uint32_t e = feature_elem(s), i = (s >> 4) & 3u;
+    uint32_t x[4];
+    float n0, n1, n2, n3;
+    noise_block(key, NS_FEATURE, step, p, k, e, i == 0u ? 0u : i - 1u, x); /* the block of n[3 i] */
+    box_muller_f(x[0], x[1], n0, n1);
+    box_muller_f(x[2], x[3], n2, n3);
+    if (i == 0u) {
+        a[0] = n0; a[1] = n1; a[2] = n2;
+    } else if (i == 3u) {
+        a[0] = n1; a[1] = n2; a[2] = n3;
+    } else {
+        uint32_t y[4];
+        float m0, m1;
+        noise_block(key, NS_FEATURE, step, p, k, e, i, y);
+        box_muller_f(y[0], y[1], m0, m1);
+        if (i == 1u) {
+            a[0] = n3; a[1] = m0; a[2] = m1;
+        } else {
+            a[0] = n2; a[1] = n3; a[2] = m0;
+        }
+    }
+}
+
+/* A group's members in order (i = 0, 1, 2, 3, the same in every lane that
+ * calls): one block for each of the first three, the normals the next
+ * member needs carried in c. */
+struct FeatureCarry {
+    float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+};
+
+template <typename Key>
+__device__ inline void feature_normals_seq(Key key, uint64_t step, uint32_t p, uint32_t k, uint32_t s, uint32_t i,
+                                           FeatureCarry& c, double a[3]) {
+    float n0 = 0.0f, n1 = 0.0f, n2 = 0.0f, n3 = 0.0f;
+    if (i != 3u) {
+        uint32_t x[4];
+        noise_block(key, NS_FEATURE, step, p, k, feature_elem(s), i, x);
+        box_muller_f(x[0], x[1], n0, n1);
+        box_muller_f(x[2], x[3], n2, n3);
+    }
+    if (i == 0u) {
+        a[0] = n0; a[1] = n1; a[2] = n2;
+        c.c0 = n3;
+    } else if (i == 1u) {
+        a[0] = c.c0; a[1] = n0; a[2] = n1;
+        c.c0 = n2; c.c1 = n3;
+    } else if (i == 2u) {
+        a[0] = c.c0; a[1] = c.c1; a[2] = n0;
+        c.c0 = n1; c.c1 = n2; c.c2 = n3;
+    } else {
+        a[0] = c.c0; a[1] = c.c1; a[2] = c.c2;
+    }
+}
+
+/* Posterior-sampling normals: six per sample, two samples to three Philox
+ * blocks.  Sample s is member i = (s >> 4) & 1 of the group whose element
+ * is posterior_elem(s) = (s >> 5) << 4 | (s & 15) (the two samples one lane
+ * visits in consecutive steps of 16); the group's twelve normals are laid
+ * out as for the landmark prediction, and member i takes n[6 i .. 6 i + 5]. */
+SCPHD_HD constexpr uint32_t posterior_elem(uint32_t s) { return ((s >> 5) << 4) | (s & 15u); }
+
+template <typename Key>
+__device__ inline void posterior_normals(Key key, uint64_t step, uint32_t p, uint32_t k, uint32_t s, double z[6]) {
+    const uint32_t e = posterior_elem(s), i = (s >> 4) & 1u;
+    uint32_t x[4], y[4];
+    float n[6];
+    noise_block(key, NS_POSTERIOR, step, p, k, e, i == 0u ? 0u : 1u, x);
+    noise_block(key, NS_POSTERIOR, step, p, k, e, i == 0u ? 1u : 2u, y);
+    if (i == 0u) { /* n0..n3 from block 0, n4 n5 from block 1's first pair */
+        box_muller_f(x[0], x[1], n[0], n[1]);
+        box_muller_f(x[2], x[3], n[2], n[3]);
+        box_muller_f(y[0], y[1], n[4], n[5]);
+    } else { /* n6 n7 from block 1's second pair, n8..n11 from block 2 */
+        box_muller_f(x[2], x[3], n[0], n[1]);
+        box_muller_f(y[0], y[1], n[2], n[3]);
+        box_muller_f(y[2], y[3], n[4], n[5]);
+    }
+#pragma unroll
+    for (int j = 0; j < 6; ++j) z[j] = (double)n[j];
+}
+
+/* The pair in order (i = 0 then 1, the same in every lane that calls): two
+ * blocks for the first, one for the second, block 1's second pair carried. */
+struct PosteriorCarry {
+    float c0 = 0.0f, c1 = 0.0f;
+};
+
+template <typename Key>
+__device__ inline void posterior_normals_seq(Key key, uint64_t step, uint32_t p, uint32_t k, uint32_t s, uint32_t i,
+                                             PosteriorCarry& c, double z[6]) {
+    const uint32_t e = posterior_elem(s);
+    float n[6];
+    uint32_t x[4];
+    if (i == 0u) {
+        noise_block(key, NS_POSTERIOR, step, p, k, e, 0u, x);
+        box_muller_f(x[0], x[1], n[0], n[1]);
+        box_muller_f(x[2], x[3], n[2], n[3]);
+        noise_block(key, NS_POSTERIOR, step, p, k, e, 1u, x);
+        box_muller_f(x[0], x[1], n[4], n[5]);
+        box_muller_f(x[2], x[3], c.c0, c.c1);
+    } else {
+        n[0] = c.c0;
+        n[1] = c.c1;
+        noise_block(key, NS_POSTERIOR, step, p, k, e, 2u, x);
+        box_muller_f(x[0], x[1], n[2], n[3]);
+        box_muller_f(x[2], x[3], n[4], n[5]);
+    }
+#pragma unroll
+    for (int j = 0; j < 6; ++j) z[j] = (double)n[j];
 }
 
 SCPHD_HD inline double safe_log(double x) { return x <= 0 ? SCPHD_LOG0 : log(x); }

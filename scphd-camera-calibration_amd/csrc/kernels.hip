@@ -200,7 +200,17 @@ template <typename Key>
 __device__ inline EuclideanPoint predict_sample(EuclideanPoint x, const LinearCVMotionModel3D& feat, double dt,
                                                 Key key, uint64_t step, uint32_t p, uint32_t k, uint32_t s) {
     double a[3];
-    normals<3>(key, NS_FEATURE, step, p, k, s, a);
+    feature_normals(key, step, p, k, s, a);
+    return feat.computeNoisyMotion(x, dt, a[0] * feat.std_ax(), a[1] * feat.std_ay(), a[2] * feat.std_az());
+}
+
+/* the same for the members of a noise group in order (phase = member) */
+template <typename Key>
+__device__ inline EuclideanPoint predict_sample_seq(EuclideanPoint x, const LinearCVMotionModel3D& feat, double dt,
+                                                    Key key, uint64_t step, uint32_t p, uint32_t k, uint32_t s,
+                                                    uint32_t phase, FeatureCarry& c) {
+    double a[3];
+    feature_normals_seq(key, step, p, k, s, phase, c, a);
     return feat.computeNoisyMotion(x, dt, a[0] * feat.std_ax(), a[1] * feat.std_ay(), a[2] * feat.std_az());
 }
 
@@ -280,10 +290,15 @@ __device__ inline void load_sample(const Planes<T>& src, int64_t si, T r[7]) {
 template <typename T, typename Key>
 __device__ inline void eval_sample(const FitArgs& a, const DisparityMeasurementModel& meas, const T r[7], Key kk,
                                    int pl, int k, int s, const CameraPose& pose, EuclideanPoint& x, DisparityPoint& d,
-                                   bool& in, double& w) {
+                                   bool& in, double& w, FeatureCarry* fc = nullptr, uint32_t phase = 0) {
     x = EuclideanPoint{(double)r[0], (double)r[1], (double)r[2], (double)r[3], (double)r[4], (double)r[5]};
     w = (double)r[6];
-    if (a.pending) x = predict_sample(x, a.feat, a.dt, kk, a.pstep, (uint32_t)pl, (uint32_t)k, (uint32_t)s);
+    if (a.pending) {
+        if (fc)
+            x = predict_sample_seq(x, a.feat, a.dt, kk, a.pstep, (uint32_t)pl, (uint32_t)k, (uint32_t)s, phase, *fc);
+        else
+            x = predict_sample(x, a.feat, a.dt, kk, a.pstep, (uint32_t)pl, (uint32_t)k, (uint32_t)s);
+    }
     d = disparity_fast(meas, pose, x);
     in = meas.inRange(d);
 }
@@ -412,6 +427,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
 
     double spd = 0.0, sw = 0.0, sw2 = 0.0, m1[6], m2[21], K[6];
     bool kw = false; /* K is a weighted sample (uniform along the row) */
+    FeatureCarry fc; /* prediction normals carried between the steps of a noise group */
 #pragma unroll
     for (int c = 0; c < 6; ++c) m1[c] = 0.0;
 #pragma unroll
@@ -442,7 +458,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
         DisparityPoint d;
         bool in = false;
         double w = 0.0;
-        if (valid) eval_sample(a, meas, cur, kk, pl, k, s, pose, x, d, in, w);
+        if (valid) eval_sample(a, meas, cur, kk, pl, k, s, pose, x, d, in, w, &fc, (uint32_t)(s0 >> 4) & 3u);
         const double q[6] = {d.u, d.v, d.d, d.vu, d.vv, d.vd};
         const double pd = in ? Pd : 0.0;
         const double wp = valid ? w * pd : 0.0;
@@ -555,7 +571,9 @@ __global__ __launch_bounds__(256) void k_nd_fill(FitArgs a, Planes<T> src, Plane
         const int64_t b0 = (int64_t)a.gblock[g] * a.S;
         const int64_t d0 = (a.nd_base + g) * (int64_t)a.S;
         const CameraPose pose = load_pose(a.pose, p);
-        /* four steps' loads issued before their arithmetic */
+        /* four steps' loads issued before their arithmetic; the four steps
+         * are one noise group (phase j) */
+        FeatureCarry fc;
         for (int s0 = 0; s0 < a.S; s0 += 64) {
             T r[4][7];
 #pragma unroll
@@ -571,7 +589,7 @@ __global__ __launch_bounds__(256) void k_nd_fill(FitArgs a, Planes<T> src, Plane
                     DisparityPoint d;
                     bool in;
                     double w;
-                    eval_sample(a, meas, r[j], kk, pl, k, s, pose, x, d, in, w);
+                    eval_sample(a, meas, r[j], kk, pl, k, s, pose, x, d, in, w, &fc, (uint32_t)j);
                     write_nd_off(meas, dst, (Off)((d0 + s) * (int64_t)sizeof(T)), x, in);
                 }
             }
@@ -1817,8 +1835,13 @@ __global__ void k_debug_normals(NoiseKey key, uint32_t stream, uint64_t step, ui
                                 double* out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double z[6];
-    normals<6>(key, stream, step, parent, gauss, (uint32_t)i, z);
+    double z[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (stream == NS_FEATURE)
+        feature_normals(key, step, parent, gauss, (uint32_t)i, z);
+    else if (stream == NS_POSTERIOR)
+        posterior_normals(key, step, parent, gauss, (uint32_t)i, z);
+    else
+        normals<6>(key, stream, step, parent, gauss, (uint32_t)i, z);
 #pragma unroll
     for (int c = 0; c < 6; ++c) out[(int64_t)i * 6 + c] = z[c];
 }
@@ -1955,9 +1978,10 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a, Planes<T> dst) {
     const CameraPose pose = load_pose(a.pose, p);
     const T w = (T)(1.0 / a.S);
     Off off = (Off)(((a.post_base + gm) * (int64_t)a.S + lane) * (int64_t)sizeof(T));
+    PosteriorCarry pc; /* normals carried between the two steps of a noise group */
     for (int s = lane; s < a.S; s += 16) {
         double z[6];
-        normals<6>(key, NS_POSTERIOR, a.ustep, (uint32_t)pl, (uint32_t)k, (uint32_t)s, z);
+        posterior_normals_seq(key, a.ustep, (uint32_t)pl, (uint32_t)k, (uint32_t)s, (uint32_t)(s >> 4) & 1u, pc, z);
         double q[6];
 #pragma unroll
         for (int r = 0; r < 6; ++r) {

@@ -10,7 +10,7 @@ world and the baseline together leaves every projection unchanged, so the
 translation's length is fixed only by the priors and is not asserted.
 
 Asserted over scene seeds 1-4 (the first four; per-seed figures of seeds
-1-8 ranged from 0.14 to 0.78 of the prior's error):
+1-8 ranged from 0.20 to 0.68 of the prior's error, 0.42 on average):
   * the rotation error of the weighted parent mean, averaged over the last
     10 frames, is below 0.85 of the prior mean's error for every seed, and
     below 0.6 of it on average over the seeds.  256 parents spread over

@@ -205,9 +205,14 @@ def test_device_noise_equals_the_oracle_bit_for_bit():
     import scphd_amd
 
     n = 4096
-    for seed, stream, step, parent, gauss in [(1, 2, 0, 0, 0), (2**64 - 59, 3, 17, 4095, 1999), (7, 1, 123456, 9, 0)]:
+    for seed, stream, step, parent, gauss in [(1, 2, 0, 0, 0), (2**64 - 59, 3, 17, 4095, 1999), (7, 1, 123456, 9, 0),
+                                              (5, 2, 99, 17, 311), (9, 3, 3, 0, 7)]:
         dev = scphd_amd.device_normals(seed, stream, step, parent, gauss, n)
-        ref = np.stack([oracle.normals(seed, stream, step, parent, gauss, e, 6) for e in range(n)])
+        if stream in (2, 3):  # the samples' normals, in groups sharing Philox blocks
+            ref = np.stack([np.pad(oracle.stream_normals(seed, stream, step, parent, gauss, s), (0, 3 if stream == 2 else 0))
+                            for s in range(n)])
+        else:
+            ref = np.stack([oracle.normals(seed, stream, step, parent, gauss, e, 6) for e in range(n)])
         np.testing.assert_array_equal(dev, ref)
 
 

@@ -499,3 +499,26 @@ def test_log_normaliser_equals_scipy_logsumexp():
         for x in l[1:]:
             acc = oracle.lse2(acc, x)
         assert oracle.lse2(acc, bpc) == pytest.approx(ref, rel=1e-14)
+
+
+def test_grouped_sample_normals_partition_the_group_blocks():
+    """Landmark prediction: samples s, s + 16, s + 32, s + 48 of a 64-sample
+    stretch take consecutive triples of their group's twelve normals (three
+    Philox blocks of element (s >> 6) << 4 | s & 15); posterior sampling:
+    samples s, s + 16 of a 32-sample stretch take the two halves.  No two
+    samples draw the same normals, and the pooled values are standard
+    normal."""
+    seed, step, p, k = 77, 5, 3, 11
+    for s0 in (0, 7, 15, 64 + 9, 128 + 15):
+        g = oracle.normals(seed, 2, step, p, k, ((s0 >> 6) << 4) | (s0 & 15), 12)
+        for m in range(4):
+            np.testing.assert_array_equal(oracle.stream_normals(seed, 2, step, p, k, s0 + 16 * m), g[3 * m:3 * m + 3])
+    for s0 in (0, 5, 15, 32 + 3, 96 + 15):
+        g = oracle.normals(seed, 3, step, p, k, ((s0 >> 5) << 4) | (s0 & 15), 12)
+        for m in range(2):
+            np.testing.assert_array_equal(oracle.stream_normals(seed, 3, step, p, k, s0 + 16 * m), g[6 * m:6 * m + 6])
+    for stream, n in ((2, 3), (3, 6)):
+        z = np.stack([oracle.stream_normals(seed, stream, step, p, k, s) for s in range(4096)])
+        assert z.shape == (4096, n)
+        assert len(np.unique(z, axis=0)) == len(z)  # no two samples draw the same normals
+        assert abs(z.mean()) < 0.03 and abs(z.std() - 1.0) < 0.03

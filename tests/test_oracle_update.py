@@ -202,7 +202,7 @@ def brute_update(P, st, zu, zv, fixed):
         wg_new, smp_new = [], []
         for k, (W, mu, C) in enumerate(merged):
             L = np.linalg.cholesky(C)
-            z = np.stack([oracle.normals(P["seed"], ST_POSTERIOR, st["n_update"], p, k, s_, 6) for s_ in range(S)])
+            z = np.stack([oracle.stream_normals(P["seed"], ST_POSTERIOR, st["n_update"], p, k, s_) for s_ in range(S)])
             q = z @ L.T + mu
             pw = to_world(P, R, t, q)
             wg_new.append(W)
