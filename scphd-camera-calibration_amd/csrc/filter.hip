@@ -20,6 +20,8 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "devmem.h"
 #include "kernels.h"
 #include "scphd.h"
@@ -250,6 +252,8 @@ struct scphd_filter {
     hipEvent_t ev[10] = {};  /* 0..5 phase boundaries, 6..7 fit kernel, 8..9 sample kernel */
     double phase_ms[8] = {};
     int lds_cap = 0;
+    /* the open profiler range of the update's current phase (0: none) */
+    roctx_range_id_t phase_range = 0;
     /* verification limits (scphd_debug_set_limits); -1 = the normal limit */
     int dbg_lds_cap = -1, dbg_heavy = -1, dbg_ncand = -1, dbg_rounds = 0, dbg_all_pairs = 0;
 };
@@ -363,8 +367,27 @@ int materialize(scphd_filter* f) {
     return f->esz == 8 ? materialize_t<double>(f) : materialize_t<float>(f);
 }
 
+/* A profiler range over a call (predict, resample): pushed at construction,
+ * popped however the call returns. */
+struct ScopedRange {
+    explicit ScopedRange(const char* m) { roctxRangePushA(m); }
+    ~ScopedRange() { roctxRangePop(); }
+};
+
+/* Phase boundary i of an update: a HIP event when timing is on, and a
+ * profiler (roctx) range per phase, as the reference marks its phases with
+ * NVTX ranges (src/scphdcameracalibration.cu:1062-2636).  The ranges span the
+ * host-side enqueueing of each phase; a failed update's open range is closed
+ * at the next one. */
 void record(scphd_filter* f, int i) {
     if (f->timing) (void)hipEventRecord(f->ev[i], f->stream);
+    static const char* const names[5] = {"scphd update: reproject + fit", "scphd update: normalisers + survivors",
+                                         "scphd update: merge + parent weights", "scphd update: posterior sampling",
+                                         "scphd update: map assembly"};
+    if (i <= 5) {
+        if (f->phase_range) roctxRangeStop(f->phase_range);
+        f->phase_range = i < 5 ? roctxRangeStartA(names[i]) : 0;
+    }
 }
 
 template <typename T>
@@ -835,6 +858,7 @@ const char* scphd_last_error(const scphd_t* f) { return f ? f->err.c_str() : "nu
 
 int scphd_predict(scphd_t* f, double dt) {
     if (check_handle(f)) return SCPHD_ERR_ARG;
+    const ScopedRange range("scphd predict");
     HIPTRY(hipSetDevice(f->dev));
     int rc = materialize(f);
     if (rc) return rc;
@@ -907,6 +931,7 @@ int scphd_update_ensemble(scphd_t* f, const double* u, const double* v, const in
 
 int scphd_resample(scphd_t* f, int32_t* resampled) {
     if (check_handle(f)) return SCPHD_ERR_ARG;
+    const ScopedRange range("scphd resample");
     HIPTRY(hipSetDevice(f->dev));
     int rc = materialize(f);
     if (rc) return rc;
