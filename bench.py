@@ -33,6 +33,7 @@ sys.path.insert(0, PKG)
 import numpy as np  # noqa: E402
 
 import scene  # noqa: E402
+import serve  # noqa: E402
 import workloads  # noqa: E402
 
 METRIC = "GM-PHD component updates/s at 4096 parents; p99 frame latency, served vs plain"
@@ -181,7 +182,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=12)
     ap.add_argument("--workload", default="c1")
-    ap.add_argument("--storage", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--storage", default="fp32", choices=["fp32", "fp64", "fp32-arith"],
+                    help="sample clouds as float (FP64 arithmetic), as double, or as float with the per-sample "
+                         "arithmetic in float")
     ap.add_argument("--cpu-parents", type=int, default=512)
     ap.add_argument("--cpu-warm", type=int, default=3, help="untimed oracle frames (maps reach their "
                                                              "steady size after 2)")
@@ -207,8 +210,8 @@ def main():
     import scphd_amd  # noqa: E402
 
     p, s = workloads.workload(args.workload)
-    p["storage"] = 0 if args.storage == "fp32" else 1
-    esz = 4 if args.storage == "fp32" else 8
+    p["storage"] = {"fp32": 0, "fp64": 1, "fp32-arith": 2}[args.storage]
+    esz = 8 if args.storage == "fp64" else 4
     frames = args.warmup + args.steps
     s = dict(s, n_frames=max(s["n_frames"], frames))
     sc = scene.generate(seed=p["seed"], **s)
@@ -220,6 +223,14 @@ def main():
     dv = DeviceArray(np.concatenate([v for _, v, _ in sets]))
 
     f = scphd_amd.Filter(p)
+    # buffers sized once, before the first frame, for 1.5 times the
+    # steady-state map size (a generation holds a step's input map and its
+    # posterior clouds): the steps then allocate nothing.  Growing the sample
+    # generations frame by frame frees and allocates blocks of gigabytes, and
+    # on these devices an allocation that follows large frees can wait
+    # 0.5-2.5 s while the driver reclaims the freed memory.
+    f.reserve(int(1.5 * p["n_particles"] * serve.expected_gaussians_per_parent(s)),
+              max(len(u) for u, _, _ in sets))
     f.set_timing(True)
     S = p["particles_per_feature"]
     lat, updates, phase = [], [], []
@@ -241,7 +252,8 @@ def main():
         st = f.update_stats()
         fb = fit_alg_bytes(st, S, esz)
         print(f"frame {k} {'timed' if k >= args.warmup else 'warmup'} {(t1 - t0) * 1e3:.2f} ms "
-              f"gaussians {f.counts()['n_gaussians']} k_fit_algorithmic_bytes {fb}", file=sys.stderr, flush=True)
+              f"gaussians {f.counts()['n_gaussians']} k_fit_algorithmic_bytes {fb} phases_ms "
+              f"{' '.join(f'{x:.2f}' for x in f.phase_ms())}", file=sys.stderr, flush=True)
         if k >= args.warmup:
             ph = f.phase_ms()
             lat.append((t1 - t0) * 1e3)
@@ -274,13 +286,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32" if args.storage == "fp32-arith" else "f64",
         "data": "synthetic",
         "config": {
             "workload": f"{args.workload}: plain one-rig filter step, {p['n_particles']} parents, "
                         f"{S} samples per Gaussian, {s['n_landmarks']} landmarks, clutter lambda {s['lam']:g}, "
-                        f"640x480 f={s['f']:g} baseline {s['baseline']:g} m, {args.storage} sample storage, "
-                        f"FP64 arithmetic",
+                        f"640x480 f={s['f']:g} baseline {s['baseline']:g} m, "
+                        + ("FP32 sample storage, FP32 per-sample arithmetic (FP64 per-Gaussian algebra, "
+                           "likelihoods, log-sum-exp and resampling)" if args.storage == "fp32-arith" else
+                           f"{args.storage} sample storage, FP64 arithmetic"),
             "n_particles": p["n_particles"],
             "samples_per_gaussian": S,
             "landmarks": s["n_landmarks"],

@@ -43,6 +43,9 @@ extern "C" {
 
 #define SCPHD_STORAGE_FP32 0     /* sample clouds stored as float  */
 #define SCPHD_STORAGE_FP64 1     /* sample clouds stored as double */
+#define SCPHD_STORAGE_FP32_ARITH 2 /* stored as float, and the per-sample work
+                                      (prediction, reprojection, moment sums,
+                                      posterior draws) computed in float     */
 
 /* Filter parameters: the keys the reference reads from its libconfig file
  * (src/scphdcameracalibration.cu:827-1005, :1145, :1246-1247, :1343, :1744,

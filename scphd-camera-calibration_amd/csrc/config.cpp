@@ -17,7 +17,7 @@
  * deviation, a = mean - sqrt(3) std, b = mean + sqrt(3) std.  `slam`
  * (src/main.cpp:77) defaults to false when absent.  Keys of this
  * implementation with no reference counterpart: seed, max_gaussians,
- * storage ("fp32" | "fp64").
+ * storage ("fp32" | "fp64" | "fp32-arith", or 0 | 1 | 2).
  */
 #include <cctype>
 #include <cmath>
@@ -440,9 +440,11 @@ int scphd_params_load(const char* cfg_path, scphd_params* p, char* errbuf, size_
     if (L.has("storage")) {
         const Value& v = ps.out.at("storage");
         if (v.kind == Value::STR)
-            q.storage = v.str == "fp64" ? SCPHD_STORAGE_FP64 : SCPHD_STORAGE_FP32;
+            q.storage = v.str == "fp64"         ? SCPHD_STORAGE_FP64
+                        : v.str == "fp32-arith" ? SCPHD_STORAGE_FP32_ARITH
+                                                : SCPHD_STORAGE_FP32;
         else
-            q.storage = v.num != 0 ? SCPHD_STORAGE_FP64 : SCPHD_STORAGE_FP32;
+            q.storage = v.num == 2 ? SCPHD_STORAGE_FP32_ARITH : v.num != 0 ? SCPHD_STORAGE_FP64 : SCPHD_STORAGE_FP32;
     }
     if (!ok) {
         copy_err(L.err, errbuf, errlen);

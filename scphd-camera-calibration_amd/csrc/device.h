@@ -215,8 +215,8 @@ __device__ inline void normals(Key key, uint32_t stream, uint64_t step, uint32_t
 SCPHD_HD constexpr uint32_t feature_elem(uint32_t s) { return ((s >> 6) << 4) | (s & 15u); }
 
 /* any one sample (one or two blocks) */
-template <typename Key>
-__device__ inline void feature_normals(Key key, uint64_t step, uint32_t p, uint32_t k, uint32_t s, double a[3]) {
+template <typename Key, typename A>
+__device__ inline void feature_normals(Key key, uint64_t step, uint32_t p, uint32_t k, uint32_t s, A a[3]) {
     const uint32_t e = feature_elem(s), i = (s >> 4) & 3u;
     uint32_t x[4];
     float n0, n1, n2, n3;
@@ -247,9 +247,9 @@ struct FeatureCarry {
     float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
 };
 
-template <typename Key>
+template <typename Key, typename A>
 __device__ inline void feature_normals_seq(Key key, uint64_t step, uint32_t p, uint32_t k, uint32_t s, uint32_t i,
-                                           FeatureCarry& c, double a[3]) {
+                                           FeatureCarry& c, A a[3]) {
     float n0 = 0.0f, n1 = 0.0f, n2 = 0.0f, n3 = 0.0f;
     if (i != 3u) {
         uint32_t x[4];
@@ -304,9 +304,9 @@ struct PosteriorCarry {
     float c0 = 0.0f, c1 = 0.0f;
 };
 
-template <typename Key>
+template <typename Key, typename A>
 __device__ inline void posterior_normals_seq(Key key, uint64_t step, uint32_t p, uint32_t k, uint32_t s, uint32_t i,
-                                             PosteriorCarry& c, double z[6]) {
+                                             PosteriorCarry& c, A z[6]) {
     const uint32_t e = posterior_elem(s);
     float n[6];
     uint32_t x[4];
@@ -325,7 +325,7 @@ __device__ inline void posterior_normals_seq(Key key, uint64_t step, uint32_t p,
         box_muller_f(x[2], x[3], n[4], n[5]);
     }
 #pragma unroll
-    for (int j = 0; j < 6; ++j) z[j] = (double)n[j];
+    for (int j = 0; j < 6; ++j) z[j] = (A)n[j];
 }
 
 SCPHD_HD inline double safe_log(double x) { return x <= 0 ? SCPHD_LOG0 : log(x); }
@@ -522,6 +522,98 @@ SCPHD_HD inline void detection_terms(const double* P, double var_u, double var_v
             r.Pn[pidx(i, j)] = s;
         }
     }
+}
+
+/* ------------------------------------------ single-precision sample work */
+/* The FP32-arithmetic variant evaluates the per-sample work -- landmark
+ * prediction, world -> disparity, the field-of-view test, posterior draws
+ * and disparity -> world -- in single precision.  Every multiply-add below
+ * is an explicit fused one and contraction is off, so each kernel that
+ * evaluates a sample gets the same bits.  Reciprocals are the hardware's
+ * single-precision estimate (v_rcp_f32, 1 ulp). */
+struct PoseF {
+    float R[9], t[3];
+};
+struct MeasF {
+    float fx, fy, u0, v0, w, h, rxy; /* rxy = fx / fy */
+};
+/* constant-velocity prediction: dt, dt^2 / 2 and the acceleration stds */
+struct MotionF {
+    float dt, h, sx, sy, sz;
+};
+
+__device__ inline PoseF pose_f(const CameraPose& c) {
+    PoseF f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) f.R[i] = (float)c.R[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) f.t[i] = (float)c.t[i];
+    return f;
+}
+__device__ inline MeasF meas_f(const DisparityMeasurementModel& m) {
+    return MeasF{(float)m.fx(), (float)m.fy(), (float)m.u0(), (float)m.v0(), (float)m.img_width(),
+                 (float)m.img_height(), (float)(m.fx() / m.fy())};
+}
+__device__ inline MotionF motion_f(const LinearCVMotionModel3D& m, double dt) {
+    return MotionF{(float)dt, (float)(0.5 * dt * dt), (float)m.std_ax(), (float)m.std_ay(), (float)m.std_az()};
+}
+
+/* computeNoisyMotion with acceleration draws n * std:
+ * p + v dt + a dt^2 / 2, v + a dt */
+__device__ inline void predict_f(float x[6], const MotionF& m, const float n[3]) {
+#pragma clang fp contract(off)
+    const float ax = n[0] * m.sx, ay = n[1] * m.sy, az = n[2] * m.sz;
+    x[0] = __builtin_fmaf(ax, m.h, __builtin_fmaf(x[3], m.dt, x[0]));
+    x[1] = __builtin_fmaf(ay, m.h, __builtin_fmaf(x[4], m.dt, x[1]));
+    x[2] = __builtin_fmaf(az, m.h, __builtin_fmaf(x[5], m.dt, x[2]));
+    x[3] = __builtin_fmaf(ax, m.dt, x[3]);
+    x[4] = __builtin_fmaf(ay, m.dt, x[4]);
+    x[5] = __builtin_fmaf(az, m.dt, x[5]);
+}
+
+/* DisparityMeasurementModel::computeMeasurement of world point x seen by
+ * camera c: p = R^T (x - t), u = u0 - fx p_x / p_z, v = v0 - fy p_y / p_z,
+ * d = -fx / p_z and the velocities' images */
+__device__ inline void disparity_f(const MeasF& m, const PoseF& c, const float x[6], float q[6]) {
+#pragma clang fp contract(off)
+    const float dx = x[0] - c.t[0], dy = x[1] - c.t[1], dz = x[2] - c.t[2];
+    const float px = __builtin_fmaf(c.R[0], dx, __builtin_fmaf(c.R[3], dy, c.R[6] * dz));
+    const float py = __builtin_fmaf(c.R[1], dx, __builtin_fmaf(c.R[4], dy, c.R[7] * dz));
+    const float pz = __builtin_fmaf(c.R[2], dx, __builtin_fmaf(c.R[5], dy, c.R[8] * dz));
+    const float vx = __builtin_fmaf(c.R[0], x[3], __builtin_fmaf(c.R[3], x[4], c.R[6] * x[5]));
+    const float vy = __builtin_fmaf(c.R[1], x[3], __builtin_fmaf(c.R[4], x[4], c.R[7] * x[5]));
+    const float vz = __builtin_fmaf(c.R[2], x[3], __builtin_fmaf(c.R[5], x[4], c.R[8] * x[5]));
+    const float iz = __builtin_amdgcn_rcpf(pz);
+    const float fz = m.fx * iz, gz = m.fy * iz, iz2 = iz * iz;
+    q[0] = __builtin_fmaf(-fz, px, m.u0);
+    q[1] = __builtin_fmaf(-gz, py, m.v0);
+    q[2] = -fz;
+    q[3] = -m.fx * __builtin_fmaf(vx, pz, -(vz * px)) * iz2;
+    q[4] = -m.fy * __builtin_fmaf(vy, pz, -(vz * py)) * iz2;
+    q[5] = m.fx * vz * iz2;
+}
+
+/* DisparityMeasurementModel::inRange */
+__device__ inline bool in_range_f(const MeasF& m, const float q[6]) {
+    return q[0] >= 0.0f && q[0] <= m.w && q[1] >= 0.0f && q[1] <= m.h && q[2] >= 0.0f;
+}
+
+/* DisparityMeasurementModel::invertMeasurement into the world frame of
+ * camera c */
+__device__ inline void world_from_disparity_f(const MeasF& m, const PoseF& c, const float q[6], float x[6]) {
+#pragma clang fp contract(off)
+    const float id = __builtin_amdgcn_rcpf(q[2]), id2 = id * id;
+    const float du = q[0] - m.u0, dv = q[1] - m.v0;
+    const float ex = du * id, ey = m.rxy * dv * id, ez = -m.fx * id;
+    const float evx = __builtin_fmaf(q[3], q[2], -(q[5] * du)) * id2;
+    const float evy = m.rxy * __builtin_fmaf(q[4], q[2], -(q[5] * dv)) * id2;
+    const float evz = m.fx * q[5] * id2;
+    x[0] = __builtin_fmaf(c.R[0], ex, __builtin_fmaf(c.R[1], ey, __builtin_fmaf(c.R[2], ez, c.t[0])));
+    x[1] = __builtin_fmaf(c.R[3], ex, __builtin_fmaf(c.R[4], ey, __builtin_fmaf(c.R[5], ez, c.t[1])));
+    x[2] = __builtin_fmaf(c.R[6], ex, __builtin_fmaf(c.R[7], ey, __builtin_fmaf(c.R[8], ez, c.t[2])));
+    x[3] = __builtin_fmaf(c.R[0], evx, __builtin_fmaf(c.R[1], evy, c.R[2] * evz));
+    x[4] = __builtin_fmaf(c.R[3], evx, __builtin_fmaf(c.R[4], evy, c.R[5] * evz));
+    x[5] = __builtin_fmaf(c.R[6], evx, __builtin_fmaf(c.R[7], evy, c.R[8] * evz));
 }
 
 }  // namespace scphd

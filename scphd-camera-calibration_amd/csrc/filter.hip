@@ -199,6 +199,7 @@ struct scphd_filter {
     std::vector<double> hbpcs;
     int64_t ens_meas = 0;
     size_t esz = 4;
+    bool arith32 = false; /* per-sample arithmetic in single precision (SCPHD_STORAGE_FP32_ARITH) */
 
     /* parents */
     DBuf<double> ext, pw, pose, loglik, cum, ext_tmp;
@@ -356,7 +357,8 @@ int materialize_t(scphd_filter* f) {
     const int nxt = 1 - f->cur;
     HIPTRY(f->gen[nxt].ensure(f->Ng, 0, f->S, f->esz, f->stream));
     HIPTRY(launch_materialize<T>(f->stream, f->gen[f->cur].planes<T>(), f->gen[nxt].planes<T>(), f->gblock.p,
-                                 f->gparent.p, f->goff.p, f->Ng, f->S, f->models.feat, f->pdt, rig_keys(f), f->pstep));
+                                 f->gparent.p, f->goff.p, f->Ng, f->S, f->models.feat, f->pdt, rig_keys(f), f->pstep,
+                                 f->arith32));
     HIPTRY(launch_iota(f->stream, f->gblock.p, f->Ng));
     f->cur = nxt;
     f->blocks_used = f->Ng;
@@ -403,10 +405,20 @@ int update_t(scphd_filter* f, int M, int fixed) {
     HIPTRY(f->keepscan.ensure(Ng + 1));
     HIPTRY(f->scantmp.ensure(scan_tmp_size(std::max<int64_t>(Ng, Np) + 1) + 16));
     if (Ng > 0) {
-        /* room for this step's posterior clouds too (the last step's count
-         * plus a quarter), so that sizing them after the merge rarely has to
-         * grow the generation and copy the non-detection clouds already in it */
-        HIPTRY(f->gen[nxt].ensure(Ng + f->kt_hint + f->kt_hint / 4, 0, S, f->esz, s));
+        /* room for this step's posterior clouds too (the last step's count,
+         * scaled by this step's measurement count, plus a quarter), so that
+         * sizing them after the merge rarely has to grow the generation and
+         * copy the non-detection clouds already in it.  The estimate is only
+         * a head start: a generation that holds Ng plus the estimate is not
+         * grown for the quarter, and where the estimate does not fit, the
+         * generation is sized for the non-detection clouds alone and grows
+         * after the merge. */
+        const int64_t hint = f->last_M > 0 ? f->kt_hint * (int64_t)M / f->last_M : 0;
+        if (f->gen[nxt].cap_blocks < Ng + hint &&
+            f->gen[nxt].ensure(Ng + hint + hint / 4, 0, S, f->esz, s) != hipSuccess) {
+            (void)hipGetLastError();
+            HIPTRY(f->gen[nxt].ensure(Ng, 0, S, f->esz, s));
+        }
         HIPTRY(f->fit.ensure((int64_t)FP_COUNT * Ng));
         HIPTRY(f->pre.ensure((int64_t)PP_COUNT * Ng));
         HIPTRY(f->lik.ensure((int64_t)LK_COUNT * (Ng + 3))); /* 3 records of padding: read four at a time */
@@ -445,6 +457,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
          * Ng of the next) lies below 4 GiB */
         fa.off32 = std::max(f->blocks_used, Ng) * (int64_t)S * (int64_t)f->esz <
                    (int64_t)0xFFFFFFFFll - 64 * (int64_t)f->esz;
+        fa.arith32 = f->arith32 ? 1 : 0;
         record(f, 6);
         HIPTRY(launch_fit<T>(s, fa, f->gen[f->cur].planes<T>(), f->gen[nxt].planes<T>()));
         record(f, 7);
@@ -621,6 +634,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
      * Kt S esz is below 4 GiB) */
     pa.post_base = 0;
     pa.off32 = Kt * (int64_t)S * (int64_t)f->esz < (int64_t)0xFFFFFFFFll - 64 * (int64_t)f->esz;
+    pa.arith32 = f->arith32 ? 1 : 0;
     Planes<T> post = f->gen[nxt].planes<T>();
     for (int c = 0; c < 7; ++c) post.p[c] += Ng * (int64_t)S;
     record(f, 8);
@@ -734,7 +748,9 @@ int create_filter(const scphd_params* p, int32_t n_rigs, bool ensemble, scphd_t*
         const scphd_params& q = p[r];
         if (q.n_particles <= 0 || q.particles_per_feature <= 0 || q.particles_per_feature > 512)
             return SCPHD_ERR_ARG;
-        if (q.storage != SCPHD_STORAGE_FP32 && q.storage != SCPHD_STORAGE_FP64) return SCPHD_ERR_ARG;
+        if (q.storage != SCPHD_STORAGE_FP32 && q.storage != SCPHD_STORAGE_FP64 &&
+            q.storage != SCPHD_STORAGE_FP32_ARITH)
+            return SCPHD_ERR_ARG;
         if (!same_shared(q, p[0])) return SCPHD_ERR_ARG;
         Np_total += q.n_particles;
     }
@@ -760,6 +776,7 @@ int create_filter(const scphd_params* p, int32_t n_rigs, bool ensemble, scphd_t*
     f->Np = (int)Np_total;
     f->S = p->particles_per_feature;
     f->esz = p->storage == SCPHD_STORAGE_FP64 ? 8 : 4;
+    f->arith32 = p->storage == SCPHD_STORAGE_FP32_ARITH;
     set_models(f);
     auto bail = [&](int code) {
         scphd_destroy(f);

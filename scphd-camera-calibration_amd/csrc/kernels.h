@@ -92,7 +92,8 @@ hipError_t launch_pose(hipStream_t s, const double* ext, double* pose, int Np, i
 template <typename T>
 hipError_t launch_materialize(hipStream_t s, Planes<T> src, Planes<T> dst, const int32_t* gblock,
                               const int32_t* gparent, const int32_t* goff, int64_t Ng, int S,
-                              LinearCVMotionModel3D feat, double dt, RigKeys key, uint64_t step);
+                              LinearCVMotionModel3D feat, double dt, RigKeys key, uint64_t step,
+                              bool arith32 = false);
 
 /* ---- update ---- */
 struct FitArgs {
@@ -120,6 +121,7 @@ struct FitArgs {
     int64_t nd_base; /* destination block of the non-detection cloud of entry g: nd_base + g */
     const float* gpd = nullptr; /* per entry: pd_g of its source term at the last update (1: posterior term) */
     int off32;       /* every source and destination plane is below 4 GiB: 32-bit byte offsets */
+    int arith32 = 0; /* per-sample arithmetic in single precision (FP32 storage only) */
 };
 
 /* k_fit: the streaming sweep; k_fit_fix: the rare cases it lists in redo[];
@@ -254,6 +256,7 @@ struct SampleArgs {
     uint64_t ustep;
     int64_t post_base;
     int off32; /* every destination plane is below 4 GiB: 32-bit byte offsets */
+    int arith32 = 0; /* draws and disparity -> world in single precision (FP32 storage only) */
 };
 template <typename T>
 hipError_t launch_sample(hipStream_t s, const SampleArgs& a, Planes<T> dst);

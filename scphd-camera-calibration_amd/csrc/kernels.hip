@@ -30,6 +30,11 @@ __device__ inline double dpp_f64(double x) {
     return __hiloint2double(hi, lo);
 }
 
+template <int CTRL>
+__device__ inline float dpp_f32(float x) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+
 __device__ inline double readlane_f64(double x, int l) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
     const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
@@ -217,7 +222,7 @@ __device__ inline EuclideanPoint predict_sample_seq(EuclideanPoint x, const Line
 template <typename T>
 __global__ void k_materialize(Planes<T> src, Planes<T> dst, const int32_t* gblock, const int32_t* gparent,
                               const int32_t* goff, int64_t Ng, int S, LinearCVMotionModel3D feat, double dt,
-                              RigKeys rk, uint64_t step) {
+                              RigKeys rk, uint64_t step, int arith32) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Ng * S) return;
     const int64_t g = i / S;
@@ -225,6 +230,24 @@ __global__ void k_materialize(Planes<T> src, Planes<T> dst, const int32_t* gbloc
     const int p = gparent[g];
     const int k = (int)(g - goff[p]);
     const int64_t si = (int64_t)gblock[g] * S + s;
+    const int64_t di = g * S + s;
+    if (arith32) { /* the single-precision prediction of the FP32-arithmetic variant */
+        float x[6], n[3];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) x[c] = (float)src.p[c][si];
+        if (!rk.keys) {
+            feature_normals(rk.key, step, (uint32_t)p, (uint32_t)k, (uint32_t)s, n);
+        } else {
+            int pl;
+            const LaneKey key = rk.of(p, pl);
+            feature_normals(key, step, (uint32_t)pl, (uint32_t)k, (uint32_t)s, n);
+        }
+        predict_f(x, motion_f(feat, dt), n);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) dst.p[c][di] = (T)x[c];
+        dst.p[6][di] = src.p[6][si];
+        return;
+    }
     EuclideanPoint x{(double)src.p[0][si], (double)src.p[1][si], (double)src.p[2][si],
                      (double)src.p[3][si], (double)src.p[4][si], (double)src.p[5][si]};
     if (!rk.keys) { /* one rig: the key stays in scalar registers */
@@ -234,7 +257,6 @@ __global__ void k_materialize(Planes<T> src, Planes<T> dst, const int32_t* gbloc
         const LaneKey key = rk.of(p, pl);
         x = predict_sample(x, feat, dt, key, step, (uint32_t)pl, (uint32_t)k, (uint32_t)s);
     }
-    const int64_t di = g * S + s;
     dst.p[0][di] = (T)x.x; dst.p[1][di] = (T)x.y; dst.p[2][di] = (T)x.z;
     dst.p[3][di] = (T)x.vx; dst.p[4][di] = (T)x.vy; dst.p[5][di] = (T)x.vz;
     dst.p[6][di] = src.p[6][si];
@@ -249,6 +271,13 @@ __device__ inline double row_sum16(double x) {
     x += dpp_f64<0x4E>(x);
     x += dpp_f64<0x141>(x);
     x += dpp_f64<0x140>(x);
+    return x;
+}
+__device__ inline float row_sum16(float x) {
+    x += dpp_f32<0xB1>(x);
+    x += dpp_f32<0x4E>(x);
+    x += dpp_f32<0x141>(x);
+    x += dpp_f32<0x140>(x);
     return x;
 }
 
@@ -310,6 +339,42 @@ __device__ inline void fit_sample(const FitArgs& a, const DisparityMeasurementMo
     T r[7];
     load_sample(src, si, r);
     eval_sample(a, meas, r, kk, pl, k, s, pose, x, d, in, w);
+}
+
+/* The FP32-arithmetic form of eval_sample: the sample's world point x (after
+ * the deferred prediction), its disparity-space point q and field-of-view
+ * flag, and its stored weight, all in single precision. */
+struct SampleCtxF {
+    MeasF m;
+    PoseF pose;
+    MotionF mo;
+};
+
+template <typename T, typename Key>
+__device__ inline void eval_sample_f(const FitArgs& a, const SampleCtxF& cx, const T r[7], Key kk, int pl, int k,
+                                     int s, float x[6], float q[6], bool& in, float& w, FeatureCarry* fc = nullptr,
+                                     uint32_t phase = 0) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) x[c] = (float)r[c];
+    w = (float)r[6];
+    if (a.pending) {
+        float n[3];
+        if (fc)
+            feature_normals_seq(kk, a.pstep, (uint32_t)pl, (uint32_t)k, (uint32_t)s, phase, *fc, n);
+        else
+            feature_normals(kk, a.pstep, (uint32_t)pl, (uint32_t)k, (uint32_t)s, n);
+        predict_f(x, cx.mo, n);
+    }
+    disparity_f(cx.m, cx.pose, x, q);
+    in = in_range_f(cx.m, q);
+}
+
+template <typename T, typename Off>
+__device__ inline void write_nd_off_f(const DisparityMeasurementModel& meas, Planes<T>& dst, Off off, const float x[6],
+                                      bool in) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) at(dst.p[c], off) = (T)x[c];
+    at(dst.p[6], off) = (T)(1.0 - (in ? meas.pd() : 0.0));
 }
 
 __device__ inline void write_fit(const FitArgs& a, int64_t g, int lane, const double* mu, const double* P, double pdg) {
@@ -390,9 +455,14 @@ struct RowMeas<true> {
 #define SCPHD_FIT_WAVES 2
 #endif
 
-template <typename T, typename Off, bool RG>
+/* A, the arithmetic type of the per-sample work: double, or float for the
+ * FP32-arithmetic variant (FP32 storage), whose moment sums are single
+ * precision up to the row sums and whose in-view count is an integer. */
+template <typename T, typename Off, bool RG, typename A = double>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_WAVES, SCPHD_FIT_WAVES))) void k_fit(
     FitArgs a, Planes<T> src, Planes<T> dst) {
+    constexpr bool F32 = std::is_same_v<A, float>;
+    static_assert(!F32 || std::is_same_v<T, float>, "single-precision arithmetic needs FP32 storage");
     /* the rows' camera poses are staged through LDS once and then held in
      * registers for the whole sweep */
     __shared__ double s_pose[16 * 12];
@@ -424,8 +494,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
      * costs only the write of a cloud nothing references if it proves wrong */
     const bool nd_write = nd_sure || (a.gpd && wg * (1.0 - (double)a.gpd[g]) >= a.min_weight);
     const int64_t d0 = (a.nd_base + g) * (int64_t)S;
+    SampleCtxF cx;
+    if constexpr (F32) cx = SampleCtxF{meas_f(meas), pose_f(pose), motion_f(a.feat, a.dt)};
+    const A PdA = (A)Pd;
 
-    double spd = 0.0, sw = 0.0, sw2 = 0.0, m1[6], m2[21], K[6];
+    [[maybe_unused]] double spd = 0.0; /* sum of pd_i (double arithmetic) */
+    [[maybe_unused]] float nin = 0.0f; /* in-view samples (single-precision arithmetic) */
+    A sw = 0, sw2 = 0, m1[6], m2[21], K[6];
     bool kw = false; /* K is a weighted sample (uniform along the row) */
     FeatureCarry fc; /* prediction normals carried between the steps of a noise group */
 #pragma unroll
@@ -454,14 +529,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
         const bool valid = FULL || s < S;
         sof += (Off)(16 * sizeof(T));
         load_sample_off(src, s + 32 < S ? sof : last, fill);
-        EuclideanPoint x;
-        DisparityPoint d;
+        A q[6] = {0, 0, 0, 0, 0, 0};
         bool in = false;
-        double w = 0.0;
-        if (valid) eval_sample(a, meas, cur, kk, pl, k, s, pose, x, d, in, w, &fc, (uint32_t)(s0 >> 4) & 3u);
-        const double q[6] = {d.u, d.v, d.d, d.vu, d.vv, d.vd};
-        const double pd = in ? Pd : 0.0;
-        const double wp = valid ? w * pd : 0.0;
+        A w = 0;
+        [[maybe_unused]] EuclideanPoint x;
+        [[maybe_unused]] float xf[6];
+        if constexpr (F32) {
+            if (valid) eval_sample_f(a, cx, cur, kk, pl, k, s, xf, q, in, w, &fc, (uint32_t)(s0 >> 4) & 3u);
+        } else {
+            DisparityPoint d;
+            if (valid) {
+                eval_sample(a, meas, cur, kk, pl, k, s, pose, x, d, in, w, &fc, (uint32_t)(s0 >> 4) & 3u);
+                q[0] = d.u; q[1] = d.v; q[2] = d.d; q[3] = d.vu; q[4] = d.vv; q[5] = d.vd;
+            }
+        }
+        const A pd = in ? PdA : (A)0;
+        const A wp = valid ? w * pd : (A)0;
         if (s0 == 0) {
             /* shift: the Gaussian's first sample, broadcast along the row
              * (the fit of a cloud with no weighted sample) */
@@ -474,7 +557,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
              * then, and a shift inside the weighted cloud keeps the
              * one-pass moments exact to rounding (a single weighted
              * sample gives zero covariance exactly) */
-            const uint32_t rb = (uint32_t)(__ballot(wp > 0.0) >> (threadIdx.x & 48)) & 0xFFFFu;
+            const uint32_t rb = (uint32_t)(__ballot(wp > (A)0) >> (threadIdx.x & 48)) & 0xFFFFu;
             if (!kw && rb != 0) {
                 const int l = __builtin_ctz(rb);
 #pragma unroll
@@ -482,19 +565,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
                 kw = true;
             }
         }
-        spd += pd;
+        if constexpr (F32)
+            nin += in ? 1.0f : 0.0f;
+        else
+            spd += pd;
         sw += wp;
         sw2 += wp * wp;
-        double dv[6];
+        A dv[6];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) dv[c] = valid ? q[c] - K[c] : 0.0;
+        for (int c = 0; c < 6; ++c) dv[c] = valid ? q[c] - K[c] : (A)0;
 #pragma unroll
         for (int c = 0; c < 6; ++c) m1[c] += wp * dv[c];
 #pragma unroll
         for (int r = 0; r < 6; ++r)
 #pragma unroll
             for (int c = 0; c <= r; ++c) m2[pidx(r, c)] += wp * dv[r] * dv[c];
-        if (nd_write && valid) write_nd_off(meas, dst, dof, x, in);
+        if (nd_write && valid) {
+            if constexpr (F32)
+                write_nd_off_f(meas, dst, dof, xf, in);
+            else
+                write_nd_off(meas, dst, dof, x, in);
+        }
         dof += (Off)(16 * sizeof(T));
     };
     const std::integral_constant<bool, true> full;
@@ -513,24 +604,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
             bB[c] = bC[c];
         }
     }
-    spd = row_sum16(spd);
-    sw = row_sum16(sw);
-    sw2 = row_sum16(sw2);
+    /* the row sums, then the rest in double */
+    if constexpr (F32)
+        spd = (double)row_sum16(nin) * Pd;
+    else
+        spd = row_sum16(spd);
+    const double sw_r = (double)row_sum16(sw);
+    const double sw2_r = (double)row_sum16(sw2);
+    double m1r[6], m2r[21];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) m1[c] = row_sum16(m1[c]);
+    for (int c = 0; c < 6; ++c) m1r[c] = (double)row_sum16(m1[c]);
 #pragma unroll
-    for (int e = 0; e < 21; ++e) m2[e] = row_sum16(m2[e]);
+    for (int e = 0; e < 21; ++e) m2r[e] = (double)row_sum16(m2[e]);
     const double pdg = spd / S;
 
     /* normalised weights w'_i / sum w': mean K + sum w (q - K); centred
      * covariance sum w (q - K)(q - K)^T - (2 - W) dm dm^T with W = sum w */
-    const double r = sw > 0 ? 1.0 / sw : 0.0;
-    const double W = sw * r, W2 = sw2 * r * r;
+    const double r = sw_r > 0 ? 1.0 / sw_r : 0.0;
+    const double W = sw_r * r, W2 = sw2_r * r * r;
     double mu[6], dm[6], P[21];
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
-        dm[c] = m1[c] * r;
-        mu[c] = K[c] + dm[c];
+        dm[c] = m1r[c] * r;
+        mu[c] = (double)K[c] + dm[c];
     }
     /* W = 0: with pd_g = 0 the fit is the first sample with zero covariance
      * (its detection terms have weight 0); with pd_g > 0 (possible only when
@@ -539,13 +635,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
 #pragma unroll
     for (int i = 0; i < 6; ++i)
 #pragma unroll
-        for (int j = 0; j <= i; ++j) P[pidx(i, j)] = (m2[pidx(i, j)] * r - (2.0 - W) * dm[i] * dm[j]) * scale;
+        for (int j = 0; j <= i; ++j) P[pidx(i, j)] = (m2r[pidx(i, j)] * r - (2.0 - W) * dm[i] * dm[j]) * scale;
     write_fit(a, g, lane, mu, P, pdg);
 
     const bool keep = wg * (1 - pdg) >= a.min_weight;
     if (lane == 0) {
         a.keep[g] = keep ? 1 : 0;
-        if (sw == 0 && spd != 0.0) a.redo[atomicAdd(a.redo_n, 1)] = (int32_t)g;
+        if (sw_r == 0 && spd != 0.0) a.redo[atomicAdd(a.redo_n, 1)] = (int32_t)g;
         if (keep && !nd_write) a.ndfill[atomicAdd(a.ndfill_n, 1)] = (int32_t)g;
     }
 }
@@ -555,8 +651,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SCPHD_FIT_W
  * weights 1 - pd_i, computed by the same code as in the sweep.  One 16-lane
  * row per listed term; workgroups stride over the list (its length is read
  * on the device). */
-template <typename T, typename Off, bool RG>
+template <typename T, typename Off, bool RG, typename A = double>
 __global__ __launch_bounds__(256) void k_nd_fill(FitArgs a, Planes<T> src, Planes<T> dst) {
+    constexpr bool F32 = std::is_same_v<A, float>;
     const int lane = threadIdx.x & 15;
     const int n = *a.ndfill_n;
     const int stride = gridDim.x * 16;
@@ -571,6 +668,8 @@ __global__ __launch_bounds__(256) void k_nd_fill(FitArgs a, Planes<T> src, Plane
         const int64_t b0 = (int64_t)a.gblock[g] * a.S;
         const int64_t d0 = (a.nd_base + g) * (int64_t)a.S;
         const CameraPose pose = load_pose(a.pose, p);
+        SampleCtxF cx;
+        if constexpr (F32) cx = SampleCtxF{meas_f(meas), pose_f(pose), motion_f(a.feat, a.dt)};
         /* four steps' loads issued before their arithmetic; the four steps
          * are one noise group (phase j) */
         FeatureCarry fc;
@@ -585,12 +684,18 @@ __global__ __launch_bounds__(256) void k_nd_fill(FitArgs a, Planes<T> src, Plane
             for (int j = 0; j < 4; ++j) {
                 const int s = s0 + 16 * j + lane;
                 if (s < a.S) {
-                    EuclideanPoint x;
-                    DisparityPoint d;
                     bool in;
-                    double w;
-                    eval_sample(a, meas, r[j], kk, pl, k, s, pose, x, d, in, w, &fc, (uint32_t)j);
-                    write_nd_off(meas, dst, (Off)((d0 + s) * (int64_t)sizeof(T)), x, in);
+                    if constexpr (F32) {
+                        float x[6], q[6], w;
+                        eval_sample_f(a, cx, r[j], kk, pl, k, s, x, q, in, w, &fc, (uint32_t)j);
+                        write_nd_off_f(meas, dst, (Off)((d0 + s) * (int64_t)sizeof(T)), x, in);
+                    } else {
+                        EuclideanPoint x;
+                        DisparityPoint d;
+                        double w;
+                        eval_sample(a, meas, r[j], kk, pl, k, s, pose, x, d, in, w, &fc, (uint32_t)j);
+                        write_nd_off(meas, dst, (Off)((d0 + s) * (int64_t)sizeof(T)), x, in);
+                    }
                 }
             }
         }
@@ -602,8 +707,9 @@ __global__ __launch_bounds__(256) void k_nd_fill(FitArgs a, Planes<T> src, Plane
  * of the samples, as the reference's normalisation of an all-zero cloud
  * gives).  Workgroups stride over the list (its length is read on the
  * device). */
-template <typename T, bool RG>
+template <typename T, bool RG, typename A = double>
 __global__ __launch_bounds__(256) void k_fit_fix(FitArgs a, Planes<T> src, Planes<T> dst) {
+    constexpr bool F32 = std::is_same_v<A, float>;
     const int lane = threadIdx.x & 15;
     const int nredo = *a.redo_n;
     const int S = a.S;
@@ -617,32 +723,51 @@ __global__ __launch_bounds__(256) void k_fit_fix(FitArgs a, Planes<T> src, Plane
         const DisparityMeasurementModel& meas = rm.m;
         const int64_t b0 = (int64_t)a.gblock[g] * S;
         const CameraPose pose = load_pose(a.pose, p);
-        double sq[6] = {0, 0, 0, 0, 0, 0};
-        for (int s0 = 0; s0 < S; s0 += 16) {
-            const int s = s0 + lane;
-            if (s < S) {
+        SampleCtxF cx;
+        if constexpr (F32) cx = SampleCtxF{meas_f(meas), pose_f(pose), motion_f(a.feat, a.dt)};
+        /* the sample's disparity-space point in the arithmetic type */
+        auto point = [&](int s, A q[6]) {
+            if constexpr (F32) {
+                T r[7];
+                load_sample(src, b0 + s, r);
+                float x[6], w;
+                bool in;
+                eval_sample_f(a, cx, r, kk, pl, k, s, x, q, in, w);
+            } else {
                 EuclideanPoint x;
                 DisparityPoint d;
                 bool in;
                 double w;
                 fit_sample(a, meas, src, b0 + s, kk, pl, k, s, pose, x, d, in, w);
-                sq[0] += d.u; sq[1] += d.v; sq[2] += d.d; sq[3] += d.vu; sq[4] += d.vv; sq[5] += d.vd;
+                q[0] = d.u; q[1] = d.v; q[2] = d.d; q[3] = d.vu; q[4] = d.vv; q[5] = d.vd;
+            }
+        };
+        A sq[6] = {0, 0, 0, 0, 0, 0};
+        for (int s0 = 0; s0 < S; s0 += 16) {
+            const int s = s0 + lane;
+            if (s < S) {
+                A q[6];
+                point(s, q);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) sq[c] += q[c];
             }
         }
-        double mu[6], cc[21], P[21];
+        A muA[6], cc[21];
+        double mu[6], P[21];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) mu[c] = row_sum16(sq[c]);
+        for (int c = 0; c < 6; ++c) {
+            muA[c] = row_sum16(sq[c]);
+            mu[c] = (double)muA[c];
+        }
 #pragma unroll
-        for (int e = 0; e < 21; ++e) cc[e] = 0.0;
+        for (int e = 0; e < 21; ++e) cc[e] = 0;
         for (int s0 = 0; s0 < S; s0 += 16) {
             const int s = s0 + lane;
             if (s < S) {
-                EuclideanPoint x;
-                DisparityPoint d;
-                bool in;
-                double w;
-                fit_sample(a, meas, src, b0 + s, kk, pl, k, s, pose, x, d, in, w);
-                const double dv[6] = {d.u - mu[0], d.v - mu[1], d.d - mu[2], d.vu - mu[3], d.vv - mu[4], d.vd - mu[5]};
+                A q[6], dv[6];
+                point(s, q);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) dv[c] = q[c] - muA[c];
 #pragma unroll
                 for (int r = 0; r < 6; ++r)
 #pragma unroll
@@ -652,7 +777,7 @@ __global__ __launch_bounds__(256) void k_fit_fix(FitArgs a, Planes<T> src, Plane
         const double W = (double)S, W2 = (double)S;
         const double scale = (W * W + W2) / W;
 #pragma unroll
-        for (int e = 0; e < 21; ++e) P[e] = row_sum16(cc[e]) * scale;
+        for (int e = 0; e < 21; ++e) P[e] = (double)row_sum16(cc[e]) * scale;
         const double pdg = a.fit[(int64_t)FP_PD * a.Ng + g];
         write_fit(a, g, lane, mu, P, pdg);
     }
@@ -1957,8 +2082,9 @@ __global__ __launch_bounds__(256) void k_estimate(const double* ext, const doubl
  * src/kernels.cu:49-80), sample weight 1/S (:2009-2010).  Destination
  * addresses are byte offsets from the plane bases (32-bit when every plane
  * is below 4 GiB). */
-template <typename T, typename Off, bool RG>
+template <typename T, typename Off, bool RG, typename A = double>
 __global__ __launch_bounds__(256) void k_sample(SampleArgs a, Planes<T> dst) {
+    constexpr bool F32 = std::is_same_v<A, float>;
     const int lane = threadIdx.x & 15;
     const int64_t gm = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
     if (gm >= a.Kt) return; /* whole rows */
@@ -1979,6 +2105,36 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a, Planes<T> dst) {
     const T w = (T)(1.0 / a.S);
     Off off = (Off)(((a.post_base + gm) * (int64_t)a.S + lane) * (int64_t)sizeof(T));
     PosteriorCarry pc; /* normals carried between the two steps of a noise group */
+    if constexpr (F32) {
+        /* the FP32-arithmetic variant: the factor and mean rounded to
+         * single precision once, each draw and its unprojection in single
+         * precision */
+        const MeasF mf = meas_f(meas);
+        const PoseF pf = pose_f(pose);
+        float Lf[21], muf[6];
+#pragma unroll
+        for (int e = 0; e < 21; ++e) Lf[e] = (float)L[e];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) muf[c] = (float)mu[c];
+        for (int s = lane; s < a.S; s += 16) {
+            float z[6], q[6], x[6];
+            posterior_normals_seq(key, a.ustep, (uint32_t)pl, (uint32_t)k, (uint32_t)s, (uint32_t)(s >> 4) & 1u, pc,
+                                  z);
+#pragma unroll
+            for (int r = 0; r < 6; ++r) {
+                float acc = muf[r];
+#pragma unroll
+                for (int c = 0; c <= r; ++c) acc = __builtin_fmaf(Lf[pidx(r, c)], z[c], acc);
+                q[r] = acc;
+            }
+            world_from_disparity_f(mf, pf, q, x);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) at(dst.p[c], off) = (T)x[c];
+            at(dst.p[6], off) = w;
+            off += (Off)(16 * sizeof(T));
+        }
+        return;
+    }
     for (int s = lane; s < a.S; s += 16) {
         double z[6];
         posterior_normals_seq(key, a.ustep, (uint32_t)pl, (uint32_t)k, (uint32_t)s, (uint32_t)(s >> 4) & 1u, pc, z);
@@ -2302,10 +2458,11 @@ hipError_t launch_pose(hipStream_t s, const double* ext, double* pose, int Np, i
 template <typename T>
 hipError_t launch_materialize(hipStream_t s, Planes<T> src, Planes<T> dst, const int32_t* gblock,
                               const int32_t* gparent, const int32_t* goff, int64_t Ng, int S,
-                              LinearCVMotionModel3D feat, double dt, RigKeys key, uint64_t step) {
+                              LinearCVMotionModel3D feat, double dt, RigKeys key, uint64_t step, bool arith32) {
     if (Ng <= 0) return hipSuccess;
+    if (arith32 && !std::is_same_v<T, float>) return hipErrorInvalidValue;
     k_materialize<T><<<nblocks(Ng * S, 256), 256, 0, s>>>(src, dst, gblock, gparent, goff, Ng, S, feat, dt, key,
-                                                          step);
+                                                          step, arith32 ? 1 : 0);
     LAUNCH_CHECK();
 }
 
@@ -2313,6 +2470,22 @@ template <typename T>
 hipError_t launch_fit(hipStream_t s, const FitArgs& a, Planes<T> src, Planes<T> dst) {
     if (a.Ng <= 0) return hipSuccess;
     const unsigned nb = nblocks(a.Ng, 16);
+    if constexpr (std::is_same_v<T, float>) {
+        if (a.arith32) {
+            if (a.rig_meas) {
+                if (a.off32)
+                    k_fit<T, uint32_t, true, float><<<nb, 256, 0, s>>>(a, src, dst);
+                else
+                    k_fit<T, uint64_t, true, float><<<nb, 256, 0, s>>>(a, src, dst);
+            } else if (a.off32)
+                k_fit<T, uint32_t, false, float><<<nb, 256, 0, s>>>(a, src, dst);
+            else
+                k_fit<T, uint64_t, false, float><<<nb, 256, 0, s>>>(a, src, dst);
+            LAUNCH_CHECK();
+        }
+    } else if (a.arith32) {
+        return hipErrorInvalidValue;
+    }
     if (a.rig_meas) {
         if (a.off32)
             k_fit<T, uint32_t, true><<<nb, 256, 0, s>>>(a, src, dst);
@@ -2329,6 +2502,17 @@ template <typename T>
 hipError_t launch_fit_fix(hipStream_t s, const FitArgs& a, Planes<T> src, Planes<T> dst) {
     if (a.Ng <= 0) return hipSuccess;
     const unsigned nb = (unsigned)std::min<int64_t>(nblocks(a.Ng, 16), 256);
+    if constexpr (std::is_same_v<T, float>) {
+        if (a.arith32) {
+            if (a.rig_meas)
+                k_fit_fix<T, true, float><<<nb, 256, 0, s>>>(a, src, dst);
+            else
+                k_fit_fix<T, false, float><<<nb, 256, 0, s>>>(a, src, dst);
+            LAUNCH_CHECK();
+        }
+    } else if (a.arith32) {
+        return hipErrorInvalidValue;
+    }
     if (a.rig_meas)
         k_fit_fix<T, true><<<nb, 256, 0, s>>>(a, src, dst);
     else
@@ -2340,6 +2524,22 @@ template <typename T>
 hipError_t launch_nd_fill(hipStream_t s, const FitArgs& a, Planes<T> src, Planes<T> dst) {
     if (a.Ng <= 0) return hipSuccess;
     const int64_t blocks = std::min<int64_t>(nblocks(a.Ng, 16), 2048); /* the list is at most Ng long */
+    if constexpr (std::is_same_v<T, float>) {
+        if (a.arith32) {
+            if (a.rig_meas) {
+                if (a.off32)
+                    k_nd_fill<T, uint32_t, true, float><<<blocks, 256, 0, s>>>(a, src, dst);
+                else
+                    k_nd_fill<T, uint64_t, true, float><<<blocks, 256, 0, s>>>(a, src, dst);
+            } else if (a.off32)
+                k_nd_fill<T, uint32_t, false, float><<<blocks, 256, 0, s>>>(a, src, dst);
+            else
+                k_nd_fill<T, uint64_t, false, float><<<blocks, 256, 0, s>>>(a, src, dst);
+            LAUNCH_CHECK();
+        }
+    } else if (a.arith32) {
+        return hipErrorInvalidValue;
+    }
     if (a.rig_meas) {
         if (a.off32)
             k_nd_fill<T, uint32_t, true><<<blocks, 256, 0, s>>>(a, src, dst);
@@ -2402,6 +2602,22 @@ template <typename T>
 hipError_t launch_sample(hipStream_t s, const SampleArgs& a, Planes<T> dst) {
     if (a.Kt <= 0) return hipSuccess;
     const unsigned nb = nblocks(a.Kt, 16);
+    if constexpr (std::is_same_v<T, float>) {
+        if (a.arith32) {
+            if (a.rig_meas) {
+                if (a.off32)
+                    k_sample<T, uint32_t, true, float><<<nb, 256, 0, s>>>(a, dst);
+                else
+                    k_sample<T, uint64_t, true, float><<<nb, 256, 0, s>>>(a, dst);
+            } else if (a.off32)
+                k_sample<T, uint32_t, false, float><<<nb, 256, 0, s>>>(a, dst);
+            else
+                k_sample<T, uint64_t, false, float><<<nb, 256, 0, s>>>(a, dst);
+            LAUNCH_CHECK();
+        }
+    } else if (a.arith32) {
+        return hipErrorInvalidValue;
+    }
     if (a.rig_meas) {
         if (a.off32)
             k_sample<T, uint32_t, true><<<nb, 256, 0, s>>>(a, dst);
@@ -2577,10 +2793,10 @@ hipError_t launch_scan(hipStream_t s, const int32_t* in, int32_t* out, int64_t n
 /* explicit instantiations */
 template hipError_t launch_materialize<float>(hipStream_t, Planes<float>, Planes<float>, const int32_t*,
                                               const int32_t*, const int32_t*, int64_t, int, LinearCVMotionModel3D,
-                                              double, RigKeys, uint64_t);
+                                              double, RigKeys, uint64_t, bool);
 template hipError_t launch_materialize<double>(hipStream_t, Planes<double>, Planes<double>, const int32_t*,
                                                const int32_t*, const int32_t*, int64_t, int, LinearCVMotionModel3D,
-                                               double, RigKeys, uint64_t);
+                                               double, RigKeys, uint64_t, bool);
 template hipError_t launch_fit<float>(hipStream_t, const FitArgs&, Planes<float>, Planes<float>);
 template hipError_t launch_fit<double>(hipStream_t, const FitArgs&, Planes<double>, Planes<double>);
 template hipError_t launch_fit_fix<float>(hipStream_t, const FitArgs&, Planes<float>, Planes<float>);

@@ -107,6 +107,16 @@ def test_config_grammar_variants(tmp_path):
     assert d["n_particles"] == 64 and d["save_all_maps"] == 1 and d["birth_var_vu0"] == 6.0
 
 
+def test_storage_key_names_the_three_variants(tmp_path):
+    """storage: "fp32" / 0 (FP32 clouds, FP64 arithmetic), "fp64" / 1,
+    "fp32-arith" / 2 (FP32 clouds and per-sample arithmetic)."""
+    body = open(os.path.join(GOLDEN, "calibration.cfg")).read()
+    for val, want in [('"fp32"', 0), ('"fp64"', 1), ('"fp32-arith"', 2), ("0", 0), ("1", 1), ("2", 2)]:
+        path = tmp_path / "s.cfg"
+        path.write_text(body + f"\nstorage = {val};\n")
+        assert scphd_amd.load_params(str(path))["storage"] == want, val
+
+
 def test_create_rejects_bad_arguments():
     p, _ = workloads.workload("c0")
     for k, v in [("n_particles", 0), ("particles_per_feature", 0), ("particles_per_feature", 1000), ("storage", 7)]:

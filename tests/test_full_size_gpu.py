@@ -3,7 +3,7 @@ that holds at every size:
   * 4096 parents, S = 200, 200 landmarks, clutter lambda 50 (the bench
     workload), with FP32 and with FP64 sample storage;
   * 16384 parents, S = 100, 300 landmarks, clutter lambda 100 (the precision
-    study), FP32 storage.
+    study), with FP32 and with FP64 sample storage.
 
 Each parent's map update depends only on that parent's own state and on
 noise counters that carry its parent index (and the Gaussian index within
@@ -54,8 +54,8 @@ def _close(a, b, rtol, atol, what):
 
 
 @pytest.mark.parametrize("name,storage,parents,samples", [("c1", 0, 4096, 200), ("c1", 1, 4096, 200),
-                                                            ("c3", 0, 16384, 100)],
-                         ids=["c1-fp32", "c1-fp64", "c3-fp32"])
+                                                            ("c3", 0, 16384, 100), ("c3", 1, 16384, 100)],
+                         ids=["c1-fp32", "c1-fp64", "c3-fp32", "c3-fp64"])
 def test_first_parents_of_the_full_size_step_equal_a_small_oracle_run(name, storage, parents, samples):
     import scphd_amd
 
