@@ -300,6 +300,32 @@ int scphd_create_ensemble(const scphd_params* rigs, int32_t n_rigs, scphd_t** ou
 int scphd_update_ensemble(scphd_t* f, const double* u, const double* v, const int32_t* moff,
                           int32_t fixed_camera);
 
+/* scphd_update_ensemble with a fixed-camera flag per rig (fixed_camera[r],
+ * n_rigs entries): rigs at different points of their frame sequence step
+ * together.  No counterpart in the reference (its driver updates one rig,
+ * src/main.cpp:52-95). */
+int scphd_update_ensemble_rigs(scphd_t* f, const double* u, const double* v, const int32_t* moff,
+                               const int32_t* fixed_camera);
+
+/* Ensemble membership, between frames (after an update or a resampling,
+ * before the next predict; SCPHD_ERR_STATE otherwise).  No counterpart in
+ * the reference: these serve many rigs from one set of launches.
+ *
+ * scphd_ensemble_add_rig appends a rig (*rig_out: its index, the last): its
+ * parents follow the others', drawn from its prior as scphd_create draws
+ * them, with an empty map.  It draws its noise with counters that start at
+ * zero when it joins, so it evolves exactly as a plain filter created with
+ * its parameters and fed the same frames.  Its parameters must agree with
+ * the ensemble's in what rigs share (scphd_create_ensemble).
+ *
+ * scphd_ensemble_remove_rig drops rig `rig` with its parents and map; the
+ * rigs after it move down one index.  The last rig cannot be removed
+ * (destroy the ensemble).  Debug outputs of the last update
+ * (scphd_get_update_debug, scphd_get_fit_debug) describe the membership of
+ * that update. */
+int scphd_ensemble_add_rig(scphd_t* f, const scphd_params* p, int32_t* rig_out);
+int scphd_ensemble_remove_rig(scphd_t* f, int32_t rig);
+
 /* scphd_get_estimate of every rig over its own parents, one launch:
  * mean12[12 r ..], cov144[144 r ..], cardinality[r] for rig r (any may be
  * NULL; a plain filter is one rig). */

@@ -86,9 +86,13 @@ struct NoiseKey {
 };
 
 /* a key that may differ between the lanes of a wavefront (ensembles: each
- * rig has its own); the same streams as the NoiseKey of the same words */
+ * rig has its own), with the count sb of the handle's steps that preceded
+ * the rig's joining: the rig draws step s of a stream with counter s - sb,
+ * the counter a filter of its own would use.  sb = 0 gives the same streams
+ * as the NoiseKey of the same words. */
 struct LaneKey {
     uint32_t k0, k1;
+    uint32_t sb = 0;
 };
 
 SCPHD_HD inline void noise_block(NoiseKey key, uint32_t stream, uint64_t step, uint32_t parent,
@@ -97,7 +101,8 @@ SCPHD_HD inline void noise_block(NoiseKey key, uint32_t stream, uint64_t step, u
 }
 SCPHD_HD inline void noise_block(LaneKey key, uint32_t stream, uint64_t step, uint32_t parent,
                                  uint32_t gauss, uint32_t elem, uint32_t block, uint32_t out[4]) {
-    philox4x32_10<true>((elem << 2) | block, gauss, parent, (uint32_t)((step << 4) | stream), key.k0, key.k1, out);
+    philox4x32_10<true>((elem << 2) | block, gauss, parent, (((uint32_t)step - key.sb) << 4) | stream, key.k0,
+                        key.k1, out);
 }
 
 /* 53-bit uniforms from two 32-bit words. */

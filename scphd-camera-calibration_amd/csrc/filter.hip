@@ -197,6 +197,15 @@ struct scphd_filter {
     DBuf<BirthModel> dbirths; /* per rig; one for a plain filter */
     std::vector<int32_t> hmoff;
     std::vector<double> hbpcs;
+    /* per rig, SC_COUNT step counts of the handle when the rig joined (a rig
+     * added between steps draws its noise with its own counts) */
+    std::vector<uint32_t> hsbase;
+    DBuf<uint32_t> dsbase;
+    bool sbase_used = false; /* some rig joined after steps were taken */
+    /* per rig, the fixed camera at this update (scphd_update_ensemble_rigs) */
+    std::vector<int32_t> hfixed;
+    DBuf<int32_t> dfixed;
+    bool rig_fixed = false;
     int64_t ens_meas = 0;
     size_t esz = 4;
     bool arith32 = false; /* per-sample arithmetic in single precision (SCPHD_STORAGE_FP32_ARITH) */
@@ -270,14 +279,17 @@ int hipfail(scphd_filter* f, hipError_t e, const char* what) {
     return fail(f, SCPHD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-/* the noise keys the kernels draw with: one, or one per rig */
-RigKeys rig_keys(const scphd_filter* f) {
+/* the noise keys the kernels draw with: one, or one per rig (with the
+ * rigs' step counts at joining, for counter class `which`) */
+RigKeys rig_keys(const scphd_filter* f, int which) {
     RigKeys r;
     r.key = f->key;
     if (f->ensemble) {
         r.keys = f->dkeys.p;
         r.prig = f->dprig.p;
         r.poff = f->dpoff.p;
+        r.sbase = f->sbase_used ? f->dsbase.p : nullptr;
+        r.which = which;
     }
     return r;
 }
@@ -286,7 +298,7 @@ RigKeys rig_keys(const scphd_filter* f) {
 template <typename Fn>
 void for_each_buf(scphd_filter* f, Fn&& fn) {
     fn(f->dkeys); fn(f->dmeas); fn(f->dpoff); fn(f->dprig); fn(f->dmoff); fn(f->drflag); fn(f->dbpcs);
-    fn(f->dbirths); fn(f->ext); fn(f->pw); fn(f->pose); fn(f->loglik); fn(f->cum); fn(f->ext_tmp); fn(f->anc);
+    fn(f->dbirths); fn(f->dsbase); fn(f->dfixed); fn(f->ext); fn(f->pw); fn(f->pose); fn(f->loglik); fn(f->cum); fn(f->ext_tmp); fn(f->anc);
     fn(f->newcnt); fn(f->goff); fn(f->gparent); fn(f->gblock); fn(f->gw); fn(f->goff2); fn(f->gparent2);
     fn(f->gblock2); fn(f->gw2); fn(f->gpd); fn(f->gpd2); fn(f->fit); fn(f->pre); fn(f->lik); fn(f->keep);
     fn(f->keepscan); fn(f->ndlist); fn(f->redo); fn(f->ndfill); fn(f->du); fn(f->dv); fn(f->eta); fn(f->nsurv);
@@ -357,7 +369,7 @@ int materialize_t(scphd_filter* f) {
     const int nxt = 1 - f->cur;
     HIPTRY(f->gen[nxt].ensure(f->Ng, 0, f->S, f->esz, f->stream));
     HIPTRY(launch_materialize<T>(f->stream, f->gen[f->cur].planes<T>(), f->gen[nxt].planes<T>(), f->gblock.p,
-                                 f->gparent.p, f->goff.p, f->Ng, f->S, f->models.feat, f->pdt, rig_keys(f), f->pstep,
+                                 f->gparent.p, f->goff.p, f->Ng, f->S, f->models.feat, f->pdt, rig_keys(f, SC_PREDICT), f->pstep,
                                  f->arith32));
     HIPTRY(launch_iota(f->stream, f->gblock.p, f->Ng));
     f->cur = nxt;
@@ -401,7 +413,8 @@ int update_t(scphd_filter* f, int M, int fixed) {
     const int nxt = 1 - f->cur;
     record(f, 0);
 
-    HIPTRY(launch_pose(s, f->ext.p, f->pose.p, Np, fixed));
+    HIPTRY(launch_pose(s, f->ext.p, f->pose.p, Np, fixed, f->rig_fixed ? f->dprig.p : nullptr,
+                       f->rig_fixed ? f->dfixed.p : nullptr));
     HIPTRY(f->keepscan.ensure(Ng + 1));
     HIPTRY(f->scantmp.ensure(scan_tmp_size(std::max<int64_t>(Ng, Np) + 1) + 16));
     if (Ng > 0) {
@@ -439,7 +452,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
         fa.feat = f->models.feat;
         fa.pending = f->pending ? 1 : 0;
         fa.dt = f->pdt;
-        fa.key = rig_keys(f);
+        fa.key = rig_keys(f, SC_PREDICT); /* the landmark prediction deferred from predict */
         fa.pstep = f->pstep;
         fa.min_weight = P.min_weight;
         fa.fit = f->fit.p;
@@ -588,7 +601,8 @@ int update_t(scphd_filter* f, int M, int fixed) {
     }
     HIPTRY(launch_merge(s, ma, merge_lds_bytes(std::max(1, std::min(maxU, lds_cap)))));
     if (!fixed)
-        HIPTRY(launch_parent_weights(s, f->pw.p, f->loglik.p, Np, f->ensemble ? f->dpoff.p : nullptr, f->n_rigs));
+        HIPTRY(launch_parent_weights(s, f->pw.p, f->loglik.p, Np, f->ensemble ? f->dpoff.p : nullptr, f->n_rigs,
+                                     f->rig_fixed ? f->dfixed.p : nullptr));
     record(f, 3);
 
     /* output counts */
@@ -627,7 +641,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
     pa.S = S;
     pa.meas = f->models.meas;
     pa.rig_meas = f->ensemble ? f->dmeas.p : nullptr;
-    pa.key = rig_keys(f);
+    pa.key = rig_keys(f, SC_UPDATE);
     pa.ustep = (uint64_t)f->n_update;
     /* the posterior clouds go to blocks [Ng, Ng + Kt): the planes are
      * passed from block Ng on, so that offsets start at 0 (32-bit while
@@ -808,6 +822,7 @@ int create_filter(const scphd_params* p, int32_t n_rigs, bool ensemble, scphd_t*
         return bail(SCPHD_ERR_HIP);
     if (f->ensemble) {
         f->rig_prm.assign(p, p + n_rigs);
+        f->hsbase.assign((size_t)SC_COUNT * n_rigs, 0u); /* every rig joins at the first step */
         std::vector<NoiseKey> keys(n_rigs);
         std::vector<int32_t> prig(Np);
         for (int r = 0; r < n_rigs; ++r) {
@@ -852,6 +867,188 @@ int scphd_create_ensemble(const scphd_params* rigs, int32_t n_rigs, scphd_t** ou
     return create_filter(rigs, n_rigs, true, out);
 }
 
+}  /* extern "C" */
+
+namespace {
+
+/* the per-rig device arrays of an ensemble from its host copies: keys,
+ * measurement models, birth models, parent offsets, parent -> rig and the
+ * step counts at joining */
+int upload_rigs(scphd_filter* f) {
+    const int R = f->n_rigs, Np = f->Np;
+    std::vector<NoiseKey> keys(R);
+    std::vector<BirthModel> births(R);
+    std::vector<int32_t> prig(std::max(Np, 1));
+    for (int r = 0; r < R; ++r) {
+        keys[r] = NoiseKey::from_seed(f->rig_prm[r].seed);
+        births[r] = birth_of(f->rig_prm[r]);
+        for (int q = f->hpoff[r]; q < f->hpoff[r + 1]; ++q) prig[q] = r;
+    }
+    hipStream_t s = f->stream;
+    HIPTRY(f->dkeys.ensure(R));
+    HIPTRY(f->dmeas.ensure(R));
+    HIPTRY(f->dbirths.ensure(R));
+    HIPTRY(f->dpoff.ensure(R + 1));
+    HIPTRY(f->dprig.ensure(std::max(Np, 1)));
+    HIPTRY(f->dmoff.ensure(R + 1));
+    HIPTRY(f->dbpcs.ensure(R));
+    HIPTRY(f->dsbase.ensure((int64_t)SC_COUNT * R));
+    HIPTRY(hipMemcpyAsync(f->dkeys.p, keys.data(), R * sizeof(NoiseKey), hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(f->dmeas.p, f->rig_meas.data(), R * sizeof(DisparityMeasurementModel),
+                          hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(f->dbirths.p, births.data(), R * sizeof(BirthModel), hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(f->dpoff.p, f->hpoff.data(), (R + 1) * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(f->dprig.p, prig.data(), Np * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(f->dsbase.p, f->hsbase.data(), (size_t)SC_COUNT * R * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, s));
+    /* the host vectors are read by the copies: wait for them */
+    HIPTRY(hipStreamSynchronize(s));
+    f->sbase_used = false;
+    for (uint32_t b : f->hsbase) f->sbase_used = f->sbase_used || b != 0;
+    return SCPHD_OK;
+}
+
+/* rigs join and leave between frames: every predict has had its update */
+int between_frames(scphd_filter* f, const char* what) {
+    if (f->n_predict != f->n_update || f->pending)
+        return fail(f, SCPHD_ERR_STATE, std::string(what) + ": between frames only (after update or resample, "
+                                                             "before predict)");
+    return SCPHD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int scphd_ensemble_add_rig(scphd_t* f, const scphd_params* p, int32_t* rig_out) {
+    if (check_handle(f) || !p) return SCPHD_ERR_ARG;
+    if (!f->ensemble) return fail(f, SCPHD_ERR_ARG, "scphd_ensemble_add_rig: not an ensemble");
+    if (p->n_particles <= 0 || !same_shared(*p, f->rig_prm[0]))
+        return fail(f, SCPHD_ERR_ARG, "scphd_ensemble_add_rig: parameters the ensemble's rigs must share differ");
+    if (int rc = between_frames(f, "scphd_ensemble_add_rig")) return rc;
+    const int64_t Np_new = (int64_t)f->Np + p->n_particles;
+    if (Np_new > (int64_t)1 << 30) return SCPHD_ERR_ARG;
+    HIPTRY(hipSetDevice(f->dev));
+    hipStream_t s = f->stream;
+    const int P0 = f->Np, N = p->n_particles, R = f->n_rigs + 1;
+    /* per-parent arrays, keeping what the current parents hold */
+    HIPTRY(grow_keep(f->ext, 12 * Np_new, 12 * (int64_t)P0, s));
+    HIPTRY(grow_keep(f->pw, Np_new, P0, s));
+    HIPTRY(grow_keep(f->loglik, Np_new, P0, s));
+    HIPTRY(grow_keep(f->anc, Np_new, P0, s));
+    HIPTRY(grow_keep(f->goff, Np_new + 1, (int64_t)P0 + 1, s));
+    HIPTRY(grow_keep(f->dneff, R, R - 1, s));
+    HIPTRY(grow_keep(f->drflag, R, R - 1, s));
+    HIPTRY(f->ext_tmp.ensure(12 * Np_new));
+    HIPTRY(f->pose.ensure(12 * Np_new));
+    HIPTRY(f->cum.ensure(Np_new));
+    HIPTRY(f->newcnt.ensure(Np_new + 1));
+    HIPTRY(f->goff2.ensure(Np_new + 1));
+    if (map_bytes(R) > map_bytes(f->n_rigs)) { /* room for the new rig's estimates */
+        HIPTRY(hipStreamSynchronize(s));
+        int32_t* h = nullptr;
+        int32_t* d = nullptr;
+        HIPTRY(hipHostMalloc((void**)&h, map_bytes(R), hipHostMallocMapped | hipHostMallocCoherent));
+        if (hipHostGetDevicePointer((void**)&d, h, 0) != hipSuccess) {
+            (void)hipHostFree(h);
+            return fail(f, SCPHD_ERR_HIP, "scphd_ensemble_add_rig: mapped read-back memory");
+        }
+        (void)hipHostFree(f->hmap);
+        f->hmap = h;
+        f->dmap = d;
+    }
+    /* the rig, its parents (no Gaussians) and its prior, drawn as a filter
+     * of its own draws it; its noise counters start at zero */
+    f->rig_prm.push_back(*p);
+    f->rig_meas.push_back(meas_of(*p));
+    f->hpoff.push_back((int32_t)Np_new);
+    const uint32_t base[SC_COUNT] = {(uint32_t)f->n_predict, (uint32_t)f->n_update, (uint32_t)f->n_resample};
+    f->hsbase.insert(f->hsbase.end(), base, base + SC_COUNT);
+    f->n_rigs = R;
+    f->Np = (int)Np_new;
+    HIPTRY(launch_prior(s, f->ext.p + 12 * (int64_t)P0, f->pw.p + P0, N, p->prior_a, p->prior_b,
+                        NoiseKey::from_seed(p->seed)));
+    HIPTRY(launch_fill_i32(s, f->goff.p + P0 + 1, (int32_t)f->Ng, N));
+    HIPTRY(launch_iota(s, f->anc.p + P0, N, P0));
+    HIPTRY(hipMemsetAsync(f->loglik.p + P0, 0, (size_t)N * sizeof(double), s));
+    if (int rc = upload_rigs(f)) return rc;
+    flush_retired(f); /* upload_rigs waited for the stream */
+    if (rig_out) *rig_out = R - 1;
+    return SCPHD_OK;
+}
+
+int scphd_ensemble_remove_rig(scphd_t* f, int32_t rig) {
+    if (check_handle(f)) return SCPHD_ERR_ARG;
+    if (!f->ensemble) return fail(f, SCPHD_ERR_ARG, "scphd_ensemble_remove_rig: not an ensemble");
+    if (rig < 0 || rig >= f->n_rigs) return fail(f, SCPHD_ERR_ARG, "scphd_ensemble_remove_rig: no such rig");
+    if (f->n_rigs == 1)
+        return fail(f, SCPHD_ERR_ARG, "scphd_ensemble_remove_rig: the last rig (destroy the ensemble instead)");
+    if (int rc = between_frames(f, "scphd_ensemble_remove_rig")) return rc;
+    HIPTRY(hipSetDevice(f->dev));
+    hipStream_t s = f->stream;
+    const int P0 = f->hpoff[rig], P1 = f->hpoff[rig + 1], Nr = P1 - P0, Np = f->Np, Np_new = Np - Nr;
+    int32_t g01[2] = {0, 0};
+    HIPTRY(hipMemcpyAsync(&g01[0], f->goff.p + P0, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipMemcpyAsync(&g01[1], f->goff.p + P1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    const int64_t G0 = g01[0], Gr = g01[1] - g01[0], Ng_new = f->Ng - Gr;
+    /* the map table without the rig's Gaussians (their sample blocks stay
+     * in the generation, unreferenced, until the next update writes the
+     * next one) */
+    HIPTRY(f->gparent2.ensure(Ng_new + 1));
+    HIPTRY(f->gblock2.ensure(Ng_new + 1));
+    HIPTRY(f->gw2.ensure(Ng_new + 1));
+    if (f->hint_ok) HIPTRY(f->gpd2.ensure(Ng_new + 1));
+    HIPTRY(launch_table_remove(s, f->gparent.p, f->gblock.p, f->gw.p, f->hint_ok ? f->gpd.p : nullptr, G0, Gr,
+                               Ng_new, P1, Nr, f->gparent2.p, f->gblock2.p, f->gw2.p,
+                               f->hint_ok ? f->gpd2.p : nullptr));
+    HIPTRY(launch_goff_remove(s, f->goff.p, P0, Nr, (int32_t)Gr, Np_new, f->goff2.p));
+    std::swap(f->goff, f->goff2);
+    std::swap(f->gparent, f->gparent2);
+    std::swap(f->gblock, f->gblock2);
+    std::swap(f->gw, f->gw2);
+    if (f->hint_ok) std::swap(f->gpd, f->gpd2);
+    /* per-parent arrays: the parents after the rig move down (through the
+     * scratch arrays, the ranges overlap) */
+    const int tail = Np - P1;
+    if (tail > 0) {
+        HIPTRY(hipMemcpyAsync(f->ext_tmp.p, f->ext.p + 12 * (int64_t)P1, (size_t)tail * 12 * sizeof(double),
+                              hipMemcpyDeviceToDevice, s));
+        HIPTRY(hipMemcpyAsync(f->ext.p + 12 * (int64_t)P0, f->ext_tmp.p, (size_t)tail * 12 * sizeof(double),
+                              hipMemcpyDeviceToDevice, s));
+        for (DBuf<double>* b : {&f->pw, &f->loglik}) {
+            HIPTRY(hipMemcpyAsync(f->cum.p, b->p + P1, (size_t)tail * sizeof(double), hipMemcpyDeviceToDevice, s));
+            HIPTRY(hipMemcpyAsync(b->p + P0, f->cum.p, (size_t)tail * sizeof(double), hipMemcpyDeviceToDevice, s));
+        }
+        HIPTRY(hipMemcpyAsync(f->newcnt.p, f->anc.p + P1, (size_t)tail * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+        HIPTRY(hipMemcpyAsync(f->anc.p + P0, f->newcnt.p, (size_t)tail * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+        HIPTRY(launch_index_shift(s, f->anc.p + P0, tail, P1, Nr));
+    }
+    /* per-rig values of the last resampling */
+    const int R = f->n_rigs;
+    std::vector<double> neff(R);
+    std::vector<int32_t> rflag(R);
+    HIPTRY(hipMemcpyAsync(neff.data(), f->dneff.p, R * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipMemcpyAsync(rflag.data(), f->drflag.p, R * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    neff.erase(neff.begin() + rig);
+    rflag.erase(rflag.begin() + rig);
+    HIPTRY(hipMemcpyAsync(f->dneff.p, neff.data(), (R - 1) * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(f->drflag.p, rflag.data(), (R - 1) * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    f->rig_prm.erase(f->rig_prm.begin() + rig);
+    f->rig_meas.erase(f->rig_meas.begin() + rig);
+    f->hsbase.erase(f->hsbase.begin() + (size_t)SC_COUNT * rig, f->hsbase.begin() + (size_t)SC_COUNT * (rig + 1));
+    f->hpoff.erase(f->hpoff.begin() + rig + 1);
+    for (int r = rig + 1; r < (int)f->hpoff.size(); ++r) f->hpoff[r] -= Nr;
+    f->n_rigs = R - 1;
+    f->Np = Np_new;
+    f->Ng = Ng_new;
+    f->prm = f->rig_prm[0];
+    if (int rc = upload_rigs(f)) return rc; /* waits for the stream */
+    flush_retired(f);
+    return SCPHD_OK;
+}
+
 void scphd_destroy(scphd_t* f) {
     if (!f) return;
     (void)hipSetDevice(f->dev);
@@ -879,7 +1076,8 @@ int scphd_predict(scphd_t* f, double dt) {
     HIPTRY(hipSetDevice(f->dev));
     int rc = materialize(f);
     if (rc) return rc;
-    HIPTRY(launch_parent_predict(f->stream, f->ext.p, f->Np, f->models.cam, dt, rig_keys(f), (uint64_t)f->n_predict));
+    HIPTRY(launch_parent_predict(f->stream, f->ext.p, f->Np, f->models.cam, dt, rig_keys(f, SC_PREDICT),
+                                 (uint64_t)f->n_predict));
     HIPTRY(launch_scale(f->stream, f->gw.p, f->Ng, f->prm.ps));
     f->pending = f->Ng > 0;
     f->pdt = dt;
@@ -915,11 +1113,23 @@ int scphd_update(scphd_t* f, const double* u, const double* v, int32_t n_meas, i
     return update_common(f, n_meas, fixed_camera ? 1 : 0);
 }
 
-int scphd_update_ensemble(scphd_t* f, const double* u, const double* v, const int32_t* moff,
-                          int32_t fixed_camera) {
+}  /* extern "C" */
+
+namespace {
+
+/* an ensemble update; rig_fixed (may be null): the fixed-camera flag per
+ * rig, else fixed_camera for every rig */
+int update_ensemble(scphd_t* f, const double* u, const double* v, const int32_t* moff, int32_t fixed_camera,
+                    const int32_t* rig_fixed) {
     if (check_handle(f) || !moff) return SCPHD_ERR_ARG;
     if (!f->ensemble) return fail(f, SCPHD_ERR_ARG, "not an ensemble: use scphd_update");
     const int R = f->n_rigs;
+    /* per-rig flags that differ go to the kernels; equal ones act as one */
+    bool mixed = false;
+    if (rig_fixed) {
+        for (int r = 1; r < R; ++r) mixed = mixed || ((rig_fixed[r] != 0) != (rig_fixed[0] != 0));
+        fixed_camera = rig_fixed[0] != 0;
+    }
     if (moff[0] != 0) return fail(f, SCPHD_ERR_ARG, "moff[0] must be 0");
     int Mmax = 0;
     for (int r = 0; r < R; ++r) {
@@ -943,7 +1153,33 @@ int scphd_update_ensemble(scphd_t* f, const double* u, const double* v, const in
     HIPTRY(hipMemcpyAsync(f->dmoff.p, f->hmoff.data(), (R + 1) * sizeof(int32_t), hipMemcpyHostToDevice,
                           f->stream));
     HIPTRY(hipMemcpyAsync(f->dbpcs.p, f->hbpcs.data(), R * sizeof(double), hipMemcpyHostToDevice, f->stream));
-    return update_common(f, Mmax, fixed_camera ? 1 : 0);
+    if (mixed) {
+        f->hfixed.resize(R);
+        for (int r = 0; r < R; ++r) f->hfixed[r] = rig_fixed[r] != 0;
+        HIPTRY(f->dfixed.ensure(R));
+        HIPTRY(hipMemcpyAsync(f->dfixed.p, f->hfixed.data(), R * sizeof(int32_t), hipMemcpyHostToDevice, f->stream));
+    }
+    f->rig_fixed = mixed;
+    /* mixed flags: the update runs the moving-camera path, and the kernels
+     * give each rig its own camera and weight rule */
+    const int rc = update_common(f, Mmax, mixed ? 0 : (fixed_camera ? 1 : 0));
+    f->rig_fixed = false;
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int scphd_update_ensemble(scphd_t* f, const double* u, const double* v, const int32_t* moff,
+                          int32_t fixed_camera) {
+    return update_ensemble(f, u, v, moff, fixed_camera, nullptr);
+}
+
+int scphd_update_ensemble_rigs(scphd_t* f, const double* u, const double* v, const int32_t* moff,
+                               const int32_t* fixed_camera) {
+    if (!fixed_camera) return SCPHD_ERR_ARG;
+    return update_ensemble(f, u, v, moff, 0, fixed_camera);
 }
 
 int scphd_resample(scphd_t* f, int32_t* resampled) {
@@ -968,6 +1204,7 @@ int scphd_resample(scphd_t* f, int32_t* resampled) {
     ra.min_neff = f->prm.min_neff;
     ra.key = f->key;
     ra.keys = f->ensemble ? f->dkeys.p : nullptr;
+    ra.sbase = f->ensemble && f->sbase_used ? f->dsbase.p : nullptr;
     ra.rstep = (uint64_t)f->n_resample;
     ra.flag = f->dflag.p;
     ra.rflag = f->drflag.p;

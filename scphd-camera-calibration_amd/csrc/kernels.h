@@ -30,20 +30,30 @@ namespace scphd {
  * prig[p] the rig of parent p; a parent draws with its rig's key and its
  * index inside the rig, so that every rig draws exactly the streams of a
  * filter of its own. */
+/* Step-counter classes of the noise streams: the camera and landmark
+ * prediction draw with the predict count, posterior sampling with the update
+ * count, resampling with the resample count. */
+enum StepCounter { SC_PREDICT = 0, SC_UPDATE = 1, SC_RESAMPLE = 2, SC_COUNT = 3 };
+
 struct RigKeys {
     NoiseKey key;
     const NoiseKey* keys = nullptr;
     const int32_t* prig = nullptr;
     const int32_t* poff = nullptr;
+    /* ensemble: per rig, SC_COUNT step counts of the handle when the rig
+     * joined (null: every rig joined at creation); the launch draws with
+     * counter class `which` */
+    const uint32_t* sbase = nullptr;
+    int which = SC_PREDICT;
     __device__ LaneKey of(int p, int& pl) const {
         if (!keys) {
             pl = p;
-            return LaneKey{key.k0, key.k1};
+            return LaneKey{key.k0, key.k1, 0u};
         }
         const int r = prig[p];
         pl = p - poff[r];
         const NoiseKey k = keys[r];
-        return LaneKey{k.k0, k.k1};
+        return LaneKey{k.k0, k.k1, sbase ? sbase[SC_COUNT * r + which] : 0u};
     }
 };
 
@@ -85,7 +95,10 @@ hipError_t launch_scale(hipStream_t s, double* x, int64_t n, double a);
 /* one rig's Np parents (weights 1 / Np) */
 hipError_t launch_prior(hipStream_t s, double* ext, double* pw, int Np, const double* a12, const double* b12,
                         NoiseKey key);
-hipError_t launch_pose(hipStream_t s, const double* ext, double* pose, int Np, int fixed);
+/* rfixed (ensembles, may be null): per rig, the fixed camera for that rig
+ * (rig of parent p: prig[p]); else `fixed` for every parent */
+hipError_t launch_pose(hipStream_t s, const double* ext, double* pose, int Np, int fixed,
+                       const int32_t* prig = nullptr, const int32_t* rfixed = nullptr);
 
 /* Apply a deferred sample prediction, writing every table entry to its own
  * block (block id = table index) of the destination generation. */
@@ -213,7 +226,18 @@ struct MergeArgs {
 hipError_t launch_merge(hipStream_t s, const MergeArgs& a, int lds_bytes);
 int merge_lds_bytes(int cap);
 int64_t merge_scratch_bytes(int64_t Stot, int Np);
-hipError_t launch_iota(hipStream_t s, int32_t* x, int64_t n);
+hipError_t launch_iota(hipStream_t s, int32_t* x, int64_t n, int32_t start = 0);
+/* Removal of a rig's parents [P0, P0 + Nr) and their Gaussians [G0, G0 + Gr)
+ * from an ensemble's map table: the entries after them move down, their
+ * parent indices by Nr (gpd and gpd_out may be null) */
+hipError_t launch_table_remove(hipStream_t s, const int32_t* gparent, const int32_t* gblock, const double* gw,
+                               const float* gpd, int64_t G0, int64_t Gr, int64_t Ng_new, int P1, int Nr,
+                               int32_t* gparent_out, int32_t* gblock_out, double* gw_out, float* gpd_out);
+/* goff without the rig's parents: goff_out[p] = goff[p] below P0, goff[p + Nr] - Gr from P0 (Np_new + 1 entries) */
+hipError_t launch_goff_remove(hipStream_t s, const int32_t* goff, int P0, int Nr, int32_t Gr, int Np_new,
+                              int32_t* goff_out);
+/* x[i] -= Nr where x[i] >= P1 (ancestor indices after a removal), n entries */
+hipError_t launch_index_shift(hipStream_t s, int32_t* x, int64_t n, int P1, int Nr);
 /* out[0..4] (device-visible host memory): off[n], the largest segment of
  * off, the number of segments above limit, *x0, *x1 */
 hipError_t launch_readout(hipStream_t s, const int32_t* off, int n, int32_t limit, const int32_t* x0,
@@ -226,8 +250,10 @@ hipError_t launch_pair_count(hipStream_t s, const int32_t* goff, const int32_t* 
 
 /* Per-rig launches: one workgroup per rig, rig r's parents [poff[r],
  * poff[r + 1]); poff == nullptr: one rig of Np parents. */
+/* rfixed (ensembles, may be null): rigs whose frame used the fixed camera
+ * keep their weights */
 hipError_t launch_parent_weights(hipStream_t s, double* pw, const double* loglik, int Np, const int32_t* poff,
-                                 int nrigs);
+                                 int nrigs, const int32_t* rfixed = nullptr);
 
 /* out[6 n]: six normals of every element 0..n-1 of one noise stream */
 hipError_t launch_debug_normals(hipStream_t s, NoiseKey key, uint32_t stream, uint64_t step, uint32_t parent,
@@ -302,6 +328,7 @@ struct ResampleArgs {
     double min_neff;
     NoiseKey key;          /* one rig */
     const NoiseKey* keys = nullptr; /* ensemble: per rig, or nullptr */
+    const uint32_t* sbase = nullptr; /* ensemble: per rig, the handle's step counts at joining (SC_COUNT each) */
     uint64_t rstep;
     int32_t* flag;  /* out: 1 if any rig resampled (zero before the launch) */
     int32_t* rflag = nullptr; /* out: per rig, 1 if resampled (may be null) */

@@ -173,9 +173,11 @@ __global__ void k_scale(double* x, int64_t n, double a) {
 
 /* Camera used by an update: the parent's extrinsics, or all-zero extrinsics
  * for the fixed camera (src/scphdcameracalibration.cu:1167-1176). */
-__global__ void k_pose(const double* ext, double* pose, int Np, int fixed) {
+__global__ void k_pose(const double* ext, double* pose, int Np, int fixed, const int32_t* prig,
+                       const int32_t* rfixed) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= Np) return;
+    if (rfixed) fixed = rfixed[prig[p]];
     CameraPose c;
     if (fixed) {
         c = CameraPose::identity();
@@ -1936,8 +1938,9 @@ __global__ __launch_bounds__(NT) void k_merge_rounds(MergeArgs a) {
  * (src/scphdcameracalibration.cu:1851-1875 multiplies in the linear
  * domain).  One workgroup. */
 __global__ __launch_bounds__(1024) void k_parent_weights(double* pw, const double* loglik, int Np,
-                                                         const int32_t* poff) {
+                                                         const int32_t* poff, const int32_t* rfixed) {
     __shared__ double red[16];
+    if (rfixed && rfixed[blockIdx.x]) return; /* this rig's frame used the fixed camera */
     if (poff) { /* one workgroup per rig */
         pw += poff[blockIdx.x];
         loglik += poff[blockIdx.x];
@@ -2223,6 +2226,8 @@ __global__ __launch_bounds__(1024) void k_resample(ResampleArgs a) {
     a.newcnt += P0;
     a.goff += P0;
     const NoiseKey key = a.keys ? a.keys[rig] : a.key;
+    /* the rig's own resample count: the handle's less its count at joining */
+    const uint64_t rstep = a.rstep - (a.sbase ? a.sbase[SC_COUNT * rig + SC_RESAMPLE] : 0u);
     const int t = threadIdx.x;
     double ss = 0.0;
     for (int p = t; p < N; p += blockDim.x) ss += a.pw[p] * a.pw[p];
@@ -2266,7 +2271,7 @@ __global__ __launch_bounds__(1024) void k_resample(ResampleArgs a) {
     }
     if (t == 0) {
         uint32_t x[4];
-        noise_block(key, NS_RESAMPLE, a.rstep, 0, 0, 0, 0, x);
+        noise_block(key, NS_RESAMPLE, rstep, 0, 0, 0, 0, x);
         s_u = uniform_co(x[0], x[1]) * (1.0 / N);
     }
     /* sequential cumulative sum */
@@ -2340,9 +2345,37 @@ __global__ void k_fill_i32(int32_t* x, int32_t v, int64_t n) {
     if (i < n) x[i] = v;
 }
 
-__global__ void k_iota(int32_t* x, int64_t n) {
+__global__ void k_iota(int32_t* x, int64_t n, int32_t start) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = (int32_t)i;
+    if (i < n) x[i] = start + (int32_t)i;
+}
+
+/* ---------------------------------------------- ensemble membership */
+/* The map table without one rig's Gaussians [G0, G0 + Gr): entry i of the
+ * result is entry i below G0, else entry i + Gr, whose parent index (at or
+ * after the removed rig's parents, which end at P1) moves down by Nr. */
+__global__ void k_table_remove(const int32_t* gparent, const int32_t* gblock, const double* gw, const float* gpd,
+                               int64_t G0, int64_t Gr, int64_t Ng_new, int P1, int Nr, int32_t* gparent_out,
+                               int32_t* gblock_out, double* gw_out, float* gpd_out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Ng_new) return;
+    const int64_t j = i < G0 ? i : i + Gr;
+    const int32_t p = gparent[j];
+    gparent_out[i] = p >= P1 ? p - Nr : p;
+    gblock_out[i] = gblock[j];
+    gw_out[i] = gw[j];
+    if (gpd_out) gpd_out[i] = gpd[j];
+}
+
+__global__ void k_goff_remove(const int32_t* goff, int P0, int Nr, int32_t Gr, int Np_new, int32_t* goff_out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > Np_new) return;
+    goff_out[p] = p < P0 ? goff[p] : goff[p + Nr] - Gr;
+}
+
+__global__ void k_index_shift(int32_t* x, int64_t n, int P1, int Nr) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && x[i] >= P1) x[i] -= Nr;
 }
 
 __global__ void k_table_resample(const int32_t* anc, const int32_t* goff_old, const int32_t* goff_new,
@@ -2450,8 +2483,10 @@ hipError_t launch_scale(hipStream_t s, double* x, int64_t n, double a) {
     LAUNCH_CHECK();
 }
 
-hipError_t launch_pose(hipStream_t s, const double* ext, double* pose, int Np, int fixed) {
-    k_pose<<<nblocks(Np, 256), 256, 0, s>>>(ext, pose, Np, fixed);
+hipError_t launch_pose(hipStream_t s, const double* ext, double* pose, int Np, int fixed, const int32_t* prig,
+                       const int32_t* rfixed) {
+    if (rfixed && !prig) return hipErrorInvalidValue;
+    k_pose<<<nblocks(Np, 256), 256, 0, s>>>(ext, pose, Np, fixed, prig, rfixed);
     LAUNCH_CHECK();
 }
 
@@ -2579,8 +2614,9 @@ hipError_t launch_merge(hipStream_t s, const MergeArgs& a, int lds_bytes) {
 }
 
 hipError_t launch_parent_weights(hipStream_t s, double* pw, const double* loglik, int Np, const int32_t* poff,
-                                 int nrigs) {
-    k_parent_weights<<<poff ? nrigs : 1, 1024, 0, s>>>(pw, loglik, Np, poff);
+                                 int nrigs, const int32_t* rfixed) {
+    if (rfixed && !poff) return hipErrorInvalidValue;
+    k_parent_weights<<<poff ? nrigs : 1, 1024, 0, s>>>(pw, loglik, Np, poff, rfixed);
     LAUNCH_CHECK();
 }
 
@@ -2750,9 +2786,31 @@ hipError_t launch_fill_i32(hipStream_t s, int32_t* x, int32_t v, int64_t n) {
     LAUNCH_CHECK();
 }
 
-hipError_t launch_iota(hipStream_t s, int32_t* x, int64_t n) {
+hipError_t launch_iota(hipStream_t s, int32_t* x, int64_t n, int32_t start) {
     if (n <= 0) return hipSuccess;
-    k_iota<<<nblocks(n, 256), 256, 0, s>>>(x, n);
+    k_iota<<<nblocks(n, 256), 256, 0, s>>>(x, n, start);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_table_remove(hipStream_t s, const int32_t* gparent, const int32_t* gblock, const double* gw,
+                               const float* gpd, int64_t G0, int64_t Gr, int64_t Ng_new, int P1, int Nr,
+                               int32_t* gparent_out, int32_t* gblock_out, double* gw_out, float* gpd_out) {
+    if (Ng_new <= 0) return hipSuccess;
+    if ((gpd_out != nullptr) != (gpd != nullptr)) return hipErrorInvalidValue;
+    k_table_remove<<<nblocks(Ng_new, 256), 256, 0, s>>>(gparent, gblock, gw, gpd, G0, Gr, Ng_new, P1, Nr,
+                                                        gparent_out, gblock_out, gw_out, gpd_out);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_goff_remove(hipStream_t s, const int32_t* goff, int P0, int Nr, int32_t Gr, int Np_new,
+                              int32_t* goff_out) {
+    k_goff_remove<<<nblocks((int64_t)Np_new + 1, 256), 256, 0, s>>>(goff, P0, Nr, Gr, Np_new, goff_out);
+    LAUNCH_CHECK();
+}
+
+hipError_t launch_index_shift(hipStream_t s, int32_t* x, int64_t n, int P1, int Nr) {
+    if (n <= 0) return hipSuccess;
+    k_index_shift<<<nblocks(n, 256), 256, 0, s>>>(x, n, P1, Nr);
     LAUNCH_CHECK();
 }
 

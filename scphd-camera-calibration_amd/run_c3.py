@@ -1,12 +1,13 @@
 #!/usr/bin/env python3
 """Precision study at scale: one rig with 16384 parents, S = 100, 300
-landmarks, clutter lambda = 100, run free on the GPU twice from one seed --
-FP32 sample storage (the plain path) and FP64 sample storage (all FP64) --
-and compared frame by frame.  At this size the FP64 CPU oracle is out of
-reach; tests/test_precision_gpu.py compares both modes against it on the
-same scene at 128 parents.
+landmarks, clutter lambda = 100, run free on the GPU from one seed with FP64
+sample storage (all FP64) and with each single-precision variant -- FP32
+sample storage (FP64 arithmetic), and FP32 storage with FP32 per-sample
+arithmetic -- each compared with the FP64 run frame by frame.  At this size
+the FP64 CPU oracle is out of reach; tests/test_precision_gpu.py compares
+all three against it on the same scene at 128 parents.
 
-    python run_c3.py [--parents 16384] [--frames 30] [--fp32-only]
+    python run_c3.py [--parents 16384] [--frames 30] [--variants fp32,fp32-arith] [--fp32-only]
 
 Maps are uncapped (max_gaussians = 0, as in the reference).  They grow
 through the run -- landmarks that leave the view keep their non-detection
@@ -61,30 +62,39 @@ def main(argv=None):
     ap.add_argument("--parents", type=int, default=16384)
     ap.add_argument("--frames", type=int, default=30)
     ap.add_argument("--fp32-only", action="store_true")
+    ap.add_argument("--variants", default="fp32,fp32-arith",
+                    help="single-precision variants compared with the FP64 run")
     args = ap.parse_args(argv)
     p, s = workloads.workload("c3", n_particles=args.parents, n_frames=args.frames, max_gaussians=0)
     sc = scene.generate(seed=p["seed"], **s)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
-    r32, t32 = run(dict(p, storage=0), s, sc, args.frames, log)
+    storage = {"fp32": 0, "fp64": 1, "fp32-arith": 2}
     if args.fp32_only:
+        r32, t32 = run(dict(p, storage=0), s, sc, args.frames, log)
         print(json.dumps(dict(summary=True, parents=args.parents, frames=args.frames, seconds_fp32=t32,
                               ms_per_frame=t32 / args.frames * 1e3, final_gaussians=r32[-1]["ng"],
                               final_estimate_fp32=list(r32[-1]["mean"][POSE]))))
         return 0
     r64, t64 = run(dict(p, storage=1), s, sc, args.frames, log)
-    worst, same_anc, same_ng = 0.0, 0, 0
-    for k, (a, b) in enumerate(zip(r32, r64)):
-        rel = float(np.linalg.norm(a["mean"][POSE] - b["mean"][POSE]) / np.linalg.norm(b["mean"][POSE]))
-        anc = a["resampled"] == b["resampled"] and (not a["resampled"] or np.array_equal(a["anc"], b["anc"]))
-        worst = max(worst, rel)
-        same_anc += anc
-        same_ng += a["ng"] == b["ng"]
-        print(json.dumps(dict(frame=k, rel_diff=rel, ng32=a["ng"], ng64=b["ng"], ancestors_equal=bool(anc),
-                              card32=a["card"], card64=b["card"])))
-    print(json.dumps(dict(summary=True, parents=args.parents, frames=args.frames, worst_extrinsic_rel_diff=worst,
-                          frames_with_equal_ancestors=same_anc, frames_with_equal_map_size=same_ng,
-                          seconds_fp32=t32, seconds_fp64=t64,
-                          final_estimate_fp32=list(r32[-1]["mean"][POSE]), final_estimate_fp64=list(r64[-1]["mean"][POSE]))))
+    summary = dict(summary=True, parents=args.parents, frames=args.frames, seconds_fp64=t64,
+                   final_estimate_fp64=list(r64[-1]["mean"][POSE]))
+    for name in args.variants.split(","):
+        rv, tv = run(dict(p, storage=storage[name]), s, sc, args.frames, log)
+        worst, same_anc, same_ng, first = 0.0, 0, 0, None
+        for k, (a, b) in enumerate(zip(rv, r64)):
+            rel = float(np.linalg.norm(a["mean"][POSE] - b["mean"][POSE]) / np.linalg.norm(b["mean"][POSE]))
+            anc = a["resampled"] == b["resampled"] and (not a["resampled"] or np.array_equal(a["anc"], b["anc"]))
+            worst = max(worst, rel)
+            same_anc += anc
+            same_ng += a["ng"] == b["ng"]
+            if first is None and not (anc and a["ng"] == b["ng"]):
+                first = k
+            print(json.dumps(dict(variant=name, frame=k, rel_diff=rel, ng=a["ng"], ng64=b["ng"],
+                                  ancestors_equal=bool(anc), card=a["card"], card64=b["card"])))
+        summary[name] = dict(worst_extrinsic_rel_diff=worst, frames_with_equal_ancestors=same_anc,
+                             frames_with_equal_map_size=same_ng, first_differing_decision=first, seconds=tv,
+                             final_estimate=list(rv[-1]["mean"][POSE]))
+    print(json.dumps(summary))
     return 0
 
 

@@ -38,7 +38,8 @@ EXPORTED = [
     "scphd_batch_submit", "scphd_batch_step", "scphd_batch_collect", "scphd_batch_get_stats",
     "scphd_batch_filter", "scphd_mat_read_measurements", "scphd_write_mat", "scphd_batch_set_output",
     "scphd_create_ensemble", "scphd_update_ensemble", "scphd_get_rig_estimates", "scphd_get_rig_resample",
-    "scphd_reserve", "scphd_debug_set_limits", "scphd_resample_noalloc",
+    "scphd_reserve", "scphd_debug_set_limits", "scphd_resample_noalloc", "scphd_update_ensemble_rigs",
+    "scphd_ensemble_add_rig", "scphd_ensemble_remove_rig",
 ]
 
 
@@ -176,6 +177,9 @@ def lib():
         "scphd_batch_set_output": (_I32, [_P, _I32, ctypes.c_char_p]),
         "scphd_create_ensemble": (_I32, [pp, _I32, ctypes.POINTER(_P)]),
         "scphd_update_ensemble": (_I32, [_P, dp, dp, ip, _I32]),
+        "scphd_update_ensemble_rigs": (_I32, [_P, dp, dp, ip, ip]),
+        "scphd_ensemble_add_rig": (_I32, [_P, pp, ip]),
+        "scphd_ensemble_remove_rig": (_I32, [_P, _I32]),
         "scphd_get_rig_estimates": (_I32, [_P, dp, dp, dp]),
         "scphd_get_rig_resample": (_I32, [_P, ip, dp]),
         "scphd_reserve": (_I32, [_P, _I64, _I32]),
@@ -434,8 +438,30 @@ class Ensemble(Filter):
         self.Np = int(self.poff[-1])
         self.S = self.params.particles_per_feature
 
+    def _set_rigs(self):
+        self.n_rigs = len(self.rig_params)
+        self.rig_np = [p.n_particles for p in self.rig_params]
+        self.poff = np.concatenate([[0], np.cumsum(self.rig_np)]).astype(int)
+        self.Np = int(self.poff[-1])
+
+    def add_rig(self, params):
+        """Append a rig between frames (scphd_ensemble_add_rig); returns its index."""
+        p = Params.from_dict(params) if isinstance(params, dict) else params
+        out = np.zeros(1, dtype=np.int32)
+        self._check(lib().scphd_ensemble_add_rig(self.h, ctypes.byref(p), _ip(out)), "ensemble_add_rig")
+        self.rig_params.append(p)
+        self._set_rigs()
+        return int(out[0])
+
+    def remove_rig(self, rig):
+        """Drop rig `rig` between frames (scphd_ensemble_remove_rig); later rigs move down one index."""
+        self._check(lib().scphd_ensemble_remove_rig(self.h, int(rig)), "ensemble_remove_rig")
+        del self.rig_params[rig]
+        self._set_rigs()
+
     def update(self, us, vs, fixed):
-        """us[r], vs[r]: rig r's measurement set."""
+        """us[r], vs[r]: rig r's measurement set; fixed: one flag for every
+        rig, or a sequence of one per rig (scphd_update_ensemble_rigs)."""
         if len(us) != self.n_rigs or len(vs) != self.n_rigs:
             raise ValueError("one measurement set per rig")
         moff = np.zeros(self.n_rigs + 1, dtype=np.int32)
@@ -444,8 +470,15 @@ class Ensemble(Filter):
         v = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64) for x in vs] + [np.zeros(1)]))
         if any(len(a) != len(b) for a, b in zip(us, vs)):
             raise ValueError("u and v differ in length")
-        self._check(lib().scphd_update_ensemble(self.h, _dp(u), _dp(v), _ip(moff), int(bool(fixed))),
-                    "update_ensemble")
+        if np.ndim(fixed) == 0:
+            self._check(lib().scphd_update_ensemble(self.h, _dp(u), _dp(v), _ip(moff), int(bool(fixed))),
+                        "update_ensemble")
+        else:
+            fx = np.ascontiguousarray([int(bool(x)) for x in fixed], dtype=np.int32)
+            if len(fx) != self.n_rigs:
+                raise ValueError("one fixed-camera flag per rig")
+            self._check(lib().scphd_update_ensemble_rigs(self.h, _dp(u), _dp(v), _ip(moff), _ip(fx)),
+                        "update_ensemble_rigs")
         self._mstride = max(len(x) for x in us) if us else 0
 
     def update_debug(self):

@@ -112,6 +112,112 @@ def test_ensemble_rigs_match_plain_filters_bit_for_bit():
     assert g0 == int(st["gcount"].sum())
 
 
+JOIN = [0, 0, 2, 3]  # the step at which each rig joins the ensemble
+RUN = [3, 6, 4, 3]   # the frames each rig streams before it leaves
+
+
+def _rig_slice(st, poff, i, S):
+    """Rig i's parents, weights and map out of an ensemble state."""
+    sl = slice(poff[i], poff[i + 1])
+    gc = st["gcount"]
+    g0, g1 = int(gc[:poff[i]].sum()), int(gc[:poff[i + 1]].sum())
+    return dict(states=st["states"][sl], weights=st["weights"][sl], gcount=gc[sl], wg=st["wg"][g0:g1],
+                samples=st["samples"][:, g0 * S:g1 * S])
+
+
+def test_rigs_joining_and_leaving_between_frames_match_plain_filters():
+    """Served form: rigs join an ensemble between frames (scphd_ensemble_add_rig)
+    and leave when their stream ends (scphd_ensemble_remove_rig), each step
+    covering every member with its own next frame and its own fixed-camera
+    flag (scphd_update_ensemble_rigs).  Rig 0 leaves from the front while
+    others stay, rig 3 joins after that.  Every rig's per-frame estimates and
+    resampling decisions, and its parents, weights and map when it leaves,
+    are bit-identical to a plain filter of that rig fed the same frames."""
+    import scphd_amd
+
+    rigs, _ = frames()
+    dt = rigs[0][1]["dt"]
+    ref = []
+    for r, (p, _, sets) in enumerate(rigs):
+        ref.append(run_alone(p, dt, sets[:RUN[r]], [sets[k][2] for k in range(RUN[r])]))
+    e = None
+    members, done = [], 0
+    est = [[] for _ in rigs]
+    rs = [[] for _ in rigs]
+    final = {}
+    step = 0
+    try:
+        while done < len(rigs):
+            # leave: members whose stream has ended (the ensemble's last rig
+            # closes it instead)
+            for r in [m for m in members if len(est[m]) == RUN[m]]:
+                i = members.index(r)
+                final[r] = _rig_slice(e.state(), e.poff, i, SAMPLES)
+                if len(members) > 1:
+                    e.remove_rig(i)
+                else:
+                    e.close()
+                    e = None
+                members.remove(r)
+                done += 1
+            for r in range(len(rigs)):
+                if JOIN[r] == step:
+                    if e is None:
+                        e = scphd_amd.Ensemble([rigs[r][0]])
+                    else:
+                        assert e.add_rig(rigs[r][0]) == len(members)
+                    members.append(r)
+            if not members:
+                step += 1
+                continue
+            ks = [len(est[r]) for r in members]
+            e.predict(dt)
+            e.update([rigs[r][2][k][0] for r, k in zip(members, ks)], [rigs[r][2][k][1] for r, k in zip(members, ks)],
+                     [rigs[r][2][k][2] for r, k in zip(members, ks)])
+            assert e.check() == 0
+            e.resample()
+            flags, _ = e.rig_resample()
+            m, c, card = e.rig_estimates()
+            for i, r in enumerate(members):
+                rs[r].append(bool(flags[i]))
+                est[r].append(np.concatenate([m[i], c[i].ravel(), [card[i]]]))
+            step += 1
+    finally:
+        if e is not None:
+            e.close()
+    for r, (rst, rest, rrs, _) in enumerate(ref):
+        for key in ("states", "weights", "gcount", "wg", "samples"):
+            np.testing.assert_array_equal(final[r][key], rst[key], err_msg=f"rig {r} {key}")
+        np.testing.assert_array_equal(np.array(est[r]), rest, err_msg=f"rig {r} estimates")
+        assert rs[r] == rrs, f"rig {r} resampling decisions"
+
+
+def test_ensemble_membership_refusals():
+    """Membership changes only between frames, never of the last rig, and
+    only with the parameters the rigs share."""
+    import scphd_amd
+
+    p, _ = workloads.rig(samples=SAMPLES, **RIGS[0])
+    q, _ = workloads.rig(samples=SAMPLES, **RIGS[1])
+    e = scphd_amd.Ensemble([p])
+    try:
+        with pytest.raises(scphd_amd.ScphdError, match="ARG"):
+            e.remove_rig(0)
+        with pytest.raises(scphd_amd.ScphdError, match="ARG"):
+            e.add_rig(dict(q, min_weight=1e-3))
+        e.predict(0.1)
+        with pytest.raises(scphd_amd.ScphdError, match="STATE"):
+            e.add_rig(q)
+        e.update([np.zeros(0)], [np.zeros(0)], True)
+        assert e.add_rig(q) == 1
+        with pytest.raises(scphd_amd.ScphdError, match="ARG"):
+            e.remove_rig(2)
+        e.remove_rig(0)
+        assert e.n_rigs == 1 and e.Np == q["n_particles"]
+    finally:
+        e.close()
+
+
 def test_ensemble_refuses_plain_updates_and_bad_offsets():
     import scphd_amd
 
