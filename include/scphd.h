@@ -319,8 +319,9 @@ int scphd_update_ensemble_rigs(scphd_t* f, const double* u, const double* v, con
  * the ensemble's in what rigs share (scphd_create_ensemble).
  *
  * scphd_ensemble_remove_rig drops rig `rig` with its parents and map; the
- * rigs after it move down one index.  The last rig cannot be removed
- * (destroy the ensemble).  Debug outputs of the last update
+ * rigs after it move down one index.  An ensemble whose last rig has left
+ * keeps its device buffers and takes new rigs; every other call on it
+ * returns SCPHD_ERR_STATE until one joins.  Debug outputs of the last update
  * (scphd_get_update_debug, scphd_get_fit_debug) describe the membership of
  * that update. */
 int scphd_ensemble_add_rig(scphd_t* f, const scphd_params* p, int32_t* rig_out);

@@ -731,6 +731,14 @@ int update_common(scphd_filter* f, int M, int fixed) {
 
 int check_handle(scphd_filter* f) { return f ? SCPHD_OK : SCPHD_ERR_ARG; }
 
+/* a handle with parents: an ensemble whose rigs have all left takes new
+ * rigs and nothing else */
+int check_live(scphd_filter* f) {
+    if (!f) return SCPHD_ERR_ARG;
+    if (f->Np == 0) return fail(f, SCPHD_ERR_STATE, "an ensemble with no rigs: add a rig first");
+    return SCPHD_OK;
+}
+
 }  // namespace
 
 /* ================================================================== ABI */
@@ -885,6 +893,8 @@ int upload_rigs(scphd_filter* f) {
         for (int q = f->hpoff[r]; q < f->hpoff[r + 1]; ++q) prig[q] = r;
     }
     hipStream_t s = f->stream;
+    f->sbase_used = false;
+    if (R == 0) return SCPHD_OK;
     HIPTRY(f->dkeys.ensure(R));
     HIPTRY(f->dmeas.ensure(R));
     HIPTRY(f->dbirths.ensure(R));
@@ -903,7 +913,6 @@ int upload_rigs(scphd_filter* f) {
                           hipMemcpyHostToDevice, s));
     /* the host vectors are read by the copies: wait for them */
     HIPTRY(hipStreamSynchronize(s));
-    f->sbase_used = false;
     for (uint32_t b : f->hsbase) f->sbase_used = f->sbase_used || b != 0;
     return SCPHD_OK;
 }
@@ -923,7 +932,7 @@ extern "C" {
 int scphd_ensemble_add_rig(scphd_t* f, const scphd_params* p, int32_t* rig_out) {
     if (check_handle(f) || !p) return SCPHD_ERR_ARG;
     if (!f->ensemble) return fail(f, SCPHD_ERR_ARG, "scphd_ensemble_add_rig: not an ensemble");
-    if (p->n_particles <= 0 || !same_shared(*p, f->rig_prm[0]))
+    if (p->n_particles <= 0 || !same_shared(*p, f->rig_prm.empty() ? f->prm : f->rig_prm[0]))
         return fail(f, SCPHD_ERR_ARG, "scphd_ensemble_add_rig: parameters the ensemble's rigs must share differ");
     if (int rc = between_frames(f, "scphd_ensemble_add_rig")) return rc;
     const int64_t Np_new = (int64_t)f->Np + p->n_particles;
@@ -981,8 +990,6 @@ int scphd_ensemble_remove_rig(scphd_t* f, int32_t rig) {
     if (check_handle(f)) return SCPHD_ERR_ARG;
     if (!f->ensemble) return fail(f, SCPHD_ERR_ARG, "scphd_ensemble_remove_rig: not an ensemble");
     if (rig < 0 || rig >= f->n_rigs) return fail(f, SCPHD_ERR_ARG, "scphd_ensemble_remove_rig: no such rig");
-    if (f->n_rigs == 1)
-        return fail(f, SCPHD_ERR_ARG, "scphd_ensemble_remove_rig: the last rig (destroy the ensemble instead)");
     if (int rc = between_frames(f, "scphd_ensemble_remove_rig")) return rc;
     HIPTRY(hipSetDevice(f->dev));
     hipStream_t s = f->stream;
@@ -1033,8 +1040,10 @@ int scphd_ensemble_remove_rig(scphd_t* f, int32_t rig) {
     HIPTRY(hipStreamSynchronize(s));
     neff.erase(neff.begin() + rig);
     rflag.erase(rflag.begin() + rig);
-    HIPTRY(hipMemcpyAsync(f->dneff.p, neff.data(), (R - 1) * sizeof(double), hipMemcpyHostToDevice, s));
-    HIPTRY(hipMemcpyAsync(f->drflag.p, rflag.data(), (R - 1) * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    if (R > 1) {
+        HIPTRY(hipMemcpyAsync(f->dneff.p, neff.data(), (R - 1) * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPTRY(hipMemcpyAsync(f->drflag.p, rflag.data(), (R - 1) * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    }
     f->rig_prm.erase(f->rig_prm.begin() + rig);
     f->rig_meas.erase(f->rig_meas.begin() + rig);
     f->hsbase.erase(f->hsbase.begin() + (size_t)SC_COUNT * rig, f->hsbase.begin() + (size_t)SC_COUNT * (rig + 1));
@@ -1043,7 +1052,7 @@ int scphd_ensemble_remove_rig(scphd_t* f, int32_t rig) {
     f->n_rigs = R - 1;
     f->Np = Np_new;
     f->Ng = Ng_new;
-    f->prm = f->rig_prm[0];
+    if (!f->rig_prm.empty()) f->prm = f->rig_prm[0]; /* else the shared values of the last rig stay */
     if (int rc = upload_rigs(f)) return rc; /* waits for the stream */
     flush_retired(f);
     return SCPHD_OK;
@@ -1071,7 +1080,7 @@ void scphd_destroy(scphd_t* f) {
 const char* scphd_last_error(const scphd_t* f) { return f ? f->err.c_str() : "null handle"; }
 
 int scphd_predict(scphd_t* f, double dt) {
-    if (check_handle(f)) return SCPHD_ERR_ARG;
+    if (int rc_live = check_live(f)) return rc_live;
     const ScopedRange range("scphd predict");
     HIPTRY(hipSetDevice(f->dev));
     int rc = materialize(f);
@@ -1088,7 +1097,8 @@ int scphd_predict(scphd_t* f, double dt) {
 
 int scphd_update_device(scphd_t* f, const double* u_device, const double* v_device, int32_t n_meas,
                         int32_t fixed_camera) {
-    if (check_handle(f) || n_meas < 0 || (n_meas > 0 && (!u_device || !v_device))) return SCPHD_ERR_ARG;
+    if (int rc_live = check_live(f)) return rc_live;
+    if (n_meas < 0 || (n_meas > 0 && (!u_device || !v_device))) return SCPHD_ERR_ARG;
     if (f->ensemble) return fail(f, SCPHD_ERR_ARG, "an ensemble: use scphd_update_ensemble");
     HIPTRY(hipSetDevice(f->dev));
     HIPTRY(f->du.ensure(std::max(n_meas, 1)));
@@ -1101,7 +1111,8 @@ int scphd_update_device(scphd_t* f, const double* u_device, const double* v_devi
 }
 
 int scphd_update(scphd_t* f, const double* u, const double* v, int32_t n_meas, int32_t fixed_camera) {
-    if (check_handle(f) || n_meas < 0 || (n_meas > 0 && (!u || !v))) return SCPHD_ERR_ARG;
+    if (int rc_live = check_live(f)) return rc_live;
+    if (n_meas < 0 || (n_meas > 0 && (!u || !v))) return SCPHD_ERR_ARG;
     if (f->ensemble) return fail(f, SCPHD_ERR_ARG, "an ensemble: use scphd_update_ensemble");
     HIPTRY(hipSetDevice(f->dev));
     HIPTRY(f->du.ensure(std::max(n_meas, 1)));
@@ -1123,6 +1134,7 @@ int update_ensemble(scphd_t* f, const double* u, const double* v, const int32_t*
                     const int32_t* rig_fixed) {
     if (check_handle(f) || !moff) return SCPHD_ERR_ARG;
     if (!f->ensemble) return fail(f, SCPHD_ERR_ARG, "not an ensemble: use scphd_update");
+    if (int rc = check_live(f)) return rc;
     const int R = f->n_rigs;
     /* per-rig flags that differ go to the kernels; equal ones act as one */
     bool mixed = false;
@@ -1183,7 +1195,7 @@ int scphd_update_ensemble_rigs(scphd_t* f, const double* u, const double* v, con
 }
 
 int scphd_resample(scphd_t* f, int32_t* resampled) {
-    if (check_handle(f)) return SCPHD_ERR_ARG;
+    if (int rc_live = check_live(f)) return rc_live;
     const ScopedRange range("scphd resample");
     HIPTRY(hipSetDevice(f->dev));
     int rc = materialize(f);
@@ -1357,7 +1369,7 @@ int scphd_resample_noalloc(scphd_t* f, int32_t* resampled) {
 }
 
 int scphd_check(scphd_t* f) {
-    if (check_handle(f)) return SCPHD_ERR_ARG;
+    if (int rc_live = check_live(f)) return rc_live;
     HIPTRY(hipSetDevice(f->dev));
     HIPTRY(launch_nan_count(f->stream, f->pw.p, f->Np, f->dmap + 8));
     HIPTRY(hipStreamSynchronize(f->stream));
@@ -1386,7 +1398,7 @@ int scphd_get_counts(scphd_t* f, scphd_counts* c) {
 }
 
 int scphd_get_parents(scphd_t* f, double* states, double* weights) {
-    if (check_handle(f)) return SCPHD_ERR_ARG;
+    if (int rc_live = check_live(f)) return rc_live;
     HIPTRY(hipSetDevice(f->dev));
     if (states)
         HIPTRY(hipMemcpyAsync(states, f->ext.p, (size_t)f->Np * 12 * sizeof(double), hipMemcpyDeviceToHost,
@@ -1398,7 +1410,7 @@ int scphd_get_parents(scphd_t* f, double* states, double* weights) {
 }
 
 int scphd_get_maps(scphd_t* f, int32_t* gcount, double* wg, double* samples) {
-    if (check_handle(f)) return SCPHD_ERR_ARG;
+    if (int rc_live = check_live(f)) return rc_live;
     HIPTRY(hipSetDevice(f->dev));
     int rc = materialize(f);
     if (rc) return rc;
@@ -1435,7 +1447,8 @@ int scphd_get_maps(scphd_t* f, int32_t* gcount, double* wg, double* samples) {
 int scphd_set_state(scphd_t* f, const double* states, const double* weights, const int32_t* gcount,
                     int64_t n_gaussians, const double* wg, const double* samples, int64_t n_predict,
                     int64_t n_update, int64_t n_resample) {
-    if (check_handle(f) || !states || !weights || !gcount || n_gaussians < 0) return SCPHD_ERR_ARG;
+    if (int rc_live = check_live(f)) return rc_live;
+    if (!states || !weights || !gcount || n_gaussians < 0) return SCPHD_ERR_ARG;
     if (n_gaussians > 0 && (!wg || !samples)) return SCPHD_ERR_ARG;
     HIPTRY(hipSetDevice(f->dev));
     const int Np = f->Np, S = f->S;
@@ -1494,7 +1507,7 @@ int scphd_get_update_stats(scphd_t* f, scphd_update_stats* st) {
 
 int scphd_get_update_debug(scphd_t* f, double* eta, double* loglik, int32_t* nmerged, int32_t* nnondetect,
                            int32_t* nsurvivors) {
-    if (check_handle(f)) return SCPHD_ERR_ARG;
+    if (int rc_live = check_live(f)) return rc_live;
     HIPTRY(hipSetDevice(f->dev));
     hipStream_t s = f->stream;
     const int Np = f->Np;
@@ -1515,7 +1528,8 @@ int scphd_get_update_debug(scphd_t* f, double* eta, double* loglik, int32_t* nme
 }
 
 int scphd_get_fit_debug(scphd_t* f, double* out) {
-    if (check_handle(f) || !out) return SCPHD_ERR_ARG;
+    if (int rc_live = check_live(f)) return rc_live;
+    if (!out) return SCPHD_ERR_ARG;
     HIPTRY(hipSetDevice(f->dev));
     const int64_t n = f->stats.n_gaussians_in;
     for (int q = 0; q < FP_COUNT && n > 0; ++q)
@@ -1526,7 +1540,8 @@ int scphd_get_fit_debug(scphd_t* f, double* out) {
 }
 
 int scphd_get_ancestors(scphd_t* f, int32_t* ancestors) {
-    if (check_handle(f) || !ancestors) return SCPHD_ERR_ARG;
+    if (int rc_live = check_live(f)) return rc_live;
+    if (!ancestors) return SCPHD_ERR_ARG;
     HIPTRY(hipSetDevice(f->dev));
     HIPTRY(hipMemcpyAsync(ancestors, f->anc.p, (size_t)f->Np * sizeof(int32_t), hipMemcpyDeviceToHost, f->stream));
     HIPTRY(hipStreamSynchronize(f->stream));
@@ -1585,7 +1600,7 @@ int scphd_set_noise(scphd_t* f, int32_t mode, uint64_t seed, const double* injec
 }
 
 int scphd_get_estimate(scphd_t* f, double* mean12, double* cov144, double* cardinality) {
-    if (check_handle(f)) return SCPHD_ERR_ARG;
+    if (int rc_live = check_live(f)) return rc_live;
     HIPTRY(hipSetDevice(f->dev));
     HIPTRY(launch_estimate(f->stream, f->ext.p, f->pw.p, f->goff.p, f->gw.p, f->Np, nullptr, 1,
                            (double*)((char*)f->dmap + MAP_EST)));
@@ -1598,7 +1613,7 @@ int scphd_get_estimate(scphd_t* f, double* mean12, double* cov144, double* cardi
 }
 
 int scphd_get_rig_estimates(scphd_t* f, double* mean12, double* cov144, double* cardinality) {
-    if (check_handle(f)) return SCPHD_ERR_ARG;
+    if (int rc_live = check_live(f)) return rc_live;
     HIPTRY(hipSetDevice(f->dev));
     HIPTRY(launch_estimate(f->stream, f->ext.p, f->pw.p, f->goff.p, f->gw.p, f->Np,
                            f->ensemble ? f->dpoff.p : nullptr, f->n_rigs,
@@ -1614,7 +1629,7 @@ int scphd_get_rig_estimates(scphd_t* f, double* mean12, double* cov144, double* 
 }
 
 int scphd_get_rig_resample(scphd_t* f, int32_t* resampled, double* neff) {
-    if (check_handle(f)) return SCPHD_ERR_ARG;
+    if (int rc_live = check_live(f)) return rc_live;
     if (f->n_resample == 0) return fail(f, SCPHD_ERR_STATE, "no resampling step yet");
     HIPTRY(hipSetDevice(f->dev));
     if (resampled)

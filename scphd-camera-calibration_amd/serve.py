@@ -20,6 +20,10 @@ first), for the throughput and per-frame latency of one rig per launch.
 run_ensemble() steps the same rigs together as one ensemble: one launch per
 kernel per frame for all of them.
 
+run_cohorts() serves the arrivals with shared launches: phase cohorts of
+rigs, each an ensemble that rigs join and leave between its ticks, every
+tick one set of launches over all of its rigs.
+
 Neither loads anything but the library; scene generation happens before the
 clock starts.
 """
@@ -149,6 +153,179 @@ def run_served(rigs, hbm_budget_bytes=0, n_workers=8, reserve_factor=1.5, progre
                 peak_active_rigs=peak_active, peak_reserved_gb=peak_res / 1e9, peak_used_gb=peak_used / 1e9,
                 refused_admissions=st["refused"], budget_gb=st["budget_bytes"] / 1e9, n_workers=n_workers,
                 paced=paced)
+    if keep_estimates:
+        out["estimates"] = [np.array([e[k] for k in sorted(e)]) for e in est]
+    return out
+
+
+def cohort_schedule(rigs, cohorts, paced=True):
+    """Phase cohorts of the served form with shared launches.  With frames
+    every T seconds, cohort c steps at the ticks (n K + c) T / K; a rig
+    arriving at t belongs to the cohort of the first tick at or after t
+    (paced) -- its frame k, ready at t + k T, is then stepped at the k-th
+    tick of that cohort after its first, at most T / K after it is ready.
+    Not paced: every frame is ready at the start and rigs go round-robin.
+    Returns, per rig, (cohort, index of its first tick in units of T / K)."""
+    if not rigs:
+        return []
+    T = rigs[0]["s"]["dt"]
+    if any(r["s"]["dt"] != T for r in rigs):
+        raise ValueError("the cohorts need one frame period")
+    out = []
+    for i, r in enumerate(rigs):
+        if paced:
+            n = int(math.ceil(r["t"] / (T / cohorts) - 1e-9))
+            out.append((n % cohorts, n))
+        else:
+            out.append((i % cohorts, i % cohorts))
+    return out
+
+
+def _per_gaussian_bytes(p):
+    """Device bytes per map entry of a rig's footprint (scphd_rig_footprint
+    between two map sizes)."""
+    q = dict(p, n_particles=1024)
+    return (scphd_amd.rig_footprint(q, 1024) - scphd_amd.rig_footprint(q, 0)) / (1024 * 1024)
+
+
+def run_cohorts(rigs, hbm_budget_bytes=200e9, cohorts=4, paced=True, reserve_factor=1.5, keep_estimates=False,
+                max_wall=None, progress=None):
+    """The served form with shared launches: rigs stream through `cohorts`
+    ensembles (cohort_schedule), one host thread each.  Every tick of a
+    cohort is one predict / update / resample of all of its rigs, each with
+    its own next frame and its own fixed-camera flag
+    (scphd_update_ensemble_rigs); a rig joins its cohort's ensemble between
+    ticks (scphd_ensemble_add_rig) once its reservation fits the cohort's
+    share of the HBM budget, and leaves after its last frame
+    (scphd_ensemble_remove_rig).  A rig that has to wait for admission
+    starts late and catches up one frame per tick.  Each cohort's ensemble
+    is created, sized for its share of the budget and emptied before the
+    clock starts, so neither ticks nor joins allocate.  Per-frame latency = completion time - the time the frame
+    was ready (arrival + k T); throughput = component updates / wall time.
+    Rigs draw their noise from counters that start at zero when they join,
+    so every rig's estimates equal those of its plain run."""
+    import threading
+
+    scphd_amd.lib()  # loaded once, before the threads
+    sched = cohort_schedule(rigs, cohorts, paced)
+    T = rigs[0]["s"]["dt"] if rigs else 1.0
+    share = hbm_budget_bytes / cohorts
+    est = [dict() for _ in rigs]
+    lat = [[] for _ in range(cohorts)]
+    pairs = [0] * cohorts
+    ticks = [0] * cohorts
+    peak = [0] * cohorts
+    refused = [0] * cohorts
+    adm_delay = [[] for _ in range(cohorts)]
+    errors = []
+    queues = []
+    ens = []
+    max_meas = max((expected_measurements(x["s"]) for x in rigs), default=1)
+    for c in range(cohorts):
+        q = sorted((i for i, (cc, _) in enumerate(sched) if cc == c), key=lambda i: sched[i][1])
+        queues.append(q)
+        e = None
+        if q:  # created with its first rig, sized for the share, emptied
+            e = scphd_amd.Ensemble([rigs[q[0]]["p"]])
+            try:
+                e.reserve(int(share / _per_gaussian_bytes(rigs[q[0]]["p"])), max_meas)
+            except scphd_amd.ScphdError:
+                pass  # the buffers grow as the maps do
+            e.remove_rig(0)
+        ens.append(e)
+    t0 = time.perf_counter()
+
+    def worker(c):
+        queue = queues[c]
+        e, members, used = ens[c], [], 0.0
+        n = c
+        try:
+            while queue or members:
+                tick_t = n * T / cohorts
+                if paced:
+                    wait = t0 + tick_t - time.perf_counter()
+                    if wait > 0:
+                        time.sleep(wait)
+                now = time.perf_counter() - t0
+                if max_wall is not None and now > max_wall:
+                    break
+                # admissions: rigs whose first tick has come, in order, while the share allows
+                while queue and (not paced or sched[queue[0]][1] <= n):
+                    i = queue[0]
+                    r = rigs[i]
+                    res = reserve_factor * scphd_amd.rig_footprint(r["p"], expected_gaussians_per_parent(r["s"]))
+                    if members and used + res > share:
+                        refused[c] += 1
+                        break
+                    e.add_rig(r["p"])
+                    queue.pop(0)
+                    used += res
+                    members.append(dict(i=i, k=0, res=res, t_join=now))
+                    adm_delay[c].append(now - (r["t"] if paced else 0.0))
+                if not members:
+                    n += cohorts
+                    continue
+                us, vs, fx = [], [], []
+                for m in members:
+                    u, v, fixed = rigs[m["i"]]["frames"][m["k"]]
+                    us.append(u)
+                    vs.append(v)
+                    fx.append(fixed)
+                e.predict(T)
+                e.update(us, vs, fx)
+                st = e.update_stats()
+                if e.check() != 0:
+                    raise RuntimeError(f"cohort {c}: nan weight")
+                e.resample()
+                mean, _, _ = e.rig_estimates()
+                done_t = time.perf_counter() - t0
+                pairs[c] += st["n_pairs"]
+                ticks[c] += 1
+                peak[c] = max(peak[c], len(members))
+                for j, m in enumerate(members):
+                    r = rigs[m["i"]]
+                    ready = (r["t"] + m["k"] * T) if paced else 0.0
+                    lat[c].append(done_t - ready)
+                    if keep_estimates:
+                        est[m["i"]][m["k"]] = mean[j].copy()
+                    m["k"] += 1
+                for j in reversed(range(len(members))):
+                    m = members[j]
+                    if m["k"] == len(rigs[m["i"]]["frames"]):
+                        e.remove_rig(j)
+                        used -= m["res"]
+                        del members[j]
+                n += cohorts
+        except Exception as ex:  # reported by the caller
+            errors.append(ex)
+        finally:
+            if e is not None:
+                e.close()
+
+    threads = [threading.Thread(target=worker, args=(c,)) for c in range(cohorts)]
+    for th in threads:
+        th.start()
+    last = time.perf_counter()
+    while any(th.is_alive() for th in threads):
+        time.sleep(0.05)
+        if progress and time.perf_counter() - last >= 5.0:
+            last = time.perf_counter()
+            progress(f"cohorts t={last - t0:.1f}s frames {sum(len(x) for x in lat)} ticks {sum(ticks)}")
+    for th in threads:
+        th.join()
+    if errors:
+        raise errors[0]
+    wall = time.perf_counter() - t0
+    allat = np.array([x for xs in lat for x in xs]) * 1e3
+    adm = np.array([x for xs in adm_delay for x in xs])
+    out = dict(rigs=len(rigs), frames=int(allat.size), wall_s=wall, cohorts=cohorts, paced=paced,
+               updates_per_s=sum(pairs) / wall if wall > 0 else 0.0, pairs=int(sum(pairs)), ticks=int(sum(ticks)),
+               latency_ms=dict(p50=_pct(allat, 50), p99=_pct(allat, 99),
+                               max=float(allat.max()) if allat.size else None),
+               admission_delay_s=dict(p50=_pct(adm, 50), p99=_pct(adm, 99),
+                                      max=float(adm.max()) if adm.size else None),
+               peak_rigs_per_cohort=int(max(peak)) if peak else 0, refused_admissions=int(sum(refused)),
+               budget_gb=hbm_budget_bytes / 1e9)
     if keep_estimates:
         out["estimates"] = [np.array([e[k] for k in sorted(e)]) for e in est]
     return out

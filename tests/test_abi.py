@@ -314,3 +314,24 @@ def test_ensemble_packing_respects_budget_and_plane_limit():
             if len(g) > 1:
                 assert foot <= budget
                 assert blocks * 100 * 4 * 1.25 <= 4 << 30
+
+
+def test_cohort_schedule_steps_every_frame_within_a_tick_of_ready():
+    """serve.cohort_schedule (host logic only): with K cohorts and frame
+    period T, rig i's frame k is stepped at tick (n_i + k K) T / K of its
+    cohort n_i mod K; that time is at or after the frame is ready
+    (arrival + k T) and less than T / K later.  Not paced: round-robin."""
+    import serve
+
+    T, K = 1.0 / 30.0, 4
+    arr = workloads.c4_arrivals(master=5, rate=20.0, horizon=2.0, samples=100, dur=(0.1, 0.3))
+    rigs = [dict(t=t, p=p, s=s, frames=[None] * s["n_frames"]) for t, p, s in arr]
+    sched = serve.cohort_schedule(rigs, K)
+    assert len(sched) == len(rigs)
+    for (c, n), r in zip(sched, rigs):
+        assert c == n % K
+        for k in range(len(r["frames"])):
+            tick = (n + k * K) * T / K
+            ready = r["t"] + k * T
+            assert ready - 1e-9 <= tick < ready + T / K + 1e-9
+    assert [c for c, _ in serve.cohort_schedule(rigs, K, paced=False)] == [i % K for i in range(len(rigs))]

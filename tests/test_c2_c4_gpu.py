@@ -18,8 +18,11 @@ relative (1e-9 absolute), weights as ratios to parent 0 within 1e-8.
 C4: rigs arriving as a Poisson process at 20 rigs/s for half a second,
 parents log-uniform in [256, 8192], landmarks 20-400, lambda 5-100, S = 100,
 frames at 30 Hz for 0.1-0.3 s each (3-9 frames), served with a 200 GB HBM
-budget: every rig completes, and every per-frame calibration estimate
-equals the one the rig gives run alone on a plain filter, bit for bit.
+budget through the batch API (one handle per rig) and through phase cohorts
+(shared launches, rigs joining and leaving ensembles between ticks; paced
+in real time and saturated): every rig completes, and every per-frame
+calibration estimate equals the one the rig gives run alone on a plain
+filter, bit for bit.
 """
 import numpy as np
 import pytest
@@ -143,12 +146,60 @@ def test_c2_rigs_in_the_batch_equal_their_oracle_slices():
         b.close()
 
 
+_ALONE = {}
+
+
+def _c4_burst():
+    """A C4 arrival burst (20 rigs/s for half a second, C4's size ranges,
+    3-9 frames each) and every rig's per-frame estimates run alone on a
+    plain filter."""
+    import scphd_amd
+    import serve
+
+    if "rigs" not in _ALONE:
+        arrivals = workloads.c4_arrivals(master=3, rate=20.0, horizon=0.5, samples=100, dur=(0.1, 0.3))
+        rigs = serve.prepare(arrivals)
+        alone = []
+        for r in rigs:
+            f = scphd_amd.Filter(r["p"])
+            est = []
+            for u, v, fixed in r["frames"]:
+                f.predict(r["s"]["dt"])
+                f.update(u, v, fixed)
+                assert f.check() == 0
+                f.resample()
+                est.append(f.estimate()[0])
+            f.close()
+            alone.append(np.array(est))
+        _ALONE["rigs"], _ALONE["alone"] = rigs, alone
+    return _ALONE["rigs"], _ALONE["alone"]
+
+
+@pytest.mark.parametrize("paced", [False, True], ids=["saturated", "paced"])
+def test_c4_cohorts_share_launches_and_equal_the_alone_runs(paced):
+    """The served form with shared launches (serve.run_cohorts): the C4
+    burst through four phase cohorts under a 200 GB budget, rigs joining
+    and leaving their cohort's ensemble between ticks.  Every rig completes
+    and every per-frame estimate equals its alone run's, bit for bit."""
+    import serve
+
+    rigs, alone = _c4_burst()
+    out = serve.run_cohorts(rigs, hbm_budget_bytes=200e9, cohorts=4, paced=paced, keep_estimates=True,
+                            max_wall=120.0)
+    print(f"C4 cohorts ({'paced' if paced else 'saturated'}): {out['rigs']} rigs, {out['frames']} frames in "
+          f"{out['ticks']} ticks, {out['updates_per_s']:.3e} updates/s, latency p50 {out['latency_ms']['p50']:.1f} "
+          f"p99 {out['latency_ms']['p99']:.1f} ms, peak {out['peak_rigs_per_cohort']} rigs per cohort")
+    assert out["frames"] == sum(len(r["frames"]) for r in rigs)
+    assert out["ticks"] < out["frames"]  # launches are shared
+    for i, est in enumerate(alone):
+        np.testing.assert_array_equal(out["estimates"][i], est, err_msg=f"rig {i}")
+
+
 def test_c4_served_rigs_complete_and_equal_their_alone_runs():
     import scphd_amd
     import serve
 
-    arrivals = workloads.c4_arrivals(master=3, rate=20.0, horizon=0.5, samples=100, dur=(0.1, 0.3))
-    rigs = serve.prepare(arrivals)
+    rigs, alone = _c4_burst()
     assert len(rigs) >= 5
     # reservations at half the steady-state estimate: these rigs stream only
     # 3-9 frames, far from their steady-state maps
@@ -159,14 +210,5 @@ def test_c4_served_rigs_complete_and_equal_their_alone_runs():
           f"peak reserved {out['peak_reserved_gb']:.1f} GB")
     assert out["rigs_done"] == len(rigs)
     assert out["frames"] == sum(len(r["frames"]) for r in rigs)
-    for i, r in enumerate(rigs):
-        f = scphd_amd.Filter(r["p"])
-        est = []
-        for u, v, fixed in r["frames"]:
-            f.predict(r["s"]["dt"])
-            f.update(u, v, fixed)
-            assert f.check() == 0
-            f.resample()
-            est.append(f.estimate()[0])
-        f.close()
-        np.testing.assert_array_equal(out["estimates"][i], np.array(est), err_msg=f"rig {i}")
+    for i, est in enumerate(alone):
+        np.testing.assert_array_equal(out["estimates"][i], est, err_msg=f"rig {i}")

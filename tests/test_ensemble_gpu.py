@@ -112,8 +112,12 @@ def test_ensemble_rigs_match_plain_filters_bit_for_bit():
     assert g0 == int(st["gcount"].sum())
 
 
-JOIN = [0, 0, 2, 3]  # the step at which each rig joins the ensemble
-RUN = [3, 6, 4, 3]   # the frames each rig streams before it leaves
+# the step at which each rig joins the ensemble, and the frames it streams
+# before it leaves: rig 2 joins beside rigs 0 and 1, rig 0 leaves from the
+# front while the others stay, all leave by step 6, rig 3 joins the empty
+# ensemble at step 7
+JOIN = [0, 0, 2, 7]
+RUN = [3, 5, 4, 3]
 
 
 def _rig_slice(st, poff, i, S):
@@ -130,7 +134,7 @@ def test_rigs_joining_and_leaving_between_frames_match_plain_filters():
     and leave when their stream ends (scphd_ensemble_remove_rig), each step
     covering every member with its own next frame and its own fixed-camera
     flag (scphd_update_ensemble_rigs).  Rig 0 leaves from the front while
-    others stay, rig 3 joins after that.  Every rig's per-frame estimates and
+    others stay; rig 3 joins after every other rig has left.  Every rig's per-frame estimates and
     resampling decisions, and its parents, weights and map when it leaves,
     are bit-identical to a plain filter of that rig fed the same frames."""
     import scphd_amd
@@ -153,18 +157,14 @@ def test_rigs_joining_and_leaving_between_frames_match_plain_filters():
             for r in [m for m in members if len(est[m]) == RUN[m]]:
                 i = members.index(r)
                 final[r] = _rig_slice(e.state(), e.poff, i, SAMPLES)
-                if len(members) > 1:
-                    e.remove_rig(i)
-                else:
-                    e.close()
-                    e = None
+                e.remove_rig(i)
                 members.remove(r)
                 done += 1
             for r in range(len(rigs)):
                 if JOIN[r] == step:
                     if e is None:
                         e = scphd_amd.Ensemble([rigs[r][0]])
-                    else:
+                    else:  # joins, also after every rig has left
                         assert e.add_rig(rigs[r][0]) == len(members)
                     members.append(r)
             if not members:
@@ -193,16 +193,20 @@ def test_rigs_joining_and_leaving_between_frames_match_plain_filters():
 
 
 def test_ensemble_membership_refusals():
-    """Membership changes only between frames, never of the last rig, and
-    only with the parameters the rigs share."""
+    """Membership changes only between frames and only with the parameters
+    the rigs share; an ensemble whose rigs have all left takes new rigs and
+    refuses steps."""
     import scphd_amd
 
     p, _ = workloads.rig(samples=SAMPLES, **RIGS[0])
     q, _ = workloads.rig(samples=SAMPLES, **RIGS[1])
     e = scphd_amd.Ensemble([p])
     try:
-        with pytest.raises(scphd_amd.ScphdError, match="ARG"):
-            e.remove_rig(0)
+        e.remove_rig(0)
+        assert e.Np == 0 and e.n_rigs == 0
+        with pytest.raises(scphd_amd.ScphdError, match="STATE"):
+            e.predict(0.1)
+        assert e.add_rig(p) == 0
         with pytest.raises(scphd_amd.ScphdError, match="ARG"):
             e.add_rig(dict(q, min_weight=1e-3))
         e.predict(0.1)
