@@ -129,8 +129,10 @@ def measured_traffic():
 def served_leg(args, seed):
     """Served calibration (scaled version of the 20 rigs/s workload): rigs
     arrive as a Poisson process, are admitted by HBM reservation, stream
-    30 Hz frames; compared with the plain path running the first rigs one at
-    a time."""
+    30 Hz frames -- through the batch API (one handle and stream per rig)
+    and through phase cohorts (shared launches: ensembles that rigs join and
+    leave between ticks), in real time and saturated; compared with the
+    plain path running the same rigs one at a time.  200 GB budget."""
     import serve  # noqa: E402
 
     arr = workloads.c4_arrivals(master=seed, rate=args.served_rate, horizon=args.served_horizon,
@@ -138,18 +140,26 @@ def served_leg(args, seed):
                                 lam=tuple(args.served_lambda), dur=tuple(args.served_duration))
     rigs = serve.prepare(arr)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
-    rt = serve.run_served(rigs, n_workers=args.served_workers, progress=log)
-    sat = serve.run_served(rigs, n_workers=args.served_workers, progress=log, paced=False)
+    budget = 200e9
+    rt = serve.run_served(rigs, hbm_budget_bytes=budget, n_workers=args.served_workers, progress=log)
+    sat = serve.run_served(rigs, hbm_budget_bytes=budget, n_workers=args.served_workers, progress=log, paced=False)
+    crt = serve.run_cohorts(rigs, hbm_budget_bytes=budget, cohorts=4, progress=log)
+    csat = serve.run_cohorts(rigs, hbm_budget_bytes=budget, cohorts=4, progress=log, paced=False)
     pl = serve.run_plain(rigs, progress=log)
     return {
         "workload": f"Poisson {args.served_rate:g} rigs/s over {args.served_horizon:g} s, parents log-uniform "
                     f"{args.served_parents}, landmarks {args.served_landmarks}, lambda {args.served_lambda}, "
                     f"30 Hz for {args.served_duration} s, S = 100, FP32 storage",
+        "hbm_budget_gb": budget / 1e9,
         "served_realtime": rt,
         "served_saturated": sat,
+        "cohorts_realtime": crt,
+        "cohorts_saturated": csat,
         "plain_one_rig_at_a_time": pl,
         "throughput_ratio_served_saturated_over_plain":
             sat["updates_per_s"] / pl["updates_per_s"] if pl["updates_per_s"] else None,
+        "throughput_ratio_cohorts_saturated_over_plain":
+            csat["updates_per_s"] / pl["updates_per_s"] if pl["updates_per_s"] else None,
     }
 
 

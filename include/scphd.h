@@ -327,6 +327,16 @@ int scphd_update_ensemble_rigs(scphd_t* f, const double* u, const double* v, con
 int scphd_ensemble_add_rig(scphd_t* f, const scphd_params* p, int32_t* rig_out);
 int scphd_ensemble_remove_rig(scphd_t* f, int32_t rig);
 
+/* Hand back device memory a handle holds beyond its map: the map's clouds
+ * move into a sample generation of the map's own size, and the spare
+ * generation and an update's work arrays are released (the next update
+ * allocates them at the sizes it needs).  Between frames only.  For
+ * long-lived handles whose maps shrink, such as ensembles whose rigs leave;
+ * buffers otherwise keep their largest size.  Results do not change;
+ * scphd_get_update_debug and scphd_get_fit_debug have nothing to report
+ * until the next update.  No counterpart in the reference. */
+int scphd_compact(scphd_t* f);
+
 /* scphd_get_estimate of every rig over its own parents, one launch:
  * mean12[12 r ..], cov144[144 r ..], cardinality[r] for rig r (any may be
  * NULL; a plain filter is one rig). */

@@ -117,11 +117,21 @@ struct Generation {
      * generation; if an allocation fails, every plane still holds the kept
      * blocks and at least the old capacity, which stays the recorded one.
      * Keeping blocks happens only while no other stream works on the
-     * generation, so after the wait an old plane goes straight to the cache. */
-    hipError_t ensure(int64_t need, int64_t keep, int S, size_t esz, hipStream_t s) {
+     * generation, so after the wait an old plane goes straight to the cache.
+     * drained: no queued work touches the generation, so with nothing to
+     * keep its planes go straight to the cache too, where the allocations
+     * that follow can reuse or free them. */
+    hipError_t ensure(int64_t need, int64_t keep, int S, size_t esz, hipStream_t s, bool drained = false) {
         if (need <= cap_blocks) return hipSuccess;
         const int64_t cap = std::max<int64_t>(need + need / 4, 16);
-        if (keep <= 0) {
+        if (keep <= 0 && drained) {
+            for (int c = 0; c < 7; ++c) {
+                if (pl[c]) cache_put(Block{pl[c], pbytes[c]});
+                pl[c] = nullptr;
+                pbytes[c] = 0;
+            }
+            cap_blocks = 0;
+        } else if (keep <= 0) {
             release(); /* nothing to keep: release first, then allocate */
         }
         int64_t got = INT64_MAX;
@@ -258,6 +268,7 @@ struct scphd_filter {
     scphd_update_stats stats{};
     int64_t last_M = 0;
     int64_t kt_hint = 0; /* merged Gaussians of the last update */
+    bool has_outputs = false; /* the last update's per-parent outputs are held (scphd_compact releases them) */
     bool timing = false;
     hipEvent_t ev[10] = {};  /* 0..5 phase boundaries, 6..7 fit kernel, 8..9 sample kernel */
     double phase_ms[8] = {};
@@ -427,10 +438,17 @@ int update_t(scphd_filter* f, int M, int fixed) {
          * generation is sized for the non-detection clouds alone and grows
          * after the merge. */
         const int64_t hint = f->last_M > 0 ? f->kt_hint * (int64_t)M / f->last_M : 0;
-        if (f->gen[nxt].cap_blocks < Ng + hint &&
-            f->gen[nxt].ensure(Ng + hint + hint / 4, 0, S, f->esz, s) != hipSuccess) {
-            (void)hipGetLastError();
-            HIPTRY(f->gen[nxt].ensure(Ng, 0, S, f->esz, s));
+        if (f->gen[nxt].cap_blocks < Ng + hint) {
+            /* the other generation holds nothing the next map needs, and
+             * between updates only the parent prediction is queued (the
+             * second stream is idle): once that has run, its planes can be
+             * handed back before the larger ones are allocated */
+            HIPTRY(hipStreamSynchronize(s));
+            flush_retired(f);
+            if (f->gen[nxt].ensure(Ng + hint + hint / 4, 0, S, f->esz, s, true) != hipSuccess) {
+                (void)hipGetLastError();
+                HIPTRY(f->gen[nxt].ensure(Ng, 0, S, f->esz, s, true));
+            }
         }
         HIPTRY(f->fit.ensure((int64_t)FP_COUNT * Ng));
         HIPTRY(f->pre.ensure((int64_t)PP_COUNT * Ng));
@@ -704,6 +722,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
     f->stats.max_survivors_per_parent = maxU;
     f->last_M = M;
     f->kt_hint = Kt;
+    f->has_outputs = true;
     if (f->timing) {
         HIPTRY(hipEventSynchronize(f->ev[5]));
         for (int i = 0; i < 5; ++i) {
@@ -1511,6 +1530,11 @@ int scphd_get_update_debug(scphd_t* f, double* eta, double* loglik, int32_t* nme
     HIPTRY(hipSetDevice(f->dev));
     hipStream_t s = f->stream;
     const int Np = f->Np;
+    /* an update of the current parents has run, and scphd_compact has not
+     * released its outputs since */
+    if (!f->has_outputs || f->nmerged.n < Np || f->outcnt.n < Np || f->nsurv.n < Np ||
+        (f->last_M > 0 && f->eta.n < (int64_t)Np * f->last_M))
+        return fail(f, SCPHD_ERR_STATE, "no update outputs for the current parents");
     if (eta && f->last_M > 0)
         HIPTRY(hipMemcpyAsync(eta, f->eta.p, (size_t)Np * f->last_M * sizeof(double), hipMemcpyDeviceToHost, s));
     if (loglik) HIPTRY(hipMemcpyAsync(loglik, f->loglik.p, (size_t)Np * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1636,6 +1660,63 @@ int scphd_get_rig_resample(scphd_t* f, int32_t* resampled, double* neff) {
         HIPTRY(hipMemcpyAsync(resampled, f->drflag.p, f->n_rigs * sizeof(int32_t), hipMemcpyDeviceToHost, f->stream));
     if (neff) HIPTRY(hipMemcpyAsync(neff, f->dneff.p, f->n_rigs * sizeof(double), hipMemcpyDeviceToHost, f->stream));
     HIPTRY(hipStreamSynchronize(f->stream));
+    return SCPHD_OK;
+}
+
+}  /* extern "C" */
+
+namespace {
+
+/* the map's clouds into the spare generation, sized for the map alone, which
+ * then becomes the current one */
+template <typename T>
+int compact_map(scphd_filter* f) {
+    const int nxt = 1 - f->cur;
+    HIPTRY(f->gen[nxt].ensure(f->Ng, 0, f->S, f->esz, f->stream, true));
+    HIPTRY(launch_gather<T>(f->stream, f->gen[f->cur].planes<T>(), f->gen[nxt].planes<T>(), f->gblock.p, f->Ng,
+                            f->S));
+    HIPTRY(launch_iota(f->stream, f->gblock.p, f->Ng));
+    f->cur = nxt;
+    f->blocks_used = f->Ng;
+    return SCPHD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int scphd_compact(scphd_t* f) {
+    if (check_handle(f)) return SCPHD_ERR_ARG;
+    if (int rc = between_frames(f, "scphd_compact")) return rc;
+    HIPTRY(hipSetDevice(f->dev));
+    hipStream_t s = f->stream;
+    HIPTRY(hipStreamSynchronize(s));
+    HIPTRY(hipStreamSynchronize(f->side));
+    flush_retired(f);
+    /* the spare generation first, so that the compacted map can use its memory */
+    f->gen[1 - f->cur].release();
+    flush_retired(f);
+    if (f->Ng > 0) {
+        int rc = f->esz == 8 ? compact_map<double>(f) : compact_map<float>(f);
+        if (rc) return rc;
+        HIPTRY(hipStreamSynchronize(s));
+    }
+    /* the generation the map came from, and the work arrays of an update
+     * (each update sizes them again); the debug outputs of the last update
+     * go with them */
+    f->gen[1 - f->cur].release();
+    for (auto* b : {&f->fit, &f->pre, &f->lik, &f->eta, &f->sw, &f->smu, &f->mw, &f->mmu, &f->mP}) b->release();
+    for (auto* b : {&f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->ndfill, &f->mcnt, &f->mncand, &f->mcand, &f->sg,
+                    &f->sm, &f->glead, &f->gsta, &f->gend, &f->gmem, &f->gparent2, &f->gblock2, &f->nsurv,
+                    &f->survoff, &f->nmerged, &f->mergeoff, &f->outcnt, &f->ngroups})
+        b->release();
+    f->gw2.release();
+    f->gpd2.release();
+    f->mscratch.release();
+    f->last_M = 0;
+    f->has_outputs = false;
+    f->stats = scphd_update_stats{};
+    flush_retired(f); /* nothing queued uses them */
     return SCPHD_OK;
 }
 

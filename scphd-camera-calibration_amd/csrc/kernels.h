@@ -349,6 +349,9 @@ hipError_t launch_table_resample(hipStream_t s, const int32_t* anc, const int32_
 /* ---- read-back ---- */
 template <typename T>
 hipError_t launch_extract(hipStream_t s, Planes<T> src, const int32_t* gblock, int64_t Ng, int S, double* out);
+/* the clouds of the Ng map entries into blocks 0 .. Ng-1 of dst */
+template <typename T>
+hipError_t launch_gather(hipStream_t s, Planes<T> src, Planes<T> dst, const int32_t* gblock, int64_t Ng, int S);
 template <typename T>
 hipError_t launch_inject(hipStream_t s, const double* in, int64_t Ng, int S, Planes<T> dst);
 

@@ -2393,6 +2393,18 @@ __global__ void k_table_resample(const int32_t* anc, const int32_t* goff_old, co
 }
 
 /* ------------------------------------------------------------ read-back */
+/* The map's clouds into consecutive blocks: block g of dst <- block
+ * gblock[g] of src, every plane. */
+template <typename T>
+__global__ void k_gather(Planes<T> src, Planes<T> dst, const int32_t* gblock, int64_t Ng, int S) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Ng * S) return;
+    const int64_t g = i / S;
+    const int64_t si = (int64_t)gblock[g] * S + (i - g * S);
+#pragma unroll
+    for (int c = 0; c < 7; ++c) dst.p[c][i] = src.p[c][si];
+}
+
 template <typename T>
 __global__ void k_extract(Planes<T> src, const int32_t* gblock, int64_t Ng, int S, double* out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2825,6 +2837,13 @@ hipError_t launch_table_resample(hipStream_t s, const int32_t* anc, const int32_
 }
 
 template <typename T>
+hipError_t launch_gather(hipStream_t s, Planes<T> src, Planes<T> dst, const int32_t* gblock, int64_t Ng, int S) {
+    if (Ng <= 0) return hipSuccess;
+    k_gather<T><<<nblocks(Ng * S, 256), 256, 0, s>>>(src, dst, gblock, Ng, S);
+    LAUNCH_CHECK();
+}
+
+template <typename T>
 hipError_t launch_extract(hipStream_t s, Planes<T> src, const int32_t* gblock, int64_t Ng, int S, double* out) {
     if (Ng <= 0) return hipSuccess;
     k_extract<T><<<nblocks(Ng * S, 256), 256, 0, s>>>(src, gblock, Ng, S, out);
@@ -2863,6 +2882,9 @@ template hipError_t launch_nd_fill<float>(hipStream_t, const FitArgs&, Planes<fl
 template hipError_t launch_nd_fill<double>(hipStream_t, const FitArgs&, Planes<double>, Planes<double>);
 template hipError_t launch_sample<float>(hipStream_t, const SampleArgs&, Planes<float>);
 template hipError_t launch_sample<double>(hipStream_t, const SampleArgs&, Planes<double>);
+template hipError_t launch_gather<float>(hipStream_t, Planes<float>, Planes<float>, const int32_t*, int64_t, int);
+template hipError_t launch_gather<double>(hipStream_t, Planes<double>, Planes<double>, const int32_t*, int64_t,
+                                          int);
 template hipError_t launch_extract<float>(hipStream_t, Planes<float>, const int32_t*, int64_t, int, double*);
 template hipError_t launch_extract<double>(hipStream_t, Planes<double>, const int32_t*, int64_t, int, double*);
 template hipError_t launch_inject<float>(hipStream_t, const double*, int64_t, int, Planes<float>);

@@ -39,7 +39,7 @@ EXPORTED = [
     "scphd_batch_filter", "scphd_mat_read_measurements", "scphd_write_mat", "scphd_batch_set_output",
     "scphd_create_ensemble", "scphd_update_ensemble", "scphd_get_rig_estimates", "scphd_get_rig_resample",
     "scphd_reserve", "scphd_debug_set_limits", "scphd_resample_noalloc", "scphd_update_ensemble_rigs",
-    "scphd_ensemble_add_rig", "scphd_ensemble_remove_rig",
+    "scphd_ensemble_add_rig", "scphd_ensemble_remove_rig", "scphd_compact",
 ]
 
 
@@ -180,6 +180,7 @@ def lib():
         "scphd_update_ensemble_rigs": (_I32, [_P, dp, dp, ip, ip]),
         "scphd_ensemble_add_rig": (_I32, [_P, pp, ip]),
         "scphd_ensemble_remove_rig": (_I32, [_P, _I32]),
+        "scphd_compact": (_I32, [_P]),
         "scphd_get_rig_estimates": (_I32, [_P, dp, dp, dp]),
         "scphd_get_rig_resample": (_I32, [_P, ip, dp]),
         "scphd_reserve": (_I32, [_P, _I64, _I32]),
@@ -410,6 +411,10 @@ class Filter:
     def reserve(self, n_gaussians, max_meas):
         """Pre-size the device buffers (scphd_reserve)."""
         self._check(lib().scphd_reserve(self.h, int(n_gaussians), int(max_meas)), "reserve")
+
+    def compact(self):
+        """Hand back device memory beyond the map (scphd_compact), between frames."""
+        self._check(lib().scphd_compact(self.h), "compact")
 
     def memory(self):
         b = _I64(0)

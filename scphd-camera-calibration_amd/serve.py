@@ -158,27 +158,23 @@ def run_served(rigs, hbm_budget_bytes=0, n_workers=8, reserve_factor=1.5, progre
     return out
 
 
-def cohort_schedule(rigs, cohorts, paced=True):
-    """Phase cohorts of the served form with shared launches.  With frames
-    every T seconds, cohort c steps at the ticks (n K + c) T / K; a rig
-    arriving at t belongs to the cohort of the first tick at or after t
-    (paced) -- its frame k, ready at t + k T, is then stepped at the k-th
-    tick of that cohort after its first, at most T / K after it is ready.
-    Not paced: every frame is ready at the start and rigs go round-robin.
-    Returns, per rig, (cohort, index of its first tick in units of T / K)."""
-    if not rigs:
-        return []
-    T = rigs[0]["s"]["dt"]
-    if any(r["s"]["dt"] != T for r in rigs):
-        raise ValueError("the cohorts need one frame period")
-    out = []
-    for i, r in enumerate(rigs):
-        if paced:
-            n = int(math.ceil(r["t"] / (T / cohorts) - 1e-9))
-            out.append((n % cohorts, n))
+def cohort_admit(pending, arrive, need, tick_t, left, empty):
+    """The rigs a cohort takes at a tick at time tick_t: in the order of
+    `pending` (arrival order), every rig that has arrived (arrive[i] <=
+    tick_t) and whose reservation need[i] fits `left`, what is left of the
+    budget; with nothing admitted (`empty`) the first arrived rig is taken
+    whatever its size, so that no rig waits forever.  Rigs that have not
+    arrived end the scan.  Returns (taken, passed over)."""
+    take, skip = [], []
+    for i in pending:
+        if arrive[i] > tick_t:
+            break
+        if need[i] <= left or (empty and not take):
+            take.append(i)
+            left -= need[i]
         else:
-            out.append((i % cohorts, i % cohorts))
-    return out
+            skip.append(i)
+    return take, skip
 
 
 def _per_gaussian_bytes(p):
@@ -190,57 +186,79 @@ def _per_gaussian_bytes(p):
 
 def run_cohorts(rigs, hbm_budget_bytes=200e9, cohorts=4, paced=True, reserve_factor=1.5, keep_estimates=False,
                 max_wall=None, progress=None):
-    """The served form with shared launches: rigs stream through `cohorts`
-    ensembles (cohort_schedule), one host thread each.  Every tick of a
-    cohort is one predict / update / resample of all of its rigs, each with
-    its own next frame and its own fixed-camera flag
-    (scphd_update_ensemble_rigs); a rig joins its cohort's ensemble between
-    ticks (scphd_ensemble_add_rig) once its reservation fits the cohort's
-    share of the HBM budget, and leaves after its last frame
-    (scphd_ensemble_remove_rig).  A rig that has to wait for admission
-    starts late and catches up one frame per tick.  Each cohort's ensemble
-    is created, sized for its share of the budget and emptied before the
-    clock starts, so neither ticks nor joins allocate.  Per-frame latency = completion time - the time the frame
-    was ready (arrival + k T); throughput = component updates / wall time.
-    Rigs draw their noise from counters that start at zero when they join,
-    so every rig's estimates equal those of its plain run."""
+    """The served form with shared launches: rigs stream through K
+    ensembles (cohorts), one host thread each, cohort c stepping at the
+    ticks (n K + c) T / K (frames every T).  K is the budget over the
+    largest rig's reservation, at most `cohorts`, so that every rig fits a
+    cohort's share.  Every tick is one
+    predict / update / resample of all of the cohort's rigs, each with its
+    own next frame and its own fixed-camera flag (scphd_update_ensemble_rigs).
+
+    Memory and admission: each cohort's ensemble is created, its buffers
+    sized for an equal share of the HBM budget, and emptied before the
+    clock starts, so that ticks and joins do not allocate.  At each of its
+    ticks a cohort takes, in arrival order, the arrived rigs whose
+    reservations (reserve_factor times scphd_rig_footprint at the expected
+    map size) fit what is left of its share (cohort_admit; smaller rigs may
+    pass one that does not fit).  Should a cohort's maps outgrow the
+    estimates, its buffers grow; once the rigs that grew them have left,
+    they are compacted (scphd_compact) and sized for the share again.
+
+    A rig is stepped from the first tick that takes it -- its phase
+    cohort's, at most T / K after it arrives, when that one has room.  It
+    joins between ticks (scphd_ensemble_add_rig), its frame k -- ready k T
+    after its arrival -- is stepped k periods after its first tick (a rig
+    that waited keeps its delay), and it leaves after its last frame
+    (scphd_ensemble_remove_rig).
+
+    Per-frame latency = completion time - the time the frame was ready
+    (arrival + k T; not paced: 0); throughput = component updates / wall
+    time.  Rigs draw their noise from counters that start at zero when they
+    join, so every rig's estimates equal those of its plain run."""
     import threading
 
     scphd_amd.lib()  # loaded once, before the threads
-    sched = cohort_schedule(rigs, cohorts, paced)
     T = rigs[0]["s"]["dt"] if rigs else 1.0
+    if any(r["s"]["dt"] != T for r in rigs):
+        raise ValueError("the cohorts need one frame period")
+    booked = [0.0]  # reservations of the admitted rigs, over all cohorts
+    arrive = [r["t"] if paced else 0.0 for r in rigs]
+    need = [reserve_factor * scphd_amd.rig_footprint(r["p"], expected_gaussians_per_parent(r["s"])) for r in rigs]
+    cohorts = max(1, min(cohorts, int(hbm_budget_bytes // max(need, default=1.0))))
     share = hbm_budget_bytes / cohorts
+    pending = sorted(range(len(rigs)), key=lambda i: arrive[i])
     est = [dict() for _ in rigs]
     lat = [[] for _ in range(cohorts)]
     pairs = [0] * cohorts
     ticks = [0] * cohorts
     peak = [0] * cohorts
-    refused = [0] * cohorts
+    waited = set()  # rigs that found no room at some tick after arriving
     adm_delay = [[] for _ in range(cohorts)]
     errors = []
-    queues = []
+    lock = threading.Lock()
+    held = [0] * cohorts  # device bytes each cohort's ensemble holds
+    peak_held = [0]
     ens = []
     max_meas = max((expected_measurements(x["s"]) for x in rigs), default=1)
-    for c in range(cohorts):
-        q = sorted((i for i, (cc, _) in enumerate(sched) if cc == c), key=lambda i: sched[i][1])
-        queues.append(q)
-        e = None
-        if q:  # created with its first rig, sized for the share, emptied
-            e = scphd_amd.Ensemble([rigs[q[0]]["p"]])
-            try:
-                e.reserve(int(share / _per_gaussian_bytes(rigs[q[0]]["p"])), max_meas)
-            except scphd_amd.ScphdError:
-                pass  # the buffers grow as the maps do
-            e.remove_rig(0)
+
+    def size_for_share(e, p):
+        try:
+            e.reserve(int(share / _per_gaussian_bytes(p)), max_meas)
+        except scphd_amd.ScphdError:
+            pass  # the buffers grow as the maps do
+
+    for c in range(min(cohorts, len(rigs))):  # created with a rig, sized, emptied
+        e = scphd_amd.Ensemble([rigs[c]["p"]])
+        size_for_share(e, rigs[c]["p"])
+        e.remove_rig(0)
         ens.append(e)
     t0 = time.perf_counter()
 
     def worker(c):
-        queue = queues[c]
-        e, members, used = ens[c], [], 0.0
+        e, members = ens[c], []
         n = c
         try:
-            while queue or members:
+            while True:
                 tick_t = n * T / cohorts
                 if paced:
                     wait = t0 + tick_t - time.perf_counter()
@@ -249,20 +267,23 @@ def run_cohorts(rigs, hbm_budget_bytes=200e9, cohorts=4, paced=True, reserve_fac
                 now = time.perf_counter() - t0
                 if max_wall is not None and now > max_wall:
                     break
-                # admissions: rigs whose first tick has come, in order, while the share allows
-                while queue and (not paced or sched[queue[0]][1] <= n):
-                    i = queue[0]
-                    r = rigs[i]
-                    res = reserve_factor * scphd_amd.rig_footprint(r["p"], expected_gaussians_per_parent(r["s"]))
-                    if members and used + res > share:
-                        refused[c] += 1
+                with lock:
+                    if not pending and not members:
                         break
-                    e.add_rig(r["p"])
-                    queue.pop(0)
-                    used += res
-                    members.append(dict(i=i, k=0, res=res, t_join=now))
-                    adm_delay[c].append(now - (r["t"] if paced else 0.0))
+                    left = min(share - sum(need[m["i"]] for m in members), hbm_budget_bytes - booked[0])
+                    # (a rig larger than the budget runs alone in a cohort)
+                    take, skip = cohort_admit(pending, arrive, need, tick_t, left, not members and booked[0] == 0.0)
+                    waited.update(skip)
+                    for i in take:
+                        pending.remove(i)
+                        booked[0] += need[i]
+                for i in take:
+                    e.add_rig(rigs[i]["p"])
+                    members.append(dict(i=i, k=0, first=n))
+                    adm_delay[c].append(tick_t - arrive[i] if paced else now)
                 if not members:
+                    if not paced:
+                        time.sleep(0.001)  # idle: leave the host to the working cohorts
                     n += cohorts
                     continue
                 us, vs, fx = [], [], []
@@ -282,37 +303,53 @@ def run_cohorts(rigs, hbm_budget_bytes=200e9, cohorts=4, paced=True, reserve_fac
                 pairs[c] += st["n_pairs"]
                 ticks[c] += 1
                 peak[c] = max(peak[c], len(members))
+                with lock:
+                    held[c] = e.memory()
+                    peak_held[0] = max(peak_held[0], sum(held))
                 for j, m in enumerate(members):
-                    r = rigs[m["i"]]
-                    ready = (r["t"] + m["k"] * T) if paced else 0.0
-                    lat[c].append(done_t - ready)
+                    lat[c].append(done_t - (arrive[m["i"]] + m["k"] * T if paced else 0.0))
                     if keep_estimates:
                         est[m["i"]][m["k"]] = mean[j].copy()
                     m["k"] += 1
+                gone = None
                 for j in reversed(range(len(members))):
                     m = members[j]
                     if m["k"] == len(rigs[m["i"]]["frames"]):
                         e.remove_rig(j)
-                        used -= m["res"]
+                        with lock:
+                            booked[0] -= need[m["i"]]
                         del members[j]
+                        gone = rigs[m["i"]]["p"]
+                # buffers keep their largest size: once the rig that grew
+                # them past the share has left, they are sized for it again
+                if gone is not None and e.memory() > 1.25 * share and sum(need[m["i"]] for m in members) <= share:
+                    e.compact()
+                    size_for_share(e, gone)
+                    with lock:
+                        held[c] = e.memory()
                 n += cohorts
-        except Exception as ex:  # reported by the caller
-            errors.append(ex)
-        finally:
-            if e is not None:
-                e.close()
+        except Exception as ex:  # reported by the caller, with the cohorts' sizes
+            with lock:
+                sizes = [(h.n_rigs, h.Np, h.counts()["n_gaussians"], round(h.memory() / 1e9, 1)) for h in ens]
+            errors.append(RuntimeError(f"cohort {c}: {ex}; per cohort (rigs, parents, Gaussians, GB held): {sizes}; "
+                                       f"reserved {booked[0] / 1e9:.1f} GB of {hbm_budget_bytes / 1e9:.0f}"))
 
-    threads = [threading.Thread(target=worker, args=(c,)) for c in range(cohorts)]
-    for th in threads:
-        th.start()
-    last = time.perf_counter()
-    while any(th.is_alive() for th in threads):
-        time.sleep(0.05)
-        if progress and time.perf_counter() - last >= 5.0:
-            last = time.perf_counter()
-            progress(f"cohorts t={last - t0:.1f}s frames {sum(len(x) for x in lat)} ticks {sum(ticks)}")
-    for th in threads:
-        th.join()
+    threads = [threading.Thread(target=worker, args=(c,)) for c in range(len(ens))]
+    try:
+        for th in threads:
+            th.start()
+        last = time.perf_counter()
+        while any(th.is_alive() for th in threads):
+            time.sleep(0.05)
+            if progress and time.perf_counter() - last >= 5.0:
+                last = time.perf_counter()
+                progress(f"cohorts t={last - t0:.1f}s frames {sum(len(x) for x in lat)} ticks {sum(ticks)} "
+                         f"waiting {len(pending)} reserved {booked[0] / 1e9:.1f} GB held {sum(held) / 1e9:.1f} GB")
+        for th in threads:
+            th.join()
+    finally:
+        for e in ens:
+            e.close()
     if errors:
         raise errors[0]
     wall = time.perf_counter() - t0
@@ -324,8 +361,8 @@ def run_cohorts(rigs, hbm_budget_bytes=200e9, cohorts=4, paced=True, reserve_fac
                                max=float(allat.max()) if allat.size else None),
                admission_delay_s=dict(p50=_pct(adm, 50), p99=_pct(adm, 99),
                                       max=float(adm.max()) if adm.size else None),
-               peak_rigs_per_cohort=int(max(peak)) if peak else 0, refused_admissions=int(sum(refused)),
-               budget_gb=hbm_budget_bytes / 1e9)
+               peak_rigs_per_cohort=int(max(peak)) if peak else 0, rigs_that_waited=len(waited),
+               budget_gb=hbm_budget_bytes / 1e9, peak_held_gb=peak_held[0] / 1e9)
     if keep_estimates:
         out["estimates"] = [np.array([e[k] for k in sorted(e)]) for e in est]
     return out
@@ -333,11 +370,12 @@ def run_cohorts(rigs, hbm_budget_bytes=200e9, cohorts=4, paced=True, reserve_fac
 
 def run_plain(rigs, progress=None):
     """The same rigs one at a time through one Filter each, frames back to back."""
-    lat, pairs, t_all = [], 0, 0.0
+    lat, pairs, t_all, gpp = [], 0, 0.0, []
     for i, r in enumerate(rigs):
         f = scphd_amd.Filter(r["p"])
-        try:  # steady-state sizes up front, as for an ensemble
-            f.reserve(r["p"]["n_particles"] * expected_gaussians_per_parent(r["s"]), expected_measurements(r["s"]))
+        try:  # steady-state sizes up front (a generation holds the input map and its posterior clouds)
+            f.reserve(int(1.5 * r["p"]["n_particles"] * expected_gaussians_per_parent(r["s"])),
+                      expected_measurements(r["s"]))
         except scphd_amd.ScphdError:
             pass
         dt = r["s"]["dt"]
@@ -354,12 +392,14 @@ def run_plain(rigs, progress=None):
             lat.append(t1 - t0)
             pairs += st["n_pairs"]
             t_all += t1 - t0
+        gpp.append(f.counts()["n_gaussians"] / r["p"]["n_particles"] / expected_gaussians_per_parent(r["s"]))
         f.close()
         if progress and (i + 1) % 10 == 0:
             progress(f"plain rigs {i + 1}/{len(rigs)}")
     lat = np.array(lat) * 1e3
     return dict(rigs=len(rigs), frames=len(lat), wall_s=t_all, updates_per_s=pairs / t_all if t_all > 0 else 0.0,
-                pairs=pairs, latency_ms=dict(p50=_pct(lat, 50), p99=_pct(lat, 99), max=float(lat.max())))
+                pairs=pairs, latency_ms=dict(p50=_pct(lat, 50), p99=_pct(lat, 99), max=float(lat.max())),
+                final_map_over_expected=dict(p50=_pct(gpp, 50), max=float(max(gpp)) if gpp else None))
 
 
 def run_ensemble(rigs, progress=None, growth=1.0):

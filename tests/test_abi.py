@@ -316,22 +316,21 @@ def test_ensemble_packing_respects_budget_and_plane_limit():
                 assert blocks * 100 * 4 * 1.25 <= 4 << 30
 
 
-def test_cohort_schedule_steps_every_frame_within_a_tick_of_ready():
-    """serve.cohort_schedule (host logic only): with K cohorts and frame
-    period T, rig i's frame k is stepped at tick (n_i + k K) T / K of its
-    cohort n_i mod K; that time is at or after the frame is ready
-    (arrival + k T) and less than T / K later.  Not paced: round-robin."""
+def test_cohort_admission_is_first_fit_in_arrival_order():
+    """serve.cohort_admit (host logic only): arrived rigs in arrival order,
+    each taken when it fits what is left of the budget, smaller ones passing
+    a rig that does not fit; the scan ends at the first rig not yet
+    arrived; with nothing admitted the first arrived rig is taken whatever its size."""
     import serve
 
-    T, K = 1.0 / 30.0, 4
-    arr = workloads.c4_arrivals(master=5, rate=20.0, horizon=2.0, samples=100, dur=(0.1, 0.3))
-    rigs = [dict(t=t, p=p, s=s, frames=[None] * s["n_frames"]) for t, p, s in arr]
-    sched = serve.cohort_schedule(rigs, K)
-    assert len(sched) == len(rigs)
-    for (c, n), r in zip(sched, rigs):
-        assert c == n % K
-        for k in range(len(r["frames"])):
-            tick = (n + k * K) * T / K
-            ready = r["t"] + k * T
-            assert ready - 1e-9 <= tick < ready + T / K + 1e-9
-    assert [c for c, _ in serve.cohort_schedule(rigs, K, paced=False)] == [i % K for i in range(len(rigs))]
+    arrive = [0.0, 0.1, 0.2, 0.3, 5.0]
+    need = [60.0, 50.0, 20.0, 10.0, 1.0]
+    pending = [0, 1, 2, 3, 4]
+    take, skip = serve.cohort_admit(pending, arrive, need, 1.0, 100.0, False)
+    assert take == [0, 2, 3] and skip == [1]
+    take, skip = serve.cohort_admit(pending, arrive, need, 0.15, 100.0, False)
+    assert take == [0] and skip == [1]
+    take, skip = serve.cohort_admit([1, 2], arrive, need, 1.0, 10.0, True)
+    assert take == [1] and skip == [2]  # empty: the head, although it exceeds the share
+    take, skip = serve.cohort_admit([4], arrive, need, 1.0, 10.0, True)
+    assert take == [] and skip == []  # not arrived yet

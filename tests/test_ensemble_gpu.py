@@ -270,6 +270,37 @@ def test_reserve_changes_no_result():
         np.testing.assert_array_equal(out[0][key], out[1][key], err_msg=key)
 
 
+def test_compact_changes_no_result_and_hands_memory_back():
+    """scphd_compact between frames (the map's clouds into a generation of
+    their own size, the spare generation and work arrays released): the
+    frames that follow give the same bits as without it, and the handle
+    holds less memory right after it."""
+    import scphd_amd
+
+    rigs, fixed = frames()
+    p, s, sets = rigs[1]
+    out = []
+    for compact in (False, True):
+        f = scphd_amd.Filter(p)
+        try:
+            f.reserve(p["n_particles"] * 400, 64)  # far more than the map needs
+            for k, (u, v, _) in enumerate(sets):
+                f.predict(s["dt"])
+                f.update(u, v, fixed[k])
+                f.resample()
+                if compact and k in (1, 3):
+                    before = f.memory()
+                    f.compact()
+                    assert f.memory() < before
+                    with pytest.raises(scphd_amd.ScphdError, match="STATE"):
+                        f.update_debug()  # released with the work arrays
+            out.append(f.state())
+        finally:
+            f.close()
+    for key in ("states", "weights", "gcount", "wg", "samples"):
+        np.testing.assert_array_equal(out[0][key], out[1][key], err_msg=key)
+
+
 BIG = [dict(seed=505, n_particles=4096, n_landmarks=200, lam=50.0, f=500.0, baseline=0.2),
        dict(seed=606, n_particles=1024, n_landmarks=120, lam=30.0, f=600.0, baseline=0.25)]
 
