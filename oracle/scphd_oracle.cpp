@@ -730,7 +730,7 @@ static int update(State& st, const double* zu, const double* zv, int M, int fixe
             pdg[k] = pdsum / S;
             Comp& f = pred[k];
             f.w = st.wg[g];
-            const double* q0 = &dsp[(size_t)k * S * 6];
+            const double* q0 = dsp.data() + (size_t)k * S * 6;
             if (pdsum == 0.0) {
                 /* D13: a cloud entirely out of view (pd_g = 0) has detection
                  * terms of weight 0, so its fit cannot reach any output; it
@@ -778,8 +778,9 @@ static int update(State& st, const double* zu, const double* zv, int M, int fixe
             const double lpd = safe_log(pdg[k]), lwg = safe_log(pred[k].w);
             for (int m = 0; m < M; ++m) lw[(size_t)m * G + k] = det_logw(pre[k], lpd, lwg, zu[m], zv[m], pred[k].m);
         }
-        double* eta = &st.eta[(size_t)p * M];
-        for (int m = 0; m < M; ++m) eta[m] = log_normaliser(&lw[(size_t)m * G], G, bpc);
+        /* (data() + offset: with M = 0 or G = 0 the vectors are empty) */
+        double* eta = st.eta.data() + (size_t)p * M;
+        for (int m = 0; m < M; ++m) eta[m] = log_normaliser(lw.data() + (size_t)m * G, G, bpc);
 
         /* single-cluster parent log-likelihood, :1813-1849 */
         double ls = 0.0;

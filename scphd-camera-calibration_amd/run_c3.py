@@ -51,7 +51,7 @@ def run(p, s, sc, frames, log):
         m, _, card = f.estimate()
         rows.append(dict(mean=m, ng=f.counts()["n_gaussians"], resampled=res, anc=f.ancestors() if res else None,
                          card=card))
-        if log and k % 20 == 0:
+        if log and k % 5 == 0:
             log(f"storage {p['storage']} frame {k} gaussians {rows[-1]['ng']} {time.perf_counter() - t0:.1f} s")
     f.close()
     return rows, time.perf_counter() - t0
