@@ -39,7 +39,7 @@ import workloads  # noqa: E402
 POSE = [0, 1, 2, 6, 7, 8]
 
 
-def run(p, s, sc, frames, log):
+def run(p, s, sc, frames, log, every=5):
     f = scphd_amd.Filter(p)
     rows = []
     t0 = time.perf_counter()
@@ -51,7 +51,7 @@ def run(p, s, sc, frames, log):
         m, _, card = f.estimate()
         rows.append(dict(mean=m, ng=f.counts()["n_gaussians"], resampled=res, anc=f.ancestors() if res else None,
                          card=card))
-        if log and k % 5 == 0:
+        if log and k % every == 0:
             log(f"storage {p['storage']} frame {k} gaussians {rows[-1]['ng']} {time.perf_counter() - t0:.1f} s")
     f.close()
     return rows, time.perf_counter() - t0
@@ -70,7 +70,7 @@ def main(argv=None):
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     storage = {"fp32": 0, "fp64": 1, "fp32-arith": 2}
     if args.fp32_only:
-        r32, t32 = run(dict(p, storage=0), s, sc, args.frames, log)
+        r32, t32 = run(dict(p, storage=0), s, sc, args.frames, log, every=1)
         print(json.dumps(dict(summary=True, parents=args.parents, frames=args.frames, seconds_fp32=t32,
                               ms_per_frame=t32 / args.frames * 1e3, final_gaussians=r32[-1]["ng"],
                               final_estimate_fp32=list(r32[-1]["mean"][POSE]))))
