@@ -240,7 +240,7 @@ struct scphd_filter {
     int64_t blocks_used = 0;
     /* per-Gaussian intermediates */
     DBuf<double> fit, pre, lik;
-    DBuf<int32_t> keep, keepscan, ndlist, redo, ndfill;
+    DBuf<int32_t> keep, keepscan, ndlist, holes, redo, ndfill;
     /* measurements, normalisers, survivors, merge */
     DBuf<double> du, dv, eta;
     int capM = 0;
@@ -267,7 +267,7 @@ struct scphd_filter {
 
     scphd_update_stats stats{};
     int64_t last_M = 0;
-    int64_t kt_hint = 0; /* merged Gaussians of the last update */
+    int64_t kt_hint = 0; /* blocks the last update's posterior clouds took beyond the dropped terms' */
     bool has_outputs = false; /* the last update's per-parent outputs are held (scphd_compact releases them) */
     bool timing = false;
     hipEvent_t ev[10] = {};  /* 0..5 phase boundaries, 6..7 fit kernel, 8..9 sample kernel */
@@ -312,7 +312,7 @@ void for_each_buf(scphd_filter* f, Fn&& fn) {
     fn(f->dbirths); fn(f->dsbase); fn(f->dfixed); fn(f->ext); fn(f->pw); fn(f->pose); fn(f->loglik); fn(f->cum); fn(f->ext_tmp); fn(f->anc);
     fn(f->newcnt); fn(f->goff); fn(f->gparent); fn(f->gblock); fn(f->gw); fn(f->goff2); fn(f->gparent2);
     fn(f->gblock2); fn(f->gw2); fn(f->gpd); fn(f->gpd2); fn(f->fit); fn(f->pre); fn(f->lik); fn(f->keep);
-    fn(f->keepscan); fn(f->ndlist); fn(f->redo); fn(f->ndfill); fn(f->du); fn(f->dv); fn(f->eta); fn(f->nsurv);
+    fn(f->keepscan); fn(f->ndlist); fn(f->holes); fn(f->redo); fn(f->ndfill); fn(f->du); fn(f->dv); fn(f->eta); fn(f->nsurv);
     fn(f->survoff); fn(f->nmerged); fn(f->mergeoff); fn(f->outcnt); fn(f->scantmp); fn(f->sg); fn(f->sm);
     fn(f->dflag); fn(f->mcnt); fn(f->mncand); fn(f->mcand); fn(f->ngroups); fn(f->glead); fn(f->gsta);
     fn(f->gend); fn(f->gmem); fn(f->sw); fn(f->smu); fn(f->mw); fn(f->mmu); fn(f->mP); fn(f->dneff);
@@ -457,6 +457,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
         HIPTRY(f->redo.ensure(Ng + 1)); /* [Ng]: the list length */
         HIPTRY(f->ndfill.ensure(Ng + 1)); /* [Ng]: the list length */
         HIPTRY(f->ndlist.ensure(Ng));
+        HIPTRY(f->holes.ensure(Ng));
         FitArgs fa;
         fa.gblock = f->gblock.p;
         fa.gparent = f->gparent.p;
@@ -645,8 +646,13 @@ int update_t(scphd_filter* f, int M, int fixed) {
     f->stats.cap_exceeded = capx;
     if (capx > 0) return fail(f, SCPHD_ERR_CAPACITY, "a parent map exceeds max_gaussians");
 
-    /* posterior samples into [Ng, Ng + Kt) of the next generation */
-    HIPTRY(f->gen[nxt].ensure(Ng + Kt, Ng, S, f->esz, s));
+    /* posterior clouds: first into the blocks of the dropped non-detection
+     * terms (k_fit wrote nothing there that is kept), the rest from block
+     * Ng on; the next generation holds Ng + max(0, Kt - holes) blocks */
+    const int64_t nholes = Ng - (Nout - Kt);
+    const int64_t over = std::max<int64_t>(0, Kt - nholes);
+    HIPTRY(launch_ndlist(s, f->keep.p, f->keepscan.p, Ng, f->ndlist.p, f->holes.p));
+    HIPTRY(f->gen[nxt].ensure(Ng + over, Ng, S, f->esz, s));
     SampleArgs pa;
     pa.mergeoff = f->mergeoff.p;
     pa.survoff = f->survoff.p;
@@ -661,21 +667,18 @@ int update_t(scphd_filter* f, int M, int fixed) {
     pa.rig_meas = f->ensemble ? f->dmeas.p : nullptr;
     pa.key = rig_keys(f, SC_UPDATE);
     pa.ustep = (uint64_t)f->n_update;
-    /* the posterior clouds go to blocks [Ng, Ng + Kt): the planes are
-     * passed from block Ng on, so that offsets start at 0 (32-bit while
-     * Kt S esz is below 4 GiB) */
-    pa.post_base = 0;
-    pa.off32 = Kt * (int64_t)S * (int64_t)f->esz < (int64_t)0xFFFFFFFFll - 64 * (int64_t)f->esz;
+    pa.post_base = Ng;
+    pa.holes = f->holes.p;
+    pa.nholes = nholes;
+    /* 32-bit byte offsets while every destination block lies below 4 GiB */
+    pa.off32 = (Ng + over) * (int64_t)S * (int64_t)f->esz < (int64_t)0xFFFFFFFFll - 64 * (int64_t)f->esz;
     pa.arith32 = f->arith32 ? 1 : 0;
-    Planes<T> post = f->gen[nxt].planes<T>();
-    for (int c = 0; c < 7; ++c) post.p[c] += Ng * (int64_t)S;
     record(f, 8);
-    HIPTRY(launch_sample<T>(s, pa, post));
+    HIPTRY(launch_sample<T>(s, pa, f->gen[nxt].planes<T>()));
     record(f, 9);
     record(f, 4);
 
     /* next map table */
-    HIPTRY(launch_ndlist(s, f->keep.p, f->keepscan.p, Ng, f->ndlist.p));
     HIPTRY(f->gparent2.ensure(Nout + 1));
     HIPTRY(f->gblock2.ensure(Nout + 1));
     HIPTRY(f->gw2.ensure(Nout + 1));
@@ -694,6 +697,8 @@ int update_t(scphd_filter* f, int M, int fixed) {
     ta.Np = Np;
     ta.Nout = Nout;
     ta.post_base = Ng;
+    ta.holes = f->holes.p;
+    ta.nholes = nholes;
     ta.nd_base = 0;
     ta.gparent = f->gparent2.p;
     ta.gblock = f->gblock2.p;
@@ -709,7 +714,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
     f->hint_ok = true;
     f->cur = nxt;
     f->Ng = Nout;
-    f->blocks_used = Ng + Kt; /* non-detection clouds below Ng, posteriors from Ng */
+    f->blocks_used = Ng + over; /* non-detection clouds and posteriors in holes below Ng, the rest from Ng */
     f->n_update++;
 
     f->stats.n_meas = f->ensemble ? f->ens_meas : M;
@@ -721,7 +726,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
     f->stats.n_gaussians_out = Nout;
     f->stats.max_survivors_per_parent = maxU;
     f->last_M = M;
-    f->kt_hint = Kt;
+    f->kt_hint = over;
     f->has_outputs = true;
     if (f->timing) {
         HIPTRY(hipEventSynchronize(f->ev[5]));
@@ -1706,7 +1711,7 @@ int scphd_compact(scphd_t* f) {
      * go with them */
     f->gen[1 - f->cur].release();
     for (auto* b : {&f->fit, &f->pre, &f->lik, &f->eta, &f->sw, &f->smu, &f->mw, &f->mmu, &f->mP}) b->release();
-    for (auto* b : {&f->keep, &f->keepscan, &f->ndlist, &f->redo, &f->ndfill, &f->mcnt, &f->mncand, &f->mcand, &f->sg,
+    for (auto* b : {&f->keep, &f->keepscan, &f->ndlist, &f->holes, &f->redo, &f->ndfill, &f->mcnt, &f->mncand, &f->mcand, &f->sg,
                     &f->sm, &f->glead, &f->gsta, &f->gend, &f->gmem, &f->gparent2, &f->gblock2, &f->nsurv,
                     &f->survoff, &f->nmerged, &f->mergeoff, &f->outcnt, &f->ngroups})
         b->release();
@@ -1738,6 +1743,7 @@ int scphd_reserve(scphd_t* f, int64_t n_gaussians, int32_t max_meas) {
     HIPTRY(f->redo.ensure(G + 1));
     HIPTRY(f->ndfill.ensure(G + 1));
     HIPTRY(f->ndlist.ensure(G));
+    HIPTRY(f->holes.ensure(G));
     HIPTRY(f->scantmp.ensure(scan_tmp_size(std::max<int64_t>(G, Np) + 1) + 16));
     /* the successor table, and the current one keeping its entries, so that
      * the table swaps of later updates and resamplings find both sized */

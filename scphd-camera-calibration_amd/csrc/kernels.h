@@ -281,7 +281,11 @@ struct SampleArgs {
     RigKeys key;
     uint64_t ustep;
     int64_t post_base;
-    int off32; /* every destination plane is below 4 GiB: 32-bit byte offsets */
+    /* merged Gaussian gm's block: holes[gm] while gm < nholes (the blocks
+     * of dropped non-detection terms), else post_base + gm - nholes */
+    const int32_t* holes = nullptr;
+    int64_t nholes = 0;
+    int off32; /* every destination offset is below 4 GiB: 32-bit byte offsets */
     int arith32 = 0; /* draws and disparity -> world in single precision (FP32 storage only) */
 };
 template <typename T>
@@ -289,7 +293,10 @@ hipError_t launch_sample(hipStream_t s, const SampleArgs& a, Planes<T> dst);
 
 hipError_t launch_outcount(hipStream_t s, const int32_t* nmerged, const int32_t* goff, const int32_t* keepscan,
                            int Np, int32_t* outcnt);
-hipError_t launch_ndlist(hipStream_t s, const int32_t* keep, const int32_t* keepscan, int64_t Ng, int32_t* ndlist);
+/* ndlist[keepscan[g]] = g for the kept entries; with holes, also
+ * holes[g - keepscan[g]] = g for the dropped ones */
+hipError_t launch_ndlist(hipStream_t s, const int32_t* keep, const int32_t* keepscan, int64_t Ng, int32_t* ndlist,
+                         int32_t* holes = nullptr);
 
 struct TableArgs {
     const int32_t* outoff;   /* Np+1 */
@@ -305,6 +312,8 @@ struct TableArgs {
     int Np;
     int64_t Nout;
     int64_t post_base;
+    const int32_t* holes = nullptr; /* posterior blocks as in SampleArgs */
+    int64_t nholes = 0;
     int64_t nd_base;
     int32_t* gparent;
     int32_t* gblock;

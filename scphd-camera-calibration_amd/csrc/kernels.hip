@@ -2077,14 +2077,22 @@ __global__ __launch_bounds__(256) void k_estimate(const double* ext, const doubl
 }
 
 /* ---------------------------------------------------- posterior samples */
+/* the block of merged Gaussian gm: a dropped non-detection term's block
+ * while there are any, else the next one after them */
+__device__ inline int64_t posterior_block(const int32_t* holes, int64_t nholes, int64_t post_base, int64_t gm) {
+    return gm < nholes ? (int64_t)holes[gm] : post_base + (gm - nholes);
+}
+
 /* Sixteen lanes (one row of a wavefront) per merged Gaussian, four per
  * wavefront: S draws x = L z + mu with L the Cholesky factor of the merged
  * covariance (sample_disparity_gaussian, src/scphdcameracalibration.cu:640-690),
  * factored once per lane and used for every sample the lane draws, mapped
  * to the world by the parent's camera (disparityToWorldKernel,
- * src/kernels.cu:49-80), sample weight 1/S (:2009-2010).  Destination
- * addresses are byte offsets from the plane bases (32-bit when every plane
- * is below 4 GiB). */
+ * src/kernels.cu:49-80), sample weight 1/S (:2009-2010).  Merged Gaussian
+ * gm's cloud goes to the block of a dropped non-detection term while there
+ * are any (holes), else after them (posterior_block).  Destination
+ * addresses are byte offsets from the plane bases (32-bit while every
+ * destination lies below 4 GiB). */
 template <typename T, typename Off, bool RG, typename A = double>
 __global__ __launch_bounds__(256) void k_sample(SampleArgs a, Planes<T> dst) {
     constexpr bool F32 = std::is_same_v<A, float>;
@@ -2106,7 +2114,7 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a, Planes<T> dst) {
     chol6(P, L);
     const CameraPose pose = load_pose(a.pose, p);
     const T w = (T)(1.0 / a.S);
-    Off off = (Off)(((a.post_base + gm) * (int64_t)a.S + lane) * (int64_t)sizeof(T));
+    Off off = (Off)((posterior_block(a.holes, a.nholes, a.post_base, gm) * (int64_t)a.S + lane) * (int64_t)sizeof(T));
     PosteriorCarry pc; /* normals carried between the two steps of a noise group */
     if constexpr (F32) {
         /* the FP32-arithmetic variant: the factor and mean rounded to
@@ -2170,10 +2178,16 @@ __global__ void k_outcount(const int32_t* nmerged, const int32_t* goff, const in
     outcnt[p] = nmerged[p] + (keepscan[goff[p + 1]] - keepscan[goff[p]]);
 }
 
-__global__ void k_ndlist(const int32_t* keep, const int32_t* keepscan, int64_t Ng, int32_t* ndlist) {
+__global__ void k_ndlist(const int32_t* keep, const int32_t* keepscan, int64_t Ng, int32_t* ndlist,
+                         int32_t* holes) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g < Ng && keep[g]) ndlist[keepscan[g]] = (int32_t)g;
+    if (g >= Ng) return;
+    if (keep[g])
+        ndlist[keepscan[g]] = (int32_t)g;
+    else if (holes)
+        holes[g - keepscan[g]] = (int32_t)g;
 }
+
 
 /* New map table per parent: [merged posterior Gaussians | kept
  * non-detection Gaussians in input order] (recombineNonDetections,
@@ -2186,7 +2200,7 @@ __global__ void k_table(TableArgs a) {
     const int j = (int)(o - a.outoff[p]);
     const int K = a.mergeoff[p + 1] - a.mergeoff[p];
     if (j < K) {
-        a.gblock[o] = (int32_t)(a.post_base + a.mergeoff[p] + j);
+        a.gblock[o] = (int32_t)posterior_block(a.holes, a.nholes, a.post_base, (int64_t)a.mergeoff[p] + j);
         a.gwo[o] = a.mw[(int64_t)a.survoff[p] + j];
         a.gpd[o] = 1.0f;
     } else {
@@ -2684,9 +2698,10 @@ hipError_t launch_outcount(hipStream_t s, const int32_t* nmerged, const int32_t*
     LAUNCH_CHECK();
 }
 
-hipError_t launch_ndlist(hipStream_t s, const int32_t* keep, const int32_t* keepscan, int64_t Ng, int32_t* ndlist) {
+hipError_t launch_ndlist(hipStream_t s, const int32_t* keep, const int32_t* keepscan, int64_t Ng, int32_t* ndlist,
+                         int32_t* holes) {
     if (Ng <= 0) return hipSuccess;
-    k_ndlist<<<nblocks(Ng, 256), 256, 0, s>>>(keep, keepscan, Ng, ndlist);
+    k_ndlist<<<nblocks(Ng, 256), 256, 0, s>>>(keep, keepscan, Ng, ndlist, holes);
     LAUNCH_CHECK();
 }
 
