@@ -127,6 +127,22 @@ SCPHD_HD inline double uniform_oc(uint32_t lo, uint32_t hi) { /* (0, 1] */
  *       (|error| < 4e-8);
  *   r = sqrt(-2 log a); z0 = r cos 2 pi c, z1 = r sin 2 pi c (single
  *   precision; box_muller widens them to double). */
+/* The correctly rounded single-precision square root of x = +-0 or x in
+ * [2^-96, 2^128): the hardware root (within 1 ulp for such x) moved by one
+ * ulp down or up when the residual x - s'^2 of that neighbour shows it is
+ * the nearer root, the same correction as the compiler's IEEE sqrt, without
+ * its scaling of tiny inputs and its special cases (+-0 pass through the
+ * correction unchanged: the neighbour below is a NaN, the one above gives a
+ * zero residual). */
+__device__ inline float sqrt_rn_normal(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float dn = __uint_as_float(__float_as_uint(s) - 1u);
+    const float up = __uint_as_float(__float_as_uint(s) + 1u);
+    float r = __builtin_fmaf(-dn, s, x) <= 0.0f ? dn : s;
+    r = __builtin_fmaf(-up, s, x) > 0.0f ? up : r;
+    return r;
+}
+
 __device__ inline void box_muller_f(uint32_t x0, uint32_t x1, float& z0, float& z1) {
 #pragma clang fp contract(off)
     const uint32_t n = x0 + 1u; /* 0 stands for a = 1 (log a = 0) */
@@ -151,7 +167,9 @@ __device__ inline void box_muller_f(uint32_t x0, uint32_t x1, float& z0, float& 
     pl = __builtin_fmaf(pl, sx, 1.0f);
     const float lf = sx * pl;
     const float L = n != 0u ? __builtin_fmaf((float)e, 0x1.62e430p-1f, lf) : 0.0f;
-    const float r = __builtin_sqrtf(-2.0f * L); /* correctly rounded (IEEE sqrt) */
+    /* -2 L is -0 (a = 1) or at least 2^-23 (a <= 1 - 2^-24): correctly
+     * rounded without the tiny-input path */
+    const float r = sqrt_rn_normal(-2.0f * L);
     const uint32_t c = x1 >> 8;
     const uint32_t oct = c >> 21;
     const uint32_t fr = c & 0x1FFFFFu;

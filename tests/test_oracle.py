@@ -86,6 +86,37 @@ def test_box_muller_polynomials_match_libm():
     assert abs(np.hypot(*oracle.box_muller(0, 0)) - math.sqrt(64 * math.log(2))) < 1e-6
 
 
+def _sqrt_corrected(x, s):
+    """The device's root correction (sqrt_rn_normal): from a hardware root s
+    within one ulp, the neighbour below when x - dn s <= 0, the one above
+    when x - up s > 0.  Each fma residual is formed exactly in float64 (the
+    product of two floats is exact there) and only its sign is used."""
+    x = np.float32(x)
+    dn = np.frombuffer(np.uint32(np.float32(s).view(np.uint32) - 1).tobytes(), np.float32)[0]
+    up = np.frombuffer(np.uint32(np.float32(s).view(np.uint32) + 1).tobytes(), np.float32)[0]
+    r = dn if float(x) - float(dn) * float(s) <= 0.0 else np.float32(s)
+    return up if float(x) - float(up) * float(s) > 0.0 else r
+
+
+def test_square_root_correction_recovers_the_correctly_rounded_root():
+    """Whichever of the three roots within one ulp the hardware returns,
+    the correction gives numpy's correctly rounded float32 sqrt, over the
+    range of -2 log a (2^-23 .. 45), with exact squares and the values next
+    to them; +-0 pass through unchanged."""
+    rng = np.random.default_rng(3)
+    xs = np.exp(rng.uniform(np.log(2.0**-23), np.log(45.0), 20000)).astype(np.float32)
+    sq = (np.arange(1, 6000, dtype=np.float32) / np.float32(128.0)) ** 2  # exact squares below 45
+    xs = np.concatenate([xs, sq, np.nextafter(sq, np.float32(0)), np.nextafter(sq, np.float32(64)),
+                         np.float32([2.0**-23, 45.0, 1.0, 2.0, 4.0])])
+    for x in xs:
+        t = np.sqrt(x)
+        for s in (np.nextafter(t, np.float32(0)), t, np.nextafter(t, np.float32(64))):
+            assert _sqrt_corrected(x, s) == t, (x, s, t)
+    for z in (np.float32(0.0), np.float32(-0.0)):
+        r = _sqrt_corrected(z, z)
+        assert r == 0 and np.signbit(r) == np.signbit(z)
+
+
 def test_normals_are_box_muller_of_philox_words():
     w = oracle.philox([(7 << 2) | 1, 3, 2, (5 << 4) | 2], [11, 0])
     z = oracle.normals(11, 2, 5, 2, 3, 7, 8)
