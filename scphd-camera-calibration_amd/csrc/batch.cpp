@@ -222,8 +222,9 @@ int64_t scphd_rig_footprint(const scphd_params* p, int32_t gpp) {
      * two tables, flags */
     const double per_gauss = Np * G * (69.0 * 8.0 + 2.0 * 16.0 + 4.0 * 4.0);
     /* survivors (at most about one per Gaussian in steady state): lists,
-     * merged means and covariances, merge groups */
-    const double per_surv = Np * G * (64.0 + 28.0 * 8.0 + 16.0);
+     * merged means and covariances, merge groups, the global-scratch merge
+     * workspace (240 B a slot) */
+    const double per_surv = Np * G * (64.0 + 28.0 * 8.0 + 16.0 + 240.0);
     const double parents = Np * 256.0;
     return (int64_t)(slack * (samples + per_gauss + per_surv + parents)) + (64LL << 20);
 }

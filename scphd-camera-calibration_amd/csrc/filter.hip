@@ -1765,6 +1765,9 @@ int scphd_reserve(scphd_t* f, int64_t n_gaussians, int32_t max_meas) {
     HIPTRY(f->mmu.ensure(6 * scap));
     HIPTRY(f->mP.ensure(21 * scap));
     for (DBuf<int32_t>* b : {&f->glead, &f->gsta, &f->gend, &f->gmem}) HIPTRY(b->ensure(G + 1));
+    /* the global-scratch merge workspace, sized like the survivor lists (a
+     * growth between frames would otherwise allocate in the update) */
+    HIPTRY(f->mscratch.ensure(merge_scratch_bytes(G + 1, (int)Np)));
     /* per (parent, measurement) and the measurement sets */
     HIPTRY(f->eta.ensure(Np * M));
     HIPTRY(f->mcnt.ensure(Np * M));
