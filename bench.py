@@ -246,6 +246,7 @@ def main():
     lat, updates, phase = [], [], []
     fit_bytes, fit_ms, smp_bytes, smp_ms = [], [], [], []
     counts = []
+    step_bytes = []  # compulsory state traffic of each timed step (below)
     t_start = None
     for k in range(frames):
         u, v, fixed = sets[k]
@@ -274,6 +275,12 @@ def main():
             smp_bytes.append(7 * esz * st["n_merged"] * S)
             smp_ms.append(ph[7])
             counts.append(st)
+            # read state k, write state k+1, read the measurements: 7 sample
+            # values and a Gaussian weight per map entry in and out, 12 FP64
+            # extrinsics + weight per parent each way, (u, v) per measurement
+            n_out = f.counts()["n_gaussians"]
+            step_bytes.append((7 * esz * S + esz) * (st["n_gaussians_in"] + n_out) + 2 * 104 * p["n_particles"]
+                              + 16 * n)
     f.sync()
     total = time.perf_counter() - t_start
     du.free()
@@ -339,6 +346,16 @@ def main():
             "k_sample_achieved_gbs": smp_gbs,
             "limiter_note": "hbm is the roofline the bytes are priced against; the kernel's limiter is VALU "
                             "issue (SQ counters in profiles/k_fit_c1_counters.txt), with HBM close behind",
+        },
+        "step_roofline": {
+            "definition": "whole step: compulsory state bytes (read state k, write state k+1, read the "
+                          "measurements: (7 sample values + weight) x esz per map entry in and out, 2 x 104 B "
+                          "per parent, 16 B per measurement) over the whole timed frame time",
+            "achieved": float(np.sum(step_bytes) / total / 1e9),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": float(np.sum(step_bytes) / total / 1e9) / HBM_PEAK_GBS,
+            "bytes_last_step": int(step_bytes[-1]),
         },
     }
     if not args.no_served:
