@@ -131,6 +131,17 @@ def test_lockstep_small(storage):
     assert rep[-1][1] > 0
 
 
+def test_lockstep_c0_full_size():
+    """The whole C0 workload, 256 parents, S = 100, 30 landmarks, lambda 10,
+    50 frames with the cameras alternating: every frame's update from the
+    oracle's state, compared after every frame (maps of ~150 Gaussians per
+    parent, ~40,000 in all)."""
+    p, s = workloads.workload("c0")
+    assert p["n_particles"] == 256 and p["particles_per_feature"] == 100 and s["n_frames"] == 50
+    rep = run_lockstep(p, s, frames=50)
+    assert min(r[1] for r in rep[5:]) > 20000
+
+
 def test_lockstep_after_warmup_fp32():
     """Lock-step frames from a state grown by 8 oracle frames (maps of
     ~100 Gaussians per parent)."""
