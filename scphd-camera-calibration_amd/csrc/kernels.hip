@@ -202,13 +202,29 @@ __device__ inline CameraPose load_pose(const double* pose, int p) {
 }
 
 /* Landmark sample prediction (predict_features functor,
- * src/scphdcameracalibration.cu:154-178) for the sample (p, k, s). */
+ * src/scphdcameracalibration.cu:154-178): the constant-velocity step with
+ * acceleration draws a[i] * std_i (LinearCVMotionModel3D::computeNoisyMotion),
+ * x + v dt + a std dt^2 / 2 and v + a std dt, the products std dt^2 / 2 and
+ * std dt formed once per kernel (loop invariants) instead of per sample. */
+__device__ inline EuclideanPoint noisy_motion(EuclideanPoint x, const LinearCVMotionModel3D& feat, double dt,
+                                              const double a[3]) {
+    const double h = 0.5 * dt * dt;
+    x.x = fma(a[0], feat.std_ax() * h, fma(x.vx, dt, x.x));
+    x.y = fma(a[1], feat.std_ay() * h, fma(x.vy, dt, x.y));
+    x.z = fma(a[2], feat.std_az() * h, fma(x.vz, dt, x.z));
+    x.vx = fma(a[0], feat.std_ax() * dt, x.vx);
+    x.vy = fma(a[1], feat.std_ay() * dt, x.vy);
+    x.vz = fma(a[2], feat.std_az() * dt, x.vz);
+    return x;
+}
+
+/* the prediction of the sample (p, k, s) */
 template <typename Key>
 __device__ inline EuclideanPoint predict_sample(EuclideanPoint x, const LinearCVMotionModel3D& feat, double dt,
                                                 Key key, uint64_t step, uint32_t p, uint32_t k, uint32_t s) {
     double a[3];
     feature_normals(key, step, p, k, s, a);
-    return feat.computeNoisyMotion(x, dt, a[0] * feat.std_ax(), a[1] * feat.std_ay(), a[2] * feat.std_az());
+    return noisy_motion(x, feat, dt, a);
 }
 
 /* the same for the members of a noise group in order (phase = member) */
@@ -218,7 +234,7 @@ __device__ inline EuclideanPoint predict_sample_seq(EuclideanPoint x, const Line
                                                     uint32_t phase, FeatureCarry& c) {
     double a[3];
     feature_normals_seq(key, step, p, k, s, phase, c, a);
-    return feat.computeNoisyMotion(x, dt, a[0] * feat.std_ax(), a[1] * feat.std_ay(), a[2] * feat.std_az());
+    return noisy_motion(x, feat, dt, a);
 }
 
 template <typename T>
