@@ -610,6 +610,7 @@ int update_t(scphd_filter* f, int M, int fixed) {
     ma.gmem = f->gmem.p;
     ma.Stot = Stot;
     ma.big_lds_up = 0;
+    ma.big_lds_bytes = 0;
     if (maxU > lds_cap) {
         HIPTRY(f->mscratch.ensure(merge_scratch_bytes(Stot, Np)));
         ma.scratch = f->mscratch.p;
@@ -617,6 +618,10 @@ int update_t(scphd_filter* f, int M, int fixed) {
         int up = 1;
         while (up < maxU) up <<= 1;
         ma.big_lds_up = up <= 4096 ? up : 0;
+        /* the sort keys (12 B per slot of next_pow2(U)), then, in the same
+         * LDS, 28 B per survivor for the walk's counts, offsets, labels,
+         * buckets and group records */
+        ma.big_lds_bytes = std::max(12 * ma.big_lds_up, ma.big_lds_up ? std::min(28 * maxU + 16, 65536) : 0);
     }
     HIPTRY(launch_merge(s, ma, merge_lds_bytes(std::max(1, std::min(maxU, lds_cap)))));
     if (!fixed)

@@ -208,6 +208,7 @@ struct MergeArgs {
     int32_t force_rounds; /* nonzero: every parent takes the round loop (verification of that path) */
     int32_t all_pairs;    /* nonzero: candidates from all pairs, as for ill-conditioned survivors (verification) */
     int32_t big_lds_up;   /* big parents with next_pow2(U) <= this sort in the launch's LDS (12 B per slot) */
+    int32_t big_lds_bytes; /* the big launch's LDS: the sort keys, then the walk's per-survivor arrays (<= 64 KB) */
     char* scratch;     /* global scratch for big parents: merge_scratch_bytes(Stot, Np) */
     double* mw;        /* merged outputs at survoff[p] + k */
     double* mmu;       /* 6 planes, stride scap */

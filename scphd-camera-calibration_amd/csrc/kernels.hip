@@ -1505,6 +1505,14 @@ __global__ __launch_bounds__(NT) void k_merge_groups(MergeArgs a) {
     __shared__ double s_red[8];
     const int Up = next_pow2(U);
     const bool keys_lds = BIG && Up <= a.big_lds_up;
+    /* once the keys are sorted, the LDS they held takes the walk's most
+     * accessed per-survivor arrays (global scratch otherwise): neighbour
+     * counts and offsets always, and when the launch's LDS allows (28 B per
+     * survivor) the component labels, bucket offsets and cursors and the
+     * per-rank group records */
+    int* lds_lab = nullptr;
+    int* lds_boff = nullptr;
+    int* lds_rinfo = nullptr;
     if (keys_lds) {
         double* key = (double*)lds;
         int* ord = (int*)(lds + (size_t)8 * Up);
@@ -1514,6 +1522,16 @@ __global__ __launch_bounds__(NT) void k_merge_groups(MergeArgs a) {
         if (s_nc <= w.ccap) {
             merge_sort_w(a, sbase, U, key, ord);
             for (int i = t; i < U; i += NT) w.ord[i] = ord[i];
+            __syncthreads(); /* the sort keys are read */
+            /* U + (U + 1) ints within the keys' 12 Up bytes */
+            w.cnt = (int*)lds;
+            w.nbo = (int*)lds + U;
+            if ((int64_t)(7 * U + 2) * 4 <= a.big_lds_bytes) {
+                lds_lab = (int*)lds + 2 * U + 1;
+                lds_boff = lds_lab + U;
+                lds_rinfo = lds_boff + 2 * U + 1;
+            }
+            for (int i = t; i < U; i += NT) w.cnt[i] = 0;
         }
     } else {
         merge_load_sort_u(a, sbase, U, w, w.key_w, w.ord);
@@ -1599,7 +1617,7 @@ __global__ __launch_bounds__(NT) void k_merge_groups(MergeArgs a) {
      * rank order by two scans.  Per rank r: rinfo[2 r] = group size | STOP,
      * rinfo[2 r + 1] = start of the members in tmp (connected leaders). */
     constexpr int STOP = 1 << 30;
-    int* rinfo = (int*)w.bu; /* free after the candidates: 8 bytes per slot */
+    int* rinfo = lds_rinfo ? lds_rinfo : (int*)w.bu; /* bu is free after the candidates: 8 bytes per slot */
     int* nl = w.cnt;         /* connected ranks, ascending; free after the lists */
     int* tmp = (int*)w.cand; /* members of connected groups; ccap >= U */
     {
@@ -1634,10 +1652,11 @@ __global__ __launch_bounds__(NT) void k_merge_groups(MergeArgs a) {
      *           by survivor index with the leader in its place. */
     {
         const int nn = s_tot;
-        int* lab = (int*)w.bv;           /* label per survivor (bv is free after the candidates) */
-        int* boff = (int*)w.bs;          /* bucket offsets [U + 1] (bs is free after the candidates) */
+        int* lab = lds_lab ? lds_lab : (int*)w.bv; /* label per survivor (bv is free after the candidates) */
+        int* gb = (int*)w.bs;            /* bs is free after the candidates */
+        int* boff = lds_boff ? lds_boff : gb; /* bucket offsets [U + 1] */
         int* bcur = boff + (U + 1);      /* scatter cursors [U] */
-        int* bkt = bcur + U;             /* connected ranks by bucket [nn] */
+        int* bkt = gb + 2 * U + 1;       /* connected ranks by bucket [nn] */
         int* seg = (int*)w.key_w;        /* non-empty buckets (key_w is free after the sorts) */
         for (int i = t; i < U; i += NT) {
             lab[i] = i;
@@ -2648,7 +2667,7 @@ hipError_t launch_survivors(hipStream_t s, const SurvArgs& a) {
 
 hipError_t launch_merge(hipStream_t s, const MergeArgs& a, int lds_bytes) {
     k_merge_groups<false><<<a.Np, NT, lds_bytes, s>>>(a);
-    if (a.scratch) k_merge_groups<true><<<a.Np, NT, 12 * a.big_lds_up, s>>>(a);
+    if (a.scratch) k_merge_groups<true><<<a.Np, NT, a.big_lds_bytes, s>>>(a);
     if (a.Stot > 0) k_merge_moments<<<nblocks(a.Stot, 128), 128, 0, s>>>(a);
     k_merge_rounds<false><<<a.Np, NT, lds_bytes, s>>>(a);
     if (a.scratch) k_merge_rounds<true><<<a.Np, NT, 0, s>>>(a);
