@@ -218,9 +218,9 @@ int64_t scphd_rig_footprint(const scphd_params* p, int32_t gpp) {
     /* two sample generations: (non-detection + posterior) blocks of the next
      * map, and the current map */
     const double samples = 2.0 * Np * G * S * 7.0 * esz;
-    /* per Gaussian: fit 28 + detection terms 33 + likelihood record 8 FP64,
+    /* per Gaussian: fit 28 + detection terms 39 + likelihood record 8 FP64,
      * two tables, flags */
-    const double per_gauss = Np * G * (69.0 * 8.0 + 2.0 * 16.0 + 4.0 * 4.0);
+    const double per_gauss = Np * G * (75.0 * 8.0 + 2.0 * 16.0 + 4.0 * 4.0);
     /* survivors (at most about one per Gaussian in steady state): lists,
      * merged means and covariances, merge groups, the global-scratch merge
      * workspace (240 B a slot) */

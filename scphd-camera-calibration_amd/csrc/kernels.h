@@ -61,9 +61,10 @@ struct RigKeys {
 enum FitPlane { FP_MU = 0, FP_P = 6, FP_PD = 27, FP_COUNT = 28 };
 /* per-Gaussian detection terms, stride Ng */
 enum PrePlane {
-    PP_K = 0,   /* gain, 12 planes */
-    PP_PN = 12, /* updated covariance: from plane 12 on, 21 packed doubles per Gaussian, contiguous */
-    PP_COUNT = 33
+    PP_MK = 0,  /* per Gaussian, PP_MK_LEN consecutive doubles: the fitted mean (6), then the gain (12) */
+    PP_MK_LEN = 18,
+    PP_PN = 18, /* updated covariance: from plane 18 on, 21 packed doubles per Gaussian, contiguous */
+    PP_COUNT = 39
 };
 /* per-Gaussian likelihood record, LK_COUNT consecutive doubles (64 B):
  * predicted measurement, S^-1 entries (s0, s1 + s2, s3), log(pd_g w_g),

@@ -825,8 +825,11 @@ __global__ __launch_bounds__(128) void k_precompute(const double* fit, const dou
     rec[LK_C] = safe_log(pdg) + safe_log(gw[g]);
     rec[LK_H] = 0.5 * safe_log(r.det);
     rec[LK_OK] = ok ? 1.0 : 0.0;
+    double* mk = pre + PP_MK * Ng + g * PP_MK_LEN;
 #pragma unroll
-    for (int i = 0; i < 12; ++i) pre[(PP_K + i) * Ng + g] = r.K[i];
+    for (int c = 0; c < 6; ++c) mk[c] = fit[(FP_MU + c) * Ng + g];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) mk[6 + i] = r.K[i];
 #pragma unroll
     for (int e = 0; e < 21; ++e) pre[PP_PN * Ng + g * 21 + e] = r.Pn[e];
 }
@@ -1105,12 +1108,9 @@ __global__ __launch_bounds__(NT) void k_survivors(SurvArgs a) {
                     a.sw[so] = w;
                     a.sg[so] = (int32_t)gi;
                     a.sm[so] = m;
+                    const double* mk = e.pre + PP_MK * e.Ng + gi * PP_MK_LEN;
 #pragma unroll
-                    for (int c = 0; c < 6; ++c) {
-                        const double mu = a.fit[(FP_MU + c) * e.Ng + gi];
-                        a.smu[c * a.cap + so] =
-                            mu + i0 * e.pre[(PP_K + c) * e.Ng + gi] + i1 * e.pre[(PP_K + 6 + c) * e.Ng + gi];
-                    }
+                    for (int c = 0; c < 6; ++c) a.smu[c * a.cap + so] = mk[c] + i0 * mk[6 + c] + i1 * mk[12 + c];
                 }
                 ++o;
             };
