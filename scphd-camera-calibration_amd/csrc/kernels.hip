@@ -807,31 +807,60 @@ __global__ __launch_bounds__(256) void k_fit_fix(FitArgs a, Planes<T> src, Plane
 __global__ __launch_bounds__(128) void k_precompute(const double* fit, const double* gw, int64_t Ng, double var_u,
                                                     double var_v,
                              double* pre, double* lik) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= Ng) return;
-    double P[21];
-#pragma unroll
-    for (int e = 0; e < 21; ++e) P[e] = fit[(FP_P + e) * Ng + g];
+    /* each Gaussian's records are formed by its thread, then written by the
+     * workgroup through LDS as one contiguous stretch (records of
+     * consecutive Gaussians are adjacent), so that the stores coalesce */
+    __shared__ double stage[128 * 21];
+    const int t = threadIdx.x;
+    const int64_t g0 = (int64_t)blockIdx.x * 128;
+    const int64_t g = g0 + t;
+    const int nb = (int)min((int64_t)128, Ng - g0);
+    const bool act = g < Ng;
     DetectionTerms r;
-    detection_terms(P, var_u, var_v, r);
-    const bool ok = is_pd6(r.Pn);
-    const double pdg = fit[FP_PD * Ng + g];
-    double* rec = lik + g * LK_COUNT;
-    rec[LK_MUU] = fit[(FP_MU + 0) * Ng + g];
-    rec[LK_MUV] = fit[(FP_MU + 1) * Ng + g];
-    rec[LK_S0] = r.s0;
-    rec[LK_S12] = r.s1 + r.s2;
-    rec[LK_S3] = r.s3;
-    rec[LK_C] = safe_log(pdg) + safe_log(gw[g]);
-    rec[LK_H] = 0.5 * safe_log(r.det);
-    rec[LK_OK] = ok ? 1.0 : 0.0;
-    double* mk = pre + PP_MK * Ng + g * PP_MK_LEN;
+    bool ok = false;
+    double pdg = 0.0, mu[6];
+    if (act) {
+        double P[21];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) mk[c] = fit[(FP_MU + c) * Ng + g];
+        for (int e = 0; e < 21; ++e) P[e] = fit[(FP_P + e) * Ng + g];
+        detection_terms(P, var_u, var_v, r);
+        ok = is_pd6(r.Pn);
+        pdg = fit[FP_PD * Ng + g];
 #pragma unroll
-    for (int i = 0; i < 12; ++i) mk[6 + i] = r.K[i];
+        for (int c = 0; c < 6; ++c) mu[c] = fit[(FP_MU + c) * Ng + g];
+    }
+    /* out[g0 L ..) <- the block's L-double records */
+    auto put = [&](double* out, int L) {
+        __syncthreads();
+        for (int e = t; e < nb * L; e += 128) out[g0 * L + e] = stage[e];
+    };
+    if (act) {
+        double* rec = stage + t * LK_COUNT;
+        rec[LK_MUU] = mu[0];
+        rec[LK_MUV] = mu[1];
+        rec[LK_S0] = r.s0;
+        rec[LK_S12] = r.s1 + r.s2;
+        rec[LK_S3] = r.s3;
+        rec[LK_C] = safe_log(pdg) + safe_log(gw[g]);
+        rec[LK_H] = 0.5 * safe_log(r.det);
+        rec[LK_OK] = ok ? 1.0 : 0.0;
+    }
+    put(lik, LK_COUNT);
+    __syncthreads();
+    if (act) {
+        double* mk = stage + t * PP_MK_LEN;
 #pragma unroll
-    for (int e = 0; e < 21; ++e) pre[PP_PN * Ng + g * 21 + e] = r.Pn[e];
+        for (int c = 0; c < 6; ++c) mk[c] = mu[c];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) mk[6 + k] = r.K[k];
+    }
+    put(pre + PP_MK * Ng, PP_MK_LEN);
+    __syncthreads();
+    if (act) {
+#pragma unroll
+        for (int e = 0; e < 21; ++e) stage[t * 21 + e] = r.Pn[e];
+    }
+    put(pre + PP_PN * Ng, 21);
 }
 
 /* ------------------------------------------- normalisers and survivors */
