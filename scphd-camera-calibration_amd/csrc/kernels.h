@@ -187,7 +187,7 @@ struct SurvArgs {
     double* sw;
     int32_t* sg;            /* table index, or -1 - rig for a birth term */
     int32_t* sm;
-    double* smu;            /* 6 planes, stride cap */
+    double* smu;            /* 6 consecutive doubles per survivor */
     int64_t cap;
     int32_t* err;           /* set to 1 when a parent's count differs from nsurv */
 };

@@ -1139,7 +1139,7 @@ __global__ __launch_bounds__(NT) void k_survivors(SurvArgs a) {
                     a.sm[so] = m;
                     const double* mk = e.pre + PP_MK * e.Ng + gi * PP_MK_LEN;
 #pragma unroll
-                    for (int c = 0; c < 6; ++c) a.smu[c * a.cap + so] = mk[c] + i0 * mk[6 + c] + i1 * mk[12 + c];
+                    for (int c = 0; c < 6; ++c) a.smu[(so) * 6 + c] = mk[c] + i0 * mk[6 + c] + i1 * mk[12 + c];
                 }
                 ++o;
             };
@@ -1173,10 +1173,10 @@ __global__ __launch_bounds__(NT) void k_survivors(SurvArgs a) {
             a.sw[so] = w;
             a.sg[so] = -1 - rig;
             a.sm[so] = m;
-            a.smu[0 * a.cap + so] = e.zu[zb + m];
-            a.smu[1 * a.cap + so] = e.zv[zb + m];
+            a.smu[(so) * 6 + 0] = e.zu[zb + m];
+            a.smu[(so) * 6 + 1] = e.zv[zb + m];
 #pragma unroll
-            for (int c = 2; c < 6; ++c) a.smu[c * a.cap + so] = birth.mean[c];
+            for (int c = 2; c < 6; ++c) a.smu[(so) * 6 + c] = birth.mean[c];
         }
         run += tot;
     }
@@ -1249,7 +1249,7 @@ __device__ inline void survivor_P(const MergeArgs& a, int64_t si, double* P) {
 
 __device__ inline void survivor_mu(const MergeArgs& a, int64_t si, double* mu) {
 #pragma unroll
-    for (int c = 0; c < 6; ++c) mu[c] = a.smu[c * a.scap + si];
+    for (int c = 0; c < 6; ++c) mu[c] = a.smu[(si) * 6 + c];
 }
 
 /* Workspace pointers of one parent's merge. */
@@ -1362,8 +1362,8 @@ __device__ inline void merge_load_sort_u(const MergeArgs& a, int64_t sbase, int 
     const int Up = next_pow2(U);
     for (int i = t; i < Up; i += NT) {
         if (i < U) {
-            w.bu[i] = a.smu[0 * a.scap + sbase + i];
-            w.bv[i] = a.smu[1 * a.scap + sbase + i];
+            w.bu[i] = a.smu[(sbase + i) * 6 + 0];
+            w.bv[i] = a.smu[(sbase + i) * 6 + 1];
             const int g = a.sg[sbase + i];
             w.bs[3 * i] = term_P(a, g, pidx(0, 0));
             w.bs[3 * i + 1] = term_P(a, g, pidx(1, 0));
@@ -1836,7 +1836,7 @@ __global__ __launch_bounds__(128) void k_merge_moments(MergeArgs a) {
         const int64_t si = sbase + a.gmem[sbase + e];
         const double w = a.sw[si];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) mu[c] += w * a.smu[c * a.scap + si];
+        for (int c = 0; c < 6; ++c) mu[c] += w * a.smu[(si) * 6 + c];
     }
 #pragma unroll
     for (int c = 0; c < 6; ++c) mu[c] = mu[c] / W;
@@ -1849,7 +1849,7 @@ __global__ __launch_bounds__(128) void k_merge_moments(MergeArgs a) {
         const int g = a.sg[si];
         double dl[6];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) dl[c] = mu[c] - a.smu[c * a.scap + si];
+        for (int c = 0; c < 6; ++c) dl[c] = mu[c] - a.smu[(si) * 6 + c];
 #pragma unroll
         for (int r = 0; r < 6; ++r)
 #pragma unroll
@@ -1872,7 +1872,7 @@ __global__ __launch_bounds__(128) void k_merge_moments(MergeArgs a) {
         const int g = a.sg[li];
         a.mw[o] = a.sw[li];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) a.mmu[c * a.scap + o] = a.smu[c * a.scap + li];
+        for (int c = 0; c < 6; ++c) a.mmu[c * a.scap + o] = a.smu[(li) * 6 + c];
 #pragma unroll
         for (int q = 0; q < 21; ++q)
             a.mP[q * a.scap + o] = term_P(a, g, q);
@@ -1923,7 +1923,7 @@ __global__ __launch_bounds__(NT) void k_merge_rounds(MergeArgs a) {
         __syncthreads();
         const int lead = s_lead;
         if (lead < 0) break;
-        if (t < 6) s_lmu[t] = a.smu[t * a.scap + sbase + lead];
+        if (t < 6) s_lmu[t] = a.smu[(sbase + lead) * 6 + t];
         if (t == 0) s_lw = a.sw[sbase + lead];
         if (t < 21) {
             const int g = a.sg[sbase + lead];
@@ -1962,7 +1962,7 @@ __global__ __launch_bounds__(NT) void k_merge_rounds(MergeArgs a) {
             double acc = 0.0;
             for (int j = 0; j < nm; ++j) {
                 const int64_t si = sbase + w.mem[j];
-                acc += a.sw[si] * a.smu[t * a.scap + si];
+                acc += a.sw[si] * a.smu[(si) * 6 + t];
             }
             s_mu[t] = acc / W;
         }
@@ -1976,8 +1976,8 @@ __global__ __launch_bounds__(NT) void k_merge_rounds(MergeArgs a) {
                 const int64_t si = sbase + w.mem[j];
                 const int g = a.sg[si];
                 const double Pe = term_P(a, g, t);
-                const double dc = s_mu[c] - a.smu[c * a.scap + si];
-                const double dr = s_mu[r] - a.smu[r * a.scap + si];
+                const double dc = s_mu[c] - a.smu[(si) * 6 + c];
+                const double dr = s_mu[r] - a.smu[(si) * 6 + r];
                 acc += a.sw[si] * (Pe + dc * dr);
             }
             s_P[t] = acc / W;
